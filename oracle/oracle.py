@@ -1,0 +1,193 @@
+"""ctypes binding of the FP64 oracle (libibx_oracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py as the checker; the product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from ibisco_amd.arrays import flatten
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB = os.path.join(HERE, "libibx_oracle.so")
+
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+
+
+class _Tables(C.Structure):
+    _fields_ = [("ntab", C.c_int), ("types", _ip), ("off", _ip), ("nrow", _ip), ("x", _dp), ("v", _dp)]
+
+
+class _Desc(C.Structure):
+    _fields_ = [
+        ("natoms", C.c_int), ("nvs", C.c_int), ("ntype", C.c_int),
+        ("mass", _dp), ("label", _ip), ("itype", _ip), ("nbonds", _ip), ("jbond", _ip),
+        ("s", _dp), ("v", _dp), ("box", C.c_double * 3),
+        ("bond", _Tables), ("angle", _Tables), ("tors", _Tables), ("nb", _Tables), ("oop", _Tables),
+        ("vs_type", _ip), ("vs_center", _ip), ("vs_off", _ip), ("vs_atoms", _ip),
+        ("nvirtbonds", C.c_int), ("virtbonds", _ip), ("nvirtangles", C.c_int), ("virtangles", _ip),
+        ("ensemble", C.c_int), ("dt", C.c_double), ("temp_in", C.c_double), ("fac", C.c_double),
+        ("rcut_atom", C.c_double), ("rlist_atom", C.c_double), ("rcut_bead", C.c_double),
+        ("rlist_bead", C.c_double), ("nonbexc_atom", C.c_int), ("nonbexc_bead", C.c_int),
+        ("nupdate", C.c_int), ("halt_drift", C.c_int), ("nmts", C.c_int), ("mts_quirk", C.c_int),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB):
+            build()
+        L = C.CDLL(LIB)
+        L.ox_create.restype = C.c_void_p
+        L.ox_create.argtypes = [C.POINTER(_Desc), C.c_char_p, C.c_int]
+        for nm in ("ox_destroy", "ox_shift", "ox_virtual_def", "ox_new_force", "ox_move", "ox_momentum"):
+            getattr(L, nm).argtypes = [C.c_void_p]
+            getattr(L, nm).restype = None
+        L.ox_update_lists.argtypes = [C.c_void_p, C.c_int]
+        L.ox_pairs.restype = C.c_int64
+        L.ox_pairs.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int64]
+        L.ox_exclusions.argtypes = [C.c_void_p, _ip, _ip, C.c_int]
+        L.ox_terms.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int]
+        L.ox_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+        L.ox_get_obs.argtypes = [C.c_void_p, _dp]
+        L.ox_get_obs.restype = None
+        L.ox_nitems.argtypes = [C.c_void_p]
+        L.ox_get_forces.argtypes = [C.c_void_p, _dp]
+        L.ox_get_forces.restype = None
+        L.ox_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        L.ox_get_state.restype = None
+        L.ox_set_state.argtypes = [C.c_void_p, _dp, _dp]
+        L.ox_set_state.restype = None
+        L.ox_table.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+OBS_KEYS = ("vnb", "vbond", "vangle", "vtors", "voop")
+
+
+class Oracle:
+    """FP64 restatement of the reference path for one system."""
+
+    def __init__(self, sysm, mts_quirk=False):
+        self.arrays = a = flatten(sysm)
+        self._keep = []
+        d = _Desc()
+        d.natoms, d.nvs, d.ntype = a["natoms"], a["nvs"], a["ntype"]
+
+        def arr(x, t):
+            x = np.ascontiguousarray(x)
+            self._keep.append(x)
+            return _p(x, t)
+
+        d.mass, d.label = arr(a["mass"], C.c_double), arr(a["label"], C.c_int)
+        d.itype, d.nbonds, d.jbond = arr(a["itype"], C.c_int), arr(a["nbonds"], C.c_int), arr(a["jbond"], C.c_int)
+        d.s, d.v = arr(a["s"], C.c_double), arr(a["v"], C.c_double)
+        for k in range(3):
+            d.box[k] = a["box"][k]
+        for kind in ("bond", "angle", "tors", "nb", "oop"):
+            t = a["tables"][kind]
+            tb = _Tables(t["ntab"], arr(t["types"], C.c_int), arr(t["off"], C.c_int), arr(t["nrow"], C.c_int),
+                         arr(t["x"], C.c_double), arr(t["v"], C.c_double))
+            setattr(d, kind, tb)
+        d.vs_type, d.vs_center = arr(a["vs_type"], C.c_int), arr(a["vs_center"], C.c_int)
+        d.vs_off, d.vs_atoms = arr(a["vs_off"], C.c_int), arr(a["vs_atoms"], C.c_int)
+        d.nvirtbonds, d.virtbonds = a["nvirtbonds"], arr(a["virtbonds"], C.c_int)
+        d.nvirtangles, d.virtangles = a["nvirtangles"], arr(a["virtangles"], C.c_int)
+        d.ensemble, d.dt, d.temp_in, d.fac = a["ensemble"], a["dt"], a["temp_in"], a["fac"]
+        d.rcut_atom, d.rlist_atom = a["rcut_atom"], a["rlist_atom"]
+        d.rcut_bead, d.rlist_bead = a["rcut_bead"], a["rlist_bead"]
+        d.nonbexc_atom, d.nonbexc_bead = a["nonbexc_atom"], a["nonbexc_bead"]
+        d.nupdate, d.halt_drift, d.nmts, d.mts_quirk = a["nupdate"], a["halt_drift"], a["nmts"], int(mts_quirk)
+        err = C.create_string_buffer(256)
+        self.L = lib()
+        self.h = self.L.ox_create(C.byref(d), err, 256)
+        if not self.h:
+            raise RuntimeError("oracle: " + err.value.decode())
+        self.natoms = a["natoms"]
+        self.nitems = self.L.ox_nitems(self.h)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.ox_destroy(self.h)
+            self.h = None
+
+    def update_lists(self, mode=0):
+        if self.L.ox_update_lists(self.h, mode):
+            raise RuntimeError("oracle: list build failed")
+
+    def pairs(self, group=0):
+        n = self.L.ox_pairs(self.h, group, None, 0)
+        out = np.zeros((max(n, 1), 2), np.int32)
+        self.L.ox_pairs(self.h, group, _p(out, C.c_int), n)
+        return out[:n]
+
+    def exclusions(self):
+        n = self.L.ox_exclusions(self.h, None, None, 0)
+        off = np.zeros(self.nitems + 1, np.int32)
+        idx = np.zeros(max(n, 1), np.int32)
+        self.L.ox_exclusions(self.h, _p(off, C.c_int), _p(idx, C.c_int), n)
+        return off, idx[:n]
+
+    def terms(self, kind):
+        n = self.L.ox_terms(self.h, kind, None, 0)
+        w = 3 if kind in (0, 2) else 4
+        out = np.zeros((max(n, 1), w), np.int32)
+        self.L.ox_terms(self.h, kind, _p(out, C.c_int), n)
+        return out[:n]
+
+    def new_force(self):
+        self.L.ox_new_force(self.h)
+
+    def run(self, nsteps, step0=0):
+        if self.L.ox_run(self.h, nsteps, step0):
+            raise RuntimeError("oracle: run failed")
+
+    def move(self):
+        self.L.ox_move(self.h)
+
+    def obs(self):
+        o = np.zeros(32)
+        self.L.ox_get_obs(self.h, _p(o, C.c_double))
+        return dict(vnb=o[0:3].copy(), vbond=o[3:6].copy(), vangle=o[6:9].copy(), vtors=o[9:12].copy(),
+                    voop=o[12:15].copy(), pt=o[15:21].copy(), ek=o[21:23].copy(), temp=o[23],
+                    temp_atom=o[24], temp_bead=o[25], volume=o[26], overext=int(o[27]))
+
+    def forces(self):
+        f = np.zeros((self.nitems, 3))
+        self.L.ox_get_forces(self.h, _p(f, C.c_double))
+        return f
+
+    def state(self):
+        s = np.zeros((self.nitems, 3)); v = np.zeros((self.natoms, 3)); r = np.zeros((self.nitems, 3))
+        self.L.ox_get_state(self.h, _p(s, C.c_double), _p(v, C.c_double), _p(r, C.c_double))
+        return s, v, r
+
+    def set_state(self, s=None, v=None):
+        s = None if s is None else np.ascontiguousarray(s, np.float64)
+        v = None if v is None else np.ascontiguousarray(v, np.float64)
+        self.L.ox_set_state(self.h, None if s is None else _p(s, C.c_double), None if v is None else _p(v, C.c_double))
+
+    def table(self, kind, t):
+        n = self.L.ox_table(self.h, kind, t, None, None, None, 0)
+        x, V, F = np.zeros(n), np.zeros(n), np.zeros(n)
+        self.L.ox_table(self.h, kind, t, _p(x, C.c_double), _p(V, C.c_double), _p(F, C.c_double), n)
+        return x, V, F

@@ -1,0 +1,143 @@
+"""Throughput of the per-step force + integration path on one MI355X.
+
+Default workload: the united-atom melt (400 chains x 250 CH2 = 100,000 sites,
+LJ 0.401 nm / 0.39 kJ/mol to 1.0 nm, harmonic bonds and angles, Ryckaert-
+Bellemans torsion table, 450 K, 1 fs, list 1.3 nm rebuilt every 10 steps),
+synthetic and equilibrated on the device before timing.  One "step" is one
+full NEW_LOOP step (list rebuild when due, nonbonded, bonded, leap-frog,
+observables) advanced through HIP graphs.
+
+Prints one JSON line: whole-job atom-steps/s, the nonbonded kernel's
+algorithmic-byte roofline fraction (HIP events on the library's stream), and
+the reference CPU path (oracle/_ref, the reference's own routines built from
+source) timed on this host's cores on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import resource
+import shutil
+import subprocess
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "atom-steps/s on 1 GPU, batched/served vs plain, at ≤1e-5 rel RMS force err"
+HBM_PEAK_GBS = 8000.0
+
+
+def _stack_unlimited():
+    try:
+        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+    except (ValueError, OSError):
+        pass
+
+
+def cpu_baseline(sysm, nsteps, threads):
+    """Times the reference's own routines (oracle/_ref/ref_harness) on this host."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    from ibisco_amd import system
+    d = tempfile.mkdtemp(prefix="ibx_ref_")
+    try:
+        system.write_dir(sysm, d)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores",
+                   OMP_STACKSIZE="256M")
+        t0 = time.time()
+        r = subprocess.run([exe, "time", str(nsteps)], cwd=d, env=env, capture_output=True, text=True,
+                           preexec_fn=_stack_unlimited, timeout=600)
+        wall = time.time() - t0
+        line = [x for x in r.stdout.splitlines() if x.startswith("LOOP_SECONDS")]
+        if r.returncode != 0 or not line:
+            return {"error": f"reference harness rc={r.returncode}: {r.stderr[-300:]}"}
+        secs = float(line[0].split()[1])
+        return {"value": sysm.natoms * nsteps / secs, "unit": "atom-steps/s", "cores": threads, "kind": "reference",
+                "sample": f"{nsteps} steps of the same {sysm.natoms}-site system (list rebuilt at step 10), "
+                          f"loop {secs:.2f} s, wall incl. set-up {wall:.1f} s, OpenMP FP32 reference routines"}
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20000)
+    ap.add_argument("--warmup", type=int, default=500)
+    ap.add_argument("--cpu-steps", type=int, default=20)
+    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--nchains", type=int, default=400)
+    ap.add_argument("--nbeads", type=int, default=250)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only (asked for {args.gpus}); replicas and jobs shard across GPUs "
+              "without communication, see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from ibisco_amd import workloads
+    from ibisco_amd.engine import Engine
+
+    sysm = workloads.config1(nchains=args.nchains, nbeads=args.nbeads)
+    n = sysm.natoms
+    print(f"[bench] {n} sites, box {sysm.box[0]:.3f} nm", file=sys.stderr, flush=True)
+    e = Engine(sysm)
+    workloads.equilibrate(e, sysm)
+    print("[bench] equilibrated", file=sys.stderr, flush=True)
+    e.step(args.warmup, graph=True)
+    e.obs()   # synchronise
+    t0 = time.perf_counter()
+    e.step(args.steps, graph=True)
+    ob = e.obs()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    value = n * args.steps / secs
+    print(f"[bench] {args.steps} steps in {secs:.3f} s, T = {ob['temp'] * 300:.1f} K", file=sys.stderr, flush=True)
+
+    # nonbonded kernel: HIP events on the library stream over an eager run
+    nt = min(200, max(20, args.steps // 10))
+    e.set_timing(True)
+    e.step(nt)
+    tm = e.timing()
+    e.set_timing(False)
+    nb_ms = tm["nonbonded"] / tm["nsteps_timed"]
+    pairs = e.pairs(0, 0)
+    prl_n = len(pairs) / n                        # half-list pairs per site
+    b_nb = 4.0 * prl_n + 28.0                     # algorithmic bytes per site-step
+    achieved = b_nb * n / (nb_ms * 1e-3) / 1e9    # GB/s
+    roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_nonbonded", "kernel_ms": round(nb_ms, 5), "bytes_per_site_step": round(b_nb, 1),
+            "half_list_pairs_per_site": round(prl_n, 2)}
+    cpu = None
+    if not args.no_cpu_baseline:
+        from ibisco_amd import units
+        pos, vel = e.state()
+        sysm.s = pos[:n].copy()
+        sysm.v = units.vel_from_reduced(vel)
+        print("[bench] timing the reference CPU path", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(sysm, args.cpu_steps, args.cpu_threads)
+    out = {
+        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded random-walk melt, equilibrated on device)",
+        "config": {"workload": "ua_melt_100k_plain", "sites": n, "chains": args.nchains, "chain_length": args.nbeads,
+                   "rc_nm": 1.0, "rlist_nm": 1.3, "nupdate": 10, "dt_fs": 1.0, "ensemble": "NVE",
+                   "path": "one system per launch, HIP graph per list interval",
+                   "precision": "FP32 forces, FP64 integration and reductions"},
+        "step_breakdown_ms": {k: round(tm[k] / tm["nsteps_timed"], 5) for k in
+                              ("nonbonded", "bonded", "integrate", "rebuild", "step")},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

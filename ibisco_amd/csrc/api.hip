@@ -1,0 +1,982 @@
+// C ABI of the ibx library: context creation, device layout, per-step
+// orchestration (eager and HIP-graph), observables and test hooks.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "ibx.h"
+#include "ibx_internal.h"
+#include "ibx_kernels.h"
+#include "kernels_api.h"
+
+using namespace ibx;
+
+namespace {
+
+thread_local std::string g_create_err;
+
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    ~DBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    hipError_t alloc(size_t count) {
+        release();
+        n = count ? count : 1;
+        return hipMalloc((void **)&p, n * sizeof(T));
+    }
+    size_t bytes() const { return p ? n * sizeof(T) : 0; }
+};
+
+int pad_to(int x, int m) { return (x + m - 1) / m * m; }
+
+struct GroupDev {
+    int nslot_pad = 0, nwg = 0, ncell_total = 0, cap = 0, max_rows = 0;
+    bool multi = false, present = false;
+    DBuf<int> mem, mem_cell, cell_count, cell_start, cell_fill, tmp, perm, slot_cell, inv_perm, list, nneb, wg_sys,
+        cell_sys;
+    DBuf<double4> r64;
+    DBuf<int4> pos4;
+    DBuf<double> part;
+};
+
+struct Timing {
+    double ms[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+};
+
+}  // namespace
+
+struct ibx_ctx {
+    int nsys = 0;
+    std::vector<HostSystem> hs;
+    std::vector<DevSys> dsys;
+    std::vector<int> item_off, slot_off[2];
+    int n_items = 0, item_nwg = 0, nupdate = 1;
+    hipStream_t st = nullptr;
+    GroupDev gr[2];
+    DBuf<DevSys> sys;
+    DBuf<int> wg_items, itype, label, excl_off, excl, term_off, term_ref, vs_of, vs_center, vs_moff, vs_members;
+    DBuf<int> tors_tab, oop_tab, nb_map, flags, step;
+    DBuf<int4> bonds, angles, tors, oops;
+    DBuf<double> s, v, mass, invmass, vs_coeff, vs_invmass, part_bd, part_mv, obs, avg, sysstate;
+    DBuf<float> fnb, fb, ftot;
+    DBuf<float4> nb_rows, bd_rows;
+    DBuf<NbTab> nb_tabs;
+    DBuf<BdTab> bd_tabs;
+    std::vector<NbTab> h_nb_tabs;
+    // rdf
+    DBuf<unsigned long long> rdf;
+    int rdf_nbin = 0, rdf_every = 0;
+    double rdf_dr = 0;
+    std::vector<int> rdf_samples;
+    // graph
+    hipGraphExec_t gexec = nullptr;
+    int graph_steps = 0;
+    // timing
+    bool timing = false;
+    Timing tm;
+    std::vector<hipEvent_t> evpool;
+    std::string err;
+    ~ibx_ctx() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto e : evpool) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+};
+
+namespace {
+
+int fail(ibx_ctx *c, int code, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (c) c->err = buf;
+    else g_create_err = buf;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                                  \
+    do {                                                                                                   \
+        hipError_t e_ = (expr);                                                                            \
+        if (e_ != hipSuccess)                                                                              \
+            return fail(ctx, e_ == hipErrorOutOfMemory ? IBX_E_NOMEM : IBX_E_DEVICE, "%s: %s", #expr,      \
+                        hipGetErrorString(e_));                                                            \
+    } while (0)
+
+template <typename T>
+int upload(ibx_ctx *c, DBuf<T> &b, const std::vector<T> &h) {
+    HIPCHK(c, b.alloc(h.size()));
+    if (!h.empty()) HIPCHK(c, hipMemcpy(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+void split_float(double x, float &hi, float &lo) {
+    hi = (float)x;
+    lo = (float)(x - (double)hi);
+}
+
+// float4 row k = {F_k, V_k, F_k+1, V_k+1}, k = 0..ndat (row ndat+1 of the table is zero)
+void append_rows(std::vector<float4> &rows, const Table &t) {
+    for (int k = 0; k <= t.ndat; ++k)
+        rows.push_back(make_float4((float)t.F[k], (float)t.V[k], (float)t.F[k + 1], (float)t.V[k + 1]));
+}
+
+int build_device(ibx_ctx *c) {
+    const int S = c->nsys;
+    c->item_off.resize(S + 1);
+    c->slot_off[0].assign(S + 1, 0);
+    c->slot_off[1].assign(S + 1, 0);
+    c->dsys.assign(S, DevSys{});
+    int off = 0;
+    for (int s = 0; s < S; ++s) {
+        c->item_off[s] = off;
+        off += pad_to(c->hs[s].nitems, kBlock);
+    }
+    c->item_off[S] = off;
+    c->n_items = off;
+    c->item_nwg = off / kBlock;
+    const int N = off;
+
+    // per-item arrays
+    std::vector<double> hs_s(3 * (size_t)N, 0.0), hv(3 * (size_t)N, 0.0), hm(N, 1.0), him(N, 1.0), hvc(N, 0.0),
+        hvim(N, 0.0);
+    std::vector<int> hit(N, -1), hlab(N, 0), hvs(N, -1), hvcen(N, -1), hwg(c->item_nwg, 0);
+    std::vector<int> hex_off(N + 1, 0), hex;
+    std::vector<int> hvm_off(N + 1, 0), hvm;
+    std::vector<int4> hb, ha, ht, ho;
+    std::vector<int> htt, hot;
+    std::vector<BdTab> hbt;
+    std::vector<float4> hbrows, hnrows;
+    std::vector<NbTab> hntab;
+    std::vector<int> hnmap;
+    // incident terms: (item, ref) pairs, ordered per item
+    std::vector<std::vector<int>> inc(N);
+
+    for (int s = 0; s < S; ++s) {
+        HostSystem &h = c->hs[s];
+        DevSys &D = c->dsys[s];
+        const int o = c->item_off[s];
+        for (int w = o / kBlock; w < c->item_off[s + 1] / kBlock; ++w) hwg[w] = s;
+        for (int i = 0; i < h.nitems; ++i) {
+            for (int k = 0; k < 3; ++k) hs_s[(size_t)k * N + o + i] = h.s[3 * (size_t)i + k];
+            if (i < h.natoms)
+                for (int k = 0; k < 3; ++k) hv[(size_t)k * N + o + i] = h.v[3 * (size_t)i + k];
+            hit[o + i] = h.itype[i];
+            hlab[o + i] = h.type_label[i];
+            hm[o + i] = h.mass[h.itype[i]];
+            him[o + i] = h.invmass[h.itype[i]];
+        }
+        // exclusions in global indices
+        for (int i = 0; i < h.nitems; ++i)
+            for (int q = h.excl_off[i]; q < h.excl_off[i + 1]; ++q) hex.push_back(o + h.excl[q]);
+        // virtual sites
+        for (int v = 0; v < h.nvs; ++v) {
+            const int vi = o + h.natoms + v;
+            hvcen[vi] = h.vs_center[v] >= 0 ? o + h.vs_center[v] : -1;
+            hvim[vi] = h.vs_invmass[v];
+            for (int q = h.vs_off[v]; q < h.vs_off[v + 1]; ++q) {
+                hvs[o + h.vs_atoms[q]] = vi;
+                hvc[o + h.vs_atoms[q]] = h.vs_coeff[q];
+            }
+        }
+        // bonded tables of this system
+        auto add_tabs = [&](const std::vector<Table> &tabs, const std::vector<int> &labs) {
+            int base = (int)hbt.size();
+            for (size_t t = 0; t < tabs.size(); ++t) {
+                BdTab b;
+                b.row_off = (int)hbrows.size();
+                b.ndat = tabs[t].ndat;
+                b.x0 = tabs[t].x0;
+                b.bin = tabs[t].bin;
+                b.inv_bin = 1.0 / tabs[t].bin;
+                b.pad = 0;
+                b.label = labs.empty() ? 1 : labs[t];
+                hbt.push_back(b);
+                append_rows(hbrows, tabs[t]);
+            }
+            return base;
+        };
+        int bbase = add_tabs(h.bt, h.bond_lab), abase = add_tabs(h.at, h.angle_lab);
+        int tbase = add_tabs(h.tt, h.tors_lab), obase = add_tabs(h.ot, h.oop_lab);
+        auto ref = [](int kind, int role, int idx) { return (kind << 29) | (role << 27) | idx; };
+        for (size_t q = 0; q < h.bonds.size(); ++q) {
+            int idx = (int)hb.size();
+            hb.push_back(make_int4(o + h.bonds[q][0], o + h.bonds[q][1], bbase + h.bond_tab[q], 0));
+            for (int r = 0; r < 2; ++r) inc[o + h.bonds[q][r]].push_back(ref(KBOND, r, idx));
+        }
+        for (size_t q = 0; q < h.angles.size(); ++q) {
+            int idx = (int)ha.size();
+            ha.push_back(make_int4(o + h.angles[q][0], o + h.angles[q][1], o + h.angles[q][2], abase + h.angle_tab[q]));
+            for (int r = 0; r < 3; ++r) inc[o + h.angles[q][r]].push_back(ref(KANGLE, r, idx));
+        }
+        for (size_t q = 0; q < h.tors.size(); ++q) {
+            int idx = (int)ht.size();
+            ht.push_back(make_int4(o + h.tors[q][0], o + h.tors[q][1], o + h.tors[q][2], o + h.tors[q][3]));
+            htt.push_back(tbase + h.tors_tab[q]);
+            for (int r = 0; r < 4; ++r) inc[o + h.tors[q][r]].push_back(ref(KTORS, r, idx));
+        }
+        for (size_t q = 0; q < h.oops.size(); ++q) {
+            int idx = (int)ho.size();
+            ho.push_back(make_int4(o + h.oops[q][0], o + h.oops[q][1], o + h.oops[q][2], o + h.oops[q][3]));
+            hot.push_back(obase + h.oop_tab[q]);
+            for (int r = 0; r < 4; ++r) inc[o + h.oops[q][r]].push_back(ref(KOOP, r, idx));
+        }
+        // nonbonded tables: fixed-point unit of this system
+        const double unit = h.box[0] / 4294967296.0;
+        D.nb_row_off = (int)hnrows.size();
+        D.nb_tab_off = (int)hntab.size();
+        D.nb_ntab = (int)h.nt.size();
+        if (D.nb_ntab > kMaxNbTabs) return fail(c, IBX_E_INVALID, "more than %d nonbonded tables", kMaxNbTabs);
+        for (auto &t : h.nt) {
+            NbTab nt;
+            nt.row_off = (int)hnrows.size();
+            nt.ndat = t.ndat;
+            split_float(unit / t.bin, nt.c_hi, nt.c_lo);
+            nt.bin = (float)t.bin;
+            nt.pad = 0.f;
+            hntab.push_back(nt);
+            append_rows(hnrows, t);
+        }
+        D.nb_nrow = (int)hnrows.size() - D.nb_row_off;
+        D.ntype = h.ntype;
+        D.nbmap_off = (int)hnmap.size();
+        if (h.nt.size() > 1 && h.ntype > kMaxNbTypes)
+            return fail(c, IBX_E_INVALID, "more than %d types with several nonbonded tables", kMaxNbTypes);
+        for (int q = 0; q < h.ntype * h.ntype; ++q) hnmap.push_back(h.inbondt[q] < 0 ? -1 : D.nb_tab_off + h.inbondt[q]);
+        // system constants
+        for (int k = 0; k < 3; ++k) {
+            D.box[k] = h.box[k];
+            D.boxinv[k] = 1.0 / h.box[k];
+            D.box2[k] = h.box[k] / 2.0;
+            D.inv_box_frac[k] = 1.0 / h.box[k];
+            D.rho[k] = (float)(h.box[k] / h.box[0]);
+        }
+        D.unit = unit;
+        D.item_off = o;
+        D.nitems_pad = c->item_off[s + 1] - o;
+        D.natoms = h.natoms;
+        D.nitems = h.nitems;
+        D.nvs = h.nvs;
+        D.item_wg_off = o / kBlock;
+        D.item_nwg = D.nitems_pad / kBlock;
+        D.hybrid = h.hybrid;
+        D.ngroup = h.ngroup;
+        D.dt = h.dt; D.temp_in = h.temp_in; D.fac = h.fac; D.ensemble = h.ensemble;
+        D.mktemp = h.mktemp; D.mktemp_atom = h.mktemp_atom; D.mktemp_bead = h.mktemp_bead;
+        D.volume = h.box[0] * h.box[1] * h.box[2];
+        D.halt_drift = h.halt_drift < 0 ? -h.halt_drift : h.halt_drift;
+        D.nmts = h.nmts; D.mts_quirk = h.mts_quirk; D.nupdate = h.nupdate;
+    }
+    for (int i = 0; i < N; ++i) {
+        hex_off[i + 1] = (int)hex.size();   // placeholder, rebuilt below
+    }
+    // exclusion CSR in global item order
+    {
+        hex.clear();
+        for (int s = 0; s < S; ++s) {
+            HostSystem &h = c->hs[s];
+            const int o = c->item_off[s];
+            for (int i = o; i < c->item_off[s + 1]; ++i) {
+                hex_off[i] = (int)hex.size();
+                int li = i - o;
+                if (li < h.nitems)
+                    for (int q = h.excl_off[li]; q < h.excl_off[li + 1]; ++q) hex.push_back(o + h.excl[q]);
+            }
+        }
+        hex_off[N] = (int)hex.size();
+    }
+    // VS member CSR per VS item
+    for (int i = 0; i < N; ++i) hvm_off[i + 1] = hvm_off[i];
+    {
+        hvm.clear();
+        std::vector<int> cnt(N + 1, 0);
+        for (int s = 0; s < S; ++s) {
+            HostSystem &h = c->hs[s];
+            const int o = c->item_off[s];
+            for (int i = o; i < c->item_off[s + 1]; ++i) {
+                hvm_off[i] = (int)hvm.size();
+                int li = i - o;
+                if (li >= h.natoms && li < h.nitems) {
+                    int v = li - h.natoms;
+                    for (int q = h.vs_off[v]; q < h.vs_off[v + 1]; ++q) hvm.push_back(o + h.vs_atoms[q]);
+                }
+            }
+        }
+        hvm_off[N] = (int)hvm.size();
+    }
+    std::vector<int> hterm_off(N + 1, 0), hterm;
+    for (int i = 0; i < N; ++i) {
+        hterm_off[i] = (int)hterm.size();
+        hterm.insert(hterm.end(), inc[i].begin(), inc[i].end());
+    }
+    hterm_off[N] = (int)hterm.size();
+
+    int rc;
+    if ((rc = upload(c, c->s, hs_s)) || (rc = upload(c, c->v, hv)) || (rc = upload(c, c->mass, hm)) ||
+        (rc = upload(c, c->invmass, him)) || (rc = upload(c, c->itype, hit)) || (rc = upload(c, c->label, hlab)) ||
+        (rc = upload(c, c->wg_items, hwg)) || (rc = upload(c, c->excl_off, hex_off)) || (rc = upload(c, c->excl, hex)) ||
+        (rc = upload(c, c->vs_of, hvs)) || (rc = upload(c, c->vs_coeff, hvc)) || (rc = upload(c, c->vs_center, hvcen)) ||
+        (rc = upload(c, c->vs_invmass, hvim)) || (rc = upload(c, c->vs_moff, hvm_off)) ||
+        (rc = upload(c, c->vs_members, hvm)) || (rc = upload(c, c->term_off, hterm_off)) ||
+        (rc = upload(c, c->term_ref, hterm)) || (rc = upload(c, c->bonds, hb)) || (rc = upload(c, c->angles, ha)) ||
+        (rc = upload(c, c->tors, ht)) || (rc = upload(c, c->tors_tab, htt)) || (rc = upload(c, c->oops, ho)) ||
+        (rc = upload(c, c->oop_tab, hot)) || (rc = upload(c, c->bd_tabs, hbt)) || (rc = upload(c, c->bd_rows, hbrows)) ||
+        (rc = upload(c, c->nb_rows, hnrows)) || (rc = upload(c, c->nb_tabs, hntab)) || (rc = upload(c, c->nb_map, hnmap)))
+        return rc;
+    c->h_nb_tabs = hntab;
+    HIPCHK(c, c->fnb.alloc(3 * (size_t)N));
+    HIPCHK(c, c->fb.alloc(3 * (size_t)N));
+    HIPCHK(c, c->ftot.alloc(3 * (size_t)N));
+    HIPCHK(c, hipMemset(c->fnb.p, 0, c->fnb.bytes()));
+    HIPCHK(c, hipMemset(c->fb.p, 0, c->fb.bytes()));
+    HIPCHK(c, hipMemset(c->ftot.p, 0, c->ftot.bytes()));
+    HIPCHK(c, c->part_bd.alloc((size_t)c->item_nwg * kBdPart));
+    HIPCHK(c, c->part_mv.alloc((size_t)c->item_nwg * kMvPart));
+    HIPCHK(c, hipMemset(c->part_bd.p, 0, c->part_bd.bytes()));
+    HIPCHK(c, hipMemset(c->part_mv.p, 0, c->part_mv.bytes()));
+    HIPCHK(c, c->obs.alloc((size_t)S * kObs));
+    HIPCHK(c, c->avg.alloc((size_t)S * kObs));
+    HIPCHK(c, c->sysstate.alloc((size_t)S * 8));
+    HIPCHK(c, hipMemset(c->obs.p, 0, c->obs.bytes()));
+    HIPCHK(c, hipMemset(c->avg.p, 0, c->avg.bytes()));
+    std::vector<double> st(8 * (size_t)S, 0.0);
+    for (int s = 0; s < S; ++s) {
+        st[8 * s] = c->hs[s].temp_atom0;
+        st[8 * s + 1] = c->hs[s].temp_bead0;
+    }
+    if ((rc = upload(c, c->sysstate, st))) return rc;
+    HIPCHK(c, c->flags.alloc(8));
+    HIPCHK(c, hipMemset(c->flags.p, 0, c->flags.bytes()));
+    HIPCHK(c, c->step.alloc(1));
+    HIPCHK(c, hipMemset(c->step.p, 0, c->step.bytes()));
+
+    // groups
+    for (int g = 0; g < 2; ++g) {
+        GroupDev &G = c->gr[g];
+        std::vector<int> mem, wg, cellsys;
+        int slot = 0, cell = 0, maxrows = 0;
+        bool any = false, multi = false;
+        double max_dens_vol = 0.0;
+        for (int s = 0; s < S; ++s) {
+            HostSystem &h = c->hs[s];
+            DevSys &D = c->dsys[s];
+            DevGroup &dg = D.g[g];
+            const int nm = (g < h.ngroup) ? (int)h.members[g].size() : 0;
+            dg.slot_off = slot;
+            dg.nmem = nm;
+            dg.nslot_pad = pad_to(nm, kBlock);
+            dg.mem_off = slot;
+            dg.wg_off = slot / kBlock;
+            dg.nwg = dg.nslot_pad / kBlock;
+            for (int k = 0; k < dg.nslot_pad; ++k) mem.push_back(k < nm ? c->item_off[s] + h.members[g][k] : -1);
+            for (int w = 0; w < dg.nwg; ++w) wg.push_back(s);
+            slot += dg.nslot_pad;
+            const double rc = h.rcut[g], rl = h.rlist[g];
+            int nc[3];
+            for (int k = 0; k < 3; ++k) {
+                nc[k] = (int)(h.box[k] / rl);           // ALLOCATEVAR2.f90:19-27
+                if (nc[k] < 1) nc[k] = 1;
+                dg.nc[k] = nc[k];
+            }
+            dg.all_pairs = (nc[0] < 3 || nc[1] < 3 || nc[2] < 3) ? 1 : 0;
+            dg.ncell = nc[0] * nc[1] * nc[2];
+            dg.cell_off = cell;
+            for (int q = 0; q < dg.ncell; ++q) cellsys.push_back(s);
+            cell += dg.ncell;
+            const double unit = D.unit;
+            dg.rl2 = rl * rl;
+            const double rl2u = (rl / unit) * (rl / unit);
+            dg.rl2u_lo = (float)(rl2u * (1.0 - 4e-6));
+            dg.rl2u_hi = (float)(rl2u * (1.0 + 4e-6));
+            dg.rc2u = (float)((rc / unit) * (rc / unit));
+            if (nm > 0) {
+                any = true;
+                double vol = h.box[0] * h.box[1] * h.box[2];
+                double nbr = 4.0 / 3.0 * M_PI * rl * rl * rl * nm / vol;
+                max_dens_vol = std::max(max_dens_vol, nbr);
+                maxrows = std::max(maxrows, D.nb_nrow);
+                multi = multi || D.nb_ntab > 1;
+            }
+        }
+        G.present = any;
+        G.nslot_pad = slot;
+        G.nwg = slot / kBlock;
+        G.ncell_total = cell;
+        G.multi = multi;
+        G.max_rows = std::min(maxrows, kLdsRows);
+        if (maxrows > kLdsRows) G.max_rows = 0;
+        if (!any) continue;
+        G.cap = pad_to((int)(1.5 * max_dens_vol) + 64, 8);
+        std::vector<int> perm_init(slot, -1);
+        if ((rc = upload(c, G.mem, mem)) || (rc = upload(c, G.wg_sys, wg)) || (rc = upload(c, G.cell_sys, cellsys)) ||
+            (rc = upload(c, G.perm, perm_init)))
+            return rc;
+        HIPCHK(c, G.mem_cell.alloc(slot));
+        HIPCHK(c, G.cell_count.alloc(cell));
+        HIPCHK(c, G.cell_start.alloc(cell));
+        HIPCHK(c, G.cell_fill.alloc(cell));
+        HIPCHK(c, G.tmp.alloc(slot));
+        HIPCHK(c, G.slot_cell.alloc(slot));
+        HIPCHK(c, G.r64.alloc(slot));
+        HIPCHK(c, G.pos4.alloc(slot));
+        HIPCHK(c, G.nneb.alloc(slot));
+        HIPCHK(c, G.part.alloc((size_t)G.nwg * kNbPart));
+        HIPCHK(c, hipMemset(G.part.p, 0, G.part.bytes()));
+        HIPCHK(c, G.list.alloc((size_t)slot * G.cap));
+        std::vector<int> inv(N, -1);
+        if ((rc = upload(c, G.inv_perm, inv))) return rc;
+    }
+    if ((rc = upload(c, c->sys, c->dsys))) return rc;
+    return 0;
+}
+
+RebuildArgs rebuild_args(ibx_ctx *c, int g) {
+    GroupDev &G = c->gr[g];
+    RebuildArgs a;
+    a.sys = c->sys.p; a.wg_sys = G.wg_sys.p; a.cell_sys = G.cell_sys.p;
+    a.g = g; a.nwg = G.nwg; a.ncell_total = G.ncell_total; a.nitems_total = c->n_items; a.cap = G.cap;
+    a.mem = G.mem.p; a.s = c->s.p; a.itype = c->itype.p; a.label = c->label.p;
+    a.mem_cell = G.mem_cell.p; a.cell_count = G.cell_count.p; a.cell_start = G.cell_start.p;
+    a.cell_fill = G.cell_fill.p; a.tmp = G.tmp.p; a.perm = G.perm.p; a.slot_cell = G.slot_cell.p;
+    a.r64 = G.r64.p; a.pos4 = G.pos4.p; a.inv_perm = G.inv_perm.p;
+    a.excl_off = c->excl_off.p; a.excl = c->excl.p; a.list = G.list.p; a.nneb = G.nneb.p; a.flags = c->flags.p;
+    return a;
+}
+
+void enqueue_rebuild(ibx_ctx *c) {
+    for (int g = 0; g < 2; ++g)
+        if (c->gr[g].present) launch_rebuild(c->st, rebuild_args(c, g));
+}
+
+void enqueue_rdf(ibx_ctx *c) {
+    if (!c->rdf_nbin || !c->gr[0].present) return;
+    GroupDev &G = c->gr[0];
+    // bins in working units of each system: identical boxes across a batch are not required,
+    // the unit differs per system, so the bin width is rescaled per system on the host side
+    float inv_dr_u = (float)(c->dsys[0].unit / c->rdf_dr);
+    launch_rdf(c->st, G.nwg, c->sys.p, G.wg_sys.p, G.perm.p, G.pos4.p, G.list.p, G.nneb.p, G.cap, c->rdf_nbin,
+               inv_dr_u, c->rdf.p);
+    for (auto &x : c->rdf_samples) ++x;
+}
+
+void enqueue_forces(ibx_ctx *c, int mode) {
+    for (int g = 0; g < 2; ++g) {
+        GroupDev &G = c->gr[g];
+        if (!G.present) continue;
+        NbArgs a;
+        a.sys = c->sys.p; a.wg_sys = G.wg_sys.p; a.g = g; a.nwg = G.nwg; a.cap = G.cap; a.nitems_total = c->n_items;
+        a.max_rows_lds = G.max_rows; a.multi = G.multi; a.perm = G.perm.p; a.pos4 = G.pos4.p; a.list = G.list.p;
+        a.nneb = G.nneb.p; a.rows = c->nb_rows.p; a.tabs = c->nb_tabs.p; a.map = c->nb_map.p; a.fnb = c->fnb.p;
+        a.part = G.part.p;
+        launch_nonbonded(c->st, a);
+    }
+}
+
+void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
+    BondedLaunch b{c->bonds.p, c->angles.p, c->tors.p, c->tors_tab.p, c->oops.p, c->oop_tab.p,
+                   c->term_off.p, c->term_ref.p, c->bd_tabs.p, c->bd_rows.p};
+    launch_bonded(c->st, c->item_nwg, c->sys.p, c->wg_items.p, b, c->s.p, c->n_items, c->fb.p, c->part_bd.p, c->flags.p);
+    if (ev) (void)hipEventRecord((*ev)[2], c->st);
+    MoveLaunch m;
+    m.s = c->s.p; m.v = c->v.p; m.fnb = c->fnb.p; m.fb = c->fb.p; m.ftot = c->ftot.p; m.itype = c->itype.p;
+    m.label = c->label.p; m.mass = c->mass.p; m.invmass = c->invmass.p; m.vs_of = c->vs_of.p;
+    m.vs_coeff = c->vs_coeff.p;
+    for (int g = 0; g < 2; ++g) {
+        m.pos4[g] = c->gr[g].present ? c->gr[g].pos4.p : nullptr;
+        m.inv_perm[g] = c->gr[g].present ? c->gr[g].inv_perm.p : nullptr;
+    }
+    if (!m.pos4[1]) { m.pos4[1] = m.pos4[0]; m.inv_perm[1] = m.inv_perm[0]; }
+    m.sysstate = c->sysstate.p; m.part = c->part_mv.p; m.step = c->step.p; m.nitems_total = c->n_items; m.mode = mode;
+    launch_move(c->st, c->item_nwg, c->sys.p, c->wg_items.p, m);
+    if (ev) (void)hipEventRecord((*ev)[3], c->st);
+    if (mode == 0 && c->gr[1].present && c->hs[0].hybrid)
+        launch_vs_place(c->st, c->item_nwg, c->sys.p, c->wg_items.p, c->s.p, c->n_items, c->label.p, c->itype.p,
+                        c->vs_center.p, c->vs_moff.p, c->vs_members.p, c->mass.p, c->vs_invmass.p, c->gr[1].pos4.p,
+                        c->gr[1].inv_perm.p);
+    launch_finalize(c->st, c->nsys, c->sys.p, c->gr[0].part.p, c->gr[1].present ? c->gr[1].part.p : c->gr[0].part.p,
+                    c->part_bd.p, c->part_mv.p, c->obs.p, c->avg.p, c->sysstate.p, c->step.p, mode);
+}
+
+int check_flags(ibx_ctx *c) {
+    int f[8];
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(f, c->flags.p, sizeof f, hipMemcpyDeviceToHost));
+    if (f[0] > 0) {
+        for (int g = 0; g < 2; ++g) {
+            GroupDev &G = c->gr[g];
+            if (!G.present || f[0] <= G.cap) continue;
+        }
+        return fail(c, IBX_E_LIST_OVERFLOW, "LIST TOO SMALL: %d neighbours exceed the list capacity", f[0]);
+    }
+    if (f[1] & 1) return fail(c, IBX_E_TABLE, "torsion angle outside its table");
+    if (f[1] & 2) return fail(c, IBX_E_TABLE, "out-of-plane angle outside its table");
+    return 0;
+}
+
+// Builds the lists; grows the capacity and rebuilds when a list overflowed.
+int rebuild_checked(ibx_ctx *c) {
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
+        enqueue_rebuild(c);
+        int f0 = 0;
+        HIPCHK(c, hipStreamSynchronize(c->st));
+        HIPCHK(c, hipMemcpy(&f0, c->flags.p, sizeof f0, hipMemcpyDeviceToHost));
+        if (f0 == 0) return 0;
+        for (int g = 0; g < 2; ++g) {
+            GroupDev &G = c->gr[g];
+            if (!G.present || f0 <= G.cap) continue;
+            G.cap = pad_to((int)(f0 * 1.25) + 16, 8);
+            HIPCHK(c, G.list.alloc((size_t)G.nslot_pad * G.cap));
+        }
+        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    }
+    return fail(c, IBX_E_LIST_OVERFLOW, "LIST TOO SMALL after resizing");
+}
+
+void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
+    if (ev) (void)hipEventRecord((*ev)[0], c->st);
+    if (step % c->nupdate == 0) {
+        enqueue_rebuild(c);
+        if (c->rdf_nbin && c->rdf_every > 0) enqueue_rdf(c);
+    }
+    if (ev) (void)hipEventRecord((*ev)[4], c->st);
+    enqueue_forces(c, 0);
+    if (ev) (void)hipEventRecord((*ev)[1], c->st);
+    enqueue_rest(c, 0, ev);
+    if (ev) (void)hipEventRecord((*ev)[5], c->st);
+}
+
+int ensure_events(ibx_ctx *c, size_t n) {
+    while (c->evpool.size() < n) {
+        hipEvent_t e;
+        HIPCHK(c, hipEventCreate(&e));
+        c->evpool.push_back(e);
+    }
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *ibx_last_error(const ibx_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int ibx_create(int nsys, const ibx_system_desc *const *desc, ibx_ctx **out) {
+    if (!out || nsys < 1 || !desc) return fail(nullptr, IBX_E_INVALID, "invalid arguments");
+    *out = nullptr;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, IBX_E_DEVICE, "no HIP device");
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return fail(nullptr, IBX_E_DEVICE, "no HIP device properties");
+    if (!strstr(prop.gcnArchName, "gfx950"))
+        return fail(nullptr, IBX_E_DEVICE, "device %s is not gfx950 (MI355X)", prop.gcnArchName);
+    std::unique_ptr<ibx_ctx> c(new ibx_ctx());
+    c->nsys = nsys;
+    c->hs.resize(nsys);
+    for (int s = 0; s < nsys; ++s) {
+        std::string err;
+        int rc = build_host_system(desc[s], c->hs[s], err);
+        if (rc) return fail(nullptr, rc, "system %d: %s", s, err.c_str());
+        if (c->hs[s].nupdate != c->hs[0].nupdate)
+            return fail(nullptr, IBX_E_INVALID, "all systems of a batch must share update_neighbour_list");
+        if (c->hs[s].hybrid && nsys > 1) return fail(nullptr, IBX_E_INVALID, "hybrid systems run one per context");
+        if (c->hs[s].nmts > 0) return fail(nullptr, IBX_E_INVALID, "the MTS cycle runs on the oracle only in this build");
+        // VIRTUAL_DEF at set-up (main.F90:148-150)
+        HostSystem &h = c->hs[s];
+        for (int v = 0; v < h.nvs; ++v) {
+            double p[3] = {0, 0, 0};
+            if (h.vs_center[v] >= 0) {
+                for (int k = 0; k < 3; ++k) p[k] = h.s[3 * (size_t)h.vs_center[v] + k];
+            } else {
+                for (int q = h.vs_off[v]; q < h.vs_off[v + 1]; ++q) {
+                    int a = h.vs_atoms[q];
+                    for (int k = 0; k < 3; ++k) p[k] += h.mass[h.itype[a]] * h.s[3 * (size_t)a + k];
+                }
+                for (int k = 0; k < 3; ++k) p[k] *= h.vs_invmass[v];
+            }
+            for (int k = 0; k < 3; ++k) h.s[3 * (size_t)(h.natoms + v) + k] = p[k];
+        }
+    }
+    c->nupdate = c->hs[0].nupdate;
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
+        return fail(nullptr, IBX_E_DEVICE, "stream creation failed");
+    int rc = build_device(c.get());
+    if (rc) { g_create_err = c->err; return rc; }
+    rc = rebuild_checked(c.get());
+    if (rc) { g_create_err = c->err; return rc; }
+    *out = c.release();
+    return 0;
+}
+
+void ibx_destroy(ibx_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipStreamSynchronize(ctx->st);
+    delete ctx;
+}
+
+int ibx_rebuild_lists(ibx_ctx *c) {
+    if (!c) return fail(nullptr, IBX_E_INVALID, "null context");
+    return rebuild_checked(c);
+}
+
+int ibx_compute_forces(ibx_ctx *c, int rebuild) {
+    if (!c) return fail(nullptr, IBX_E_INVALID, "null context");
+    if (rebuild) {
+        int rc = rebuild_checked(c);
+        if (rc) return rc;
+    }
+    HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
+    enqueue_forces(c, 1);
+    enqueue_rest(c, 1, nullptr);
+    return check_flags(c);
+}
+
+int ibx_step(ibx_ctx *c, int nsteps, int step0) {
+    if (!c || nsteps < 0) return fail(c, IBX_E_INVALID, "invalid arguments");
+    HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
+    HIPCHK(c, hipMemcpyAsync(c->step.p, &step0, sizeof(int), hipMemcpyHostToDevice, c->st));
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<hipEvent_t> ev;
+    const int per = 6;
+    if (c->timing) {
+        int rc = ensure_events(c, (size_t)per * nsteps);
+        if (rc) return rc;
+    }
+    for (int k = 0; k < nsteps; ++k) {
+        std::vector<hipEvent_t> e;
+        if (c->timing) e.assign(c->evpool.begin() + per * k, c->evpool.begin() + per * (k + 1));
+        enqueue_step(c, step0 + 1 + k, c->timing ? &e : nullptr);
+    }
+    int rc = check_flags(c);
+    if (c->timing) {
+        for (int k = 0; k < nsteps; ++k) {
+            hipEvent_t *e = c->evpool.data() + per * k;
+            float t;
+            if (hipEventElapsedTime(&t, e[4], e[1]) == hipSuccess) c->tm.ms[0] += t;
+            if (hipEventElapsedTime(&t, e[1], e[2]) == hipSuccess) c->tm.ms[1] += t;
+            if (hipEventElapsedTime(&t, e[2], e[3]) == hipSuccess) c->tm.ms[2] += t;
+            if (hipEventElapsedTime(&t, e[0], e[4]) == hipSuccess) c->tm.ms[3] += t;
+            if (hipEventElapsedTime(&t, e[0], e[5]) == hipSuccess) c->tm.ms[4] += t;
+            c->tm.ms[5] += 1;
+            if ((step0 + 1 + k) % c->nupdate == 0) c->tm.ms[7] += 1;
+        }
+        c->tm.ms[6] += nsteps;
+    }
+    return rc;
+}
+
+int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
+    if (!c || nsteps < 0) return fail(c, IBX_E_INVALID, "invalid arguments");
+    // eager head until the next rebuild step, graph blocks of nupdate steps, eager tail
+    int head = 0;
+    while (head < nsteps && (step0 + 1 + head) % c->nupdate != 0) ++head;
+    int rc = ibx_step(c, head, step0);
+    if (rc) return rc;
+    int done = head;
+    int nblk = (nsteps - done) / c->nupdate;
+    if (nblk > 0) {
+        if (!c->gexec || c->graph_steps != c->nupdate) {
+            if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+            hipGraph_t graph;
+            HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+            for (int k = 0; k < c->nupdate; ++k) enqueue_step(c, c->nupdate + k, nullptr);   // block starts at a rebuild
+            HIPCHK(c, hipStreamEndCapture(c->st, &graph));
+            hipError_t e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (e != hipSuccess) return fail(c, IBX_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+            c->graph_steps = c->nupdate;
+        }
+        HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
+        int cur = step0 + done;
+        HIPCHK(c, hipMemcpyAsync(c->step.p, &cur, sizeof(int), hipMemcpyHostToDevice, c->st));
+        for (int b = 0; b < nblk; ++b) HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
+        rc = check_flags(c);
+        if (rc) return rc;
+        done += nblk * c->nupdate;
+    }
+    return ibx_step(c, nsteps - done, step0 + done);
+}
+
+int ibx_get_observables(ibx_ctx *c, int sys, double obs[32]) {
+    if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(obs, c->obs.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
+    double a[kObs];
+    HIPCHK(c, hipMemcpy(a, c->avg.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
+    obs[26] = c->dsys[sys].volume;
+    obs[28] = a[28];
+    return 0;
+}
+
+int ibx_get_averages(ibx_ctx *c, int sys, double sums[32], int reset) {
+    if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(sums, c->avg.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
+    if (reset) HIPCHK(c, hipMemset(c->avg.p + (size_t)sys * kObs, 0, kObs * sizeof(double)));
+    return 0;
+}
+
+int ibx_get_forces(ibx_ctx *c, int sys, double *f) {
+    if (!c || sys < 0 || sys >= c->nsys || !f) return fail(c, IBX_E_INVALID, "invalid arguments");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const HostSystem &h = c->hs[sys];
+    const int o = c->item_off[sys], N = c->n_items;
+    std::vector<float> tmp(h.nitems);
+    for (int k = 0; k < 3; ++k) {
+        HIPCHK(c, hipMemcpy(tmp.data(), c->ftot.p + (size_t)k * N + o, h.nitems * sizeof(float), hipMemcpyDeviceToHost));
+        for (int i = 0; i < h.nitems; ++i) f[3 * (size_t)i + k] = tmp[i];
+    }
+    return 0;
+}
+
+int ibx_get_state(ibx_ctx *c, int sys, double *s, double *v) {
+    if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    const HostSystem &h = c->hs[sys];
+    const int o = c->item_off[sys], N = c->n_items;
+    std::vector<double> tmp(h.nitems);
+    double st[8];
+    HIPCHK(c, hipMemcpy(st, c->sysstate.p + 8 * (size_t)sys, sizeof st, hipMemcpyDeviceToHost));
+    for (int k = 0; k < 3; ++k) {
+        if (s) {
+            HIPCHK(c, hipMemcpy(tmp.data(), c->s.p + (size_t)k * N + o, h.nitems * sizeof(double), hipMemcpyDeviceToHost));
+            for (int i = 0; i < h.nitems; ++i) s[3 * (size_t)i + k] = tmp[i];
+        }
+        if (v) {
+            HIPCHK(c, hipMemcpy(tmp.data(), c->v.p + (size_t)k * N + o, h.natoms * sizeof(double), hipMemcpyDeviceToHost));
+            // a MOMENTUM correction decided at the last step is applied lazily by the next move
+            for (int i = 0; i < h.natoms; ++i) v[3 * (size_t)i + k] = tmp[i] - st[2 + k] * h.invmass[h.itype[i]];
+        }
+    }
+    return 0;
+}
+
+int ibx_set_state(ibx_ctx *c, int sys, const double *s, const double *v) {
+    if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HostSystem &h = c->hs[sys];
+    const int o = c->item_off[sys], N = c->n_items;
+    std::vector<double> tmp(h.nitems);
+    if (s) {
+        for (int i = 0; i < h.natoms; ++i)
+            for (int k = 0; k < 3; ++k) h.s[3 * (size_t)i + k] = s[3 * (size_t)i + k];
+        for (int vv = 0; vv < h.nvs; ++vv) {
+            double p[3] = {0, 0, 0};
+            if (h.vs_center[vv] >= 0) {
+                for (int k = 0; k < 3; ++k) p[k] = h.s[3 * (size_t)h.vs_center[vv] + k];
+            } else {
+                for (int q = h.vs_off[vv]; q < h.vs_off[vv + 1]; ++q) {
+                    int a = h.vs_atoms[q];
+                    for (int k = 0; k < 3; ++k) p[k] += h.mass[h.itype[a]] * h.s[3 * (size_t)a + k];
+                }
+                for (int k = 0; k < 3; ++k) p[k] *= h.vs_invmass[vv];
+            }
+            for (int k = 0; k < 3; ++k) h.s[3 * (size_t)(h.natoms + vv) + k] = p[k];
+        }
+        for (int k = 0; k < 3; ++k) {
+            for (int i = 0; i < h.nitems; ++i) tmp[i] = h.s[3 * (size_t)i + k];
+            HIPCHK(c, hipMemcpy(c->s.p + (size_t)k * N + o, tmp.data(), h.nitems * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    if (v) {
+        for (int k = 0; k < 3; ++k) {
+            for (int i = 0; i < h.natoms; ++i) tmp[i] = v[3 * (size_t)i + k];
+            HIPCHK(c, hipMemcpy(c->v.p + (size_t)k * N + o, tmp.data(), h.natoms * sizeof(double), hipMemcpyHostToDevice));
+        }
+    }
+    HIPCHK(c, hipMemset(c->sysstate.p + 8 * (size_t)sys + 2, 0, 3 * sizeof(double)));
+    // refresh the sorted copies with the current order (lists unchanged)
+    for (int g = 0; g < 2; ++g) {
+        if (!c->gr[g].present) continue;
+        RebuildArgs a = rebuild_args(c, g);
+        launch_pack_only(c->st, a);
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    return 0;
+}
+
+int ibx_set_nb_table(ibx_ctx *c, int sys, int t, int nrow, const double *x, const double *v) {
+    if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
+    HostSystem &h = c->hs[sys];
+    if (t < 0 || t >= (int)h.nt.size()) return fail(c, IBX_E_INVALID, "invalid table index");
+    // same load + FTABLE as set-up
+    ibx_tables one{1, nullptr, nullptr, nullptr, x, v};
+    (void)one;
+    Table nt;
+    {
+        nt.x.assign(nrow + 4, 0.0);
+        nt.V.assign(nrow + 4, 0.0);
+        nt.F.assign(nrow + 4, 0.0);
+        double bin = x[1] - x[0];
+        int k;
+        if (x[0] != 0.0) {
+            nt.x[0] = 0.0; nt.V[0] = v[0]; nt.x[1] = bin; nt.V[1] = v[0]; nt.x[2] = x[1]; nt.V[2] = v[1];
+            for (k = 3; k - 1 < nrow; ++k) { nt.x[k] = x[k - 1]; nt.V[k] = v[k - 1]; }
+        } else {
+            for (k = 0; k < nrow; ++k) { nt.x[k] = x[k]; nt.V[k] = v[k]; }
+        }
+        nt.ndat = k - 1;
+        nt.bin = bin;
+        for (int j = 1; j <= nt.ndat; ++j) nt.F[j] = -(nt.V[j + 1] - nt.V[j - 1]) / nt.bin / nt.x[j] / 2.0;
+        nt.F[0] = nt.F[1];
+    }
+    if (nt.ndat != h.nt[t].ndat) return fail(c, IBX_E_INVALID, "replacement table must keep the row count");
+    h.nt[t] = nt;
+    std::vector<float4> rows;
+    append_rows(rows, nt);
+    const NbTab &hdr = c->h_nb_tabs[c->dsys[sys].nb_tab_off + t];
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(c->nb_rows.p + hdr.row_off, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice));
+    NbTab nh = hdr;
+    split_float(c->dsys[sys].unit / nt.bin, nh.c_hi, nh.c_lo);
+    nh.bin = (float)nt.bin;
+    c->h_nb_tabs[c->dsys[sys].nb_tab_off + t] = nh;
+    HIPCHK(c, hipMemcpy(c->nb_tabs.p + c->dsys[sys].nb_tab_off + t, &nh, sizeof nh, hipMemcpyHostToDevice));
+    return 0;
+}
+
+int ibx_set_thermostat(ibx_ctx *c, int ensemble, double temp_in, double fac, double dt) {
+    if (!c || (ensemble != 0 && ensemble != 1)) return fail(c, IBX_E_INVALID, "invalid thermostat");
+    for (auto &D : c->dsys) {
+        D.ensemble = ensemble;
+        D.temp_in = temp_in;
+        D.fac = fac;
+        D.dt = dt;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    HIPCHK(c, hipMemcpy(c->sys.p, c->dsys.data(), c->dsys.size() * sizeof(DevSys), hipMemcpyHostToDevice));
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    return 0;
+}
+
+int ibx_get_pairs(ibx_ctx *c, int sys, int group, int *out, int64_t cap, int64_t *count) {
+    if (!c || sys < 0 || sys >= c->nsys || group < 0 || group > 1 || !count)
+        return fail(c, IBX_E_INVALID, "invalid arguments");
+    *count = 0;
+    GroupDev &G = c->gr[group];
+    if (!G.present) return 0;
+    const DevGroup &dg = c->dsys[sys].g[group];
+    if (dg.nmem == 0) return 0;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<int> perm(dg.nslot_pad), nneb(dg.nslot_pad), list((size_t)dg.nslot_pad * G.cap);
+    HIPCHK(c, hipMemcpy(perm.data(), G.perm.p + dg.slot_off, perm.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(nneb.data(), G.nneb.p + dg.slot_off, nneb.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(list.data(), G.list.p + (size_t)dg.slot_off * G.cap, list.size() * sizeof(int),
+                        hipMemcpyDeviceToHost));
+    const int o = c->item_off[sys];
+    std::vector<std::pair<int, int>> pr;
+    for (int i = 0; i < dg.nslot_pad; ++i) {
+        if (perm[i] < 0) continue;
+        const int I = perm[i] - o;
+        const int *lp = list.data() + (size_t)(i >> 6) * G.cap * 64 + (i & 63);
+        for (int k = 0; k < std::min(nneb[i], G.cap); ++k) {
+            const int j = lp[(size_t)k * 64] - dg.slot_off;
+            const int J = perm[j] - o;
+            if (I < J) pr.emplace_back(I, J);
+        }
+    }
+    std::sort(pr.begin(), pr.end());
+    *count = (int64_t)pr.size();
+    if (out && cap >= *count)
+        for (size_t q = 0; q < pr.size(); ++q) { out[2 * q] = pr[q].first; out[2 * q + 1] = pr[q].second; }
+    return 0;
+}
+
+int ibx_rdf_enable(ibx_ctx *c, int nbin, double dr, int every) {
+    if (!c || nbin < 0 || nbin > 8192 || (nbin > 0 && !(dr > 0))) return fail(c, IBX_E_INVALID, "invalid RDF settings");
+    c->rdf_nbin = nbin;
+    c->rdf_dr = dr;
+    c->rdf_every = every;
+    c->rdf_samples.assign(c->nsys, 0);
+    if (nbin) {
+        for (int s = 1; s < c->nsys; ++s)
+            if (std::fabs(c->dsys[s].unit - c->dsys[0].unit) > 0)
+                return fail(c, IBX_E_INVALID, "the device RDF needs equal box lengths across a batch");
+        HIPCHK(c, c->rdf.alloc((size_t)c->nsys * nbin));
+        HIPCHK(c, hipMemset(c->rdf.p, 0, c->rdf.bytes()));
+    }
+    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    return 0;
+}
+
+int ibx_rdf_get(ibx_ctx *c, int sys, double *g, int reset) {
+    if (!c || sys < 0 || sys >= c->nsys || !c->rdf_nbin) return fail(c, IBX_E_INVALID, "RDF not enabled");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    std::vector<unsigned long long> h(c->rdf_nbin);
+    HIPCHK(c, hipMemcpy(h.data(), c->rdf.p + (size_t)sys * c->rdf_nbin, h.size() * sizeof(unsigned long long),
+                        hipMemcpyDeviceToHost));
+    const HostSystem &hs = c->hs[sys];
+    const double n = (double)hs.members[0].size();
+    const double vol = hs.box[0] * hs.box[1] * hs.box[2];
+    const double samples = std::max(1, c->rdf_samples[sys]);
+    for (int b = 0; b < c->rdf_nbin; ++b) {
+        const double r0 = b * c->rdf_dr, r1 = (b + 1) * c->rdf_dr;
+        const double shell = 4.0 / 3.0 * M_PI * (r1 * r1 * r1 - r0 * r0 * r0);
+        g[b] = (double)h[b] / (samples * n * (n / vol) * shell);
+    }
+    if (reset) {
+        HIPCHK(c, hipMemset(c->rdf.p + (size_t)sys * c->rdf_nbin, 0, c->rdf_nbin * sizeof(unsigned long long)));
+        c->rdf_samples[sys] = 0;
+    }
+    return 0;
+}
+
+int ibx_get_timing(ibx_ctx *c, double ms[8]) {
+    if (!c) return fail(nullptr, IBX_E_INVALID, "null context");
+    for (int k = 0; k < 8; ++k) ms[k] = c->tm.ms[k];
+    return 0;
+}
+
+int ibx_get_info(ibx_ctx *c, int sys, int64_t info[8]) {
+    if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
+    const HostSystem &h = c->hs[sys];
+    size_t bytes = c->s.bytes() + c->v.bytes() + c->fnb.bytes() + c->fb.bytes() + c->ftot.bytes() + c->mass.bytes() +
+                   c->invmass.bytes() + c->excl.bytes() + c->term_ref.bytes();
+    for (auto &G : c->gr)
+        bytes += G.list.bytes() + G.pos4.bytes() + G.r64.bytes() + G.perm.bytes() + G.nneb.bytes() + G.mem.bytes();
+    info[0] = h.natoms;
+    info[1] = h.nvs;
+    info[2] = c->gr[0].cap;
+    info[3] = c->gr[1].cap;
+    info[4] = c->dsys[sys].g[0].nslot_pad;
+    info[5] = (int64_t)bytes;
+    info[6] = c->nsys;
+    info[7] = c->n_items;
+    return 0;
+}
+
+// Enables per-kernel-class device timing of ibx_step (off by default).
+int ibx_set_timing(ibx_ctx *c, int on) {
+    if (!c) return fail(nullptr, IBX_E_INVALID, "null context");
+    c->timing = on != 0;
+    for (double &x : c->tm.ms) x = 0.0;
+    return 0;
+}
+
+int ibx_load_dir(const char *dir, ibx_ctx **out) {
+    DescStorage st;
+    std::string err;
+    int rc = read_input_dir(dir, st, err);
+    if (rc) return fail(nullptr, rc, "%s", err.c_str());
+    const ibx_system_desc *d = &st.d;
+    return ibx_create(1, &d, out);
+}
+
+}  // extern "C"

@@ -1,0 +1,76 @@
+// Internal host-side data model of the ibx library (not part of the C ABI).
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "ibx.h"
+
+namespace ibx {
+
+// A tabulated interaction after the load-time shift of RDINTERACT.f90:450-537
+// and the differentiation of FTABLE.f90:9-91.  Rows 0..ndat hold data; row
+// ndat+1 is the zero row the reference reads past the end of its tables.
+struct Table {
+    int ndat = 0;
+    double bin = 0.0;
+    double x0 = 0.0;
+    std::vector<double> x, V, F;
+};
+
+
+
+struct HostSystem {
+    int natoms = 0, nvs = 0, nitems = 0, ntype = 0;
+    bool hybrid = false;
+    std::vector<double> mass, invmass;
+    std::vector<int> tlabel;            // per type: 1 atom, 2 bead
+    std::vector<int> itype, type_label; // per item: 1 atom, 2 bead, 3 virtual site
+    std::vector<int> nbonds, jbond;     // per atom, 4 slots
+    std::vector<double> s, v;           // s: nitems*3, v: natoms*3
+    double box[3] = {0, 0, 0};
+
+    std::vector<Table> bt, at, tt, nt, ot;
+    std::vector<int> inbondt;           // ntype*ntype -> nb table or -1
+    std::vector<int> bond_lab, angle_lab, tors_lab, oop_lab;
+
+    // Bonded terms in the reference's evaluation order (owner ascending).
+    std::vector<std::array<int, 4>> bonds, angles, tors, oops;  // member indices, -1 unused
+    std::vector<int> bond_tab, angle_tab, tors_tab, oop_tab;
+
+    // CONNECTED_TO as CSR over items.
+    std::vector<int> excl_off, excl;
+
+    // virtual sites
+    std::vector<int> vs_type, vs_center, vs_off, vs_atoms;
+    std::vector<double> vs_invmass;     // per VS
+    std::vector<double> vs_coeff;       // per member entry: m_j / M_vs
+    std::vector<int> vs_of_atom;        // per atom: owning VS or -1
+
+    // nonbonded groups: 0 = ATOM list, 1 = BEAD list (beads then VS)
+    std::vector<int> members[2];
+    double rcut[2] = {0, 0}, rlist[2] = {0, 0};
+    int ngroup = 1;
+
+    int ensemble = 0, nupdate = 1, halt_drift = 1, nmts = 0, mts_quirk = 0;
+    double dt = 0, temp_in = 0, fac = 0;
+    int numatoms = 0, numbeads = 0;
+    double mktemp = 0, mktemp_atom = 0, mktemp_bead = 0;
+    double temp_atom0 = 0, temp_bead0 = 0;
+};
+
+// Set-up of main.F90:107-234 from a descriptor (no device work).
+int build_host_system(const ibx_system_desc *d, HostSystem &hs, std::string &err);
+
+// Reference input directory -> descriptor storage (ibx_load_dir).
+struct DescStorage {
+    ibx_system_desc d{};
+    std::vector<double> mass, s, v, tx[5], tv[5];
+    std::vector<int> label, itype, nbonds, jbond, ttypes[5], toff[5], tnrow[5];
+    std::vector<int> vs_type, vs_center, vs_off, vs_atoms, virtbonds, virtangles;
+    int ntrj = 100, nstep = 0;
+};
+int read_input_dir(const char *dir, DescStorage &st, std::string &err);
+
+}  // namespace ibx

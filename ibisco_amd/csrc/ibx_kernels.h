@@ -1,0 +1,85 @@
+// Device data layout shared by the kernels and the host orchestration.
+//
+// A context holds S systems.  Items (atoms then virtual sites) of system s
+// occupy the padded range [item_off, item_off + nitems_pad); each nonbonded
+// group g keeps a cell-sorted copy of its members in the padded slot range
+// [slot_off[g], slot_off[g] + nslot_pad[g]).  Ranges are padded to the block
+// size so that every workgroup belongs to exactly one system.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace ibx {
+
+constexpr int kBlock = 256;        // threads per workgroup for every kernel
+constexpr int kWave = 64;
+constexpr int kLdsRows = 3072;     // float4 rows of nonbonded table staged in LDS (48 KiB)
+constexpr int kNbPart = 16;        // doubles per workgroup partial (nonbonded)
+constexpr int kBdPart = 24;        // doubles per workgroup partial (bonded: energies, overextension, virial)
+constexpr int kMvPart = 24;        // doubles per workgroup partial (integrate)
+constexpr int kObs = 32;
+constexpr int kMaxNbTabs = 64;     // nonbonded tables per system
+constexpr int kMaxNbTypes = 32;    // particle types per system with several nonbonded tables
+enum TermKind { KBOND = 0, KANGLE = 1, KTORS = 2, KOOP = 3 };
+
+struct DevGroup {
+    int slot_off, nslot_pad, nmem;
+    int mem_off;                   // offset into the member list
+    int cell_off, ncell;           // global cell range
+    int nc[3];
+    int all_pairs;                 // fewer than 3 cells in a dimension: scan every cell once
+    double rl2;                    // rlist^2 (FP64 list criterion)
+    float rl2u_lo, rl2u_hi;        // FP32 pre-classification thresholds, working units^2
+    float rc2u;                    // rcut^2 in working units^2
+    int wg_off, nwg;               // workgroup range of this system in the group's grid
+};
+
+struct DevSys {
+    double box[3], boxinv[3], box2[3];
+    double inv_box_frac[3];        // 1/L per dimension for fixed-point conversion
+    double unit;                   // working length unit: L_x / 2^32 (nm)
+    float rho[3];                  // L_k / L_x (1 for a cubic box)
+    int item_off, nitems_pad, natoms, nitems, nvs;
+    int item_wg_off, item_nwg;
+    int hybrid;
+    DevGroup g[2];
+    int ngroup;
+    // nonbonded tables of this system
+    int nb_row_off, nb_nrow;       // rows in the global float4 table array
+    int nb_ntab, nb_tab_off;       // table headers
+    int ntype, nbmap_off;          // ntype*ntype map -> global table header index
+    // integration
+    double dt, temp_in, fac;
+    int ensemble;
+    double mktemp, mktemp_atom, mktemp_bead;
+    double volume;
+    int halt_drift, nmts, mts_quirk, nupdate;
+    int vs_off;                    // offset into the VS arrays
+    int bead_off;                  // offset into MTS bead buffers
+    int nbead;                     // members of group 1 (beads+VS) for MTS
+};
+
+struct NbTab {                     // one nonbonded table as seen by a system
+    int row_off, ndat;
+    float c_hi, c_lo;              // unit/bin split in two floats
+    float bin;
+    float pad;
+};
+
+struct BdTab {                     // one bonded table (bond/angle/torsion/OOP)
+    int row_off, ndat;
+    double x0, bin, inv_bin;
+    int label;                     // energy class 1..3
+    int pad;
+};
+
+// Fixed-point fractional coordinate: floor(frac(s/L) * 2^32).
+__device__ __forceinline__ uint32_t to_fixed(double s, double inv_l) {
+    double t = s * inv_l;
+    t = t - floor(t);
+    double u = t * 4294967296.0;
+    if (u >= 4294967296.0) u = 0.0;
+    return (uint32_t)u;
+}
+
+}  // namespace ibx

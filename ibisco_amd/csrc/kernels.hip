@@ -1,0 +1,1009 @@
+// gfx950 kernels of the per-step path: cell sort and Verlet-list rebuild,
+// tabulated nonbonded forces, bonded forces, leap-frog integration and the
+// per-system reductions.  Launch wrappers are at the end of the file.
+#include <hip/hip_runtime.h>
+
+#include "ibx_kernels.h"
+#include "kernels_api.h"
+
+namespace ibx {
+
+// ------------------------------------------------------------------------------------------------
+// helpers
+
+// SHIFT.f90:37-43 in FP64 with no contraction, so the folded coordinates are
+// bit-identical to the oracle's.
+__device__ __forceinline__ void fold3(const double s[3], const double box2[3], double r[3]) {
+#pragma clang fp contract(off)
+    bool out = false;
+    for (int k = 0; k < 3; ++k) out |= (s[k] > box2[k]) || (s[k] < -box2[k]);
+    if (out) {
+        for (int k = 0; k < 3; ++k) {
+            double x = s[k] / box2[k];
+            double sp = x - 2.0 * trunc(x / 2.0);
+            r[k] = (sp - 2.0 * trunc(sp)) * box2[k];
+        }
+    } else {
+        r[0] = s[0]; r[1] = s[1]; r[2] = s[2];
+    }
+}
+
+// Pair criterion of NEW_NEIGHBOUR_WITHLIST.f90:64-69 in FP64, no contraction.
+__device__ __forceinline__ double exact_r2(const double4 a, const double4 b, const DevSys &S) {
+#pragma clang fp contract(off)
+    double d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z;
+    d0 = d0 - round(d0 * S.boxinv[0]) * S.box[0];
+    d1 = d1 - round(d1 * S.boxinv[1]) * S.box[1];
+    d2 = d2 - round(d2 * S.boxinv[2]) * S.box[2];
+    double r2 = d0 * d0;
+    r2 = r2 + d1 * d1;
+    r2 = r2 + d2 * d2;
+    return r2;
+}
+
+__device__ __forceinline__ double wave_sum(double x) {
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    return x;
+}
+
+// Deterministic block reduction of n doubles per thread; thread 0 gets the sums.
+template <int N>
+__device__ __forceinline__ void block_sum(double (&v)[N], double *lds /* N*4 */) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double x = wave_sum(v[k]);
+        if (lane == 0) lds[k * 4 + wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = ((lds[k * 4] + lds[k * 4 + 1]) + lds[k * 4 + 2]) + lds[k * 4 + 3];
+    }
+    __syncthreads();
+}
+
+constexpr int kMultiTabsLds(bool multi) { return multi ? kMaxNbTabs : 1; }
+
+__device__ __forceinline__ int pos_type(int w) { return w & 0xffff; }
+__device__ __forceinline__ int pos_label(int w) { return (w >> 16) & 0xff; }
+
+// ------------------------------------------------------------------------------------------------
+// list rebuild (replaces LINKS.f90, MAPS.f90, NEW_NEIGHBOUR_WITHLIST.f90)
+
+__global__ void k_zero_int(int *p, int n) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Cell of every member from its folded FP64 position (LINKS.f90:16-18, clamped).
+__global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
+                                                       int g, const int *__restrict__ mem, const double *__restrict__ s,
+                                                       int nitems_total, int *__restrict__ mem_cell,
+                                                       int *__restrict__ cell_count) {
+    const DevSys &S = sys[wg_sys[blockIdx.x]];
+    const DevGroup &G = S.g[g];
+    int m = blockIdx.x * kBlock + threadIdx.x;
+    int it = mem[m];
+    if (it < 0) return;
+    double sp[3] = {s[it], s[nitems_total + it], s[2 * nitems_total + it]}, r[3];
+    fold3(sp, S.box2, r);
+    int c[3];
+    for (int k = 0; k < 3; ++k) {
+        int ck = (int)((r[k] * S.boxinv[k] + 0.5) * G.nc[k]);
+        c[k] = ck < 0 ? 0 : (ck >= G.nc[k] ? G.nc[k] - 1 : ck);
+    }
+    int cell = (c[0] * G.nc[1] + c[1]) * G.nc[2] + c[2];
+    mem_cell[m] = cell;
+    atomicAdd(&cell_count[G.cell_off + cell], 1);
+}
+
+// Exclusive scan of the cell counts of the whole group grid (one workgroup).
+__global__ void __launch_bounds__(1024) k_scan_cells(const int *__restrict__ count, int *__restrict__ start,
+                                                     int *__restrict__ fill, int n) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (n + 1023) / 1024;
+    const int a = t * per, b = min(n, a + per);
+    int sum = 0;
+    for (int i = a; i < b; ++i) sum += count[i];
+    part[t] = sum;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int x = t >= off ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += x;
+        __syncthreads();
+    }
+    int run = part[t] - sum;
+    for (int i = a; i < b; ++i) {
+        start[i] = run;
+        fill[i] = 0;
+        run += count[i];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_place(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, int g,
+                                                  const int *__restrict__ mem, const int *__restrict__ mem_cell,
+                                                  const int *__restrict__ cell_start, int *__restrict__ cell_fill,
+                                                  int *__restrict__ tmp) {
+    const DevSys &S = sys[wg_sys[blockIdx.x]];
+    const DevGroup &G = S.g[g];
+    int m = blockIdx.x * kBlock + threadIdx.x;
+    if (mem[m] < 0) return;
+    int gc = G.cell_off + mem_cell[m];
+    int local = cell_start[gc] - cell_start[G.cell_off];
+    int pos = atomicAdd(&cell_fill[gc], 1);
+    tmp[G.slot_off + local + pos] = m;
+}
+
+// Orders each cell by member index so the sorted layout (and every sum over
+// it) is deterministic.  One workgroup per cell.
+__global__ void __launch_bounds__(kBlock) k_sort_cells(const int *__restrict__ cell_sys, const DevSys *__restrict__ sys, int g,
+                                                       const int *__restrict__ cell_start, const int *__restrict__ cell_count,
+                                                       const int *__restrict__ tmp, const int *__restrict__ mem,
+                                                       int *__restrict__ perm, int *__restrict__ slot_cell,
+                                                       int *__restrict__ flags) {
+    __shared__ int vals[1024];
+    const int gc = blockIdx.x;
+    const DevSys &S = sys[cell_sys[gc]];
+    const DevGroup &G = S.g[g];
+    const int n = cell_count[gc];
+    if (n == 0) return;
+    const int base = G.slot_off + cell_start[gc] - cell_start[G.cell_off];
+    const int lc = gc - G.cell_off;
+    if (n > 1024) {   // too crowded to sort in LDS: keep arrival order, flag it
+        if (threadIdx.x == 0) atomicMax(&flags[2], n);
+        for (int i = threadIdx.x; i < n; i += kBlock) {
+            perm[base + i] = mem[tmp[base + i]];
+            slot_cell[base + i] = lc;
+        }
+        return;
+    }
+    for (int i = threadIdx.x; i < n; i += kBlock) vals[i] = tmp[base + i];
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += kBlock) {
+        int v = vals[i], rank = 0;
+        for (int k = 0; k < n; ++k) rank += vals[k] < v;
+        perm[base + rank] = mem[v];
+        slot_cell[base + rank] = lc;
+    }
+}
+
+// Sorted copies of the positions: folded FP64 (list criterion) and fixed point
+// (nonbonded kernel), plus the item -> slot map.
+__global__ void __launch_bounds__(kBlock) k_pack(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, int g,
+                                                 const int *__restrict__ perm, const double *__restrict__ s,
+                                                 int nitems_total, const int *__restrict__ itype,
+                                                 const int *__restrict__ label, double4 *__restrict__ r64,
+                                                 int4 *__restrict__ pos4, int *__restrict__ inv_perm) {
+    const DevSys &S = sys[wg_sys[blockIdx.x]];
+    int slot = blockIdx.x * kBlock + threadIdx.x;
+    int it = perm[slot];
+    if (it < 0) {
+        pos4[slot] = make_int4(0, 0, 0, -1);
+        r64[slot] = make_double4(0, 0, 0, 0);
+        return;
+    }
+    double sp[3] = {s[it], s[nitems_total + it], s[2 * nitems_total + it]}, r[3];
+    fold3(sp, S.box2, r);
+    r64[slot] = make_double4(r[0], r[1], r[2], 0.0);
+    pos4[slot] = make_int4((int)to_fixed(sp[0], S.inv_box_frac[0]), (int)to_fixed(sp[1], S.inv_box_frac[1]),
+                           (int)to_fixed(sp[2], S.inv_box_frac[2]), itype[it] | (label[it] << 16));
+    inv_perm[it] = slot;
+}
+
+// Full Verlet list per sorted slot: every member j != i of the 27 neighbouring
+// cells (each cell once) with r^2 < rlist^2, not excluded, not VS-VS.  The set
+// of unordered pairs equals the reference's half list bit for bit: a cheap
+// fixed-point test settles pairs far from the boundary and the FP64 criterion
+// decides the rest.  Layout: list[((slot/64)*cap + k)*64 + slot%64].
+__global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, int g,
+                                                       const int *__restrict__ perm, const int4 *__restrict__ pos4,
+                                                       const double4 *__restrict__ r64, const int *__restrict__ slot_cell,
+                                                       const int *__restrict__ cell_start, const int *__restrict__ cell_count,
+                                                       const int *__restrict__ excl_off, const int *__restrict__ excl,
+                                                       int cap, int *__restrict__ list, int *__restrict__ nneb,
+                                                       int *__restrict__ flags) {
+    const DevSys &S = sys[wg_sys[blockIdx.x]];
+    const DevGroup &G = S.g[g];
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int I = perm[i];
+    if (I < 0) { nneb[i] = 0; return; }
+    const int4 pi = pos4[i];
+    const double4 ri = r64[i];
+    const int li = pos_label(pi.w);
+    const int ex0 = excl_off[I], ex1 = excl_off[I + 1];
+    const int lc = slot_cell[i];
+    const int cz = lc % G.nc[2], cy = (lc / G.nc[2]) % G.nc[1], cx = lc / (G.nc[2] * G.nc[1]);
+    const int cs0 = cell_start[G.cell_off];
+    int* out = list + (size_t)(i >> 6) * cap * 64 + (i & 63);
+    int cnt = 0;
+    const int ncell_scan = G.all_pairs ? G.ncell : 27;
+    for (int q = 0; q < ncell_scan; ++q) {
+        int cc;
+        if (G.all_pairs) {
+            cc = q;
+        } else {
+            int ox = q / 9 - 1, oy = (q / 3) % 3 - 1, oz = q % 3 - 1;
+            int x = cx + ox, y = cy + oy, z = cz + oz;
+            x += (x < 0) ? G.nc[0] : 0; x -= (x >= G.nc[0]) ? G.nc[0] : 0;
+            y += (y < 0) ? G.nc[1] : 0; y -= (y >= G.nc[1]) ? G.nc[1] : 0;
+            z += (z < 0) ? G.nc[2] : 0; z -= (z >= G.nc[2]) ? G.nc[2] : 0;
+            cc = (x * G.nc[1] + y) * G.nc[2] + z;
+        }
+        const int gc = G.cell_off + cc;
+        const int j0 = G.slot_off + cell_start[gc] - cs0, j1 = j0 + cell_count[gc];
+        for (int j = j0; j < j1; ++j) {
+            if (j == i) continue;
+            const int4 pj = pos4[j];
+            float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x) * S.rho[0];
+            float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y) * S.rho[1];
+            float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z) * S.rho[2];
+            float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            if (r2u > G.rl2u_hi) continue;
+            if (r2u >= G.rl2u_lo) {
+                if (!(exact_r2(ri, r64[j], S) < G.rl2)) continue;
+            }
+            const int J = perm[j];
+            bool skip = S.hybrid && li == 3 && pos_label(pj.w) == 3;
+            for (int e = ex0; e < ex1 && !skip; ++e) skip = (excl[e] == J);
+            if (skip) continue;
+            if (cnt < cap) out[(size_t)cnt * 64] = j;
+            ++cnt;
+        }
+    }
+    nneb[i] = cnt;
+    if (cnt > cap) atomicMax(&flags[0], cnt);
+}
+
+// ------------------------------------------------------------------------------------------------
+// nonbonded forces (replaces NONBONDED_FORCE.F90:11-109)
+//
+// One thread per sorted slot walks its full list; positions are int32 fixed
+// point, so the minimum image is the wrap-around of the integer difference.
+// Tables (F/r and V, interleaved with the next row) are staged in LDS.  Each
+// pair is visited from both ends; energies and virial are halved.
+
+// Contribution of one listed neighbour; `cls` selects the energy class.
+template <bool kMulti, int kGroup>
+__device__ __forceinline__ void nb_pair(const int4 pi, const int4 pj, int li, float rc2u, const float rho[3],
+                                        const float4 *__restrict__ rows, const NbTab *__restrict__ tabs,
+                                        const int *__restrict__ map, int ntype, const NbTab &t0, float &fx,
+                                        float &fy, float &fz, float (&e)[3], float (&w)[6]) {
+    const float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x) * rho[0];
+    const float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y) * rho[1];
+    const float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z) * rho[2];
+    const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+    if (r2 < rc2u) {
+        NbTab tb = t0;
+        if (kMulti) tb = tabs[map[pos_type(pi.w) * ntype + pos_type(pj.w)]];
+        const float ru = sqrtf(r2);
+        const float tt = fmaf(ru, tb.c_hi, ru * tb.c_lo);
+        int kr = (int)tt;
+        kr = kr > tb.ndat ? tb.ndat : kr;
+        const float a = tt - (float)kr;
+        const float4 row = rows[tb.row_off + kr];
+        const float F = fmaf(a, row.z - row.x, row.x);
+        const float V = fmaf(a, row.w - row.y, row.y);
+        fx = fmaf(F, dx, fx);
+        fy = fmaf(F, dy, fy);
+        fz = fmaf(F, dz, fz);
+        if (kGroup == 0) {
+            e[0] += V;
+        } else {
+            const int cls = (li == 2 && pos_label(pj.w) == 2) ? 1 : 2;
+            e[cls] += V;
+        }
+        const float fdx = F * dx, fdy = F * dy;
+        w[0] = fmaf(fdx, dx, w[0]);
+        w[1] = fmaf(fdy, dy, w[1]);
+        w[2] = fmaf(F * dz, dz, w[2]);
+        w[3] = fmaf(fdy, dx, w[3]);
+        w[4] = fmaf(fdx, dz, w[4]);
+        w[5] = fmaf(fdy, dz, w[5]);
+    }
+}
+
+// Nonbonded forces (replaces NONBONDED_FORCE.F90:11-109).
+//
+// One thread per sorted slot walks its full list; positions are int32 fixed
+// point, so the minimum image is the wrap-around of the integer difference.
+// Tables (F/r and V, interleaved with the next row) are staged in LDS when
+// they fit (kLds), else read through the cache.  Each pair is visited from
+// both ends; energies and virial are halved.  The neighbour loop is unrolled
+// by four so the index and position loads of four pairs are in flight together.
+template <bool kMulti, bool kLds, int kGroup>
+__global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
+                                                      const int *__restrict__ perm, const int4 *__restrict__ pos4,
+                                                      const int *__restrict__ list, const int *__restrict__ nneb, int cap,
+                                                      const float4 *__restrict__ nb_rows, const NbTab *__restrict__ nb_tabs,
+                                                      const int *__restrict__ nb_map, int nitems_total,
+                                                      float *__restrict__ fnb, double *__restrict__ part) {
+    extern __shared__ float4 lds_rows[];
+    __shared__ NbTab tabs[kMultiTabsLds(kMulti)];
+    __shared__ int map[kMulti ? kMaxNbTypes * kMaxNbTypes : 1];
+    __shared__ double red[9 * 4];
+    const DevSys &S = sys[wg_sys[blockIdx.x]];
+    const DevGroup &G = S.g[kGroup];
+    if (kLds)
+        for (int r = threadIdx.x; r < S.nb_nrow; r += kBlock) lds_rows[r] = nb_rows[S.nb_row_off + r];
+    for (int t = threadIdx.x; t < S.nb_ntab && t < kMultiTabsLds(kMulti); t += kBlock) {
+        NbTab tb = nb_tabs[S.nb_tab_off + t];
+        tb.row_off -= S.nb_row_off;
+        tabs[t] = tb;
+    }
+    if (kMulti)
+        for (int t = threadIdx.x; t < S.ntype * S.ntype; t += kBlock) {
+            const int m = nb_map[S.nbmap_off + t];
+            map[t] = m < 0 ? 0 : m - S.nb_tab_off;
+        }
+    __syncthreads();
+    const float4 *rows = kLds ? lds_rows : nb_rows + S.nb_row_off;
+
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int I = perm[i];
+    float fx = 0.f, fy = 0.f, fz = 0.f;
+    float e[3] = {0.f, 0.f, 0.f};
+    float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    if (I >= 0) {
+        const int4 pi = pos4[i];
+        const int li = pos_label(pi.w);
+        const float rc2u = G.rc2u;
+        const float rho[3] = {S.rho[0], S.rho[1], S.rho[2]};
+        const NbTab t0 = tabs[0];
+        const int nt = S.ntype;
+        const int n = min(nneb[i], cap);
+        const int *lp = list + (size_t)(i >> 6) * cap * 64 + (i & 63);
+        int k = 0;
+        for (; k + 4 <= n; k += 4) {
+            const int j0 = lp[(size_t)k * 64], j1 = lp[(size_t)(k + 1) * 64];
+            const int j2 = lp[(size_t)(k + 2) * 64], j3 = lp[(size_t)(k + 3) * 64];
+            const int4 p0 = pos4[j0], p1 = pos4[j1], p2 = pos4[j2], p3 = pos4[j3];
+            nb_pair<kMulti, kGroup>(pi, p0, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
+            nb_pair<kMulti, kGroup>(pi, p1, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
+            nb_pair<kMulti, kGroup>(pi, p2, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
+            nb_pair<kMulti, kGroup>(pi, p3, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
+        }
+        for (; k < n; ++k)
+            nb_pair<kMulti, kGroup>(pi, pos4[lp[(size_t)k * 64]], li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e,
+                                    w);
+        const float u = (float)S.unit;
+        fnb[I] = fx * u;
+        fnb[nitems_total + I] = fy * u;
+        fnb[2 * nitems_total + I] = fz * u;
+    }
+    double acc[9] = {e[0], e[1], e[2], w[0], w[1], w[2], w[3], w[4], w[5]};
+    block_sum<9>(acc, red);
+    if (threadIdx.x == 0) {
+        double *p = part + (size_t)blockIdx.x * kNbPart;
+        const double u2 = S.unit * S.unit;
+        for (int k = 0; k < 3; ++k) p[k] = 0.5 * acc[k];
+        for (int k = 3; k < 9; ++k) p[k] = 0.5 * acc[k] * u2;
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// bonded forces (replaces BONDED_FORCE.F90 with BONDS/ANGLES/TORSIONS/OUTOFPLANES.inc)
+//
+// Gather form: each item evaluates every term it takes part in and keeps its
+// own share, so no atomics are needed and the sums are deterministic.  The
+// energy of a term is counted by its first member.  Geometry differences are
+// taken in FP64 from the unwrapped coordinates, the rest in FP32.
+
+struct BondedArgs {
+    const int4 *bonds;    // i, j, table, 0
+    const int4 *angles;   // i, j, k, table
+    const int4 *tors;     // i, j, k, l
+    const int *tors_tab;
+    const int4 *oops;
+    const int *oop_tab;
+    const int *term_off;  // per item CSR
+    const int *term_ref;  // kind<<29 | role<<27 | index
+    const BdTab *tabs;    // all bonded table headers
+    const float4 *rows;   // all bonded rows {F_k, V_k, F_k+1, V_k+1}
+};
+
+// Table row index from a non-negative abscissa ratio, clamped to [0, ndat]
+// (NaN -> 0); ratios at or past ndat+1 set `bit` in *flag.
+template <typename T>
+__device__ __forceinline__ int row_index(T x, int ndat, int *flag, int bit) {
+    if (!(x >= (T)0)) return 0;
+    if (x >= (T)(ndat + 1)) {
+        if (flag) atomicOr(flag, bit);
+        return ndat;
+    }
+    return min((int)x, ndat);
+}
+
+template <typename T>
+__device__ __forceinline__ void lerp_row(const float4 *rows, const BdTab &t, int ni, T alpha, T &F, T &V) {
+    const float4 r = rows[t.row_off + ni];
+    F = (T)r.x * ((T)1 - alpha) + alpha * (T)r.z;
+    V = (T)r.y * ((T)1 - alpha) + alpha * (T)r.w;
+}
+
+template <typename T>
+__device__ __forceinline__ void dvec(const double *s, int n, int a, int b, T d[3]) {
+    d[0] = (T)(s[a] - s[b]);
+    d[1] = (T)(s[n + a] - s[n + b]);
+    d[2] = (T)(s[2 * n + a] - s[2 * n + b]);
+}
+
+template <typename T>
+__device__ __forceinline__ T nrm(const T a[3]) { return sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]); }
+
+template <typename T>
+__device__ __forceinline__ void cross(const T a[3], const T b[3], T c[3]) {
+    c[0] = a[1] * b[2] - a[2] * b[1];
+    c[1] = -a[0] * b[2] + a[2] * b[0];
+    c[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// BONDS.inc:8-56.  Returns the force on member `role` and the energy.
+template <typename T>
+__device__ __forceinline__ void bond_term(const BondedArgs &A, const double *s, int n, int idx, T fm[4][3], T &V,
+                                          bool &over) {
+    const int4 b = A.bonds[idx];
+    const BdTab t = A.tabs[b.z];
+    T d[3];
+    dvec(s, n, b.x, b.y, d);
+    const T rij = nrm(d);
+    const T xr = (rij - (T)t.x0) * (T)t.inv_bin;
+    over = xr >= (T)(t.ndat + 1);              // BONDS.inc:21-31: warn, clamp, continue
+    const int ni = row_index(xr, t.ndat, (int *)nullptr, 0);
+    const T alpha = (rij - ((T)t.x0 + (T)ni * (T)t.bin)) * (T)t.inv_bin;
+    T F;
+    lerp_row(A.rows, t, ni, alpha, F, V);
+    for (int k = 0; k < 3; ++k) {
+        fm[0][k] = F * d[k];
+        fm[1][k] = -F * d[k];
+    }
+}
+
+// ANGLES.inc:2-75
+template <typename T>
+__device__ __forceinline__ void angle_term(const BondedArgs &A, const double *s, int n, int idx, T fm[4][3], T &V) {
+    const T R2D = (T)180 / (T)3.14159265358;
+    const int4 a4 = A.angles[idx];
+    const BdTab t = A.tabs[a4.w];
+    T ij[3], kj[3], M[3], N[3], Sv[3];
+    dvec(s, n, a4.x, a4.y, ij);
+    dvec(s, n, a4.z, a4.y, kj);
+    const T rij = nrm(ij), rkj = nrm(kj);
+    T ct = (ij[0] * kj[0] + ij[1] * kj[1] + ij[2] * kj[2]) / rij / rkj;
+    ct = ct > (T)1 ? (T)1 : (ct < (T)-1 ? (T)-1 : ct);
+    const T theta = acos(ct) * R2D;
+    M[0] = -(ij[1] * kj[2] - ij[2] * kj[1]);
+    M[1] = -(-ij[0] * kj[2] + ij[2] * kj[0]);
+    M[2] = -(ij[0] * kj[1] - ij[1] * kj[0]);
+    const T rm = nrm(M);
+    for (int k = 0; k < 3; ++k) M[k] /= rm;
+    N[0] = (M[1] * ij[2] - M[2] * ij[1]) / rij;
+    N[1] = (-M[0] * ij[2] + M[2] * ij[0]) / rij;
+    N[2] = (M[0] * ij[1] - M[1] * ij[0]) / rij;
+    Sv[0] = -(M[1] * kj[2] - M[2] * kj[1]) / rkj;
+    Sv[1] = -(-M[0] * kj[2] + M[2] * kj[0]) / rkj;
+    Sv[2] = -(M[0] * kj[1] - M[1] * kj[0]) / rkj;
+    const T xr = theta * (T)t.inv_bin;
+    const int ni = row_index(xr, t.ndat, (int *)nullptr, 0);
+    T F;
+    if (theta == (T)180) {
+        const float4 r = A.rows[t.row_off + ni];
+        F = (T)r.x;
+        V = (T)r.y;
+    } else {
+        lerp_row(A.rows, t, ni, (theta - (T)ni * (T)t.bin) * (T)t.inv_bin, F, V);
+    }
+    for (int k = 0; k < 3; ++k) {
+        const T fa = F * N[k] / rij, fk = F * Sv[k] / rkj;
+        fm[0][k] = fa;
+        fm[1][k] = -(fa + fk);
+        fm[2][k] = fk;
+    }
+}
+
+// TORSIONS.inc:1-152
+template <typename T>
+__device__ __forceinline__ void torsion_term(const BondedArgs &A, const double *s, int n, int idx, T fm[4][3], T &V,
+                                             int *flags) {
+    const T R2D = (T)180 / (T)3.14159265358;
+    const int4 q4 = A.tors[idx];
+    const BdTab t = A.tabs[A.tors_tab[idx]];
+    T ij[3], jk[3], kl[3], M[3], N[3];
+    dvec(s, n, q4.x, q4.y, ij);
+    dvec(s, n, q4.y, q4.z, jk);
+    dvec(s, n, q4.z, q4.w, kl);
+    T rij = nrm(ij);
+    const T rjk = nrm(jk);
+    T rkl = nrm(kl);
+    cross(ij, jk, M);
+    T rm = nrm(M);
+    if (rm == (T)0) {           // TORSIONS.inc:47-59: perturb x of atom i
+        ij[0] = (T)(s[q4.x] * 0.99 - s[q4.y]);
+        rij = nrm(ij);
+        cross(ij, jk, M);
+        rm = nrm(M);
+    }
+    for (int k = 0; k < 3; ++k) M[k] /= rm;
+    const T cosm = (ij[0] * jk[0] + ij[1] * jk[1] + ij[2] * jk[2]) / rij / rjk;
+    const T sinm = sqrt(fmax((T)0, (T)1 - cosm * cosm));
+    const T cotm = cosm / sinm;
+    cross(jk, kl, N);
+    T rn = nrm(N);
+    if (rn == (T)0) {           // TORSIONS.inc:75-87: perturb x of atom l
+        kl[0] = (T)(s[q4.z] - s[q4.w] * 0.99);
+        rkl = nrm(kl);
+        cross(jk, kl, N);
+        rn = nrm(N);
+    }
+    for (int k = 0; k < 3; ++k) N[k] /= rn;
+    const T cosn = (jk[0] * kl[0] + jk[1] * kl[1] + jk[2] * kl[2]) / rjk / rkl;
+    const T sinn = sqrt(fmax((T)0, (T)1 - cosn * cosn));
+    const T cotn = cosn / sinn;
+    T cost = M[0] * N[0] + M[1] * N[1] + M[2] * N[2];
+    const T signt = -(M[0] * kl[0] + M[1] * kl[1] + M[2] * kl[2]);
+    cost = cost > (T)1 ? (T)1 : (cost < (T)-1 ? (T)-1 : cost);
+    T phi = acos(cost) * R2D;
+    if (signt < (T)0) phi = (T)360 - phi;
+    const int ni = row_index(phi * (T)t.inv_bin, t.ndat, flags, 1);
+    T F;
+    if (phi == (T)360) {
+        const float4 r = A.rows[t.row_off + ni];
+        F = (T)r.x;
+        V = (T)r.y;
+    } else {
+        lerp_row(A.rows, t, ni, (phi - (T)ni * (T)t.bin) * (T)t.inv_bin, F, V);
+    }
+    for (int k = 0; k < 3; ++k) {
+        const T f1 = -F * M[k] / sinm / rij;
+        const T f4 = F * N[k] / sinn / rkl;
+        const T f12 = F * (cotm * M[k] + cotn * N[k]) / rjk;
+        fm[0][k] = f1;
+        fm[1][k] = -f1 + f12;
+        fm[2][k] = -f4 - f12;
+        fm[3][k] = f4;
+    }
+}
+
+// OUTOFPLANES.inc:1-141
+template <typename T>
+__device__ __forceinline__ void oop_term(const BondedArgs &A, const double *s, int n, int idx, T fm[4][3], T &V,
+                                         int *flags) {
+    const T R2D = (T)180 / (T)3.14159265358;
+    const int4 q4 = A.oops[idx];
+    const BdTab t = A.tabs[A.oop_tab[idx]];
+    T ij[3], jk[3], kl[3], M[3], N[3], mN[3];
+    dvec(s, n, q4.x, q4.y, ij);
+    dvec(s, n, q4.z, q4.y, jk);
+    dvec(s, n, q4.z, q4.w, kl);
+    const T rij = nrm(ij), rjk = nrm(jk), rkl = nrm(kl);
+    cross(ij, jk, M);
+    const T rm = nrm(M);
+    for (int k = 0; k < 3; ++k) M[k] /= rm;
+    const T cosm = (ij[0] * jk[0] + ij[1] * jk[1] + ij[2] * jk[2]) / rij / rjk;
+    const T sinm = sqrt(fmax((T)0, (T)1 - cosm * cosm)), cotm = cosm / sinm;
+    cross(jk, kl, N);
+    const T rn = nrm(N);
+    for (int k = 0; k < 3; ++k) N[k] /= rn;
+    const T cosn = (jk[0] * kl[0] + jk[1] * kl[1] + jk[2] * kl[2]) / rjk / rkl;
+    const T sinn = sqrt(fmax((T)0, (T)1 - cosn * cosn)), cotn = cosn / sinn;
+    T cost = M[0] * N[0] + M[1] * N[1] + M[2] * N[2];
+    cross(M, N, mN);
+    const T signt = copysign((T)1, mN[0] * jk[0] + mN[1] * jk[1] + mN[2] * jk[2]);
+    cost = cost > (T)1 ? (T)1 : (cost < (T)-1 ? (T)-1 : cost);
+    const T phi = signt * acos(cost) * R2D;
+    int ni = row_index(rint(((T)180 + phi) * (T)t.inv_bin), t.ndat, flags, 2);
+    T alpha = fabs(fabs(phi) - fabs((T)ni * (T)t.bin)) * (T)t.inv_bin;
+    if (alpha > (T)1) {
+        ni = min(ni + 1, t.ndat);
+        alpha = fabs(fabs(phi) - fabs((T)ni * (T)t.bin)) * (T)t.inv_bin;
+    }
+    T F;
+    lerp_row(A.rows, t, ni, alpha, F, V);
+    for (int k = 0; k < 3; ++k) {
+        const T f1 = F * M[k] / sinm / rij;
+        const T f4 = -F * N[k] / sinn / rkl;
+        const T f12 = F * (cotm * M[k] + cotn * N[k]) / rjk;
+        fm[0][k] = f1;
+        fm[1][k] = -f1 + f12;
+        fm[2][k] = -f4 - f12;
+        fm[3][k] = f4;
+    }
+}
+
+// Bonded forces (replaces BONDED_FORCE.F90 with BONDS/ANGLES/TORSIONS/OUTOFPLANES.inc).
+//
+// Gather form: each item evaluates every term it takes part in and keeps its
+// own share, so no atomics are needed and the sums are deterministic.  The
+// first member of a term evaluates it in FP64 and counts its energy and its
+// virial; the other members evaluate it in FP32 for their force only.
+// Geometry differences are always taken in FP64 from the unwrapped coordinates.
+//
+// Virial: NEW_FORCE.F90:73-80 sums (F - F_nb) . s over atoms with unwrapped
+// coordinates.  Because the member forces of a term sum to zero, the same sum
+// is accumulated per term as sum_m f_m (s_m - s_first) in FP64, which does not
+// lose precision to the size of the unwrapped coordinates.  Order of the six
+// components as the reference: 11 22 33, then F_y s_x, F_z s_x, F_z s_y.
+__device__ __forceinline__ void term_members(const BondedArgs &A, int kind, int idx, int m[4], int &nm) {
+    if (kind == KBOND) { const int4 b = A.bonds[idx]; m[0] = b.x; m[1] = b.y; nm = 2; }
+    else if (kind == KANGLE) { const int4 a = A.angles[idx]; m[0] = a.x; m[1] = a.y; m[2] = a.z; nm = 3; }
+    else if (kind == KTORS) { const int4 q = A.tors[idx]; m[0] = q.x; m[1] = q.y; m[2] = q.z; m[3] = q.w; nm = 4; }
+    else { const int4 q = A.oops[idx]; m[0] = q.x; m[1] = q.y; m[2] = q.z; m[3] = q.w; nm = 4; }
+}
+
+__global__ void __launch_bounds__(kBlock) k_bonded(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
+                                                   BondedArgs A, const double *__restrict__ s, int nitems_total,
+                                                   float *__restrict__ fb, double *__restrict__ part,
+                                                   int *__restrict__ flags) {
+    __shared__ double red[19 * 4];
+    (void)sys; (void)wg_sys;
+    const int it = blockIdx.x * kBlock + threadIdx.x;
+    const int n = nitems_total;
+    double fsum[3] = {0, 0, 0};
+    double en[19];
+    for (int k = 0; k < 19; ++k) en[k] = 0.0;
+    const int t0 = A.term_off[it], t1 = A.term_off[it + 1];
+    for (int q = t0; q < t1; ++q) {
+        const int ref = A.term_ref[q];
+        const int kind = (ref >> 29) & 3, role = (ref >> 27) & 3, idx = ref & ((1 << 27) - 1);
+        if (role == 0) {
+            double fm[4][3], V;
+            int cls;
+            if (kind == KBOND) {
+                bool over;
+                bond_term<double>(A, s, n, idx, fm, V, over);
+                cls = A.tabs[A.bonds[idx].z].label - 1;
+                if (over) en[12] += 1.0;
+            } else if (kind == KANGLE) {
+                angle_term<double>(A, s, n, idx, fm, V);
+                cls = 3 + A.tabs[A.angles[idx].w].label - 1;
+            } else if (kind == KTORS) {
+                torsion_term<double>(A, s, n, idx, fm, V, flags);
+                cls = 6 + A.tabs[A.tors_tab[idx]].label - 1;
+            } else {
+                oop_term<double>(A, s, n, idx, fm, V, flags);
+                cls = 9 + A.tabs[A.oop_tab[idx]].label - 1;
+            }
+            en[cls] += V;
+            for (int k = 0; k < 3; ++k) fsum[k] += fm[0][k];
+            int mem[4], nm;
+            term_members(A, kind, idx, mem, nm);
+            for (int r = 1; r < nm; ++r) {
+                const double dx = s[mem[r]] - s[mem[0]], dy = s[n + mem[r]] - s[n + mem[0]],
+                             dz = s[2 * n + mem[r]] - s[2 * n + mem[0]];
+                en[13] += fm[r][0] * dx;
+                en[14] += fm[r][1] * dy;
+                en[15] += fm[r][2] * dz;
+                en[16] += fm[r][1] * dx;
+                en[17] += fm[r][2] * dx;
+                en[18] += fm[r][2] * dy;
+            }
+        } else {
+            float fm[4][3], V;
+            if (kind == KBOND) {
+                bool over;
+                bond_term<float>(A, s, n, idx, fm, V, over);
+            } else if (kind == KANGLE) {
+                angle_term<float>(A, s, n, idx, fm, V);
+            } else if (kind == KTORS) {
+                torsion_term<float>(A, s, n, idx, fm, V, flags);
+            } else {
+                oop_term<float>(A, s, n, idx, fm, V, flags);
+            }
+            for (int k = 0; k < 3; ++k) fsum[k] += (double)fm[role][k];
+        }
+    }
+    fb[it] = (float)fsum[0];
+    fb[n + it] = (float)fsum[1];
+    fb[2 * n + it] = (float)fsum[2];
+    block_sum<19>(en, red);
+    if (threadIdx.x == 0) {
+        double *p = part + (size_t)blockIdx.x * kBdPart;
+        for (int k = 0; k < 19; ++k) p[k] = en[k];
+    }
+}
+
+// ------------------------------------------------------------------------------------------------
+// integration (replaces MOVE.F90:9-118; MOMENTUM.f90 is applied here too)
+//
+// mode 0: leap-frog step; mode 1: forces only (combine, bonded virial, total force).
+// The total force on an atom is its nonbonded plus bonded force plus the mass
+// share of the forces on its virtual site (DISTRIBUTE_VSFORCE.f90:21-38); the
+// bonded virial is sum over atoms of (F - F_nb) . s (NEW_FORCE.F90:73-80).
+
+struct MoveArgs {
+    double *s, *v;
+    const float *fnb, *fb;
+    float *ftot;
+    const int *itype, *label;
+    const double *mass, *invmass;
+    const int *vs_of;
+    const double *vs_coeff;
+    int4 *pos4[2];
+    const int *inv_perm[2];
+    const double *sysstate;   // per system: temp_atom, temp_bead, pending momentum x,y,z
+    double *part;
+    int *step;
+    int nitems_total;
+    int mode;
+};
+
+__global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, MoveArgs A) {
+    __shared__ double red[21 * 4];
+    const int s_id = wg_sys[blockIdx.x];
+    const DevSys &S = sys[s_id];
+    const int it = blockIdx.x * kBlock + threadIdx.x;
+    const int n = A.nitems_total;
+    if (A.mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.step, 1);
+    double acc[21];
+    for (int k = 0; k < 21; ++k) acc[k] = 0.0;
+    const int ty = A.itype[it];
+    const int lab = A.label[it];
+    if (A.mode == 1 && ty >= 0 && lab == 3)
+        for (int k = 0; k < 3; ++k) A.ftot[k * n + it] = 0.0f;   // distributed (DISTRIBUTE_VSFORCE.f90:40)
+    if (ty >= 0 && lab != 3) {
+        double F[3];
+        for (int k = 0; k < 3; ++k) F[k] = (double)A.fnb[k * n + it] + (double)A.fb[k * n + it];
+        const int vs = A.vs_of[it];
+        if (vs >= 0) {
+            const double c = A.vs_coeff[it];
+            for (int k = 0; k < 3; ++k) F[k] += (double)A.fnb[k * n + vs] * c + (double)A.fb[k * n + vs] * c;
+        }
+        double sp[3] = {A.s[it], A.s[n + it], A.s[2 * n + it]};
+        if (A.mode == 1) {
+            for (int k = 0; k < 3; ++k) A.ftot[k * n + it] = (float)F[k];
+        } else {
+            const double *st = A.sysstate + (size_t)s_id * 8;
+            const int pop = (lab == 2) ? 1 : 0;
+            double lam = 1.0;
+            if (S.ensemble == 1) lam = sqrt(1.0 + S.fac * ((S.temp_in / st[pop]) - 1.0));
+            const double m = A.mass[it], im = A.invmass[it];
+            double vt[3];
+            for (int k = 0; k < 3; ++k) {
+                const double v0 = A.v[k * n + it] - st[2 + k] * im;   // deferred MOMENTUM
+                const double v1 = (v0 + S.dt * F[k] * im) * lam;
+                A.v[k * n + it] = v1;
+                sp[k] += S.dt * v1;
+                A.s[k * n + it] = sp[k];
+                vt[k] = 0.5 * (v1 + v0);
+                acc[18 + k] = m * v1;
+            }
+            double *kk = acc + 6 + 6 * pop;
+            kk[0] = m * vt[0] * vt[0];
+            kk[1] = m * vt[1] * vt[1];
+            kk[2] = m * vt[2] * vt[2];
+            kk[3] = m * vt[0] * vt[1];
+            kk[4] = m * vt[0] * vt[2];
+            kk[5] = m * vt[1] * vt[2];
+            // fixed-point copy for the next nonbonded evaluation
+            const int g = (S.hybrid && lab == 2) ? 1 : 0;
+            const int slot = A.inv_perm[g][it];
+            if (slot >= 0) {
+                int4 p;
+                p.x = (int)to_fixed(sp[0], S.inv_box_frac[0]);
+                p.y = (int)to_fixed(sp[1], S.inv_box_frac[1]);
+                p.z = (int)to_fixed(sp[2], S.inv_box_frac[2]);
+                p.w = ty | (lab << 16);
+                A.pos4[g][slot] = p;
+            }
+        }
+    }
+    block_sum<21>(acc, red);
+    if (threadIdx.x == 0) {
+        double *p = A.part + (size_t)blockIdx.x * kMvPart;
+        for (int k = 0; k < 21; ++k) p[k] = acc[k];
+    }
+}
+
+// VIRTUAL_DEF.f90:8-50 after the atoms moved (mass and vs_invmass are per item): centre-atom copy or mass-weighted
+// centre (all three components accumulated), then the sorted fixed-point copy.
+__global__ void __launch_bounds__(kBlock) k_vs_place(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
+                                                     double *__restrict__ s, int nitems_total, const int *__restrict__ label,
+                                                     const int *__restrict__ itype, const int *__restrict__ vs_center,
+                                                     const int *__restrict__ vs_moff, const int *__restrict__ vs_members,
+                                                     const double *__restrict__ mass, const double *__restrict__ vs_invmass,
+                                                     int4 *__restrict__ pos4, const int *__restrict__ inv_perm) {
+    const DevSys &S = sys[wg_sys[blockIdx.x]];
+    const int it = blockIdx.x * kBlock + threadIdx.x;
+    const int n = nitems_total;
+    if (itype[it] < 0 || label[it] != 3) return;
+    double p[3];
+    const int c = vs_center[it];
+    if (c >= 0) {
+        for (int k = 0; k < 3; ++k) p[k] = s[k * n + c];
+    } else {
+        double sum[3] = {0, 0, 0};
+        for (int q = vs_moff[it]; q < vs_moff[it + 1]; ++q) {
+            const int a = vs_members[q];
+            const double m = mass[a];
+            for (int k = 0; k < 3; ++k) sum[k] += m * s[k * n + a];
+        }
+        for (int k = 0; k < 3; ++k) p[k] = sum[k] * vs_invmass[it];
+    }
+    for (int k = 0; k < 3; ++k) s[k * n + it] = p[k];
+    const int slot = inv_perm[it];
+    if (slot >= 0)
+        pos4[slot] = make_int4((int)to_fixed(p[0], S.inv_box_frac[0]), (int)to_fixed(p[1], S.inv_box_frac[1]),
+                               (int)to_fixed(p[2], S.inv_box_frac[2]), itype[it] | (3 << 16));
+}
+
+// Per-system reduction of the workgroup partials into the observables
+// (AVERAGE.f90:10-47 window sums included).  One workgroup per system.
+__global__ void __launch_bounds__(kBlock) k_finalize(const DevSys *__restrict__ sys, const double *__restrict__ part_nb0,
+                                                     const double *__restrict__ part_nb1, const double *__restrict__ part_bd,
+                                                     const double *__restrict__ part_mv, double *__restrict__ obs,
+                                                     double *__restrict__ avg, double *__restrict__ sysstate,
+                                                     const int *__restrict__ step, int mode) {
+    __shared__ double red[40 * 4];
+    const int s_id = blockIdx.x;
+    const DevSys &S = sys[s_id];
+    double acc[40];
+    for (int k = 0; k < 40; ++k) acc[k] = 0.0;
+    for (int g = 0; g < S.ngroup; ++g) {
+        const double *pp = g == 0 ? part_nb0 : part_nb1;
+        for (int w = S.g[g].wg_off + threadIdx.x; w < S.g[g].wg_off + S.g[g].nwg; w += kBlock)
+            for (int k = 0; k < 9; ++k) acc[k] += pp[(size_t)w * kNbPart + k];
+    }
+    for (int w = S.item_wg_off + threadIdx.x; w < S.item_wg_off + S.item_nwg; w += kBlock) {
+        for (int k = 0; k < 13; ++k) acc[9 + k] += part_bd[(size_t)w * kBdPart + k];
+        for (int k = 0; k < 6; ++k) acc[22 + k] += part_bd[(size_t)w * kBdPart + 13 + k];
+    }
+    // integrate partials: (unused 6), kinetic tensor of each population (6+6), momentum (3)
+    double mom[3] = {0, 0, 0};
+    for (int w = S.item_wg_off + threadIdx.x; w < S.item_wg_off + S.item_nwg; w += kBlock) {
+        for (int k = 0; k < 18; ++k) acc[22 + k] += part_mv[(size_t)w * kMvPart + k];
+        for (int k = 0; k < 3; ++k) mom[k] += part_mv[(size_t)w * kMvPart + 18 + k];
+    }
+    block_sum<40>(acc, red);
+    block_sum<3>(mom, red);
+    if (threadIdx.x != 0) return;
+    double *o = obs + (size_t)s_id * kObs;
+    double *st = sysstate + (size_t)s_id * 8;
+    for (int k = 0; k < 3; ++k) o[k] = acc[k];                 // V_NB
+    for (int k = 0; k < 12; ++k) o[3 + k] = acc[9 + k];        // V_BOND, V_ANGLE, V_TORSION, V_OOP
+    double pt[6];
+    for (int k = 0; k < 6; ++k) pt[k] = acc[3 + k] + acc[22 + k];
+    o[27] = acc[21];
+    if (mode == 1) {
+        for (int k = 0; k < 6; ++k) o[15 + k] = pt[k];
+        return;
+    }
+    double ek0 = 0.5 * (acc[28] + acc[29] + acc[30]);
+    double ek1 = 0.5 * (acc[34] + acc[35] + acc[36]);
+    for (int k = 0; k < 6; ++k) o[15 + k] = (pt[k] + acc[28 + k] + acc[34 + k]) * (1.0 / S.volume);
+    o[21] = ek0;
+    o[22] = ek1;
+    o[23] = (ek0 + ek1) * S.mktemp;
+    o[24] = ek0 * S.mktemp_atom;
+    o[25] = ek1 * S.mktemp_bead;
+    o[26] = S.volume;
+    st[0] = o[24];
+    st[1] = o[25];
+    const int cur = *step;
+    if (cur % S.halt_drift == 0) {
+        for (int k = 0; k < 3; ++k) st[2 + k] = mom[k] / (double)S.natoms;
+    } else {
+        st[2] = st[3] = st[4] = 0.0;
+    }
+    double *a = avg + (size_t)s_id * kObs;
+    for (int k = 0; k < 26; ++k) a[k] += o[k];
+    a[27] += o[27];
+    a[28] += 1.0;
+}
+
+// Radial distribution function histogram over the current list (group 0).
+__global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
+                                                const int *__restrict__ perm, const int4 *__restrict__ pos4,
+                                                const int *__restrict__ list, const int *__restrict__ nneb, int cap,
+                                                int nbin, float inv_dr_u, unsigned long long *__restrict__ hist) {
+    extern __shared__ unsigned int h[];
+    const int s_id = wg_sys[blockIdx.x];
+    const DevSys &S = sys[s_id];
+    for (int b = threadIdx.x; b < nbin; b += kBlock) h[b] = 0;
+    __syncthreads();
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (perm[i] >= 0) {
+        const int4 pi = pos4[i];
+        const int nn = min(nneb[i], cap);
+        const int *lp = list + (size_t)(i >> 6) * cap * 64 + (i & 63);
+        for (int k = 0; k < nn; ++k) {
+            const int4 pj = pos4[lp[(size_t)k * 64]];
+            const float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x) * S.rho[0];
+            const float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y) * S.rho[1];
+            const float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z) * S.rho[2];
+            const int b = (int)(sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz))) * inv_dr_u);
+            if (b < nbin) atomicAdd(&h[b], 1u);
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbin; b += kBlock)
+        if (h[b]) atomicAdd(&hist[(size_t)s_id * nbin + b], (unsigned long long)h[b]);
+}
+
+// ------------------------------------------------------------------------------------------------
+// launch wrappers
+
+void launch_zero_int(hipStream_t st, int *p, int n) {
+    if (n > 0) hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+}
+
+void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
+    launch_zero_int(st, a.cell_count, a.ncell_total);
+    hipLaunchKernelGGL(k_cell_count, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.mem, a.s, a.nitems_total,
+                       a.mem_cell, a.cell_count);
+    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, a.cell_count, a.cell_start, a.cell_fill, a.ncell_total);
+    hipLaunchKernelGGL(k_place, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.mem, a.mem_cell, a.cell_start,
+                       a.cell_fill, a.tmp);
+    hipLaunchKernelGGL(k_sort_cells, dim3(a.ncell_total), dim3(kBlock), 0, st, a.cell_sys, a.sys, a.g, a.cell_start,
+                       a.cell_count, a.tmp, a.mem, a.perm, a.slot_cell, a.flags);
+    hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
+                       a.itype, a.label, a.r64, a.pos4, a.inv_perm);
+    hipLaunchKernelGGL(k_build_list, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.pos4, a.r64,
+                       a.slot_cell, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap, a.list, a.nneb, a.flags);
+}
+
+void launch_pack_only(hipStream_t st, const RebuildArgs &a) {
+    hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
+                       a.itype, a.label, a.r64, a.pos4, a.inv_perm);
+}
+
+template <bool M, bool L, int G>
+static void nb_go(hipStream_t st, const NbArgs &a, size_t lds) {
+    hipLaunchKernelGGL((k_nonbonded<M, L, G>), dim3(a.nwg), dim3(kBlock), lds, st, a.sys, a.wg_sys, a.perm, a.pos4,
+                       a.list, a.nneb, a.cap, a.rows, a.tabs, a.map, a.nitems_total, a.fnb, a.part);
+}
+
+template <int G>
+static void nb_dispatch(hipStream_t st, const NbArgs &a) {
+    const bool lds = a.max_rows_lds > 0;
+    const size_t bytes = lds ? (size_t)a.max_rows_lds * sizeof(float4) : 0;
+    if (a.multi) {
+        if (lds) nb_go<true, true, G>(st, a, bytes);
+        else nb_go<true, false, G>(st, a, bytes);
+    } else {
+        if (lds) nb_go<false, true, G>(st, a, bytes);
+        else nb_go<false, false, G>(st, a, bytes);
+    }
+}
+
+void launch_nonbonded(hipStream_t st, const NbArgs &a) {
+    if (a.g == 0) nb_dispatch<0>(st, a);
+    else nb_dispatch<1>(st, a);
+}
+
+void launch_bonded(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const BondedLaunch &b,
+                   const double *s, int nitems_total, float *fb, double *part, int *flags) {
+    BondedArgs A{b.bonds, b.angles, b.tors, b.tors_tab, b.oops, b.oop_tab, b.term_off, b.term_ref, b.tabs, b.rows};
+    hipLaunchKernelGGL(k_bonded, dim3(nwg), dim3(kBlock), 0, st, sys, wg_sys, A, s, nitems_total, fb, part, flags);
+}
+
+void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const MoveLaunch &m) {
+    MoveArgs A;
+    A.s = m.s; A.v = m.v; A.fnb = m.fnb; A.fb = m.fb; A.ftot = m.ftot; A.itype = m.itype; A.label = m.label;
+    A.mass = m.mass; A.invmass = m.invmass; A.vs_of = m.vs_of; A.vs_coeff = m.vs_coeff;
+    A.pos4[0] = m.pos4[0]; A.pos4[1] = m.pos4[1]; A.inv_perm[0] = m.inv_perm[0]; A.inv_perm[1] = m.inv_perm[1];
+    A.sysstate = m.sysstate; A.part = m.part; A.step = m.step; A.nitems_total = m.nitems_total; A.mode = m.mode;
+    hipLaunchKernelGGL(k_move, dim3(nwg), dim3(kBlock), 0, st, sys, wg_sys, A);
+}
+
+void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, double *s, int nitems_total,
+                     const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,
+                     const double *mass, const double *vs_invmass, int4 *pos4, const int *inv_perm) {
+    hipLaunchKernelGGL(k_vs_place, dim3(nwg), dim3(kBlock), 0, st, sys, wg_sys, s, nitems_total, label, itype, vs_center,
+                       vs_moff, vs_members, mass, vs_invmass, pos4, inv_perm);
+}
+
+void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,
+                     const double *pbd, const double *pmv, double *obs, double *avg, double *sysstate, const int *step,
+                     int mode) {
+    hipLaunchKernelGGL(k_finalize, dim3(nsys), dim3(kBlock), 0, st, sys, p0, p1, pbd, pmv, obs, avg, sysstate, step, mode);
+}
+
+void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
+                const int *list, const int *nneb, int cap, int nbin, float inv_dr_u, unsigned long long *hist) {
+    hipLaunchKernelGGL(k_rdf, dim3(nwg), dim3(kBlock), (size_t)nbin * sizeof(unsigned int), st, sys, wg_sys, perm, pos4,
+                       list, nneb, cap, nbin, inv_dr_u, hist);
+}
+
+}  // namespace ibx

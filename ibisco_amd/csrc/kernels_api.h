@@ -1,0 +1,82 @@
+// Launch wrappers of kernels.hip (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "ibx_kernels.h"
+
+namespace ibx {
+
+struct RebuildArgs {
+    const DevSys *sys;
+    const int *wg_sys;
+    const int *cell_sys;
+    int g, nwg, ncell_total, nitems_total, cap;
+    const int *mem;
+    const double *s;
+    const int *itype, *label;
+    int *mem_cell, *cell_count, *cell_start, *cell_fill, *tmp, *perm, *slot_cell;
+    double4 *r64;
+    int4 *pos4;
+    int *inv_perm;
+    const int *excl_off, *excl;
+    int *list, *nneb, *flags;
+};
+
+struct NbArgs {
+    const DevSys *sys;
+    const int *wg_sys;
+    int g, nwg, cap, nitems_total, max_rows_lds;
+    bool multi;
+    const int *perm;
+    const int4 *pos4;
+    const int *list, *nneb;
+    const float4 *rows;
+    const NbTab *tabs;
+    const int *map;
+    float *fnb;
+    double *part;
+};
+
+struct BondedLaunch {
+    const int4 *bonds, *angles, *tors;
+    const int *tors_tab;
+    const int4 *oops;
+    const int *oop_tab;
+    const int *term_off, *term_ref;
+    const BdTab *tabs;
+    const float4 *rows;
+};
+
+struct MoveLaunch {
+    double *s, *v;
+    const float *fnb, *fb;
+    float *ftot;
+    const int *itype, *label;
+    const double *mass, *invmass;
+    const int *vs_of;
+    const double *vs_coeff;
+    int4 *pos4[2];
+    const int *inv_perm[2];
+    const double *sysstate;
+    double *part;
+    int *step;
+    int nitems_total, mode;
+};
+
+void launch_zero_int(hipStream_t st, int *p, int n);
+void launch_rebuild(hipStream_t st, const RebuildArgs &a);
+void launch_pack_only(hipStream_t st, const RebuildArgs &a);
+void launch_nonbonded(hipStream_t st, const NbArgs &a);
+void launch_bonded(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const BondedLaunch &b,
+                   const double *s, int nitems_total, float *fb, double *part, int *flags);
+void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const MoveLaunch &m);
+void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, double *s, int nitems_total,
+                     const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,
+                     const double *mass, const double *vs_invmass, int4 *pos4, const int *inv_perm);
+void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,
+                     const double *pbd, const double *pmv, double *obs, double *avg, double *sysstate, const int *step,
+                     int mode);
+void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
+                const int *list, const int *nneb, int cap, int nbin, float inv_dr_u, unsigned long long *hist);
+
+}  // namespace ibx

@@ -1,0 +1,256 @@
+"""Python host binding of libibx.so (ctypes over the C ABI of include/ibx.h).
+
+There is no CPU fallback: constructing an :class:`Engine` without the built
+library or without a gfx950 device raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .arrays import flatten
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libibx.so")
+
+_ip = C.POINTER(C.c_int)
+_dp = C.POINTER(C.c_double)
+
+
+class IbxTables(C.Structure):
+    _fields_ = [("ntab", C.c_int), ("types", _ip), ("off", _ip), ("nrow", _ip), ("x", _dp), ("v", _dp)]
+
+
+class IbxSystemDesc(C.Structure):
+    _fields_ = [
+        ("natoms", C.c_int), ("nvs", C.c_int), ("ntype", C.c_int),
+        ("mass", _dp), ("label", _ip), ("itype", _ip), ("nbonds", _ip), ("jbond", _ip),
+        ("s", _dp), ("v", _dp), ("box", C.c_double * 3),
+        ("bond", IbxTables), ("angle", IbxTables), ("tors", IbxTables), ("nb", IbxTables), ("oop", IbxTables),
+        ("vs_type", _ip), ("vs_center", _ip), ("vs_off", _ip), ("vs_atoms", _ip),
+        ("nvirtbonds", C.c_int), ("virtbonds", _ip), ("nvirtangles", C.c_int), ("virtangles", _ip),
+        ("ensemble", C.c_int), ("dt", C.c_double), ("temp_in", C.c_double), ("fac", C.c_double),
+        ("rcut_atom", C.c_double), ("rlist_atom", C.c_double), ("rcut_bead", C.c_double),
+        ("rlist_bead", C.c_double), ("nonbexc_atom", C.c_int), ("nonbexc_bead", C.c_int),
+        ("nupdate", C.c_int), ("halt_drift", C.c_int), ("nmts", C.c_int), ("mts_quirk", C.c_int),
+    ]
+
+
+EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_step", "ibx_step_graph",
+           "ibx_compute_forces", "ibx_rebuild_lists", "ibx_get_observables", "ibx_get_averages",
+           "ibx_get_forces", "ibx_get_state", "ibx_set_state", "ibx_set_nb_table", "ibx_set_thermostat",
+           "ibx_get_pairs", "ibx_rdf_enable", "ibx_rdf_get", "ibx_get_timing", "ibx_set_timing",
+           "ibx_get_info")
+
+_lib = None
+
+
+def load_library(path=LIB_PATH):
+    """Loads libibx.so and declares its signatures (no device work)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `python -m ibisco_amd.build`")
+    L = C.CDLL(path)
+    vp = C.c_void_p
+    L.ibx_create.argtypes = [C.c_int, C.POINTER(C.POINTER(IbxSystemDesc)), C.POINTER(vp)]
+    L.ibx_load_dir.argtypes = [C.c_char_p, C.POINTER(vp)]
+    L.ibx_destroy.argtypes = [vp]
+    L.ibx_destroy.restype = None
+    L.ibx_last_error.argtypes = [vp]
+    L.ibx_last_error.restype = C.c_char_p
+    for nm in ("ibx_step", "ibx_step_graph"):
+        getattr(L, nm).argtypes = [vp, C.c_int, C.c_int]
+    L.ibx_compute_forces.argtypes = [vp, C.c_int]
+    L.ibx_rebuild_lists.argtypes = [vp]
+    L.ibx_get_observables.argtypes = [vp, C.c_int, _dp]
+    L.ibx_get_averages.argtypes = [vp, C.c_int, _dp, C.c_int]
+    L.ibx_get_forces.argtypes = [vp, C.c_int, _dp]
+    L.ibx_get_state.argtypes = [vp, C.c_int, _dp, _dp]
+    L.ibx_set_state.argtypes = [vp, C.c_int, _dp, _dp]
+    L.ibx_set_nb_table.argtypes = [vp, C.c_int, C.c_int, C.c_int, _dp, _dp]
+    L.ibx_set_thermostat.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double]
+    L.ibx_get_pairs.argtypes = [vp, C.c_int, C.c_int, _ip, C.c_int64, C.POINTER(C.c_int64)]
+    L.ibx_rdf_enable.argtypes = [vp, C.c_int, C.c_double, C.c_int]
+    L.ibx_rdf_get.argtypes = [vp, C.c_int, _dp, C.c_int]
+    L.ibx_get_timing.argtypes = [vp, _dp]
+    L.ibx_set_timing.argtypes = [vp, C.c_int]
+    L.ibx_get_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+    _lib = L
+    return L
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+class IbxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"ibx error {code}: {msg}")
+        self.code = code
+
+
+def make_desc(sysm, keep, mts_quirk=False):
+    """ibx_system_desc for one System; arrays it points to are appended to ``keep``."""
+    a = flatten(sysm)
+    d = IbxSystemDesc()
+    d.natoms, d.nvs, d.ntype = a["natoms"], a["nvs"], a["ntype"]
+
+    def arr(x, t):
+        x = np.ascontiguousarray(x)
+        keep.append(x)
+        return _p(x, t)
+
+    d.mass, d.label = arr(a["mass"], C.c_double), arr(a["label"], C.c_int)
+    d.itype, d.nbonds, d.jbond = arr(a["itype"], C.c_int), arr(a["nbonds"], C.c_int), arr(a["jbond"], C.c_int)
+    d.s, d.v = arr(a["s"], C.c_double), arr(a["v"], C.c_double)
+    for k in range(3):
+        d.box[k] = a["box"][k]
+    for kind in ("bond", "angle", "tors", "nb", "oop"):
+        t = a["tables"][kind]
+        setattr(d, kind, IbxTables(t["ntab"], arr(t["types"], C.c_int), arr(t["off"], C.c_int),
+                                   arr(t["nrow"], C.c_int), arr(t["x"], C.c_double), arr(t["v"], C.c_double)))
+    d.vs_type, d.vs_center = arr(a["vs_type"], C.c_int), arr(a["vs_center"], C.c_int)
+    d.vs_off, d.vs_atoms = arr(a["vs_off"], C.c_int), arr(a["vs_atoms"], C.c_int)
+    d.nvirtbonds, d.virtbonds = a["nvirtbonds"], arr(a["virtbonds"], C.c_int)
+    d.nvirtangles, d.virtangles = a["nvirtangles"], arr(a["virtangles"], C.c_int)
+    d.ensemble, d.dt, d.temp_in, d.fac = a["ensemble"], a["dt"], a["temp_in"], a["fac"]
+    d.rcut_atom, d.rlist_atom = a["rcut_atom"], a["rlist_atom"]
+    d.rcut_bead, d.rlist_bead = a["rcut_bead"], a["rlist_bead"]
+    d.nonbexc_atom, d.nonbexc_bead = a["nonbexc_atom"], a["nonbexc_bead"]
+    d.nupdate, d.halt_drift, d.nmts, d.mts_quirk = a["nupdate"], a["halt_drift"], a["nmts"], int(mts_quirk)
+    keep.append(d)
+    return d, a
+
+
+OBS_SLICES = dict(vnb=(0, 3), vbond=(3, 6), vangle=(6, 9), vtors=(9, 12), voop=(12, 15), pt=(15, 21), ek=(21, 23))
+
+
+def obs_dict(o):
+    d = {k: o[a:b].copy() for k, (a, b) in OBS_SLICES.items()}
+    d.update(temp=o[23], temp_atom=o[24], temp_bead=o[25], volume=o[26], overext=int(o[27]), nsamples=int(o[28]))
+    return d
+
+
+class Engine:
+    """One context: a single system or a batch of independent replicas on the GPU."""
+
+    def __init__(self, systems, mts_quirk=False, load_dir=None):
+        self.L = load_library()
+        self.h = C.c_void_p()
+        if load_dir is not None:
+            rc = self.L.ibx_load_dir(load_dir.encode(), C.byref(self.h))
+            self.nsys = 1
+            self.natoms = None
+        else:
+            if not isinstance(systems, (list, tuple)):
+                systems = [systems]
+            self._keep = []
+            descs = [make_desc(s, self._keep, mts_quirk) for s in systems]
+            self.arrays = [a for _, a in descs]
+            ptrs = (C.POINTER(IbxSystemDesc) * len(descs))(*[C.pointer(d) for d, _ in descs])
+            rc = self.L.ibx_create(len(descs), ptrs, C.byref(self.h))
+            self.nsys = len(descs)
+            self.natoms = [a["natoms"] for a in self.arrays]
+            self.nitems = [a["natoms"] + a["nvs"] for a in self.arrays]
+        if rc != 0:
+            raise IbxError(rc, self.L.ibx_last_error(None).decode())
+        if self.natoms is None:
+            info = self.info(0)
+            self.natoms = [int(info[0])]
+            self.nitems = [int(info[0] + info[1])]
+        self.step_count = 0
+
+    def close(self):
+        if getattr(self, "h", None) and self.h.value:
+            self.L.ibx_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _chk(self, rc):
+        if rc != 0:
+            raise IbxError(rc, self.L.ibx_last_error(self.h).decode())
+
+    def step(self, nsteps, graph=False):
+        fn = self.L.ibx_step_graph if graph else self.L.ibx_step
+        self._chk(fn(self.h, nsteps, self.step_count))
+        self.step_count += nsteps
+
+    def compute_forces(self, rebuild=True):
+        self._chk(self.L.ibx_compute_forces(self.h, int(rebuild)))
+
+    def rebuild_lists(self):
+        self._chk(self.L.ibx_rebuild_lists(self.h))
+
+    def obs(self, sys=0):
+        o = np.zeros(32)
+        self._chk(self.L.ibx_get_observables(self.h, sys, _p(o, C.c_double)))
+        return obs_dict(o)
+
+    def averages(self, sys=0, reset=False):
+        o = np.zeros(32)
+        self._chk(self.L.ibx_get_averages(self.h, sys, _p(o, C.c_double), int(reset)))
+        return o
+
+    def forces(self, sys=0):
+        f = np.zeros((self.nitems[sys], 3))
+        self._chk(self.L.ibx_get_forces(self.h, sys, _p(f, C.c_double)))
+        return f
+
+    def state(self, sys=0):
+        s = np.zeros((self.nitems[sys], 3))
+        v = np.zeros((self.natoms[sys], 3))
+        self._chk(self.L.ibx_get_state(self.h, sys, _p(s, C.c_double), _p(v, C.c_double)))
+        return s, v
+
+    def set_state(self, s=None, v=None, sys=0):
+        s = None if s is None else np.ascontiguousarray(s, np.float64)
+        v = None if v is None else np.ascontiguousarray(v, np.float64)
+        self._chk(self.L.ibx_set_state(self.h, sys, None if s is None else _p(s, C.c_double),
+                                       None if v is None else _p(v, C.c_double)))
+
+    def set_nb_table(self, x, v_reduced, sys=0, t=0):
+        x = np.ascontiguousarray(x, np.float64)
+        v = np.ascontiguousarray(v_reduced, np.float64)
+        self._chk(self.L.ibx_set_nb_table(self.h, sys, t, len(x), _p(x, C.c_double), _p(v, C.c_double)))
+
+    def set_thermostat(self, ensemble, temp_in, fac, dt):
+        self._chk(self.L.ibx_set_thermostat(self.h, ensemble, temp_in, fac, dt))
+
+    def pairs(self, sys=0, group=0):
+        n = C.c_int64()
+        self._chk(self.L.ibx_get_pairs(self.h, sys, group, None, 0, C.byref(n)))
+        out = np.zeros((max(n.value, 1), 2), np.int32)
+        self._chk(self.L.ibx_get_pairs(self.h, sys, group, _p(out, C.c_int), n.value, C.byref(n)))
+        return out[:n.value]
+
+    def rdf_enable(self, nbin, dr, every=1):
+        self._chk(self.L.ibx_rdf_enable(self.h, nbin, dr, every))
+        self._rdf_nbin = nbin
+
+    def rdf(self, sys=0, reset=False):
+        g = np.zeros(self._rdf_nbin)
+        self._chk(self.L.ibx_rdf_get(self.h, sys, _p(g, C.c_double), int(reset)))
+        return g
+
+    def set_timing(self, on=True):
+        self._chk(self.L.ibx_set_timing(self.h, int(on)))
+
+    def timing(self):
+        t = np.zeros(8)
+        self._chk(self.L.ibx_get_timing(self.h, _p(t, C.c_double)))
+        return dict(nonbonded=t[0], bonded=t[1], integrate=t[2], rebuild=t[3], step=t[4], nsteps_timed=t[6],
+                    rebuilds=t[7])
+
+    def info(self, sys=0):
+        i = (C.c_int64 * 8)()
+        self._chk(self.L.ibx_get_info(self.h, sys, i))
+        return list(i)

@@ -200,6 +200,79 @@ def _grow_chains(rng, nchains, nbeads, box, b, theta0_deg, spread_deg=8.0):
     return pos.reshape(-1, 3)
 
 
+def _grow_chains_avoiding(rng, nchains, nbeads, box, b, theta0_deg, dmin, spread_deg=8.0, tries=40):
+    """Random-walk chains whose beads keep dmin from every earlier bead (bonded
+    neighbours and 1-3 partners aside); a spatial hash keeps it O(N).  Meant for
+    the small systems of the tests; growth falls back to the best of its tries."""
+    L = np.asarray(box, dtype=np.float64)
+    nc = np.maximum(1, np.floor(L / dmin).astype(int))
+    cs = L / nc
+    grid = {}
+    out = np.empty((nchains * nbeads, 3))
+
+    def key(p):
+        return tuple((np.floor((p / L + 0.5) % 1.0 * nc).astype(int)) % nc)
+
+    def min_dist(p, skip):
+        k = key(p)
+        best = 1e9
+        for dx in (-1, 0, 1):
+            for dy in (-1, 0, 1):
+                for dz in (-1, 0, 1):
+                    cell = ((k[0] + dx) % nc[0], (k[1] + dy) % nc[1], (k[2] + dz) % nc[2])
+                    for q in grid.get(cell, ()):
+                        if q in skip:
+                            continue
+                        d = out[q] - p
+                        d -= np.round(d / L) * L
+                        best = min(best, float(np.sqrt((d * d).sum())))
+        return best
+
+    def add(idx):
+        grid.setdefault(key(out[idx]), []).append(idx)
+
+    n = 0
+    for _c in range(nchains):
+        first = n
+        best_p, best_d = None, -1.0
+        for _t in range(tries):
+            p = rng.uniform(-0.5, 0.5, 3) * L
+            dd = min_dist(p, ())
+            if dd > best_d:
+                best_p, best_d = p, dd
+            if dd >= dmin:
+                break
+        out[n] = best_p
+        add(n)
+        n += 1
+        u = rng.normal(size=3)
+        u /= np.linalg.norm(u)
+        for k in range(1, nbeads):
+            best_p, best_d, best_u = None, -1.0, None
+            for _t in range(tries):
+                if k == 1:
+                    un = rng.normal(size=3)
+                else:
+                    turn = np.deg2rad(180.0 - theta0_deg + rng.uniform(-spread_deg, spread_deg))
+                    a = rng.normal(size=3)
+                    a -= (a @ u) * u
+                    a /= np.linalg.norm(a)
+                    un = np.cos(turn) * u + np.sin(turn) * a
+                un = un / np.linalg.norm(un)
+                p = out[n - 1] + b * un
+                dd = min_dist(p, {n - 1, n - 2} if k > 1 else {n - 1})
+                if dd > best_d:
+                    best_p, best_d, best_u = p, dd, un
+                if dd >= dmin:
+                    break
+            out[n] = best_p
+            u = best_u
+            add(n)
+            n += 1
+        del first
+    return out
+
+
 def maxwell_velocities(rng, mass_kgmol, temp_k):
     """Per-component Gaussian velocities (nm/ps) at temp_k with zero total momentum."""
     m = np.asarray(mass_kgmol, dtype=np.float64)
@@ -212,10 +285,12 @@ def maxwell_velocities(rng, mass_kgmol, temp_k):
 def polymer_melt(nchains, nbeads, density, mass_kgmol, temp_k, seed, *, sigma, eps, rc, dr_nb,
                  bond_r0, bond_k, angle_theta0, angle_k, bond_rmax=None, bond_dr=None,
                  torsion=False, label="B", nb_table=None, dt=0.005, rlist=None, nupdate=10,
-                 nstep=1000, non_bonded=4, name="melt"):
+                 nstep=1000, non_bonded=4, name="melt", avoid=None):
     """Linear homopolymer melt of one particle type in a periodic cube.
 
-    ``density`` is in particles per nm^3.  Bonds and angles are harmonic
+    ``density`` is in particles per nm^3.  ``avoid`` (a fraction of sigma) grows
+    the chains with a minimum distance between non-bonded beads; without it the
+    walks overlap freely.  Bonds and angles are harmonic
     (V = k/2 (x-x0)^2, k in kJ/mol/nm^2 and kJ/mol/rad^2), the nonbonded
     table is shifted-force LJ unless ``nb_table`` (r, V) is given.
     """
@@ -223,7 +298,10 @@ def polymer_melt(nchains, nbeads, density, mass_kgmol, temp_k, seed, *, sigma, e
     n = nchains * nbeads
     L = (n / density) ** (1.0 / 3.0)
     box = np.array([L, L, L])
-    s = _grow_chains(rng, nchains, nbeads, box, bond_r0, angle_theta0)
+    if avoid:
+        s = _grow_chains_avoiding(rng, nchains, nbeads, box, bond_r0, angle_theta0, avoid * sigma)
+    else:
+        s = _grow_chains(rng, nchains, nbeads, box, bond_r0, angle_theta0)
     itype = np.zeros(n, dtype=np.int32)
     v = maxwell_velocities(rng, np.full(n, mass_kgmol), temp_k)
     jbond = []

@@ -30,6 +30,10 @@ def kjmol_to_reduced(e):
     return e * 1000.0 / NA / ESCALE
 
 
+def reduced_to_kjmol(e):
+    return e * NA * ESCALE / 1000.0
+
+
 def mass_to_reduced(m_kgmol):
     return m_kgmol / NA / MASSSCALE
 

@@ -1,0 +1,117 @@
+"""HIP path against the FP64 oracle on identical inputs (needs a gfx950 device).
+
+Tolerances (north-star gates): pair sets and exclusions bit-exact; per-item
+forces within 1e-5 relative RMS; energies and virial within 1e-7 relative to
+the magnitude of their totals.
+"""
+import numpy as np
+import pytest
+
+from ibisco_amd import system
+from ibisco_amd.engine import Engine
+
+pytestmark = pytest.mark.gpu
+
+FORCE_RTOL = 1e-5
+ENERGY_RTOL = 1e-7
+
+
+def _sys_cg(n_chains=40, seed=3, **kw):
+    return system.cg_melt(nchains=n_chains, nbeads=50, seed=seed, avoid=0.8, **kw)
+
+
+def _sys_ua(seed=5):
+    return system.ua_melt(nchains=20, nbeads=100, seed=seed, avoid=0.6)
+
+
+CASES = {
+    "cg2000": lambda: _sys_cg(),
+    "ua2000_torsion": lambda: _sys_ua(),
+    "cg600_two_cells": lambda: system.cg_melt(nchains=12, nbeads=50, seed=7, avoid=0.8),
+}
+
+
+def _rel_rms(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _check_energy(name, g, o, scale):
+    err = np.abs(np.asarray(g) - np.asarray(o))
+    assert np.all(err <= ENERGY_RTOL * scale), f"{name}: gpu={g} oracle={o} scale={scale}"
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_pairs_bit_exact(case, oracle_mod):
+    s = CASES[case]()
+    o = oracle_mod.Oracle(s)
+    e = Engine(s)
+    po, pg = o.pairs(0), e.pairs(0, 0)
+    print(case, "pairs", len(po), len(pg))
+    assert len(po) == len(pg)
+    assert np.array_equal(po, pg)
+
+
+@pytest.mark.parametrize("case", sorted(CASES))
+def test_forces_energy_virial(case, oracle_mod):
+    s = CASES[case]()
+    o = oracle_mod.Oracle(s)
+    o.new_force()
+    e = Engine(s)
+    e.compute_forces(rebuild=True)
+    fo, fg = o.forces(), e.forces()
+    oo, og = o.obs(), e.obs()
+    rr = _rel_rms(fg, fo)
+    print(case, "force rel rms", rr)
+    for k in ("vnb", "vbond", "vangle", "vtors", "pt"):
+        print(case, k, og[k], oo[k])
+    assert rr <= FORCE_RTOL
+    epot = sum(float(np.abs(oo[k]).sum()) for k in ("vnb", "vbond", "vangle", "vtors", "voop"))
+    for k in ("vnb", "vbond", "vangle", "vtors", "voop"):
+        _check_energy(k, og[k], oo[k], max(float(np.abs(oo[k]).sum()), 1e-3 * epot))
+    _check_energy("virial", og["pt"], oo["pt"], float(np.abs(oo["pt"][:3]).sum()))
+
+
+def test_trajectory_short_nve(oracle_mod):
+    s = _sys_cg(dt=0.001)
+    o = oracle_mod.Oracle(s)
+    e = Engine(s)
+    o.run(20)
+    e.step(20)
+    so, vo, _ = o.state()
+    sg, vg = e.state()
+    ob, eb = o.obs(), e.obs()
+    print("pos max abs", np.abs(sg - so).max(), "vel rel", _rel_rms(vg, vo), "ek", eb["ek"], ob["ek"])
+    assert np.abs(sg - so).max() < 1e-6
+    assert _rel_rms(vg, vo) < 1e-5
+    assert abs(eb["ek"][0] - ob["ek"][0]) <= 1e-6 * abs(ob["ek"][0])
+
+
+def test_batch_equals_alone():
+    systems = [_sys_cg(n_chains=24, seed=11), _sys_cg(n_chains=40, seed=12), _sys_cg(n_chains=31, seed=13)]
+    for k, sy in enumerate(systems):
+        r, v = system.corrected_cg_table(k)
+        sy.nb_tables[0] = system.Table((0, 0), r, v, "nb1")
+        sy.control.dt = 0.002
+    batch = Engine(systems)
+    batch.step(25)
+    for k, sy in enumerate(systems):
+        alone = Engine(sy)
+        alone.step(25)
+        sa, va = alone.state()
+        sb, vb = batch.state(k)
+        assert np.array_equal(sa, sb), k
+        assert np.array_equal(va, vb), k
+        oa, ob = alone.obs(), batch.obs(k)
+        assert np.array_equal(oa["vnb"], ob["vnb"]) and np.array_equal(oa["pt"], ob["pt"]), k
+
+
+def test_graph_equals_eager():
+    s = _sys_cg(dt=0.002)
+    a, b = Engine(s), Engine(s)
+    a.step(35)
+    b.step(35, graph=True)
+    sa, va = a.state()
+    sb, vb = b.state()
+    assert np.array_equal(sa, sb)
+    assert np.array_equal(va, vb)
+    assert np.array_equal(a.obs()["pt"], b.obs()["pt"])

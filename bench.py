@@ -87,7 +87,7 @@ def main():
     n = sysm.natoms
     print(f"[bench] {n} sites, box {sysm.box[0]:.3f} nm", file=sys.stderr, flush=True)
     e = Engine(sysm)
-    workloads.equilibrate(e, sysm)
+    workloads.equilibrate(e, sysm, stage_steps=200, final_steps=1000)
     print("[bench] equilibrated", file=sys.stderr, flush=True)
     e.step(args.warmup, graph=True)
     e.obs()   # synchronise

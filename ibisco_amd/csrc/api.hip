@@ -70,11 +70,13 @@ struct ibx_ctx {
     hipStream_t st = nullptr;
     GroupDev gr[2];
     DBuf<DevSys> sys;
-    DBuf<int> wg_items, itype, label, excl_off, excl, term_off, term_ref, vs_of, vs_center, vs_moff, vs_members;
-    DBuf<int> tors_tab, oop_tab, nb_map, flags, step;
-    DBuf<int4> bonds, angles, tors, oops;
+    DBuf<int> wg_items, itype, label, excl_off, excl, vs_of, vs_center, vs_moff, vs_members;
+    DBuf<int> term_meta, term_slot, inc_off, inc_slot, wg_terms, nb_map, flags, step;
+    DBuf<int4> term_mem;
+    DBuf<float> contrib;
+    int nterms = 0, term_nwg = 0, ncontrib = 0;
     DBuf<double> s, v, mass, invmass, vs_coeff, vs_invmass, part_bd, part_mv, obs, avg, sysstate;
-    DBuf<float> fnb, fb, ftot;
+    DBuf<float> fnb, ftot;
     DBuf<float4> nb_rows, bd_rows;
     DBuf<NbTab> nb_tabs;
     DBuf<BdTab> bd_tabs;
@@ -160,13 +162,14 @@ int build_device(ibx_ctx *c) {
     std::vector<int> hit(N, -1), hlab(N, 0), hvs(N, -1), hvcen(N, -1), hwg(c->item_nwg, 0);
     std::vector<int> hex_off(N + 1, 0), hex;
     std::vector<int> hvm_off(N + 1, 0), hvm;
-    std::vector<int4> hb, ha, ht, ho;
-    std::vector<int> htt, hot;
+    std::vector<int4> htm;            // term members
+    std::vector<int> hmeta, hslot, hwgt;
     std::vector<BdTab> hbt;
+    int nslot = 0;
     std::vector<float4> hbrows, hnrows;
     std::vector<NbTab> hntab;
     std::vector<int> hnmap;
-    // incident terms: (item, ref) pairs, ordered per item
+    // contribution slots of the bonded terms each item takes part in, in term order
     std::vector<std::vector<int>> inc(N);
 
     for (int s = 0; s < S; ++s) {
@@ -215,29 +218,29 @@ int build_device(ibx_ctx *c) {
         };
         int bbase = add_tabs(h.bt, h.bond_lab), abase = add_tabs(h.at, h.angle_lab);
         int tbase = add_tabs(h.tt, h.tors_lab), obase = add_tabs(h.ot, h.oop_lab);
-        auto ref = [](int kind, int role, int idx) { return (kind << 29) | (role << 27) | idx; };
-        for (size_t q = 0; q < h.bonds.size(); ++q) {
-            int idx = (int)hb.size();
-            hb.push_back(make_int4(o + h.bonds[q][0], o + h.bonds[q][1], bbase + h.bond_tab[q], 0));
-            for (int r = 0; r < 2; ++r) inc[o + h.bonds[q][r]].push_back(ref(KBOND, r, idx));
+        // terms of this system, padded to whole workgroups (one system per workgroup)
+        const int term0 = (int)hmeta.size();
+        auto add_term = [&](int kind, int tab, const std::array<int, 4> &m, int nm) {
+            int4 g4 = make_int4(o + m[0], o + m[1], nm > 2 ? o + m[2] : -1, nm > 3 ? o + m[3] : -1);
+            htm.push_back(g4);
+            hmeta.push_back(kind | (tab << 2));
+            hslot.push_back(nslot);
+            for (int r = 0; r < nm; ++r) inc[o + m[r]].push_back(nslot + r);
+            nslot += nm;
+        };
+        for (size_t q = 0; q < h.bonds.size(); ++q) add_term(KBOND, bbase + h.bond_tab[q], h.bonds[q], 2);
+        for (size_t q = 0; q < h.angles.size(); ++q) add_term(KANGLE, abase + h.angle_tab[q], h.angles[q], 3);
+        for (size_t q = 0; q < h.tors.size(); ++q) add_term(KTORS, tbase + h.tors_tab[q], h.tors[q], 4);
+        for (size_t q = 0; q < h.oops.size(); ++q) add_term(KOOP, obase + h.oop_tab[q], h.oops[q], 4);
+        int nt_pad = pad_to(std::max(1, (int)hmeta.size() - term0), kBlock);
+        while ((int)hmeta.size() < term0 + nt_pad) {
+            htm.push_back(make_int4(-1, -1, -1, -1));
+            hmeta.push_back(-1);
+            hslot.push_back(0);
         }
-        for (size_t q = 0; q < h.angles.size(); ++q) {
-            int idx = (int)ha.size();
-            ha.push_back(make_int4(o + h.angles[q][0], o + h.angles[q][1], o + h.angles[q][2], abase + h.angle_tab[q]));
-            for (int r = 0; r < 3; ++r) inc[o + h.angles[q][r]].push_back(ref(KANGLE, r, idx));
-        }
-        for (size_t q = 0; q < h.tors.size(); ++q) {
-            int idx = (int)ht.size();
-            ht.push_back(make_int4(o + h.tors[q][0], o + h.tors[q][1], o + h.tors[q][2], o + h.tors[q][3]));
-            htt.push_back(tbase + h.tors_tab[q]);
-            for (int r = 0; r < 4; ++r) inc[o + h.tors[q][r]].push_back(ref(KTORS, r, idx));
-        }
-        for (size_t q = 0; q < h.oops.size(); ++q) {
-            int idx = (int)ho.size();
-            ho.push_back(make_int4(o + h.oops[q][0], o + h.oops[q][1], o + h.oops[q][2], o + h.oops[q][3]));
-            hot.push_back(obase + h.oop_tab[q]);
-            for (int r = 0; r < 4; ++r) inc[o + h.oops[q][r]].push_back(ref(KOOP, r, idx));
-        }
+        D.term_wg_off = term0 / kBlock;
+        D.term_nwg = nt_pad / kBlock;
+        for (int w = 0; w < D.term_nwg; ++w) hwgt.push_back(s);
         // nonbonded tables: fixed-point unit of this system
         const double unit = h.box[0] / 4294967296.0;
         D.nb_row_off = (int)hnrows.size();
@@ -321,12 +324,15 @@ int build_device(ibx_ctx *c) {
         }
         hvm_off[N] = (int)hvm.size();
     }
-    std::vector<int> hterm_off(N + 1, 0), hterm;
+    std::vector<int> hinc_off(N + 1, 0), hinc;
     for (int i = 0; i < N; ++i) {
-        hterm_off[i] = (int)hterm.size();
-        hterm.insert(hterm.end(), inc[i].begin(), inc[i].end());
+        hinc_off[i] = (int)hinc.size();
+        hinc.insert(hinc.end(), inc[i].begin(), inc[i].end());
     }
-    hterm_off[N] = (int)hterm.size();
+    hinc_off[N] = (int)hinc.size();
+    c->nterms = (int)hmeta.size();
+    c->term_nwg = c->nterms / kBlock;
+    c->ncontrib = std::max(1, nslot);
 
     int rc;
     if ((rc = upload(c, c->s, hs_s)) || (rc = upload(c, c->v, hv)) || (rc = upload(c, c->mass, hm)) ||
@@ -334,20 +340,20 @@ int build_device(ibx_ctx *c) {
         (rc = upload(c, c->wg_items, hwg)) || (rc = upload(c, c->excl_off, hex_off)) || (rc = upload(c, c->excl, hex)) ||
         (rc = upload(c, c->vs_of, hvs)) || (rc = upload(c, c->vs_coeff, hvc)) || (rc = upload(c, c->vs_center, hvcen)) ||
         (rc = upload(c, c->vs_invmass, hvim)) || (rc = upload(c, c->vs_moff, hvm_off)) ||
-        (rc = upload(c, c->vs_members, hvm)) || (rc = upload(c, c->term_off, hterm_off)) ||
-        (rc = upload(c, c->term_ref, hterm)) || (rc = upload(c, c->bonds, hb)) || (rc = upload(c, c->angles, ha)) ||
-        (rc = upload(c, c->tors, ht)) || (rc = upload(c, c->tors_tab, htt)) || (rc = upload(c, c->oops, ho)) ||
-        (rc = upload(c, c->oop_tab, hot)) || (rc = upload(c, c->bd_tabs, hbt)) || (rc = upload(c, c->bd_rows, hbrows)) ||
+        (rc = upload(c, c->vs_members, hvm)) || (rc = upload(c, c->inc_off, hinc_off)) ||
+        (rc = upload(c, c->inc_slot, hinc)) || (rc = upload(c, c->term_mem, htm)) || (rc = upload(c, c->term_meta, hmeta)) ||
+        (rc = upload(c, c->term_slot, hslot)) || (rc = upload(c, c->wg_terms, hwgt)) ||
+        (rc = upload(c, c->bd_tabs, hbt)) || (rc = upload(c, c->bd_rows, hbrows)) ||
         (rc = upload(c, c->nb_rows, hnrows)) || (rc = upload(c, c->nb_tabs, hntab)) || (rc = upload(c, c->nb_map, hnmap)))
         return rc;
     c->h_nb_tabs = hntab;
     HIPCHK(c, c->fnb.alloc(3 * (size_t)N));
-    HIPCHK(c, c->fb.alloc(3 * (size_t)N));
+    HIPCHK(c, c->contrib.alloc(3 * (size_t)c->ncontrib));
+    HIPCHK(c, hipMemset(c->contrib.p, 0, c->contrib.bytes()));
     HIPCHK(c, c->ftot.alloc(3 * (size_t)N));
     HIPCHK(c, hipMemset(c->fnb.p, 0, c->fnb.bytes()));
-    HIPCHK(c, hipMemset(c->fb.p, 0, c->fb.bytes()));
     HIPCHK(c, hipMemset(c->ftot.p, 0, c->ftot.bytes()));
-    HIPCHK(c, c->part_bd.alloc((size_t)c->item_nwg * kBdPart));
+    HIPCHK(c, c->part_bd.alloc((size_t)c->term_nwg * kBdPart));
     HIPCHK(c, c->part_mv.alloc((size_t)c->item_nwg * kMvPart));
     HIPCHK(c, hipMemset(c->part_bd.p, 0, c->part_bd.bytes()));
     HIPCHK(c, hipMemset(c->part_mv.p, 0, c->part_mv.bytes()));
@@ -437,7 +443,7 @@ int build_device(ibx_ctx *c) {
         HIPCHK(c, G.r64.alloc(slot));
         HIPCHK(c, G.pos4.alloc(slot));
         HIPCHK(c, G.nneb.alloc(slot));
-        HIPCHK(c, G.part.alloc((size_t)G.nwg * kNbPart));
+        HIPCHK(c, G.part.alloc((size_t)G.nwg * (kBlock / kNbAtomsPerBlock) * kNbPart));
         HIPCHK(c, hipMemset(G.part.p, 0, G.part.bytes()));
         HIPCHK(c, G.list.alloc((size_t)slot * G.cap));
         std::vector<int> inv(N, -1);
@@ -452,6 +458,8 @@ RebuildArgs rebuild_args(ibx_ctx *c, int g) {
     RebuildArgs a;
     a.sys = c->sys.p; a.wg_sys = G.wg_sys.p; a.cell_sys = G.cell_sys.p;
     a.g = g; a.nwg = G.nwg; a.ncell_total = G.ncell_total; a.nitems_total = c->n_items; a.cap = G.cap;
+    a.cubic = true;
+    for (const DevSys &D : c->dsys) a.cubic = a.cubic && D.rho[1] == 1.0f && D.rho[2] == 1.0f;
     a.mem = G.mem.p; a.s = c->s.p; a.itype = c->itype.p; a.label = c->label.p;
     a.mem_cell = G.mem_cell.p; a.cell_count = G.cell_count.p; a.cell_start = G.cell_start.p;
     a.cell_fill = G.cell_fill.p; a.tmp = G.tmp.p; a.perm = G.perm.p; a.slot_cell = G.slot_cell.p;
@@ -490,12 +498,13 @@ void enqueue_forces(ibx_ctx *c, int mode) {
 }
 
 void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
-    BondedLaunch b{c->bonds.p, c->angles.p, c->tors.p, c->tors_tab.p, c->oops.p, c->oop_tab.p,
-                   c->term_off.p, c->term_ref.p, c->bd_tabs.p, c->bd_rows.p};
-    launch_bonded(c->st, c->item_nwg, c->sys.p, c->wg_items.p, b, c->s.p, c->n_items, c->fb.p, c->part_bd.p, c->flags.p);
+    TermLaunch b{c->term_mem.p, c->term_meta.p, c->term_slot.p, c->bd_tabs.p, c->bd_rows.p};
+    launch_terms(c->st, c->term_nwg, c->wg_terms.p, b, c->s.p, c->n_items, c->nterms, c->contrib.p, c->ncontrib,
+                 c->part_bd.p, c->flags.p);
     if (ev) (void)hipEventRecord((*ev)[2], c->st);
     MoveLaunch m;
-    m.s = c->s.p; m.v = c->v.p; m.fnb = c->fnb.p; m.fb = c->fb.p; m.ftot = c->ftot.p; m.itype = c->itype.p;
+    m.s = c->s.p; m.v = c->v.p; m.fnb = c->fnb.p; m.contrib = c->contrib.p; m.inc_off = c->inc_off.p;
+    m.inc_slot = c->inc_slot.p; m.ncontrib = c->ncontrib; m.ftot = c->ftot.p; m.itype = c->itype.p;
     m.label = c->label.p; m.mass = c->mass.p; m.invmass = c->invmass.p; m.vs_of = c->vs_of.p;
     m.vs_coeff = c->vs_coeff.p;
     for (int g = 0; g < 2; ++g) {
@@ -885,9 +894,8 @@ int ibx_get_pairs(ibx_ctx *c, int sys, int group, int *out, int64_t cap, int64_t
     for (int i = 0; i < dg.nslot_pad; ++i) {
         if (perm[i] < 0) continue;
         const int I = perm[i] - o;
-        const int *lp = list.data() + (size_t)(i >> 6) * G.cap * 64 + (i & 63);
         for (int k = 0; k < std::min(nneb[i], G.cap); ++k) {
-            const int j = lp[(size_t)k * 64] - dg.slot_off;
+            const int j = list[list_pos(i, k, G.cap)] - dg.slot_off;
             const int J = perm[j] - o;
             if (I < J) pr.emplace_back(I, J);
         }
@@ -947,8 +955,8 @@ int ibx_get_timing(ibx_ctx *c, double ms[8]) {
 int ibx_get_info(ibx_ctx *c, int sys, int64_t info[8]) {
     if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
     const HostSystem &h = c->hs[sys];
-    size_t bytes = c->s.bytes() + c->v.bytes() + c->fnb.bytes() + c->fb.bytes() + c->ftot.bytes() + c->mass.bytes() +
-                   c->invmass.bytes() + c->excl.bytes() + c->term_ref.bytes();
+    size_t bytes = c->s.bytes() + c->v.bytes() + c->fnb.bytes() + c->contrib.bytes() + c->ftot.bytes() +
+                   c->mass.bytes() + c->invmass.bytes() + c->excl.bytes() + c->inc_slot.bytes() + c->term_mem.bytes();
     for (auto &G : c->gr)
         bytes += G.list.bytes() + G.pos4.bytes() + G.r64.bytes() + G.perm.bytes() + G.nneb.bytes() + G.mem.bytes();
     info[0] = h.natoms;

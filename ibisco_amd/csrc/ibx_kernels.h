@@ -21,6 +21,20 @@ constexpr int kObs = 32;
 constexpr int kMaxNbTabs = 64;     // nonbonded tables per system
 constexpr int kMaxNbTypes = 32;    // particle types per system with several nonbonded tables
 enum TermKind { KBOND = 0, KANGLE = 1, KTORS = 2, KOOP = 3 };
+constexpr int kLanes = 4;                 // lanes per atom in the nonbonded kernel
+constexpr int kAtomsPerWave = kWave / kLanes;
+constexpr int kNbAtomsPerBlock = kBlock / kLanes;
+constexpr int kListBlock = 128;           // threads per workgroup of the list build (one home cell)
+constexpr int kCandLds = 1024;            // candidate positions staged in LDS per chunk
+constexpr int kMaxScanCells = 1024;       // neighbour cells scanned per home cell
+
+// Position of entry k of sorted slot i in the Verlet list.  Entries of the
+// kAtomsPerWave slots of one wave are interleaved so that the kLanes lanes
+// serving a slot, and the slots of a wave, read consecutive words.
+__host__ __device__ __forceinline__ size_t list_pos(int slot, int k, int cap) {
+    return (size_t)(slot / kAtomsPerWave) * kAtomsPerWave * cap + (size_t)(k / kLanes) * kWave +
+           (slot % kAtomsPerWave) * kLanes + (k % kLanes);
+}
 
 struct DevGroup {
     int slot_off, nslot_pad, nmem;
@@ -41,6 +55,7 @@ struct DevSys {
     float rho[3];                  // L_k / L_x (1 for a cubic box)
     int item_off, nitems_pad, natoms, nitems, nvs;
     int item_wg_off, item_nwg;
+    int term_wg_off, term_nwg;     // workgroups of this system's bonded terms
     int hybrid;
     DevGroup g[2];
     int ngroup;

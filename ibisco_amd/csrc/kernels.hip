@@ -193,68 +193,104 @@ __global__ void __launch_bounds__(kBlock) k_pack(const DevSys *__restrict__ sys,
     inv_perm[it] = slot;
 }
 
-// Full Verlet list per sorted slot: every member j != i of the 27 neighbouring
-// cells (each cell once) with r^2 < rlist^2, not excluded, not VS-VS.  The set
-// of unordered pairs equals the reference's half list bit for bit: a cheap
-// fixed-point test settles pairs far from the boundary and the FP64 criterion
-// decides the rest.  Layout: list[((slot/64)*cap + k)*64 + slot%64].
-__global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, int g,
-                                                       const int *__restrict__ perm, const int4 *__restrict__ pos4,
-                                                       const double4 *__restrict__ r64, const int *__restrict__ slot_cell,
-                                                       const int *__restrict__ cell_start, const int *__restrict__ cell_count,
-                                                       const int *__restrict__ excl_off, const int *__restrict__ excl,
-                                                       int cap, int *__restrict__ list, int *__restrict__ nneb,
-                                                       int *__restrict__ flags) {
-    const DevSys &S = sys[wg_sys[blockIdx.x]];
+// Full Verlet list of every member of one home cell (one workgroup per cell):
+// every member j != i of the 27 neighbouring cells (each cell once) with
+// r^2 < rlist^2, not excluded, not VS-VS.  The set of unordered pairs equals
+// the reference's half list bit for bit: a cheap fixed-point test settles
+// pairs far from the boundary and the FP64 criterion decides the rest.  The
+// candidate cells are staged in LDS one at a time; a list is ordered by cell
+// then by slot, so it is deterministic.
+template <bool kCubic>
+__global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ cell_sys,
+                                                           int g, const int *__restrict__ perm,
+                                                           const int4 *__restrict__ pos4, const double4 *__restrict__ r64,
+                                                           const int *__restrict__ cell_start,
+                                                           const int *__restrict__ cell_count,
+                                                           const int *__restrict__ excl_off, const int *__restrict__ excl,
+                                                           int cap, int *__restrict__ list, int *__restrict__ nneb,
+                                                           int *__restrict__ flags) {
+    __shared__ int4 cand[kCandLds];
+    __shared__ int cells[kMaxScanCells];
+    __shared__ int ncells;
+    const int gc = blockIdx.x;
+    const int nhome = cell_count[gc];
+    if (nhome == 0) return;
+    const DevSys &S = sys[cell_sys[gc]];
     const DevGroup &G = S.g[g];
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    const int I = perm[i];
-    if (I < 0) { nneb[i] = 0; return; }
-    const int4 pi = pos4[i];
-    const double4 ri = r64[i];
-    const int li = pos_label(pi.w);
-    const int ex0 = excl_off[I], ex1 = excl_off[I + 1];
-    const int lc = slot_cell[i];
-    const int cz = lc % G.nc[2], cy = (lc / G.nc[2]) % G.nc[1], cx = lc / (G.nc[2] * G.nc[1]);
     const int cs0 = cell_start[G.cell_off];
-    int* out = list + (size_t)(i >> 6) * cap * 64 + (i & 63);
-    int cnt = 0;
-    const int ncell_scan = G.all_pairs ? G.ncell : 27;
-    for (int q = 0; q < ncell_scan; ++q) {
-        int cc;
+    const int home0 = G.slot_off + cell_start[gc] - cs0;
+    const int lc = gc - G.cell_off;
+    if (threadIdx.x == 0) {
         if (G.all_pairs) {
-            cc = q;
+            const int nc = min(G.ncell, kMaxScanCells);
+            if (G.ncell > kMaxScanCells) atomicMax(&flags[3], G.ncell);
+            for (int q = 0; q < nc; ++q) cells[q] = q;
+            ncells = nc;
         } else {
-            int ox = q / 9 - 1, oy = (q / 3) % 3 - 1, oz = q % 3 - 1;
-            int x = cx + ox, y = cy + oy, z = cz + oz;
-            x += (x < 0) ? G.nc[0] : 0; x -= (x >= G.nc[0]) ? G.nc[0] : 0;
-            y += (y < 0) ? G.nc[1] : 0; y -= (y >= G.nc[1]) ? G.nc[1] : 0;
-            z += (z < 0) ? G.nc[2] : 0; z -= (z >= G.nc[2]) ? G.nc[2] : 0;
-            cc = (x * G.nc[1] + y) * G.nc[2] + z;
-        }
-        const int gc = G.cell_off + cc;
-        const int j0 = G.slot_off + cell_start[gc] - cs0, j1 = j0 + cell_count[gc];
-        for (int j = j0; j < j1; ++j) {
-            if (j == i) continue;
-            const int4 pj = pos4[j];
-            float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x) * S.rho[0];
-            float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y) * S.rho[1];
-            float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z) * S.rho[2];
-            float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            if (r2u > G.rl2u_hi) continue;
-            if (r2u >= G.rl2u_lo) {
-                if (!(exact_r2(ri, r64[j], S) < G.rl2)) continue;
+            const int cz = lc % G.nc[2], cy = (lc / G.nc[2]) % G.nc[1], cx = lc / (G.nc[2] * G.nc[1]);
+            for (int q = 0; q < 27; ++q) {
+                int x = cx + q / 9 - 1, y = cy + (q / 3) % 3 - 1, z = cz + q % 3 - 1;
+                x += (x < 0) ? G.nc[0] : 0; x -= (x >= G.nc[0]) ? G.nc[0] : 0;
+                y += (y < 0) ? G.nc[1] : 0; y -= (y >= G.nc[1]) ? G.nc[1] : 0;
+                z += (z < 0) ? G.nc[2] : 0; z -= (z >= G.nc[2]) ? G.nc[2] : 0;
+                cells[q] = (x * G.nc[1] + y) * G.nc[2] + z;
             }
-            const int J = perm[j];
-            bool skip = S.hybrid && li == 3 && pos_label(pj.w) == 3;
-            for (int e = ex0; e < ex1 && !skip; ++e) skip = (excl[e] == J);
-            if (skip) continue;
-            if (cnt < cap) out[(size_t)cnt * 64] = j;
-            ++cnt;
+            ncells = 27;
         }
     }
-    nneb[i] = cnt;
-    if (cnt > cap) atomicMax(&flags[0], cnt);
+    __syncthreads();
+    const int nscan = ncells;
+    const float rho0 = S.rho[0], rho1 = S.rho[1], rho2 = S.rho[2];
+    const float lo = G.rl2u_lo, hi = G.rl2u_hi;
+    for (int hb = 0; hb < nhome; hb += kListBlock) {
+        const bool valid = hb + (int)threadIdx.x < nhome;
+        const int i = home0 + hb + threadIdx.x;
+        int4 pi = make_int4(0, 0, 0, 0);
+        double4 ri = make_double4(0, 0, 0, 0);
+        int I = -1, li = 0, ex0 = 0, ex1 = 0;
+        if (valid) {
+            pi = pos4[i];
+            ri = r64[i];
+            I = perm[i];
+            li = pos_label(pi.w);
+            ex0 = excl_off[I];
+            ex1 = excl_off[I + 1];
+        }
+        int cnt = 0;
+        for (int q = 0; q < nscan; ++q) {
+            const int gcc = G.cell_off + cells[q];
+            const int j0 = G.slot_off + cell_start[gcc] - cs0, nj = cell_count[gcc];
+            for (int jb = 0; jb < nj; jb += kCandLds) {
+                const int m = min(kCandLds, nj - jb);
+                __syncthreads();
+                for (int t = threadIdx.x; t < m; t += kListBlock) cand[t] = pos4[j0 + jb + t];
+                __syncthreads();
+                if (!valid) continue;
+                for (int t = 0; t < m; ++t) {
+                    const int j = j0 + jb + t;
+                    if (j == i) continue;
+                    const int4 pj = cand[t];
+                    float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
+                    float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
+                    float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
+                    if (!kCubic) { dx *= rho0; dy *= rho1; dz *= rho2; }
+                    const float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                    if (r2u > hi) continue;
+                    if (r2u >= lo && !(exact_r2(ri, r64[j], S) < G.rl2)) continue;
+                    const int J = perm[j];
+                    bool skip = S.hybrid && li == 3 && pos_label(pj.w) == 3;
+                    for (int e = ex0; e < ex1 && !skip; ++e) skip = (excl[e] == J);
+                    if (skip) continue;
+                    if (cnt < cap) list[list_pos(i, cnt, cap)] = j;
+                    ++cnt;
+                }
+            }
+        }
+        if (valid) {
+            nneb[i] = cnt;
+            if (cnt > cap) atomicMax(&flags[0], cnt);
+        }
+    }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -324,7 +360,7 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
     __shared__ NbTab tabs[kMultiTabsLds(kMulti)];
     __shared__ int map[kMulti ? kMaxNbTypes * kMaxNbTypes : 1];
     __shared__ double red[9 * 4];
-    const DevSys &S = sys[wg_sys[blockIdx.x]];
+    const DevSys &S = sys[wg_sys[blockIdx.x / (kBlock / kNbAtomsPerBlock)]];
     const DevGroup &G = S.g[kGroup];
     if (kLds)
         for (int r = threadIdx.x; r < S.nb_nrow; r += kBlock) lds_rows[r] = nb_rows[S.nb_row_off + r];
@@ -341,7 +377,9 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
     __syncthreads();
     const float4 *rows = kLds ? lds_rows : nb_rows + S.nb_row_off;
 
-    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int a = lane / kLanes, q = lane % kLanes;
+    const int i = blockIdx.x * kNbAtomsPerBlock + (threadIdx.x / kWave) * kAtomsPerWave + a;
     const int I = perm[i];
     float fx = 0.f, fy = 0.f, fz = 0.f;
     float e[3] = {0.f, 0.f, 0.f};
@@ -354,20 +392,29 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
         const NbTab t0 = tabs[0];
         const int nt = S.ntype;
         const int n = min(nneb[i], cap);
-        const int *lp = list + (size_t)(i >> 6) * cap * 64 + (i & 63);
-        int k = 0;
-        for (; k + 4 <= n; k += 4) {
-            const int j0 = lp[(size_t)k * 64], j1 = lp[(size_t)(k + 1) * 64];
-            const int j2 = lp[(size_t)(k + 2) * 64], j3 = lp[(size_t)(k + 3) * 64];
+        const int *lp = list + list_pos(i, q, cap);   // entry q + kLanes*m is at lp[m*kWave]
+        const int nm = n > q ? (n - q + kLanes - 1) / kLanes : 0;
+        int m = 0;
+        for (; m + 4 <= nm; m += 4) {
+            const int j0 = lp[(size_t)m * kWave], j1 = lp[(size_t)(m + 1) * kWave];
+            const int j2 = lp[(size_t)(m + 2) * kWave], j3 = lp[(size_t)(m + 3) * kWave];
             const int4 p0 = pos4[j0], p1 = pos4[j1], p2 = pos4[j2], p3 = pos4[j3];
             nb_pair<kMulti, kGroup>(pi, p0, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
             nb_pair<kMulti, kGroup>(pi, p1, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
             nb_pair<kMulti, kGroup>(pi, p2, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
             nb_pair<kMulti, kGroup>(pi, p3, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
         }
-        for (; k < n; ++k)
-            nb_pair<kMulti, kGroup>(pi, pos4[lp[(size_t)k * 64]], li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e,
-                                    w);
+        for (; m < nm; ++m)
+            nb_pair<kMulti, kGroup>(pi, pos4[lp[(size_t)m * kWave]], li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz,
+                                    e, w);
+    }
+    // the kLanes lanes of a slot hold partial forces: butterfly-sum them
+    for (int off = 1; off < kLanes; off <<= 1) {
+        fx += __shfl_xor(fx, off, kWave);
+        fy += __shfl_xor(fy, off, kWave);
+        fz += __shfl_xor(fz, off, kWave);
+    }
+    if (I >= 0 && q == 0) {
         const float u = (float)S.unit;
         fnb[I] = fx * u;
         fnb[nitems_total + I] = fy * u;
@@ -391,15 +438,10 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
 // energy of a term is counted by its first member.  Geometry differences are
 // taken in FP64 from the unwrapped coordinates, the rest in FP32.
 
-struct BondedArgs {
-    const int4 *bonds;    // i, j, table, 0
-    const int4 *angles;   // i, j, k, table
-    const int4 *tors;     // i, j, k, l
-    const int *tors_tab;
-    const int4 *oops;
-    const int *oop_tab;
-    const int *term_off;  // per item CSR
-    const int *term_ref;  // kind<<29 | role<<27 | index
+struct TermArgs {
+    const int4 *mem;      // members (unused -1)
+    const int *meta;      // kind | table << 2, -1 for padding
+    const int *slot;      // first contribution slot of the term
     const BdTab *tabs;    // all bonded table headers
     const float4 *rows;   // all bonded rows {F_k, V_k, F_k+1, V_k+1}
 };
@@ -442,10 +484,8 @@ __device__ __forceinline__ void cross(const T a[3], const T b[3], T c[3]) {
 
 // BONDS.inc:8-56.  Returns the force on member `role` and the energy.
 template <typename T>
-__device__ __forceinline__ void bond_term(const BondedArgs &A, const double *s, int n, int idx, T fm[4][3], T &V,
-                                          bool &over) {
-    const int4 b = A.bonds[idx];
-    const BdTab t = A.tabs[b.z];
+__device__ __forceinline__ void bond_term(const BdTab &t, const float4 *rows, const int4 b, const double *s, int n,
+                                          T fm[4][3], T &V, bool &over) {
     T d[3];
     dvec(s, n, b.x, b.y, d);
     const T rij = nrm(d);
@@ -454,7 +494,7 @@ __device__ __forceinline__ void bond_term(const BondedArgs &A, const double *s, 
     const int ni = row_index(xr, t.ndat, (int *)nullptr, 0);
     const T alpha = (rij - ((T)t.x0 + (T)ni * (T)t.bin)) * (T)t.inv_bin;
     T F;
-    lerp_row(A.rows, t, ni, alpha, F, V);
+    lerp_row(rows, t, ni, alpha, F, V);
     for (int k = 0; k < 3; ++k) {
         fm[0][k] = F * d[k];
         fm[1][k] = -F * d[k];
@@ -463,10 +503,9 @@ __device__ __forceinline__ void bond_term(const BondedArgs &A, const double *s, 
 
 // ANGLES.inc:2-75
 template <typename T>
-__device__ __forceinline__ void angle_term(const BondedArgs &A, const double *s, int n, int idx, T fm[4][3], T &V) {
+__device__ __forceinline__ void angle_term(const BdTab &t, const float4 *rows, const int4 a4, const double *s, int n,
+                                           T fm[4][3], T &V) {
     const T R2D = (T)180 / (T)3.14159265358;
-    const int4 a4 = A.angles[idx];
-    const BdTab t = A.tabs[a4.w];
     T ij[3], kj[3], M[3], N[3], Sv[3];
     dvec(s, n, a4.x, a4.y, ij);
     dvec(s, n, a4.z, a4.y, kj);
@@ -489,11 +528,11 @@ __device__ __forceinline__ void angle_term(const BondedArgs &A, const double *s,
     const int ni = row_index(xr, t.ndat, (int *)nullptr, 0);
     T F;
     if (theta == (T)180) {
-        const float4 r = A.rows[t.row_off + ni];
+        const float4 r = rows[t.row_off + ni];
         F = (T)r.x;
         V = (T)r.y;
     } else {
-        lerp_row(A.rows, t, ni, (theta - (T)ni * (T)t.bin) * (T)t.inv_bin, F, V);
+        lerp_row(rows, t, ni, (theta - (T)ni * (T)t.bin) * (T)t.inv_bin, F, V);
     }
     for (int k = 0; k < 3; ++k) {
         const T fa = F * N[k] / rij, fk = F * Sv[k] / rkj;
@@ -505,11 +544,9 @@ __device__ __forceinline__ void angle_term(const BondedArgs &A, const double *s,
 
 // TORSIONS.inc:1-152
 template <typename T>
-__device__ __forceinline__ void torsion_term(const BondedArgs &A, const double *s, int n, int idx, T fm[4][3], T &V,
-                                             int *flags) {
+__device__ __forceinline__ void torsion_term(const BdTab &t, const float4 *rows, const int4 q4, const double *s, int n,
+                                             T fm[4][3], T &V, int *flags) {
     const T R2D = (T)180 / (T)3.14159265358;
-    const int4 q4 = A.tors[idx];
-    const BdTab t = A.tabs[A.tors_tab[idx]];
     T ij[3], jk[3], kl[3], M[3], N[3];
     dvec(s, n, q4.x, q4.y, ij);
     dvec(s, n, q4.y, q4.z, jk);
@@ -549,11 +586,11 @@ __device__ __forceinline__ void torsion_term(const BondedArgs &A, const double *
     const int ni = row_index(phi * (T)t.inv_bin, t.ndat, flags, 1);
     T F;
     if (phi == (T)360) {
-        const float4 r = A.rows[t.row_off + ni];
+        const float4 r = rows[t.row_off + ni];
         F = (T)r.x;
         V = (T)r.y;
     } else {
-        lerp_row(A.rows, t, ni, (phi - (T)ni * (T)t.bin) * (T)t.inv_bin, F, V);
+        lerp_row(rows, t, ni, (phi - (T)ni * (T)t.bin) * (T)t.inv_bin, F, V);
     }
     for (int k = 0; k < 3; ++k) {
         const T f1 = -F * M[k] / sinm / rij;
@@ -568,11 +605,9 @@ __device__ __forceinline__ void torsion_term(const BondedArgs &A, const double *
 
 // OUTOFPLANES.inc:1-141
 template <typename T>
-__device__ __forceinline__ void oop_term(const BondedArgs &A, const double *s, int n, int idx, T fm[4][3], T &V,
-                                         int *flags) {
+__device__ __forceinline__ void oop_term(const BdTab &t, const float4 *rows, const int4 q4, const double *s, int n,
+                                         T fm[4][3], T &V, int *flags) {
     const T R2D = (T)180 / (T)3.14159265358;
-    const int4 q4 = A.oops[idx];
-    const BdTab t = A.tabs[A.oop_tab[idx]];
     T ij[3], jk[3], kl[3], M[3], N[3], mN[3];
     dvec(s, n, q4.x, q4.y, ij);
     dvec(s, n, q4.z, q4.y, jk);
@@ -600,7 +635,7 @@ __device__ __forceinline__ void oop_term(const BondedArgs &A, const double *s, i
         alpha = fabs(fabs(phi) - fabs((T)ni * (T)t.bin)) * (T)t.inv_bin;
     }
     T F;
-    lerp_row(A.rows, t, ni, alpha, F, V);
+    lerp_row(rows, t, ni, alpha, F, V);
     for (int k = 0; k < 3; ++k) {
         const T f1 = F * M[k] / sinm / rij;
         const T f4 = -F * N[k] / sinn / rkl;
@@ -614,89 +649,66 @@ __device__ __forceinline__ void oop_term(const BondedArgs &A, const double *s, i
 
 // Bonded forces (replaces BONDED_FORCE.F90 with BONDS/ANGLES/TORSIONS/OUTOFPLANES.inc).
 //
-// Gather form: each item evaluates every term it takes part in and keeps its
-// own share, so no atomics are needed and the sums are deterministic.  The
-// first member of a term evaluates it in FP64 and counts its energy and its
-// virial; the other members evaluate it in FP32 for their force only.
-// Geometry differences are always taken in FP64 from the unwrapped coordinates.
+// One thread per term evaluates it in FP64 and stores the force on each member
+// in its own contribution slot; the integration kernel gathers an item's slots
+// in a fixed order, so there are no atomics and the sums are deterministic.
+// Geometry differences are taken in FP64 from the unwrapped coordinates.
 //
 // Virial: NEW_FORCE.F90:73-80 sums (F - F_nb) . s over atoms with unwrapped
 // coordinates.  Because the member forces of a term sum to zero, the same sum
 // is accumulated per term as sum_m f_m (s_m - s_first) in FP64, which does not
-// lose precision to the size of the unwrapped coordinates.  Order of the six
-// components as the reference: 11 22 33, then F_y s_x, F_z s_x, F_z s_y.
-__device__ __forceinline__ void term_members(const BondedArgs &A, int kind, int idx, int m[4], int &nm) {
-    if (kind == KBOND) { const int4 b = A.bonds[idx]; m[0] = b.x; m[1] = b.y; nm = 2; }
-    else if (kind == KANGLE) { const int4 a = A.angles[idx]; m[0] = a.x; m[1] = a.y; m[2] = a.z; nm = 3; }
-    else if (kind == KTORS) { const int4 q = A.tors[idx]; m[0] = q.x; m[1] = q.y; m[2] = q.z; m[3] = q.w; nm = 4; }
-    else { const int4 q = A.oops[idx]; m[0] = q.x; m[1] = q.y; m[2] = q.z; m[3] = q.w; nm = 4; }
-}
-
-__global__ void __launch_bounds__(kBlock) k_bonded(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
-                                                   BondedArgs A, const double *__restrict__ s, int nitems_total,
-                                                   float *__restrict__ fb, double *__restrict__ part,
-                                                   int *__restrict__ flags) {
+// lose precision to the size of the unwrapped coordinates.  Component order as
+// the reference: 11 22 33, then F_y s_x, F_z s_x, F_z s_y.
+__global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys, TermArgs A, const double *__restrict__ s,
+                                                  int nitems_total, int nterms, float *__restrict__ contrib,
+                                                  int ncontrib, double *__restrict__ part, int *__restrict__ flags) {
     __shared__ double red[19 * 4];
-    (void)sys; (void)wg_sys;
-    const int it = blockIdx.x * kBlock + threadIdx.x;
+    (void)wg_sys;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
     const int n = nitems_total;
-    double fsum[3] = {0, 0, 0};
     double en[19];
     for (int k = 0; k < 19; ++k) en[k] = 0.0;
-    const int t0 = A.term_off[it], t1 = A.term_off[it + 1];
-    for (int q = t0; q < t1; ++q) {
-        const int ref = A.term_ref[q];
-        const int kind = (ref >> 29) & 3, role = (ref >> 27) & 3, idx = ref & ((1 << 27) - 1);
-        if (role == 0) {
-            double fm[4][3], V;
-            int cls;
-            if (kind == KBOND) {
-                bool over;
-                bond_term<double>(A, s, n, idx, fm, V, over);
-                cls = A.tabs[A.bonds[idx].z].label - 1;
-                if (over) en[12] += 1.0;
-            } else if (kind == KANGLE) {
-                angle_term<double>(A, s, n, idx, fm, V);
-                cls = 3 + A.tabs[A.angles[idx].w].label - 1;
-            } else if (kind == KTORS) {
-                torsion_term<double>(A, s, n, idx, fm, V, flags);
-                cls = 6 + A.tabs[A.tors_tab[idx]].label - 1;
-            } else {
-                oop_term<double>(A, s, n, idx, fm, V, flags);
-                cls = 9 + A.tabs[A.oop_tab[idx]].label - 1;
-            }
-            en[cls] += V;
-            for (int k = 0; k < 3; ++k) fsum[k] += fm[0][k];
-            int mem[4], nm;
-            term_members(A, kind, idx, mem, nm);
-            for (int r = 1; r < nm; ++r) {
-                const double dx = s[mem[r]] - s[mem[0]], dy = s[n + mem[r]] - s[n + mem[0]],
-                             dz = s[2 * n + mem[r]] - s[2 * n + mem[0]];
-                en[13] += fm[r][0] * dx;
-                en[14] += fm[r][1] * dy;
-                en[15] += fm[r][2] * dz;
-                en[16] += fm[r][1] * dx;
-                en[17] += fm[r][2] * dx;
-                en[18] += fm[r][2] * dy;
-            }
+    const int meta = t < nterms ? A.meta[t] : -1;
+    if (meta >= 0) {
+        const int kind = meta & 3;
+        const BdTab tb = A.tabs[meta >> 2];
+        const int4 m4 = A.mem[t];
+        double fm[4][3], V;
+        int nm;
+        if (kind == KBOND) {
+            bool over;
+            bond_term<double>(tb, A.rows, m4, s, n, fm, V, over);
+            if (over) en[12] += 1.0;
+            nm = 2;
+        } else if (kind == KANGLE) {
+            angle_term<double>(tb, A.rows, m4, s, n, fm, V);
+            nm = 3;
+        } else if (kind == KTORS) {
+            torsion_term<double>(tb, A.rows, m4, s, n, fm, V, flags);
+            nm = 4;
         } else {
-            float fm[4][3], V;
-            if (kind == KBOND) {
-                bool over;
-                bond_term<float>(A, s, n, idx, fm, V, over);
-            } else if (kind == KANGLE) {
-                angle_term<float>(A, s, n, idx, fm, V);
-            } else if (kind == KTORS) {
-                torsion_term<float>(A, s, n, idx, fm, V, flags);
-            } else {
-                oop_term<float>(A, s, n, idx, fm, V, flags);
-            }
-            for (int k = 0; k < 3; ++k) fsum[k] += (double)fm[role][k];
+            oop_term<double>(tb, A.rows, m4, s, n, fm, V, flags);
+            nm = 4;
+        }
+        en[3 * kind + tb.label - 1] += V;
+        const int mem[4] = {m4.x, m4.y, m4.z, m4.w};
+        const int sl = A.slot[t];
+        for (int r = 0; r < nm; ++r) {
+            contrib[sl + r] = (float)fm[r][0];
+            contrib[ncontrib + sl + r] = (float)fm[r][1];
+            contrib[2 * ncontrib + sl + r] = (float)fm[r][2];
+        }
+        for (int r = 1; r < nm; ++r) {
+            const double dx = s[mem[r]] - s[mem[0]], dy = s[n + mem[r]] - s[n + mem[0]],
+                         dz = s[2 * n + mem[r]] - s[2 * n + mem[0]];
+            en[13] += fm[r][0] * dx;
+            en[14] += fm[r][1] * dy;
+            en[15] += fm[r][2] * dz;
+            en[16] += fm[r][1] * dx;
+            en[17] += fm[r][2] * dx;
+            en[18] += fm[r][2] * dy;
         }
     }
-    fb[it] = (float)fsum[0];
-    fb[n + it] = (float)fsum[1];
-    fb[2 * n + it] = (float)fsum[2];
     block_sum<19>(en, red);
     if (threadIdx.x == 0) {
         double *p = part + (size_t)blockIdx.x * kBdPart;
@@ -714,7 +726,10 @@ __global__ void __launch_bounds__(kBlock) k_bonded(const DevSys *__restrict__ sy
 
 struct MoveArgs {
     double *s, *v;
-    const float *fnb, *fb;
+    const float *fnb;
+    const float *contrib;   // bonded contribution slots (SoA x, y, z)
+    const int *inc_off, *inc_slot;
+    int ncontrib;
     float *ftot;
     const int *itype, *label;
     const double *mass, *invmass;
@@ -743,12 +758,22 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     if (A.mode == 1 && ty >= 0 && lab == 3)
         for (int k = 0; k < 3; ++k) A.ftot[k * n + it] = 0.0f;   // distributed (DISTRIBUTE_VSFORCE.f90:40)
     if (ty >= 0 && lab != 3) {
-        double F[3];
-        for (int k = 0; k < 3; ++k) F[k] = (double)A.fnb[k * n + it] + (double)A.fb[k * n + it];
+        double F[3], fbv[3] = {0, 0, 0};
+        for (int q = A.inc_off[it]; q < A.inc_off[it + 1]; ++q) {
+            const int sl = A.inc_slot[q];
+            for (int k = 0; k < 3; ++k) fbv[k] += (double)A.contrib[k * A.ncontrib + sl];
+        }
+        for (int k = 0; k < 3; ++k) F[k] = (double)A.fnb[k * n + it] + fbv[k];
         const int vs = A.vs_of[it];
         if (vs >= 0) {
+            // the virtual site's nonbonded and bonded forces, spread by mass fraction
             const double c = A.vs_coeff[it];
-            for (int k = 0; k < 3; ++k) F[k] += (double)A.fnb[k * n + vs] * c + (double)A.fb[k * n + vs] * c;
+            double fv[3] = {0, 0, 0};
+            for (int q = A.inc_off[vs]; q < A.inc_off[vs + 1]; ++q) {
+                const int sl = A.inc_slot[q];
+                for (int k = 0; k < 3; ++k) fv[k] += (double)A.contrib[k * A.ncontrib + sl];
+            }
+            for (int k = 0; k < 3; ++k) F[k] += ((double)A.fnb[k * n + vs] + fv[k]) * c;
         }
         double sp[3] = {A.s[it], A.s[n + it], A.s[2 * n + it]};
         if (A.mode == 1) {
@@ -842,10 +867,11 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const DevSys *__restrict__ 
     for (int k = 0; k < 40; ++k) acc[k] = 0.0;
     for (int g = 0; g < S.ngroup; ++g) {
         const double *pp = g == 0 ? part_nb0 : part_nb1;
-        for (int w = S.g[g].wg_off + threadIdx.x; w < S.g[g].wg_off + S.g[g].nwg; w += kBlock)
+        constexpr int per = kBlock / kNbAtomsPerBlock;   // nonbonded workgroups per 256-slot block
+        for (int w = per * S.g[g].wg_off + threadIdx.x; w < per * (S.g[g].wg_off + S.g[g].nwg); w += kBlock)
             for (int k = 0; k < 9; ++k) acc[k] += pp[(size_t)w * kNbPart + k];
     }
-    for (int w = S.item_wg_off + threadIdx.x; w < S.item_wg_off + S.item_nwg; w += kBlock) {
+    for (int w = S.term_wg_off + threadIdx.x; w < S.term_wg_off + S.term_nwg; w += kBlock) {
         for (int k = 0; k < 13; ++k) acc[9 + k] += part_bd[(size_t)w * kBdPart + k];
         for (int k = 0; k < 6; ++k) acc[22 + k] += part_bd[(size_t)w * kBdPart + 13 + k];
     }
@@ -906,9 +932,8 @@ __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, 
     if (perm[i] >= 0) {
         const int4 pi = pos4[i];
         const int nn = min(nneb[i], cap);
-        const int *lp = list + (size_t)(i >> 6) * cap * 64 + (i & 63);
         for (int k = 0; k < nn; ++k) {
-            const int4 pj = pos4[lp[(size_t)k * 64]];
+            const int4 pj = pos4[list[list_pos(i, k, cap)]];
             const float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x) * S.rho[0];
             const float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y) * S.rho[1];
             const float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z) * S.rho[2];
@@ -939,8 +964,14 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
                        a.cell_count, a.tmp, a.mem, a.perm, a.slot_cell, a.flags);
     hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
                        a.itype, a.label, a.r64, a.pos4, a.inv_perm);
-    hipLaunchKernelGGL(k_build_list, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.pos4, a.r64,
-                       a.slot_cell, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap, a.list, a.nneb, a.flags);
+    if (a.cubic)
+        hipLaunchKernelGGL(k_build_list<true>, dim3(a.ncell_total), dim3(kListBlock), 0, st, a.sys, a.cell_sys, a.g,
+                           a.perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap, a.list, a.nneb,
+                           a.flags);
+    else
+        hipLaunchKernelGGL(k_build_list<false>, dim3(a.ncell_total), dim3(kListBlock), 0, st, a.sys, a.cell_sys, a.g,
+                           a.perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap, a.list, a.nneb,
+                           a.flags);
 }
 
 void launch_pack_only(hipStream_t st, const RebuildArgs &a) {
@@ -950,7 +981,7 @@ void launch_pack_only(hipStream_t st, const RebuildArgs &a) {
 
 template <bool M, bool L, int G>
 static void nb_go(hipStream_t st, const NbArgs &a, size_t lds) {
-    hipLaunchKernelGGL((k_nonbonded<M, L, G>), dim3(a.nwg), dim3(kBlock), lds, st, a.sys, a.wg_sys, a.perm, a.pos4,
+    hipLaunchKernelGGL((k_nonbonded<M, L, G>), dim3(a.nwg * (kBlock / kNbAtomsPerBlock)), dim3(kBlock), lds, st, a.sys, a.wg_sys, a.perm, a.pos4,
                        a.list, a.nneb, a.cap, a.rows, a.tabs, a.map, a.nitems_total, a.fnb, a.part);
 }
 
@@ -972,15 +1003,17 @@ void launch_nonbonded(hipStream_t st, const NbArgs &a) {
     else nb_dispatch<1>(st, a);
 }
 
-void launch_bonded(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const BondedLaunch &b,
-                   const double *s, int nitems_total, float *fb, double *part, int *flags) {
-    BondedArgs A{b.bonds, b.angles, b.tors, b.tors_tab, b.oops, b.oop_tab, b.term_off, b.term_ref, b.tabs, b.rows};
-    hipLaunchKernelGGL(k_bonded, dim3(nwg), dim3(kBlock), 0, st, sys, wg_sys, A, s, nitems_total, fb, part, flags);
+void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &b, const double *s, int nitems_total,
+                  int nterms, float *contrib, int ncontrib, double *part, int *flags) {
+    TermArgs A{b.mem, b.meta, b.slot, b.tabs, b.rows};
+    hipLaunchKernelGGL(k_terms, dim3(nwg), dim3(kBlock), 0, st, wg_sys, A, s, nitems_total, nterms, contrib, ncontrib,
+                       part, flags);
 }
 
 void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const MoveLaunch &m) {
     MoveArgs A;
-    A.s = m.s; A.v = m.v; A.fnb = m.fnb; A.fb = m.fb; A.ftot = m.ftot; A.itype = m.itype; A.label = m.label;
+    A.s = m.s; A.v = m.v; A.fnb = m.fnb; A.contrib = m.contrib; A.inc_off = m.inc_off; A.inc_slot = m.inc_slot;
+    A.ncontrib = m.ncontrib; A.ftot = m.ftot; A.itype = m.itype; A.label = m.label;
     A.mass = m.mass; A.invmass = m.invmass; A.vs_of = m.vs_of; A.vs_coeff = m.vs_coeff;
     A.pos4[0] = m.pos4[0]; A.pos4[1] = m.pos4[1]; A.inv_perm[0] = m.inv_perm[0]; A.inv_perm[1] = m.inv_perm[1];
     A.sysstate = m.sysstate; A.part = m.part; A.step = m.step; A.nitems_total = m.nitems_total; A.mode = m.mode;

@@ -11,6 +11,7 @@ struct RebuildArgs {
     const int *wg_sys;
     const int *cell_sys;
     int g, nwg, ncell_total, nitems_total, cap;
+    bool cubic;
     const int *mem;
     const double *s;
     const int *itype, *label;
@@ -37,19 +38,19 @@ struct NbArgs {
     double *part;
 };
 
-struct BondedLaunch {
-    const int4 *bonds, *angles, *tors;
-    const int *tors_tab;
-    const int4 *oops;
-    const int *oop_tab;
-    const int *term_off, *term_ref;
+struct TermLaunch {
+    const int4 *mem;
+    const int *meta, *slot;
     const BdTab *tabs;
     const float4 *rows;
 };
 
 struct MoveLaunch {
     double *s, *v;
-    const float *fnb, *fb;
+    const float *fnb;
+    const float *contrib;
+    const int *inc_off, *inc_slot;
+    int ncontrib;
     float *ftot;
     const int *itype, *label;
     const double *mass, *invmass;
@@ -67,8 +68,8 @@ void launch_zero_int(hipStream_t st, int *p, int n);
 void launch_rebuild(hipStream_t st, const RebuildArgs &a);
 void launch_pack_only(hipStream_t st, const RebuildArgs &a);
 void launch_nonbonded(hipStream_t st, const NbArgs &a);
-void launch_bonded(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const BondedLaunch &b,
-                   const double *s, int nitems_total, float *fb, double *part, int *flags);
+void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &b, const double *s, int nitems_total,
+                  int nterms, float *contrib, int ncontrib, double *part, int *flags);
 void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const MoveLaunch &m);
 void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, double *s, int nitems_total,
                      const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,

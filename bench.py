@@ -74,7 +74,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--nchains", type=int, default=400)
     ap.add_argument("--nbeads", type=int, default=250)
+    ap.add_argument("--eager", action="store_true",
+                    help="launch kernel by kernel instead of through HIP graphs (profiling runs)")
+    ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
+    if args.calib_copy:
+        import torch
+        n = args.calib_copy * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(5):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes_read": a.numel() * 4, "calib_copy_bytes_written": b.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); replicas and jobs shard across GPUs "
               "without communication, see DESIGN.md", file=sys.stderr)
@@ -89,10 +103,10 @@ def main():
     e = Engine(sysm)
     workloads.equilibrate(e, sysm, stage_steps=200, final_steps=1000)
     print("[bench] equilibrated", file=sys.stderr, flush=True)
-    e.step(args.warmup, graph=True)
+    e.step(args.warmup, graph=not args.eager)
     e.obs()   # synchronise
     t0 = time.perf_counter()
-    e.step(args.steps, graph=True)
+    e.step(args.steps, graph=not args.eager)
     ob = e.obs()
     t1 = time.perf_counter()
     secs = t1 - t0
@@ -129,7 +143,8 @@ def main():
         "data": "synthetic (seeded random-walk melt, equilibrated on device)",
         "config": {"workload": "ua_melt_100k_plain", "sites": n, "chains": args.nchains, "chain_length": args.nbeads,
                    "rc_nm": 1.0, "rlist_nm": 1.3, "nupdate": 10, "dt_fs": 1.0, "ensemble": "NVE",
-                   "path": "one system per launch, HIP graph per list interval",
+                   "path": "one system per launch, " + ("eager launches" if args.eager else
+                                                         "HIP graph per list interval"),
                    "precision": "FP32 forces, FP64 integration and reductions"},
         "step_breakdown_ms": {k: round(tm[k] / tm["nsteps_timed"], 5) for k in
                               ("nonbonded", "bonded", "integrate", "rebuild", "step")},

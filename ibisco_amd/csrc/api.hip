@@ -78,6 +78,7 @@ struct ibx_ctx {
     DBuf<double> s, v, mass, invmass, vs_coeff, vs_invmass, part_bd, part_mv, obs, avg, sysstate;
     DBuf<float> fnb, ftot;
     DBuf<float4> nb_rows, bd_rows;
+    DBuf<double> bd_x;
     DBuf<NbTab> nb_tabs;
     DBuf<BdTab> bd_tabs;
     std::vector<NbTab> h_nb_tabs;
@@ -167,6 +168,7 @@ int build_device(ibx_ctx *c) {
     std::vector<BdTab> hbt;
     int nslot = 0;
     std::vector<float4> hbrows, hnrows;
+    std::vector<double> hbx;
     std::vector<NbTab> hntab;
     std::vector<int> hnmap;
     // contribution slots of the bonded terms each item takes part in, in term order
@@ -213,6 +215,7 @@ int build_device(ibx_ctx *c) {
                 b.label = labs.empty() ? 1 : labs[t];
                 hbt.push_back(b);
                 append_rows(hbrows, tabs[t]);
+                for (int k = 0; k <= tabs[t].ndat; ++k) hbx.push_back(tabs[t].x[k]);
             }
             return base;
         };
@@ -343,7 +346,7 @@ int build_device(ibx_ctx *c) {
         (rc = upload(c, c->vs_members, hvm)) || (rc = upload(c, c->inc_off, hinc_off)) ||
         (rc = upload(c, c->inc_slot, hinc)) || (rc = upload(c, c->term_mem, htm)) || (rc = upload(c, c->term_meta, hmeta)) ||
         (rc = upload(c, c->term_slot, hslot)) || (rc = upload(c, c->wg_terms, hwgt)) ||
-        (rc = upload(c, c->bd_tabs, hbt)) || (rc = upload(c, c->bd_rows, hbrows)) ||
+        (rc = upload(c, c->bd_tabs, hbt)) || (rc = upload(c, c->bd_rows, hbrows)) || (rc = upload(c, c->bd_x, hbx)) ||
         (rc = upload(c, c->nb_rows, hnrows)) || (rc = upload(c, c->nb_tabs, hntab)) || (rc = upload(c, c->nb_map, hnmap)))
         return rc;
     c->h_nb_tabs = hntab;
@@ -498,7 +501,7 @@ void enqueue_forces(ibx_ctx *c, int mode) {
 }
 
 void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
-    TermLaunch b{c->term_mem.p, c->term_meta.p, c->term_slot.p, c->bd_tabs.p, c->bd_rows.p};
+    TermLaunch b{c->term_mem.p, c->term_meta.p, c->term_slot.p, c->bd_tabs.p, c->bd_rows.p, c->bd_x.p};
     launch_terms(c->st, c->term_nwg, c->wg_terms.p, b, c->s.p, c->n_items, c->nterms, c->contrib.p, c->ncontrib,
                  c->part_bd.p, c->flags.p);
     if (ev) (void)hipEventRecord((*ev)[2], c->st);

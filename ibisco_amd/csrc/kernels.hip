@@ -301,24 +301,33 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restr
 // Tables (F/r and V, interleaved with the next row) are staged in LDS.  Each
 // pair is visited from both ends; energies and virial are halved.
 
-// Contribution of one listed neighbour; `cls` selects the energy class.
+// Contribution of one listed neighbour.  The cutoff test and the force
+// direction use FP32 displacements; for pairs inside the cutoff the distance
+// that indexes the table is recomputed in FP64 from the exact integer
+// displacement, so the interpolated energy is not limited by the FP32
+// rounding of r (which alone would cost ~1e-7 of the total energy).
 template <bool kMulti, int kGroup>
 __device__ __forceinline__ void nb_pair(const int4 pi, const int4 pj, int li, float rc2u, const float rho[3],
-                                        const float4 *__restrict__ rows, const NbTab *__restrict__ tabs,
-                                        const int *__restrict__ map, int ntype, const NbTab &t0, float &fx,
-                                        float &fy, float &fz, float (&e)[3], float (&w)[6]) {
-    const float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x) * rho[0];
-    const float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y) * rho[1];
-    const float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z) * rho[2];
+                                        const double rhod[3], const float4 *__restrict__ rows,
+                                        const NbTab *__restrict__ tabs, const int *__restrict__ map, int ntype,
+                                        const NbTab &t0, float &fx, float &fy, float &fz, float (&e)[3],
+                                        float (&w)[6]) {
+    const int ix = (int)((uint32_t)pi.x - (uint32_t)pj.x);
+    const int iy = (int)((uint32_t)pi.y - (uint32_t)pj.y);
+    const int iz = (int)((uint32_t)pi.z - (uint32_t)pj.z);
+    const float dx = (float)ix * rho[0];
+    const float dy = (float)iy * rho[1];
+    const float dz = (float)iz * rho[2];
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
     if (r2 < rc2u) {
         NbTab tb = t0;
         if (kMulti) tb = tabs[map[pos_type(pi.w) * ntype + pos_type(pj.w)]];
-        const float ru = sqrtf(r2);
-        const float tt = fmaf(ru, tb.c_hi, ru * tb.c_lo);
+        const double ddx = (double)ix * rhod[0], ddy = (double)iy * rhod[1], ddz = (double)iz * rhod[2];
+        const double rd = sqrt(fma(ddx, ddx, fma(ddy, ddy, ddz * ddz)));
+        const double tt = rd * ((double)tb.c_hi + (double)tb.c_lo);
         int kr = (int)tt;
         kr = kr > tb.ndat ? tb.ndat : kr;
-        const float a = tt - (float)kr;
+        const float a = (float)(tt - (double)kr);
         const float4 row = rows[tb.row_off + kr];
         const float F = fmaf(a, row.z - row.x, row.x);
         const float V = fmaf(a, row.w - row.y, row.y);
@@ -389,6 +398,7 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
         const int li = pos_label(pi.w);
         const float rc2u = G.rc2u;
         const float rho[3] = {S.rho[0], S.rho[1], S.rho[2]};
+        const double rhod[3] = {S.box[0] / S.box[0], S.box[1] / S.box[0], S.box[2] / S.box[0]};
         const NbTab t0 = tabs[0];
         const int nt = S.ntype;
         const int n = min(nneb[i], cap);
@@ -399,13 +409,13 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
             const int j0 = lp[(size_t)m * kWave], j1 = lp[(size_t)(m + 1) * kWave];
             const int j2 = lp[(size_t)(m + 2) * kWave], j3 = lp[(size_t)(m + 3) * kWave];
             const int4 p0 = pos4[j0], p1 = pos4[j1], p2 = pos4[j2], p3 = pos4[j3];
-            nb_pair<kMulti, kGroup>(pi, p0, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
-            nb_pair<kMulti, kGroup>(pi, p1, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
-            nb_pair<kMulti, kGroup>(pi, p2, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
-            nb_pair<kMulti, kGroup>(pi, p3, li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
+            nb_pair<kMulti, kGroup>(pi, p0, li, rc2u, rho, rhod, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
+            nb_pair<kMulti, kGroup>(pi, p1, li, rc2u, rho, rhod, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
+            nb_pair<kMulti, kGroup>(pi, p2, li, rc2u, rho, rhod, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
+            nb_pair<kMulti, kGroup>(pi, p3, li, rc2u, rho, rhod, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
         }
         for (; m < nm; ++m)
-            nb_pair<kMulti, kGroup>(pi, pos4[lp[(size_t)m * kWave]], li, rc2u, rho, rows, tabs, map, nt, t0, fx, fy, fz,
+            nb_pair<kMulti, kGroup>(pi, pos4[lp[(size_t)m * kWave]], li, rc2u, rho, rhod, rows, tabs, map, nt, t0, fx, fy, fz,
                                     e, w);
     }
     // the kLanes lanes of a slot hold partial forces: butterfly-sum them
@@ -444,6 +454,7 @@ struct TermArgs {
     const int *slot;      // first contribution slot of the term
     const BdTab *tabs;    // all bonded table headers
     const float4 *rows;   // all bonded rows {F_k, V_k, F_k+1, V_k+1}
+    const double *xs;     // abscissa of every bonded row
 };
 
 // Table row index from a non-negative abscissa ratio, clamped to [0, ndat]
@@ -484,15 +495,15 @@ __device__ __forceinline__ void cross(const T a[3], const T b[3], T c[3]) {
 
 // BONDS.inc:8-56.  Returns the force on member `role` and the energy.
 template <typename T>
-__device__ __forceinline__ void bond_term(const BdTab &t, const float4 *rows, const int4 b, const double *s, int n,
-                                          T fm[4][3], T &V, bool &over) {
+__device__ __forceinline__ void bond_term(const BdTab &t, const float4 *rows, const double *xs, const int4 b,
+                                          const double *s, int n, T fm[4][3], T &V, bool &over) {
     T d[3];
     dvec(s, n, b.x, b.y, d);
     const T rij = nrm(d);
     const T xr = (rij - (T)t.x0) * (T)t.inv_bin;
     over = xr >= (T)(t.ndat + 1);              // BONDS.inc:21-31: warn, clamp, continue
     const int ni = row_index(xr, t.ndat, (int *)nullptr, 0);
-    const T alpha = (rij - ((T)t.x0 + (T)ni * (T)t.bin)) * (T)t.inv_bin;
+    const T alpha = (rij - (T)xs[t.row_off + ni]) * (T)t.inv_bin;   // RBOND(TIJ,NI), the table's own abscissa
     T F;
     lerp_row(rows, t, ni, alpha, F, V);
     for (int k = 0; k < 3; ++k) {
@@ -503,8 +514,8 @@ __device__ __forceinline__ void bond_term(const BdTab &t, const float4 *rows, co
 
 // ANGLES.inc:2-75
 template <typename T>
-__device__ __forceinline__ void angle_term(const BdTab &t, const float4 *rows, const int4 a4, const double *s, int n,
-                                           T fm[4][3], T &V) {
+__device__ __forceinline__ void angle_term(const BdTab &t, const float4 *rows, const double *xs, const int4 a4,
+                                           const double *s, int n, T fm[4][3], T &V) {
     const T R2D = (T)180 / (T)3.14159265358;
     T ij[3], kj[3], M[3], N[3], Sv[3];
     dvec(s, n, a4.x, a4.y, ij);
@@ -532,7 +543,7 @@ __device__ __forceinline__ void angle_term(const BdTab &t, const float4 *rows, c
         F = (T)r.x;
         V = (T)r.y;
     } else {
-        lerp_row(rows, t, ni, (theta - (T)ni * (T)t.bin) * (T)t.inv_bin, F, V);
+        lerp_row(rows, t, ni, (theta - (T)xs[t.row_off + ni]) * (T)t.inv_bin, F, V);
     }
     for (int k = 0; k < 3; ++k) {
         const T fa = F * N[k] / rij, fk = F * Sv[k] / rkj;
@@ -544,8 +555,8 @@ __device__ __forceinline__ void angle_term(const BdTab &t, const float4 *rows, c
 
 // TORSIONS.inc:1-152
 template <typename T>
-__device__ __forceinline__ void torsion_term(const BdTab &t, const float4 *rows, const int4 q4, const double *s, int n,
-                                             T fm[4][3], T &V, int *flags) {
+__device__ __forceinline__ void torsion_term(const BdTab &t, const float4 *rows, const double *xs, const int4 q4,
+                                             const double *s, int n, T fm[4][3], T &V, int *flags) {
     const T R2D = (T)180 / (T)3.14159265358;
     T ij[3], jk[3], kl[3], M[3], N[3];
     dvec(s, n, q4.x, q4.y, ij);
@@ -590,7 +601,7 @@ __device__ __forceinline__ void torsion_term(const BdTab &t, const float4 *rows,
         F = (T)r.x;
         V = (T)r.y;
     } else {
-        lerp_row(rows, t, ni, (phi - (T)ni * (T)t.bin) * (T)t.inv_bin, F, V);
+        lerp_row(rows, t, ni, (phi - (T)xs[t.row_off + ni]) * (T)t.inv_bin, F, V);
     }
     for (int k = 0; k < 3; ++k) {
         const T f1 = -F * M[k] / sinm / rij;
@@ -605,8 +616,8 @@ __device__ __forceinline__ void torsion_term(const BdTab &t, const float4 *rows,
 
 // OUTOFPLANES.inc:1-141
 template <typename T>
-__device__ __forceinline__ void oop_term(const BdTab &t, const float4 *rows, const int4 q4, const double *s, int n,
-                                         T fm[4][3], T &V, int *flags) {
+__device__ __forceinline__ void oop_term(const BdTab &t, const float4 *rows, const double *xs, const int4 q4,
+                                         const double *s, int n, T fm[4][3], T &V, int *flags) {
     const T R2D = (T)180 / (T)3.14159265358;
     T ij[3], jk[3], kl[3], M[3], N[3], mN[3];
     dvec(s, n, q4.x, q4.y, ij);
@@ -629,10 +640,10 @@ __device__ __forceinline__ void oop_term(const BdTab &t, const float4 *rows, con
     cost = cost > (T)1 ? (T)1 : (cost < (T)-1 ? (T)-1 : cost);
     const T phi = signt * acos(cost) * R2D;
     int ni = row_index(rint(((T)180 + phi) * (T)t.inv_bin), t.ndat, flags, 2);
-    T alpha = fabs(fabs(phi) - fabs((T)ni * (T)t.bin)) * (T)t.inv_bin;
+    T alpha = fabs(fabs(phi) - fabs((T)xs[t.row_off + ni])) * (T)t.inv_bin;
     if (alpha > (T)1) {
         ni = min(ni + 1, t.ndat);
-        alpha = fabs(fabs(phi) - fabs((T)ni * (T)t.bin)) * (T)t.inv_bin;
+        alpha = fabs(fabs(phi) - fabs((T)xs[t.row_off + ni])) * (T)t.inv_bin;
     }
     T F;
     lerp_row(rows, t, ni, alpha, F, V);
@@ -677,17 +688,17 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
         int nm;
         if (kind == KBOND) {
             bool over;
-            bond_term<double>(tb, A.rows, m4, s, n, fm, V, over);
+            bond_term<double>(tb, A.rows, A.xs, m4, s, n, fm, V, over);
             if (over) en[12] += 1.0;
             nm = 2;
         } else if (kind == KANGLE) {
-            angle_term<double>(tb, A.rows, m4, s, n, fm, V);
+            angle_term<double>(tb, A.rows, A.xs, m4, s, n, fm, V);
             nm = 3;
         } else if (kind == KTORS) {
-            torsion_term<double>(tb, A.rows, m4, s, n, fm, V, flags);
+            torsion_term<double>(tb, A.rows, A.xs, m4, s, n, fm, V, flags);
             nm = 4;
         } else {
-            oop_term<double>(tb, A.rows, m4, s, n, fm, V, flags);
+            oop_term<double>(tb, A.rows, A.xs, m4, s, n, fm, V, flags);
             nm = 4;
         }
         en[3 * kind + tb.label - 1] += V;
@@ -1005,7 +1016,7 @@ void launch_nonbonded(hipStream_t st, const NbArgs &a) {
 
 void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &b, const double *s, int nitems_total,
                   int nterms, float *contrib, int ncontrib, double *part, int *flags) {
-    TermArgs A{b.mem, b.meta, b.slot, b.tabs, b.rows};
+    TermArgs A{b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs};
     hipLaunchKernelGGL(k_terms, dim3(nwg), dim3(kBlock), 0, st, wg_sys, A, s, nitems_total, nterms, contrib, ncontrib,
                        part, flags);
 }

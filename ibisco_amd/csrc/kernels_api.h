@@ -43,6 +43,7 @@ struct TermLaunch {
     const int *meta, *slot;
     const BdTab *tabs;
     const float4 *rows;
+    const double *xs;
 };
 
 struct MoveLaunch {

@@ -28,17 +28,23 @@ __device__ __forceinline__ void fold3(const double s[3], const double box2[3], d
     }
 }
 
-// Pair criterion of NEW_NEIGHBOUR_WITHLIST.f90:64-69 in FP64, no contraction.
-__device__ __forceinline__ double exact_r2(const double4 a, const double4 b, const DevSys &S) {
+// Pair criterion of NEW_NEIGHBOUR_WITHLIST.f90:64-69 in FP64: d = ri - rj; d -= ANINT(d/L)*L; r2 = dx*dx + dy*dy + dz*dz, each operation rounded on its own as in
+// NEW_NEIGHBOUR_WITHLIST.f90:64-69 (FP64 operands; no contraction into FMA)
+__device__ __forceinline__ double exact_r2(const double4 a, const double4 b, double l0, double l1, double l2,
+                                           double li0, double li1, double li2) {
 #pragma clang fp contract(off)
     double d0 = a.x - b.x, d1 = a.y - b.y, d2 = a.z - b.z;
-    d0 = d0 - round(d0 * S.boxinv[0]) * S.box[0];
-    d1 = d1 - round(d1 * S.boxinv[1]) * S.box[1];
-    d2 = d2 - round(d2 * S.boxinv[2]) * S.box[2];
+    d0 = d0 - round(d0 * li0) * l0;
+    d1 = d1 - round(d1 * li1) * l1;
+    d2 = d2 - round(d2 * li2) * l2;
     double r2 = d0 * d0;
     r2 = r2 + d1 * d1;
     r2 = r2 + d2 * d2;
     return r2;
+}
+
+__device__ __forceinline__ double exact_r2(const double4 a, const double4 b, const DevSys &S) {
+    return exact_r2(a, b, S.box[0], S.box[1], S.box[2], S.boxinv[0], S.boxinv[1], S.boxinv[2]);
 }
 
 __device__ __forceinline__ double wave_sum(double x) {
@@ -210,6 +216,7 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restr
                                                            int cap, int *__restrict__ list, int *__restrict__ nneb,
                                                            int *__restrict__ flags) {
     __shared__ int4 cand[kCandLds];
+    __shared__ int cand_item[kCandLds];
     __shared__ int cells[kMaxScanCells];
     __shared__ int ncells;
     const int gc = blockIdx.x;
@@ -217,9 +224,15 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restr
     if (nhome == 0) return;
     const DevSys &S = sys[cell_sys[gc]];
     const DevGroup &G = S.g[g];
-    const int cs0 = cell_start[G.cell_off];
-    const int home0 = G.slot_off + cell_start[gc] - cs0;
-    const int lc = gc - G.cell_off;
+    // system constants in registers: the list stores below may not alias them
+    const int cell_off = G.cell_off, slot_off = G.slot_off;
+    const int cs0 = cell_start[cell_off];
+    const int home0 = slot_off + cell_start[gc] - cs0;
+    const int lc = gc - cell_off;
+    const double box0 = S.box[0], box1 = S.box[1], box2 = S.box[2];
+    const double bi0 = S.boxinv[0], bi1 = S.boxinv[1], bi2 = S.boxinv[2];
+    const double rl2 = G.rl2;
+    const bool hybrid = S.hybrid != 0;
     if (threadIdx.x == 0) {
         if (G.all_pairs) {
             const int nc = min(G.ncell, kMaxScanCells);
@@ -247,39 +260,46 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restr
         const int i = home0 + hb + threadIdx.x;
         int4 pi = make_int4(0, 0, 0, 0);
         double4 ri = make_double4(0, 0, 0, 0);
-        int I = -1, li = 0, ex0 = 0, ex1 = 0;
+        int li = 0, ex0 = 0, ex1 = 0;
+        int ex[8] = {-1, -1, -1, -1, -1, -1, -1, -1};   // the first exclusions, in registers
         if (valid) {
             pi = pos4[i];
             ri = r64[i];
-            I = perm[i];
+            const int I = perm[i];
             li = pos_label(pi.w);
             ex0 = excl_off[I];
             ex1 = excl_off[I + 1];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) ex[e] = (ex0 + e < ex1) ? excl[ex0 + e] : -1;
         }
         int cnt = 0;
         for (int q = 0; q < nscan; ++q) {
-            const int gcc = G.cell_off + cells[q];
-            const int j0 = G.slot_off + cell_start[gcc] - cs0, nj = cell_count[gcc];
+            const int gcc = cell_off + cells[q];
+            const int j0 = slot_off + cell_start[gcc] - cs0, nj = cell_count[gcc];
             for (int jb = 0; jb < nj; jb += kCandLds) {
                 const int m = min(kCandLds, nj - jb);
                 __syncthreads();
-                for (int t = threadIdx.x; t < m; t += kListBlock) cand[t] = pos4[j0 + jb + t];
+                for (int t = threadIdx.x; t < m; t += kListBlock) {
+                    cand[t] = pos4[j0 + jb + t];
+                    cand_item[t] = perm[j0 + jb + t];
+                }
                 __syncthreads();
                 if (!valid) continue;
                 for (int t = 0; t < m; ++t) {
                     const int j = j0 + jb + t;
-                    if (j == i) continue;
                     const int4 pj = cand[t];
                     float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
                     float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
                     float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
                     if (!kCubic) { dx *= rho0; dy *= rho1; dz *= rho2; }
                     const float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    if (r2u > hi) continue;
-                    if (r2u >= lo && !(exact_r2(ri, r64[j], S) < G.rl2)) continue;
-                    const int J = perm[j];
-                    bool skip = S.hybrid && li == 3 && pos_label(pj.w) == 3;
-                    for (int e = ex0; e < ex1 && !skip; ++e) skip = (excl[e] == J);
+                    if (r2u > hi || j == i) continue;
+                    if (r2u >= lo && !(exact_r2(ri, r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2)) continue;
+                    const int J = cand_item[t];
+                    bool skip = hybrid && li == 3 && pos_label(pj.w) == 3;
+#pragma unroll
+                    for (int e = 0; e < 8; ++e) skip |= (ex[e] == J);
+                    for (int e = ex0 + 8; e < ex1 && !skip; ++e) skip = (excl[e] == J);
                     if (skip) continue;
                     if (cnt < cap) list[list_pos(i, cnt, cap)] = j;
                     ++cnt;

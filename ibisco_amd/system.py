@@ -600,3 +600,73 @@ def read_dir(path: str) -> System:
         if os.path.exists(va_):
             sysm.virtangles = np.loadtxt(va_, dtype=np.int64, ndmin=2)[:, :3].astype(np.int32) - 1
     return sysm
+
+
+# ----------------------------------------------------------------------------------------------
+# compact archive of a System (numpy .npz, no pickled objects)
+
+
+def to_arrays(sysm: System, f32=False) -> dict:
+    """Flat numpy arrays describing a System (inverse: :func:`from_arrays`).
+    With f32, positions, velocities and tables are kept in single precision,
+    which is exactly what the reference's REAL*4 readers see."""
+    ft = np.float32 if f32 else np.float64
+    out = {
+        "box": np.asarray(sysm.box, np.float64), "type_mass": np.asarray(sysm.type_mass, np.float64),
+        "type_names": np.array(sysm.type_names), "type_label": np.array(sysm.type_label),
+        "itype": np.asarray(sysm.itype, np.int32), "s": np.asarray(sysm.s, ft),
+        "v": np.asarray(sysm.v, ft),
+        "jbond_n": np.array([len(j) for j in sysm.jbond], np.int32),
+        "jbond": np.array([x for j in sysm.jbond for x in j], np.int32),
+        "mol_sizes": np.asarray(sysm.mol_sizes, np.int32), "mol_names": np.array(sysm.mol_names),
+    }
+    c = sysm.control
+    for k, val in vars(c).items():
+        out["ctl_" + k] = np.array(val)
+    for kind in ("bond", "angle", "torsion", "nb", "oop"):
+        tabs = getattr(sysm, kind + "_tables")
+        out[f"{kind}_n"] = np.array(len(tabs), np.int32)
+        for q, t in enumerate(tabs):
+            out[f"{kind}{q}_types"] = np.asarray(t.types, np.int32)
+            out[f"{kind}{q}_x"] = np.asarray(t.x, ft)
+            out[f"{kind}{q}_v"] = np.asarray(t.v, ft)
+            out[f"{kind}{q}_name"] = np.array(t.name)
+    if sysm.nvs:
+        out["vs_type"] = np.asarray(sysm.vs_type, np.int32)
+        out["vs_center"] = np.asarray(sysm.vs_center, np.int32)
+        out["vs_n"] = np.array([len(a) for a in sysm.vs_atoms], np.int32)
+        out["vs_atoms"] = np.concatenate(sysm.vs_atoms).astype(np.int32)
+        if sysm.virtbonds is not None:
+            out["virtbonds"] = np.asarray(sysm.virtbonds, np.int32)
+        if sysm.virtangles is not None:
+            out["virtangles"] = np.asarray(sysm.virtangles, np.int32)
+    return out
+
+
+def from_arrays(a) -> System:
+    off = np.concatenate([[0], np.cumsum(a["jbond_n"])])
+    jb = [list(map(int, a["jbond"][off[i]:off[i + 1]])) for i in range(len(a["jbond_n"]))]
+    c = Control()
+    for k in vars(c):
+        if "ctl_" + k in a:
+            val = a["ctl_" + k].item()
+            setattr(c, k, type(getattr(c, k))(val))
+    sysm = System(box=np.asarray(a["box"]), type_names=[str(x) for x in a["type_names"]],
+                  type_mass=np.asarray(a["type_mass"]), type_label=[str(x) for x in a["type_label"]],
+                  itype=np.asarray(a["itype"], np.int32), s=np.asarray(a["s"], np.float64),
+                  v=np.asarray(a["v"], np.float64), jbond=jb,
+                  mol_sizes=[int(x) for x in a["mol_sizes"]], mol_names=[str(x) for x in a["mol_names"]], control=c)
+    for kind in ("bond", "angle", "torsion", "nb", "oop"):
+        tabs = []
+        for q in range(int(a[f"{kind}_n"])):
+            tabs.append(Table(tuple(int(x) for x in a[f"{kind}{q}_types"]), np.asarray(a[f"{kind}{q}_x"], np.float64),
+                              np.asarray(a[f"{kind}{q}_v"], np.float64), str(a[f"{kind}{q}_name"])))
+        setattr(sysm, kind + "_tables", tabs)
+    if "vs_type" in a:
+        sysm.vs_type = np.asarray(a["vs_type"], np.int32)
+        sysm.vs_center = np.asarray(a["vs_center"], np.int32)
+        vo = np.concatenate([[0], np.cumsum(a["vs_n"])])
+        sysm.vs_atoms = [np.asarray(a["vs_atoms"][vo[i]:vo[i + 1]], np.int32) for i in range(len(a["vs_n"]))]
+        sysm.virtbonds = np.asarray(a["virtbonds"], np.int32) if "virtbonds" in a else None
+        sysm.virtangles = np.asarray(a["virtangles"], np.int32) if "virtangles" in a else None
+    return sysm

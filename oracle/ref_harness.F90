@@ -88,6 +88,9 @@ PROGRAM REF_HARNESS
   TEMP_ATOM = EK(1) * MKTEMP_ATOM
   TEMP_BEAD = EK(2) * MKTEMP_BEAD
 
+  ! SETLIS.f90:17-24 zeroes the per-atom term counts only when bond types exist;
+  ! a system without bonds (twoatombox) would otherwise read them uninitialised
+  NIJK = 0; NOANGLEIJK = 0; NIJKL = 0; FNIJKL = 0; NOOPIJKL = 0
   CALL SETLIS()
   IF (ISTOP == 1) STOP 'Failed in SETLIS'
   IF (IBRDESCR .eq. 0) THEN

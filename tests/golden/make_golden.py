@@ -1,0 +1,202 @@
+"""Writes the golden fixtures in tests/golden/ from the reference's own routines.
+
+Runs only where the reference sources are present (/root/reference): builds
+oracle/_ref with oracle/Makefile.ref, then for every case writes the inputs in
+the reference's file formats, runs oracle/_ref/ref_harness (the reference's
+subroutines driven by this repo's harness), and stores inputs and outputs as
+.npz data: forces, energies and virial after one NEW_FORCE, hashes and counts
+of the neighbour pair sets, exclusion lists and enumerated terms, and states
+after short runs.  The atomisticPA case also keeps, as numbers, the expected
+coordinates and velocities of the first three atoms that the reference's
+testsuite compares against (testsuite/tests.py:47-68).
+
+Inputs are stored in single precision, which is what the reference's REAL*4
+readers see, so the oracle and the reference start from identical numbers.
+
+    python tests/golden/make_golden.py
+"""
+from __future__ import annotations
+
+import hashlib
+import os
+import re
+import resource
+import shutil
+import subprocess
+import sys
+import tempfile
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from ibisco_amd import system, units  # noqa: E402
+
+REF = "/root/reference"
+HERE = os.path.dirname(os.path.abspath(__file__))
+HARNESS = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
+
+
+def pair_hash(p):
+    p = np.ascontiguousarray(np.asarray(p, dtype="<i4").reshape(-1, 2))
+    return hashlib.sha256(p.tobytes()).hexdigest()
+
+
+def canonical_pairs(p):
+    p = np.sort(np.asarray(p, np.int64), axis=1)
+    order = np.lexsort((p[:, 1], p[:, 0]))
+    return p[order].astype(np.int32)
+
+
+def item_hashes(pairs, nitems):
+    """Per item, an 8-byte BLAKE2b digest of its ascending partner list (both directions of every pair)."""
+    p = np.asarray(pairs, np.int64).reshape(-1, 2)
+    d = np.concatenate([p, p[:, ::-1]])
+    d = d[np.lexsort((d[:, 1], d[:, 0]))]
+    cut = np.searchsorted(d[:, 0], np.arange(nitems + 1))
+    part = d[:, 1].astype("<i4")
+    out = np.zeros(nitems, np.uint64)
+    for i in range(nitems):
+        out[i] = int.from_bytes(hashlib.blake2b(part[cut[i]:cut[i + 1]].tobytes(), digest_size=8).digest(), "little")
+    return out
+
+
+def list_hash(counts, flat):
+    h = hashlib.sha256()
+    h.update(np.ascontiguousarray(np.asarray(counts, "<i4")).tobytes())
+    h.update(np.ascontiguousarray(np.asarray(flat, "<i4")).tobytes())
+    return h.hexdigest()
+
+
+def _stack():
+    resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
+
+
+def run_harness(d, *args):
+    env = dict(os.environ, OMP_NUM_THREADS="8", OMP_STACKSIZE="256M")
+    r = subprocess.run([HARNESS, *args], cwd=d, env=env, capture_output=True, text=True, preexec_fn=_stack)
+    if r.returncode != 0:
+        raise RuntimeError(f"harness {args} failed in {d}: {r.stderr[-500:]}")
+
+
+def parse_force_dump(path):
+    b = open(path, "rb").read()
+    off = 0
+
+    def rd(dt, n):
+        nonlocal off
+        a = np.frombuffer(b, dt, n, off)
+        off += a.nbytes
+        return a
+
+    nitems, natoms = (int(x) for x in rd("<i4", 2))
+    out = {"nitems": nitems, "natoms": natoms}
+    out["forces"] = rd("<f4", 3 * nitems).reshape(-1, 3).copy()
+    out["energies"] = rd("<f8", 15).copy()
+    out["pt"] = rd("<f4", 6).copy()
+    rd("<f4", 3 * nitems)
+    for g in (0, 1):
+        n = int(rd("<i4", 1)[0])
+        p = rd("<i4", 2 * n).reshape(-1, 2) - 1
+        cp = canonical_pairs(p)
+        out[f"pairs{g}_count"] = n
+        out[f"pairs{g}_hash"] = pair_hash(cp)
+        out[f"pairs{g}_items"] = item_hashes(cp, nitems)
+    conn = rd("<i4", nitems).copy()
+    flat = rd("<i4", int(conn.sum())) - 1
+    out["excl_hash"] = list_hash(conn, flat)
+    n = int(rd("<i4", 1)[0])
+    ang = rd("<i4", 3 * n).reshape(-1, 3) - 1
+    out["angles_count"] = n
+    out["angles_hash"] = pair_hash(ang.reshape(-1, 2)) if n else ""
+    n = int(rd("<i4", 1)[0])
+    tor = rd("<i4", 4 * n).reshape(-1, 4) - 1
+    out["torsions_count"] = n
+    out["torsions_hash"] = pair_hash(tor.reshape(-1, 2)) if n else ""
+    return out
+
+
+def parse_state(path):
+    b = open(path, "rb").read()
+    nitems, natoms = np.frombuffer(b, "<i4", 2)
+    s = np.frombuffer(b, "<f4", 3 * nitems, 8).reshape(-1, 3)
+    v = np.frombuffer(b, "<f4", 3 * natoms, 8 + 12 * nitems).reshape(-1, 3)
+    return s.copy(), v.copy()
+
+
+def single(sysm):
+    """The system as the reference's REAL*4 readers see it."""
+    return system.from_arrays(system.to_arrays(sysm, f32=True))
+
+
+def testsuite_case(name):
+    d = tempfile.mkdtemp(prefix="ibx_ts_")
+    src = os.path.join(REF, "testsuite", name)
+    for f in os.listdir(src):
+        shutil.copy(os.path.join(src, f), d)
+        os.chmod(os.path.join(d, f), 0o644)
+    with open(os.path.join(d, "control"), "a") as fh:
+        fh.write("mts 0\n")   # rev 79 needs it (RDCONTROL.f90:485-497); the shipped files say "MTS N"
+    sysm = system.read_dir(d)
+    shutil.rmtree(d)
+    return sysm
+
+
+def testsuite_expected():
+    """The atomistic expected arrays of testsuite/tests.py:47-68 (numbers only)."""
+    txt = open(os.path.join(REF, "testsuite", "tests.py")).read()
+    block = txt[txt.index("class reference_atomistic_data"):txt.index("class AtomisticTest")]
+    out = {}
+    for key in ("ref_coords_short", "ref_velos_short", "ref_coords_long", "ref_velos_long"):
+        seg = block[block.index(key):]
+        seg = seg[seg.index("[["):seg.index("]]") + 2]
+        nums = [float(x) for x in re.findall(r"-?\d+\.\d+(?:[eE][-+]?\d+)?", seg)]
+        out[key] = np.array(nums).reshape(3, 3)
+    return out
+
+
+def make_case(name, sysm, runs=()):
+    sysm = single(sysm)
+    d = tempfile.mkdtemp(prefix="ibx_gold_")
+    try:
+        system.write_dir(sysm, d)
+        run_harness(d, "force")
+        res = parse_force_dump(os.path.join(d, "ref_dump.bin"))
+        data = {"ref_" + k: np.array(v) for k, v in res.items()}
+        for n in runs:
+            run_harness(d, "run", str(n))
+            s, v = parse_state(os.path.join(d, "ref_state.bin"))
+            obs = np.fromfile(os.path.join(d, "ref_obs.bin"), "<f8").reshape(n, -1)
+            keep = min(len(v), 200)
+            data[f"run{n}_s"] = s[:keep]
+            data[f"run{n}_v"] = v[:keep]
+            data[f"run{n}_obs"] = obs
+        data.update({"sys_" + k: v for k, v in system.to_arrays(sysm, f32=True).items()})
+        np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **data)
+        print(f"{name}: {res['nitems']} items, {res['pairs0_count']} + {res['pairs1_count']} pairs")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
+def main():
+    subprocess.run(["make", "-s", "-j8", "-f", os.path.join(ROOT, "oracle", "Makefile.ref")], check=True)
+    make_case("cg2000", system.cg_melt(nchains=40, nbeads=50, seed=3, avoid=0.8), runs=(10,))
+    ua = system.ua_melt(nchains=20, nbeads=100, seed=5, avoid=0.6)   # 3 cells per edge at rlist 1.3
+    ua.control.non_bonded = 5
+    make_case("ua2000_excl5", ua)
+    st = system.cg_melt(nchains=30, nbeads=50, seed=9, avoid=0.8)
+    st.s = st.s + np.random.default_rng(9).normal(0.0, 0.03, st.s.shape)   # strained bonds and angles
+    make_case("cg1500_strained", st)
+    tb = testsuite_case("twoatombox")
+    make_case("twoatombox", tb, runs=(40,))
+    make_case("hybridPA", testsuite_case("hybridPA"))
+    apa = testsuite_case("atomisticPA")
+    make_case("atomisticPA", apa, runs=(10, 40))
+    exp = testsuite_expected()
+    np.savez_compressed(os.path.join(HERE, "atomisticPA_expected.npz"), **exp)
+    print("atomisticPA expected arrays:", {k: v.shape for k, v in exp.items()})
+
+
+if __name__ == "__main__":
+    main()

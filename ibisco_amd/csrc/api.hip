@@ -48,7 +48,8 @@ int pad_to(int x, int m) { return (x + m - 1) / m * m; }
 struct GroupDev {
     int nslot_pad = 0, nwg = 0, ncell_total = 0, cap = 0, max_rows = 0;
     bool multi = false, present = false;
-    DBuf<int> mem, mem_cell, cell_count, cell_start, cell_fill, tmp, perm, slot_cell, inv_perm, list, nneb, wg_sys,
+    DBuf<int2> nneb;
+    DBuf<int> mem, mem_cell, cell_count, cell_start, cell_fill, tmp, perm, slot_cell, inv_perm, list, wg_sys,
         cell_sys;
     DBuf<double4> r64;
     DBuf<int4> pos4;
@@ -80,6 +81,7 @@ struct ibx_ctx {
     DBuf<float4> nb_rows, bd_rows;
     DBuf<double> bd_x;
     DBuf<NbTab> nb_tabs;
+    DBuf<float> nb_xoff;        // x_k/bin - k per nonbonded row (non-uniform tables)
     DBuf<BdTab> bd_tabs;
     std::vector<NbTab> h_nb_tabs;
     // rdf
@@ -130,15 +132,30 @@ int upload(ibx_ctx *c, DBuf<T> &b, const std::vector<T> &h) {
     return 0;
 }
 
-void split_float(double x, float &hi, float &lo) {
-    hi = (float)x;
-    lo = (float)(x - (double)hi);
-}
-
 // float4 row k = {F_k, V_k, F_k+1, V_k+1}, k = 0..ndat (row ndat+1 of the table is zero)
 void append_rows(std::vector<float4> &rows, const Table &t) {
     for (int k = 0; k <= t.ndat; ++k)
         rows.push_back(make_float4((float)t.F[k], (float)t.V[k], (float)t.F[k + 1], (float)t.V[k + 1]));
+}
+
+// Header of a nonbonded table in a system of working unit `unit`, and the
+// offsets x_k/bin - k of its rows.  NONBONDED_FORCE.F90:66-68 takes
+// NI = INT(r/bin) and alpha = (r - x_NI)/bin with the file's own abscissas;
+// where those are not exactly k*bin (abscissas rounded to single precision in
+// the file, for instance) the kernel subtracts the offset from alpha.
+NbTab nb_header(const Table &t, double unit, int row_off, std::vector<float> &xoff) {
+    NbTab nt;
+    nt.row_off = row_off;
+    nt.ndat = t.ndat;
+    nt.c = unit / t.bin;
+    nt.bin = (float)t.bin;
+    nt.uniform = 1;
+    for (int k = 0; k <= t.ndat; ++k) {
+        const double o = t.x[k] / t.bin - k;
+        xoff.push_back((float)o);
+        if (std::fabs(o) > 1e-9) nt.uniform = 0;
+    }
+    return nt;
 }
 
 int build_device(ibx_ctx *c) {
@@ -170,6 +187,7 @@ int build_device(ibx_ctx *c) {
     std::vector<float4> hbrows, hnrows;
     std::vector<double> hbx;
     std::vector<NbTab> hntab;
+    std::vector<float> hnxoff;
     std::vector<int> hnmap;
     // contribution slots of the bonded terms each item takes part in, in term order
     std::vector<std::vector<int>> inc(N);
@@ -251,13 +269,7 @@ int build_device(ibx_ctx *c) {
         D.nb_ntab = (int)h.nt.size();
         if (D.nb_ntab > kMaxNbTabs) return fail(c, IBX_E_INVALID, "more than %d nonbonded tables", kMaxNbTabs);
         for (auto &t : h.nt) {
-            NbTab nt;
-            nt.row_off = (int)hnrows.size();
-            nt.ndat = t.ndat;
-            split_float(unit / t.bin, nt.c_hi, nt.c_lo);
-            nt.bin = (float)t.bin;
-            nt.pad = 0.f;
-            hntab.push_back(nt);
+            hntab.push_back(nb_header(t, unit, (int)hnrows.size(), hnxoff));
             append_rows(hnrows, t);
         }
         D.nb_nrow = (int)hnrows.size() - D.nb_row_off;
@@ -347,7 +359,8 @@ int build_device(ibx_ctx *c) {
         (rc = upload(c, c->inc_slot, hinc)) || (rc = upload(c, c->term_mem, htm)) || (rc = upload(c, c->term_meta, hmeta)) ||
         (rc = upload(c, c->term_slot, hslot)) || (rc = upload(c, c->wg_terms, hwgt)) ||
         (rc = upload(c, c->bd_tabs, hbt)) || (rc = upload(c, c->bd_rows, hbrows)) || (rc = upload(c, c->bd_x, hbx)) ||
-        (rc = upload(c, c->nb_rows, hnrows)) || (rc = upload(c, c->nb_tabs, hntab)) || (rc = upload(c, c->nb_map, hnmap)))
+        (rc = upload(c, c->nb_rows, hnrows)) || (rc = upload(c, c->nb_tabs, hntab)) || (rc = upload(c, c->nb_map, hnmap)) ||
+        (rc = upload(c, c->nb_xoff, hnxoff)))
         return rc;
     c->h_nb_tabs = hntab;
     HIPCHK(c, c->fnb.alloc(3 * (size_t)N));
@@ -412,9 +425,16 @@ int build_device(ibx_ctx *c) {
             const double unit = D.unit;
             dg.rl2 = rl * rl;
             const double rl2u = (rl / unit) * (rl / unit);
-            dg.rl2u_lo = (float)(rl2u * (1.0 - 4e-6));
-            dg.rl2u_hi = (float)(rl2u * (1.0 + 4e-6));
+            const double lband = 4e-6 + 16.0 * unit / rl;
+            dg.rl2u_lo = (float)(rl2u * (1.0 - lband));
+            dg.rl2u_hi = (float)(rl2u * (1.0 + lband));
             dg.rc2u = (float)((rc / unit) * (rc / unit));
+            // FP32 r^2 from the fixed-point displacement is within ~2e-7 of the FP64 value, plus the
+            // fixed-point rounding of the two positions (a few units over a distance rc)
+            const double band = 1e-6 + 16.0 * unit / rc;
+            dg.rc2 = rc * rc;
+            dg.rc2u_lo = (float)((rc / unit) * (rc / unit) * (1.0 - band));
+            dg.rc2u_hi = (float)((rc / unit) * (rc / unit) * (1.0 + band));
             if (nm > 0) {
                 any = true;
                 double vol = h.box[0] * h.box[1] * h.box[2];
@@ -494,8 +514,10 @@ void enqueue_forces(ibx_ctx *c, int mode) {
         NbArgs a;
         a.sys = c->sys.p; a.wg_sys = G.wg_sys.p; a.g = g; a.nwg = G.nwg; a.cap = G.cap; a.nitems_total = c->n_items;
         a.max_rows_lds = G.max_rows; a.multi = G.multi; a.perm = G.perm.p; a.pos4 = G.pos4.p; a.list = G.list.p;
-        a.nneb = G.nneb.p; a.rows = c->nb_rows.p; a.tabs = c->nb_tabs.p; a.map = c->nb_map.p; a.fnb = c->fnb.p;
-        a.part = G.part.p;
+        a.nneb = G.nneb.p; a.rows = c->nb_rows.p; a.xoff = c->nb_xoff.p; a.tabs = c->nb_tabs.p; a.map = c->nb_map.p;
+        a.fnb = c->fnb.p; a.part = G.part.p; a.s = c->s.p;
+        a.cubic = true;
+        for (const DevSys &D : c->dsys) a.cubic = a.cubic && D.rho[1] == 1.0f && D.rho[2] == 1.0f;
         launch_nonbonded(c->st, a);
     }
 }
@@ -530,13 +552,8 @@ int check_flags(ibx_ctx *c) {
     int f[8];
     HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, hipMemcpy(f, c->flags.p, sizeof f, hipMemcpyDeviceToHost));
-    if (f[0] > 0) {
-        for (int g = 0; g < 2; ++g) {
-            GroupDev &G = c->gr[g];
-            if (!G.present || f[0] <= G.cap) continue;
-        }
+    if (f[0] > 0)
         return fail(c, IBX_E_LIST_OVERFLOW, "LIST TOO SMALL: %d neighbours exceed the list capacity", f[0]);
-    }
     if (f[1] & 1) return fail(c, IBX_E_TABLE, "torsion angle outside its table");
     if (f[1] & 2) return fail(c, IBX_E_TABLE, "out-of-plane angle outside its table");
     return 0;
@@ -852,13 +869,13 @@ int ibx_set_nb_table(ibx_ctx *c, int sys, int t, int nrow, const double *x, cons
     if (nt.ndat != h.nt[t].ndat) return fail(c, IBX_E_INVALID, "replacement table must keep the row count");
     h.nt[t] = nt;
     std::vector<float4> rows;
+    std::vector<float> xoff;
     append_rows(rows, nt);
     const NbTab &hdr = c->h_nb_tabs[c->dsys[sys].nb_tab_off + t];
+    const NbTab nh = nb_header(nt, c->dsys[sys].unit, hdr.row_off, xoff);
     HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, hipMemcpy(c->nb_rows.p + hdr.row_off, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice));
-    NbTab nh = hdr;
-    split_float(c->dsys[sys].unit / nt.bin, nh.c_hi, nh.c_lo);
-    nh.bin = (float)nt.bin;
+    HIPCHK(c, hipMemcpy(c->nb_xoff.p + hdr.row_off, xoff.data(), xoff.size() * sizeof(float), hipMemcpyHostToDevice));
     c->h_nb_tabs[c->dsys[sys].nb_tab_off + t] = nh;
     HIPCHK(c, hipMemcpy(c->nb_tabs.p + c->dsys[sys].nb_tab_off + t, &nh, sizeof nh, hipMemcpyHostToDevice));
     return 0;
@@ -887,9 +904,10 @@ int ibx_get_pairs(ibx_ctx *c, int sys, int group, int *out, int64_t cap, int64_t
     const DevGroup &dg = c->dsys[sys].g[group];
     if (dg.nmem == 0) return 0;
     HIPCHK(c, hipStreamSynchronize(c->st));
-    std::vector<int> perm(dg.nslot_pad), nneb(dg.nslot_pad), list((size_t)dg.nslot_pad * G.cap);
+    std::vector<int> perm(dg.nslot_pad), list((size_t)dg.nslot_pad * G.cap);
+    std::vector<int2> nneb(dg.nslot_pad);
     HIPCHK(c, hipMemcpy(perm.data(), G.perm.p + dg.slot_off, perm.size() * sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(nneb.data(), G.nneb.p + dg.slot_off, nneb.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, hipMemcpy(nneb.data(), G.nneb.p + dg.slot_off, nneb.size() * sizeof(int2), hipMemcpyDeviceToHost));
     HIPCHK(c, hipMemcpy(list.data(), G.list.p + (size_t)dg.slot_off * G.cap, list.size() * sizeof(int),
                         hipMemcpyDeviceToHost));
     const int o = c->item_off[sys];
@@ -897,8 +915,9 @@ int ibx_get_pairs(ibx_ctx *c, int sys, int group, int *out, int64_t cap, int64_t
     for (int i = 0; i < dg.nslot_pad; ++i) {
         if (perm[i] < 0) continue;
         const int I = perm[i] - o;
-        for (int k = 0; k < std::min(nneb[i], G.cap); ++k) {
-            const int j = list[list_pos(i, k, G.cap)] - dg.slot_off;
+        const int n_in = std::min(nneb[i].x, G.cap), n_out = std::min(nneb[i].y, G.cap - n_in);
+        for (int k = 0; k < n_in + n_out; ++k) {
+            const int j = list[list_pos(i, k < n_in ? k : G.cap - 1 - (k - n_in), G.cap)] - dg.slot_off;
             const int J = perm[j] - o;
             if (I < J) pr.emplace_back(I, J);
         }

@@ -24,13 +24,18 @@ enum TermKind { KBOND = 0, KANGLE = 1, KTORS = 2, KOOP = 3 };
 constexpr int kLanes = 4;                 // lanes per atom in the nonbonded kernel
 constexpr int kAtomsPerWave = kWave / kLanes;
 constexpr int kNbAtomsPerBlock = kBlock / kLanes;
-constexpr int kListBlock = 128;           // threads per workgroup of the list build (one home cell)
-constexpr int kCandLds = 1024;            // candidate positions staged in LDS per chunk
+constexpr int kListBlock = 256;           // threads per workgroup of the list build (one home cell)
+constexpr int kListLanes = 16;            // lanes testing the candidates of one home member
+constexpr int kListAtoms = kListBlock / kListLanes;
+constexpr int kCandLds = 2048;            // candidates staged in LDS per chunk
 constexpr int kMaxScanCells = 1024;       // neighbour cells scanned per home cell
 
 // Position of entry k of sorted slot i in the Verlet list.  Entries of the
 // kAtomsPerWave slots of one wave are interleaved so that the kLanes lanes
 // serving a slot, and the slots of a wave, read consecutive words.
+// A slot's cap entries hold two runs: nneb.x pairs that were inside rc when
+// the list was built, from k = 0 up, and nneb.y pairs with rc <= r < rlist,
+// from k = cap - 1 down.
 __host__ __device__ __forceinline__ size_t list_pos(int slot, int k, int cap) {
     return (size_t)(slot / kAtomsPerWave) * kAtomsPerWave * cap + (size_t)(k / kLanes) * kWave +
            (slot % kAtomsPerWave) * kLanes + (k % kLanes);
@@ -45,6 +50,8 @@ struct DevGroup {
     double rl2;                    // rlist^2 (FP64 list criterion)
     float rl2u_lo, rl2u_hi;        // FP32 pre-classification thresholds, working units^2
     float rc2u;                    // rcut^2 in working units^2
+    float rc2u_lo, rc2u_hi;        // band around rc2u where the FP64 cutoff test decides
+    double rc2;                    // rcut^2 (FP64 cutoff criterion)
     int wg_off, nwg;               // workgroup range of this system in the group's grid
 };
 
@@ -76,9 +83,9 @@ struct DevSys {
 
 struct NbTab {                     // one nonbonded table as seen by a system
     int row_off, ndat;
-    float c_hi, c_lo;              // unit/bin split in two floats
+    double c;                      // unit / bin: table coordinate per working length unit
     float bin;
-    float pad;
+    int uniform;                   // abscissas are k*bin (else nb_xoff holds x_k/bin - k per row)
 };
 
 struct BdTab {                     // one bonded table (bond/angle/torsion/OOP)

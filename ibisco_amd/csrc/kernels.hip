@@ -203,21 +203,27 @@ __global__ void __launch_bounds__(kBlock) k_pack(const DevSys *__restrict__ sys,
 // every member j != i of the 27 neighbouring cells (each cell once) with
 // r^2 < rlist^2, not excluded, not VS-VS.  The set of unordered pairs equals
 // the reference's half list bit for bit: a cheap fixed-point test settles
-// pairs far from the boundary and the FP64 criterion decides the rest.  The
-// candidate cells are staged in LDS one at a time; a list is ordered by cell
-// then by slot, so it is deterministic.
+// pairs far from the boundary and the FP64 criterion decides the rest.
+//
+// The members of the scanned cells are staged in LDS in cell order (in chunks
+// when they do not fit) and kListLanes lanes test one home member against
+// consecutive candidates.  Hits are compacted with a wave ballot, so a slot's
+// runs stay in candidate order and the list is deterministic.  Pairs inside rc
+// when the list is built fill the front run, the others the back run.
 template <bool kCubic>
 __global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ cell_sys,
                                                            int g, const int *__restrict__ perm,
+                                                           const int *__restrict__ inv_perm,
                                                            const int4 *__restrict__ pos4, const double4 *__restrict__ r64,
                                                            const int *__restrict__ cell_start,
                                                            const int *__restrict__ cell_count,
                                                            const int *__restrict__ excl_off, const int *__restrict__ excl,
-                                                           int cap, int *__restrict__ list, int *__restrict__ nneb,
+                                                           int cap, int *__restrict__ list, int2 *__restrict__ nneb,
                                                            int *__restrict__ flags) {
     __shared__ int4 cand[kCandLds];
-    __shared__ int cand_item[kCandLds];
+    __shared__ int cand_slot[kCandLds];
     __shared__ int cells[kMaxScanCells];
+    __shared__ int cbase[kMaxScanCells + 1];
     __shared__ int ncells;
     const int gc = blockIdx.x;
     const int nhome = cell_count[gc];
@@ -233,12 +239,14 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restr
     const double bi0 = S.boxinv[0], bi1 = S.boxinv[1], bi2 = S.boxinv[2];
     const double rl2 = G.rl2;
     const bool hybrid = S.hybrid != 0;
+    const float rho0 = S.rho[0], rho1 = S.rho[1], rho2 = S.rho[2];
+    const float lo = G.rl2u_lo, hi = G.rl2u_hi, rc2u = G.rc2u;
     if (threadIdx.x == 0) {
+        int nc;
         if (G.all_pairs) {
-            const int nc = min(G.ncell, kMaxScanCells);
+            nc = min(G.ncell, kMaxScanCells);
             if (G.ncell > kMaxScanCells) atomicMax(&flags[3], G.ncell);
             for (int q = 0; q < nc; ++q) cells[q] = q;
-            ncells = nc;
         } else {
             const int cz = lc % G.nc[2], cy = (lc / G.nc[2]) % G.nc[1], cx = lc / (G.nc[2] * G.nc[1]);
             for (int q = 0; q < 27; ++q) {
@@ -248,20 +256,45 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restr
                 z += (z < 0) ? G.nc[2] : 0; z -= (z >= G.nc[2]) ? G.nc[2] : 0;
                 cells[q] = (x * G.nc[1] + y) * G.nc[2] + z;
             }
-            ncells = 27;
+            nc = 27;
         }
+        int b = 0;
+        for (int q = 0; q < nc; ++q) {
+            cbase[q] = b;
+            b += cell_count[cell_off + cells[q]];
+        }
+        cbase[nc] = b;
+        ncells = nc;
     }
     __syncthreads();
-    const int nscan = ncells;
-    const float rho0 = S.rho[0], rho1 = S.rho[1], rho2 = S.rho[2];
-    const float lo = G.rl2u_lo, hi = G.rl2u_hi;
-    for (int hb = 0; hb < nhome; hb += kListBlock) {
-        const bool valid = hb + (int)threadIdx.x < nhome;
-        const int i = home0 + hb + threadIdx.x;
+    const int nscan = ncells, total = cbase[nscan];
+    const int nchunk = (total + kCandLds - 1) / kCandLds;
+    // stage candidates [c0, c0 + kCandLds) of the scanned cells, concatenated in cell order
+    auto stage = [&](int c0) {
+        const int c1 = min(c0 + kCandLds, total);
+        for (int t = c0 + (int)threadIdx.x; t < c1; t += kListBlock) {
+            int a = 0, b = nscan;
+            while (b - a > 1) {
+                const int mid = (a + b) >> 1;
+                if (cbase[mid] <= t) a = mid; else b = mid;
+            }
+            const int j = slot_off + cell_start[cell_off + cells[a]] - cs0 + (t - cbase[a]);
+            cand[t - c0] = pos4[j];
+            cand_slot[t - c0] = j;
+        }
+        __syncthreads();
+    };
+    const int sub = threadIdx.x & (kListLanes - 1), grp = threadIdx.x / kListLanes;
+    const int gshift = (threadIdx.x & (kWave - 1)) & ~(kListLanes - 1);   // this group's bits in a wave ballot
+    const unsigned below = (1u << sub) - 1u;
+    if (nchunk == 1) stage(0);
+    for (int hb = 0; hb < nhome; hb += kListAtoms) {
+        const bool valid = hb + grp < nhome;
+        const int i = home0 + hb + grp;
         int4 pi = make_int4(0, 0, 0, 0);
         double4 ri = make_double4(0, 0, 0, 0);
         int li = 0, ex0 = 0, ex1 = 0;
-        int ex[8] = {-1, -1, -1, -1, -1, -1, -1, -1};   // the first exclusions, in registers
+        int ex[8] = {-1, -1, -1, -1, -1, -1, -1, -1};   // slots of the first exclusions
         if (valid) {
             pi = pos4[i];
             ri = r64[i];
@@ -270,45 +303,55 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restr
             ex0 = excl_off[I];
             ex1 = excl_off[I + 1];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) ex[e] = (ex0 + e < ex1) ? excl[ex0 + e] : -1;
+            for (int e = 0; e < 8; ++e) ex[e] = (ex0 + e < ex1) ? inv_perm[excl[ex0 + e]] : -1;
         }
-        int cnt = 0;
-        for (int q = 0; q < nscan; ++q) {
-            const int gcc = cell_off + cells[q];
-            const int j0 = slot_off + cell_start[gcc] - cs0, nj = cell_count[gcc];
-            for (int jb = 0; jb < nj; jb += kCandLds) {
-                const int m = min(kCandLds, nj - jb);
+        int n_in = 0, n_out = 0;
+        for (int ch = 0; ch < nchunk; ++ch) {
+            if (nchunk > 1) {
                 __syncthreads();
-                for (int t = threadIdx.x; t < m; t += kListBlock) {
-                    cand[t] = pos4[j0 + jb + t];
-                    cand_item[t] = perm[j0 + jb + t];
-                }
-                __syncthreads();
-                if (!valid) continue;
-                for (int t = 0; t < m; ++t) {
-                    const int j = j0 + jb + t;
+                stage(ch * kCandLds);
+            }
+            const int m = min(kCandLds, total - ch * kCandLds);
+            for (int t0 = 0; t0 < m; t0 += kListLanes) {
+                const int t = t0 + sub;
+                bool hit = false, inner = false;
+                int j = -1;
+                if (valid && t < m) {
                     const int4 pj = cand[t];
+                    j = cand_slot[t];
                     float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
                     float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
                     float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
                     if (!kCubic) { dx *= rho0; dy *= rho1; dz *= rho2; }
                     const float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    if (r2u > hi || j == i) continue;
-                    if (r2u >= lo && !(exact_r2(ri, r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2)) continue;
-                    const int J = cand_item[t];
-                    bool skip = hybrid && li == 3 && pos_label(pj.w) == 3;
+                    if (r2u <= hi && j != i &&
+                        (r2u < lo || exact_r2(ri, r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2)) {
+                        bool skip = hybrid && li == 3 && pos_label(pj.w) == 3;
 #pragma unroll
-                    for (int e = 0; e < 8; ++e) skip |= (ex[e] == J);
-                    for (int e = ex0 + 8; e < ex1 && !skip; ++e) skip = (excl[e] == J);
-                    if (skip) continue;
-                    if (cnt < cap) list[list_pos(i, cnt, cap)] = j;
-                    ++cnt;
+                        for (int e = 0; e < 8; ++e) skip |= (ex[e] == j);
+                        for (int e = ex0 + 8; e < ex1 && !skip; ++e) skip = (inv_perm[excl[e]] == j);
+                        hit = !skip;
+                        inner = r2u < rc2u;
+                    }
                 }
+                const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
+                const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
+                if (hit) {
+                    if (inner) {
+                        const int k = n_in + __popc(gi & below);
+                        if (k < cap) list[list_pos(i, k, cap)] = j;
+                    } else {
+                        const int k = n_out + __popc(go & below);
+                        if (k < cap) list[list_pos(i, cap - 1 - k, cap)] = j;
+                    }
+                }
+                n_in += __popc(gi);
+                n_out += __popc(go);
             }
         }
-        if (valid) {
-            nneb[i] = cnt;
-            if (cnt > cap) atomicMax(&flags[0], cnt);
+        if (valid && sub == 0) {
+            nneb[i] = make_int2(n_in, n_out);
+            if (n_in + n_out > cap) atomicMax(&flags[0], n_in + n_out);
         }
     }
 }
@@ -321,69 +364,107 @@ __global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restr
 // Tables (F/r and V, interleaved with the next row) are staged in LDS.  Each
 // pair is visited from both ends; energies and virial are halved.
 
-// Contribution of one listed neighbour.  The cutoff test and the force
-// direction use FP32 displacements; for pairs inside the cutoff the distance
-// that indexes the table is recomputed in FP64 from the exact integer
-// displacement, so the interpolated energy is not limited by the FP32
-// rounding of r (which alone would cost ~1e-7 of the total energy).
-template <bool kMulti, int kGroup>
-__device__ __forceinline__ void nb_pair(const int4 pi, const int4 pj, int li, float rc2u, const float rho[3],
-                                        const double rhod[3], const float4 *__restrict__ rows,
-                                        const NbTab *__restrict__ tabs, const int *__restrict__ map, int ntype,
-                                        const NbTab &t0, float &fx, float &fy, float &fz, float (&e)[3],
-                                        float (&w)[6]) {
+// What a pair needs besides the two positions: the tables, the FP32 cutoff
+// band, and what the exact cutoff test reads.
+struct NbCtx {
+    const float4 *rows;            // {F_k, V_k, F_k+1, V_k+1}
+    const float *xoff;             // x_k/bin - k of non-uniform tables (global rows)
+    const NbTab *tabs;
+    const int *map;
+    int ntype, xoff_base;
+    NbTab t0;
+    float rho[3];
+    double rhod[3];
+    float band_lo, band_hi;        // FP32 r^2 below band_lo is inside rc, at or above band_hi outside
+    // exact cutoff test
+    const DevSys *S;
+    const double *s;
+    const int *perm;
+    int n, I, g;
+};
+
+// r^2 < rc^2 decided as NONBONDED_FORCE.F90:58-63 does, from the folded FP64
+// positions (SHIFT.f90:37-43) of items I and J.  Rare: only pairs within FP32
+// rounding of the cutoff get here.
+__device__ __noinline__ bool nb_inside_exact(const DevSys *__restrict__ S, int g, const double *__restrict__ s,
+                                             int n, int I, int J) {
+    const double si[3] = {s[I], s[n + I], s[2 * n + I]}, sj[3] = {s[J], s[n + J], s[2 * n + J]};
+    double ri[3], rj[3];
+    fold3(si, S->box2, ri);
+    fold3(sj, S->box2, rj);
+    const double4 a = make_double4(ri[0], ri[1], ri[2], 0.0), b = make_double4(rj[0], rj[1], rj[2], 0.0);
+    return exact_r2(a, b, *S) < S->g[g].rc2;
+}
+
+// Contribution of one listed neighbour (NONBONDED_FORCE.F90:55-98).  The
+// cutoff test and the force direction use FP32 displacements from the int32
+// fixed-point positions; pairs within FP32 rounding of rc are settled by the
+// FP64 test.  For pairs inside the cutoff the table coordinate r/bin is formed
+// in FP64 (one Newton step from the FP32 root on the exact integer r^2), so
+// alpha = (r - x_k)/bin is exact to FP32 and the interpolated energy is not
+// limited by the rounding of r.
+template <bool kMulti, int kGroup, bool kCubic>
+__device__ __forceinline__ void nb_pair(NbCtx &C, const int4 pi, int j, const int4 pj, int li, float &fx, float &fy,
+                                        float &fz, float (&e)[3], float (&w)[6]) {
     const int ix = (int)((uint32_t)pi.x - (uint32_t)pj.x);
     const int iy = (int)((uint32_t)pi.y - (uint32_t)pj.y);
     const int iz = (int)((uint32_t)pi.z - (uint32_t)pj.z);
-    const float dx = (float)ix * rho[0];
-    const float dy = (float)iy * rho[1];
-    const float dz = (float)iz * rho[2];
+    float dx = (float)ix, dy = (float)iy, dz = (float)iz;
+    if (!kCubic) { dx *= C.rho[0]; dy *= C.rho[1]; dz *= C.rho[2]; }
     const float r2 = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-    if (r2 < rc2u) {
-        NbTab tb = t0;
-        if (kMulti) tb = tabs[map[pos_type(pi.w) * ntype + pos_type(pj.w)]];
-        const double ddx = (double)ix * rhod[0], ddy = (double)iy * rhod[1], ddz = (double)iz * rhod[2];
-        const double rd = sqrt(fma(ddx, ddx, fma(ddy, ddy, ddz * ddz)));
-        const double tt = rd * ((double)tb.c_hi + (double)tb.c_lo);
-        int kr = (int)tt;
-        kr = kr > tb.ndat ? tb.ndat : kr;
-        const float a = (float)(tt - (double)kr);
-        const float4 row = rows[tb.row_off + kr];
-        const float F = fmaf(a, row.z - row.x, row.x);
-        const float V = fmaf(a, row.w - row.y, row.y);
-        fx = fmaf(F, dx, fx);
-        fy = fmaf(F, dy, fy);
-        fz = fmaf(F, dz, fz);
-        if (kGroup == 0) {
-            e[0] += V;
-        } else {
-            const int cls = (li == 2 && pos_label(pj.w) == 2) ? 1 : 2;
-            e[cls] += V;
-        }
-        const float fdx = F * dx, fdy = F * dy;
-        w[0] = fmaf(fdx, dx, w[0]);
-        w[1] = fmaf(fdy, dy, w[1]);
-        w[2] = fmaf(F * dz, dz, w[2]);
-        w[3] = fmaf(fdy, dx, w[3]);
-        w[4] = fmaf(fdx, dz, w[4]);
-        w[5] = fmaf(fdy, dz, w[5]);
+    if (r2 >= C.band_hi) return;
+    if (r2 >= C.band_lo && !nb_inside_exact(C.S, C.g, C.s, C.n, C.I, C.perm[j])) return;
+    NbTab tb = C.t0;
+    if (kMulti) tb = C.tabs[C.map[pos_type(pi.w) * C.ntype + pos_type(pj.w)]];
+    double ddx = (double)ix, ddy = (double)iy, ddz = (double)iz;
+    if (!kCubic) { ddx *= C.rhod[0]; ddy *= C.rhod[1]; ddz *= C.rhod[2]; }
+    const double r2d = fma(ddx, ddx, fma(ddy, ddy, ddz * ddz));
+    const float rs = __builtin_amdgcn_rsqf(r2);
+    const double r0 = r2 > 0.f ? (double)(r2 * rs) : 0.0;      // FP32 root, then one FP64 Newton step
+    const double rd = fma(fma(-r0, r0, r2d), r2 > 0.f ? (double)(0.5f * rs) : 0.0, r0);
+    const double tt = rd * tb.c;
+    int kr = (int)tt;
+    kr = kr > tb.ndat ? tb.ndat : kr;
+    float a = (float)(tt - (double)kr);
+    if (!tb.uniform) a -= C.xoff[C.xoff_base + tb.row_off + kr];
+    const float4 row = C.rows[tb.row_off + kr];
+    const float F = fmaf(a, row.z - row.x, row.x);
+    const float V = fmaf(a, row.w - row.y, row.y);
+    fx = fmaf(F, dx, fx);
+    fy = fmaf(F, dy, fy);
+    fz = fmaf(F, dz, fz);
+    if (kGroup == 0) {
+        e[0] += V;
+    } else {
+        const int cls = (li == 2 && pos_label(pj.w) == 2) ? 1 : 2;
+        e[cls] += V;
     }
+    const float fdx = F * dx, fdy = F * dy;
+    w[0] = fmaf(fdx, dx, w[0]);
+    w[1] = fmaf(fdy, dy, w[1]);
+    w[2] = fmaf(F * dz, dz, w[2]);
+    w[3] = fmaf(fdy, dx, w[3]);
+    w[4] = fmaf(fdx, dz, w[4]);
+    w[5] = fmaf(fdy, dz, w[5]);
 }
 
 // Nonbonded forces (replaces NONBONDED_FORCE.F90:11-109).
 //
-// One thread per sorted slot walks its full list; positions are int32 fixed
+// kLanes lanes per sorted slot walk its full list, first the run of pairs
+// that were inside rc when the list was built, then the rest, so the lanes
+// of a wave mostly agree on the cutoff branch.  Positions are int32 fixed
 // point, so the minimum image is the wrap-around of the integer difference.
-// Tables (F/r and V, interleaved with the next row) are staged in LDS when
-// they fit (kLds), else read through the cache.  Each pair is visited from
-// both ends; energies and virial are halved.  The neighbour loop is unrolled
-// by four so the index and position loads of four pairs are in flight together.
-template <bool kMulti, bool kLds, int kGroup>
+// Tables are staged in LDS when they fit (kLds), else read through the cache.
+// Each pair is visited from both ends; energies and virial are halved.  The
+// loops are unrolled by four so four index and position loads are in flight.
+template <bool kMulti, bool kLds, int kGroup, bool kCubic>
 __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
                                                       const int *__restrict__ perm, const int4 *__restrict__ pos4,
-                                                      const int *__restrict__ list, const int *__restrict__ nneb, int cap,
-                                                      const float4 *__restrict__ nb_rows, const NbTab *__restrict__ nb_tabs,
-                                                      const int *__restrict__ nb_map, int nitems_total,
+                                                      const int *__restrict__ list, const int2 *__restrict__ nneb,
+                                                      int cap, const float4 *__restrict__ nb_rows,
+                                                      const float *__restrict__ nb_xoff,
+                                                      const NbTab *__restrict__ nb_tabs, const int *__restrict__ nb_map,
+                                                      int nitems_total, const double *__restrict__ s,
                                                       float *__restrict__ fnb, double *__restrict__ part) {
     extern __shared__ float4 lds_rows[];
     __shared__ NbTab tabs[kMultiTabsLds(kMulti)];
@@ -404,7 +485,6 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
             map[t] = m < 0 ? 0 : m - S.nb_tab_off;
         }
     __syncthreads();
-    const float4 *rows = kLds ? lds_rows : nb_rows + S.nb_row_off;
 
     const int lane = threadIdx.x & (kWave - 1);
     const int a = lane / kLanes, q = lane % kLanes;
@@ -414,29 +494,53 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
     float e[3] = {0.f, 0.f, 0.f};
     float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
     if (I >= 0) {
+        NbCtx C;
+        C.rows = kLds ? lds_rows : nb_rows + S.nb_row_off;
+        C.xoff = nb_xoff;
+        C.xoff_base = S.nb_row_off;
+        C.tabs = tabs;
+        C.map = map;
+        C.ntype = S.ntype;
+        C.t0 = tabs[0];
+        for (int k = 0; k < 3; ++k) {
+            C.rho[k] = S.rho[k];
+            C.rhod[k] = S.box[k] / S.box[0];
+        }
+        C.band_lo = G.rc2u_lo;
+        C.band_hi = G.rc2u_hi;
+        C.S = &S;
+        C.s = s;
+        C.perm = perm;
+        C.n = nitems_total;
+        C.I = I;
+        C.g = kGroup;
         const int4 pi = pos4[i];
         const int li = pos_label(pi.w);
-        const float rc2u = G.rc2u;
-        const float rho[3] = {S.rho[0], S.rho[1], S.rho[2]};
-        const double rhod[3] = {S.box[0] / S.box[0], S.box[1] / S.box[0], S.box[2] / S.box[0]};
-        const NbTab t0 = tabs[0];
-        const int nt = S.ntype;
-        const int n = min(nneb[i], cap);
-        const int *lp = list + list_pos(i, q, cap);   // entry q + kLanes*m is at lp[m*kWave]
-        const int nm = n > q ? (n - q + kLanes - 1) / kLanes : 0;
-        int m = 0;
-        for (; m + 4 <= nm; m += 4) {
-            const int j0 = lp[(size_t)m * kWave], j1 = lp[(size_t)(m + 1) * kWave];
-            const int j2 = lp[(size_t)(m + 2) * kWave], j3 = lp[(size_t)(m + 3) * kWave];
-            const int4 p0 = pos4[j0], p1 = pos4[j1], p2 = pos4[j2], p3 = pos4[j3];
-            nb_pair<kMulti, kGroup>(pi, p0, li, rc2u, rho, rhod, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
-            nb_pair<kMulti, kGroup>(pi, p1, li, rc2u, rho, rhod, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
-            nb_pair<kMulti, kGroup>(pi, p2, li, rc2u, rho, rhod, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
-            nb_pair<kMulti, kGroup>(pi, p3, li, rc2u, rho, rhod, rows, tabs, map, nt, t0, fx, fy, fz, e, w);
+        const int2 nn = nneb[i];
+        const int n_in = min(nn.x, cap), n_out = min(nn.y, cap - n_in);
+        // front run: entry q + kLanes*m at lp[m*kWave]; back run: entry q + kLanes*m at lb[-m*kWave]
+        const int *lp = list + list_pos(i, q, cap);
+        const int *lb = list + list_pos(i, cap - 1 - q, cap);
+        for (int run = 0; run < 2; ++run) {
+            const int n = run == 0 ? n_in : n_out;
+            const int *l0 = run == 0 ? lp : lb;
+            const ptrdiff_t step = run == 0 ? kWave : -kWave;
+            const int nm = n > q ? (n - q + kLanes - 1) / kLanes : 0;
+            int m = 0;
+            for (; m + 4 <= nm; m += 4) {
+                const int j0 = l0[m * step], j1 = l0[(m + 1) * step];
+                const int j2 = l0[(m + 2) * step], j3 = l0[(m + 3) * step];
+                const int4 p0 = pos4[j0], p1 = pos4[j1], p2 = pos4[j2], p3 = pos4[j3];
+                nb_pair<kMulti, kGroup, kCubic>(C, pi, j0, p0, li, fx, fy, fz, e, w);
+                nb_pair<kMulti, kGroup, kCubic>(C, pi, j1, p1, li, fx, fy, fz, e, w);
+                nb_pair<kMulti, kGroup, kCubic>(C, pi, j2, p2, li, fx, fy, fz, e, w);
+                nb_pair<kMulti, kGroup, kCubic>(C, pi, j3, p3, li, fx, fy, fz, e, w);
+            }
+            for (; m < nm; ++m) {
+                const int j = l0[m * step];
+                nb_pair<kMulti, kGroup, kCubic>(C, pi, j, pos4[j], li, fx, fy, fz, e, w);
+            }
         }
-        for (; m < nm; ++m)
-            nb_pair<kMulti, kGroup>(pi, pos4[lp[(size_t)m * kWave]], li, rc2u, rho, rhod, rows, tabs, map, nt, t0, fx, fy, fz,
-                                    e, w);
     }
     // the kLanes lanes of a slot hold partial forces: butterfly-sum them
     for (int off = 1; off < kLanes; off <<= 1) {
@@ -952,7 +1056,7 @@ __global__ void __launch_bounds__(kBlock) k_finalize(const DevSys *__restrict__ 
 // Radial distribution function histogram over the current list (group 0).
 __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
                                                 const int *__restrict__ perm, const int4 *__restrict__ pos4,
-                                                const int *__restrict__ list, const int *__restrict__ nneb, int cap,
+                                                const int *__restrict__ list, const int2 *__restrict__ nneb, int cap,
                                                 int nbin, float inv_dr_u, unsigned long long *__restrict__ hist) {
     extern __shared__ unsigned int h[];
     const int s_id = wg_sys[blockIdx.x];
@@ -962,9 +1066,10 @@ __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, 
     const int i = blockIdx.x * kBlock + threadIdx.x;
     if (perm[i] >= 0) {
         const int4 pi = pos4[i];
-        const int nn = min(nneb[i], cap);
-        for (int k = 0; k < nn; ++k) {
-            const int4 pj = pos4[list[list_pos(i, k, cap)]];
+        const int2 nn = nneb[i];
+        const int n_in = min(nn.x, cap), n_out = min(nn.y, cap - n_in);
+        for (int k = 0; k < n_in + n_out; ++k) {
+            const int4 pj = pos4[list[list_pos(i, k < n_in ? k : cap - 1 - (k - n_in), cap)]];
             const float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x) * S.rho[0];
             const float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y) * S.rho[1];
             const float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z) * S.rho[2];
@@ -997,12 +1102,12 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
                        a.itype, a.label, a.r64, a.pos4, a.inv_perm);
     if (a.cubic)
         hipLaunchKernelGGL(k_build_list<true>, dim3(a.ncell_total), dim3(kListBlock), 0, st, a.sys, a.cell_sys, a.g,
-                           a.perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap, a.list, a.nneb,
-                           a.flags);
+                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap,
+                           a.list, a.nneb, a.flags);
     else
         hipLaunchKernelGGL(k_build_list<false>, dim3(a.ncell_total), dim3(kListBlock), 0, st, a.sys, a.cell_sys, a.g,
-                           a.perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap, a.list, a.nneb,
-                           a.flags);
+                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap,
+                           a.list, a.nneb, a.flags);
 }
 
 void launch_pack_only(hipStream_t st, const RebuildArgs &a) {
@@ -1010,10 +1115,17 @@ void launch_pack_only(hipStream_t st, const RebuildArgs &a) {
                        a.itype, a.label, a.r64, a.pos4, a.inv_perm);
 }
 
-template <bool M, bool L, int G>
+template <bool M, bool L, int G, bool K>
 static void nb_go(hipStream_t st, const NbArgs &a, size_t lds) {
-    hipLaunchKernelGGL((k_nonbonded<M, L, G>), dim3(a.nwg * (kBlock / kNbAtomsPerBlock)), dim3(kBlock), lds, st, a.sys, a.wg_sys, a.perm, a.pos4,
-                       a.list, a.nneb, a.cap, a.rows, a.tabs, a.map, a.nitems_total, a.fnb, a.part);
+    hipLaunchKernelGGL((k_nonbonded<M, L, G, K>), dim3(a.nwg * (kBlock / kNbAtomsPerBlock)), dim3(kBlock), lds, st,
+                       a.sys, a.wg_sys, a.perm, a.pos4, a.list, a.nneb, a.cap, a.rows, a.xoff, a.tabs, a.map,
+                       a.nitems_total, a.s, a.fnb, a.part);
+}
+
+template <bool M, bool L, int G>
+static void nb_cubic(hipStream_t st, const NbArgs &a, size_t lds) {
+    if (a.cubic) nb_go<M, L, G, true>(st, a, lds);
+    else nb_go<M, L, G, false>(st, a, lds);
 }
 
 template <int G>
@@ -1021,11 +1133,11 @@ static void nb_dispatch(hipStream_t st, const NbArgs &a) {
     const bool lds = a.max_rows_lds > 0;
     const size_t bytes = lds ? (size_t)a.max_rows_lds * sizeof(float4) : 0;
     if (a.multi) {
-        if (lds) nb_go<true, true, G>(st, a, bytes);
-        else nb_go<true, false, G>(st, a, bytes);
+        if (lds) nb_cubic<true, true, G>(st, a, bytes);
+        else nb_cubic<true, false, G>(st, a, bytes);
     } else {
-        if (lds) nb_go<false, true, G>(st, a, bytes);
-        else nb_go<false, false, G>(st, a, bytes);
+        if (lds) nb_cubic<false, true, G>(st, a, bytes);
+        else nb_cubic<false, false, G>(st, a, bytes);
     }
 }
 
@@ -1065,7 +1177,7 @@ void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *
 }
 
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
-                const int *list, const int *nneb, int cap, int nbin, float inv_dr_u, unsigned long long *hist) {
+                const int *list, const int2 *nneb, int cap, int nbin, float inv_dr_u, unsigned long long *hist) {
     hipLaunchKernelGGL(k_rdf, dim3(nwg), dim3(kBlock), (size_t)nbin * sizeof(unsigned int), st, sys, wg_sys, perm, pos4,
                        list, nneb, cap, nbin, inv_dr_u, hist);
 }

@@ -20,19 +20,24 @@ struct RebuildArgs {
     int4 *pos4;
     int *inv_perm;
     const int *excl_off, *excl;
-    int *list, *nneb, *flags;
+    int *list;
+    int2 *nneb;
+    int *flags;
 };
 
 struct NbArgs {
     const DevSys *sys;
     const int *wg_sys;
     int g, nwg, cap, nitems_total, max_rows_lds;
-    bool multi;
+    bool multi, cubic;
     const int *perm;
     const int4 *pos4;
-    const int *list, *nneb;
+    const int *list;
+    const int2 *nneb;
     const float4 *rows;
+    const float *xoff;
     const NbTab *tabs;
+    const double *s;
     const int *map;
     float *fnb;
     double *part;
@@ -79,6 +84,6 @@ void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *
                      const double *pbd, const double *pmv, double *obs, double *avg, double *sysstate, const int *step,
                      int mode);
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
-                const int *list, const int *nneb, int cap, int nbin, float inv_dr_u, unsigned long long *hist);
+                const int *list, const int2 *nneb, int cap, int nbin, float inv_dr_u, unsigned long long *hist);
 
 }  // namespace ibx

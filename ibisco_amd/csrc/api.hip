@@ -413,11 +413,11 @@ int build_device(ibx_ctx *c) {
             const double rc = h.rcut[g], rl = h.rlist[g];
             int nc[3];
             for (int k = 0; k < 3; ++k) {
-                nc[k] = (int)(h.box[k] / rl);           // ALLOCATEVAR2.f90:19-27
+                nc[k] = (int)(2.0 * h.box[k] / rl);     // cells of edge >= rlist/2 (the reference: rlist)
                 if (nc[k] < 1) nc[k] = 1;
                 dg.nc[k] = nc[k];
             }
-            dg.all_pairs = (nc[0] < 3 || nc[1] < 3 || nc[2] < 3) ? 1 : 0;
+            dg.all_pairs = (nc[0] < kMinCells || nc[1] < kMinCells || nc[2] < kMinCells) ? 1 : 0;
             dg.ncell = nc[0] * nc[1] * nc[2];
             dg.cell_off = cell;
             for (int q = 0; q < dg.ncell; ++q) cellsys.push_back(s);

@@ -24,11 +24,10 @@ enum TermKind { KBOND = 0, KANGLE = 1, KTORS = 2, KOOP = 3 };
 constexpr int kLanes = 4;                 // lanes per atom in the nonbonded kernel
 constexpr int kAtomsPerWave = kWave / kLanes;
 constexpr int kNbAtomsPerBlock = kBlock / kLanes;
-constexpr int kListBlock = 256;           // threads per workgroup of the list build (one home cell)
-constexpr int kListLanes = 16;            // lanes testing the candidates of one home member
-constexpr int kListAtoms = kListBlock / kListLanes;
-constexpr int kCandLds = 2048;            // candidates staged in LDS per chunk
-constexpr int kMaxScanCells = 1024;       // neighbour cells scanned per home cell
+constexpr int kListLanes = 16;            // lanes testing the candidates of one member in the list build
+constexpr int kListAtoms = kBlock / kListLanes;
+constexpr int kListRanges = 64;           // candidate slot ranges per member (25 columns, at most 2 each)
+constexpr int kMinCells = 5;              // cells per axis for the +-2 cell stencil
 
 // Position of entry k of sorted slot i in the Verlet list.  Entries of the
 // kAtomsPerWave slots of one wave are interleaved so that the kLanes lanes
@@ -46,7 +45,7 @@ struct DevGroup {
     int mem_off;                   // offset into the member list
     int cell_off, ncell;           // global cell range
     int nc[3];
-    int all_pairs;                 // fewer than 3 cells in a dimension: scan every cell once
+    int all_pairs;                 // fewer than kMinCells cells along an axis: every member scans the group
     double rl2;                    // rlist^2 (FP64 list criterion)
     float rl2u_lo, rl2u_hi;        // FP32 pre-classification thresholds, working units^2
     float rc2u;                    // rcut^2 in working units^2

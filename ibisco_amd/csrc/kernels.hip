@@ -199,160 +199,201 @@ __global__ void __launch_bounds__(kBlock) k_pack(const DevSys *__restrict__ sys,
     inv_perm[it] = slot;
 }
 
-// Full Verlet list of every member of one home cell (one workgroup per cell):
-// every member j != i of the 27 neighbouring cells (each cell once) with
-// r^2 < rlist^2, not excluded, not VS-VS.  The set of unordered pairs equals
-// the reference's half list bit for bit: a cheap fixed-point test settles
-// pairs far from the boundary and the FP64 criterion decides the rest.
+// Gap along one axis between a point at fraction f of its cell and the cell d
+// cells away (cell edge a); 0 for its own cell.
+__device__ __forceinline__ float cell_gap(int d, float f, float a) {
+    return d > 0 ? ((float)d - f) * a : (d < 0 ? ((float)(-d - 1) + f) * a : 0.f);
+}
+
+// Full Verlet list of every group member (replaces NEW_NEIGHBOUR_WITHLIST.f90:5-127
+// with LINKS/MAPS): every member j != i with r^2 < rlist^2, not excluded, not
+// VS-VS.  The set of unordered pairs equals the reference's half list bit for
+// bit: a fixed-point test settles pairs far from the boundary and the FP64
+// criterion of the reference decides the rest.
 //
-// The members of the scanned cells are staged in LDS in cell order (in chunks
-// when they do not fit) and kListLanes lanes test one home member against
-// consecutive candidates.  Hits are compacted with a wave ballot, so a slot's
-// runs stay in candidate order and the list is deterministic.  Pairs inside rc
-// when the list is built fill the front run, the others the back run.
+// Cells have an edge of at least rlist/2 and are stored x-major, so the cells
+// of one (x, y) column are consecutive slots.  For member i the columns within
+// two cells whose distance from i is below rlist give at most two slot ranges
+// each (the z extent reachable from i, split where it wraps), and kListLanes
+// lanes walk their concatenation, 16 candidates per step.  Hits are compacted
+// with a wave ballot, so each list is in candidate order and deterministic.
+// Pairs inside rc when the list is built fill the front run, the others the
+// back run.  With fewer than five cells along an axis every member scans the
+// whole group once.
 template <bool kCubic>
-__global__ void __launch_bounds__(kListBlock) k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ cell_sys,
-                                                           int g, const int *__restrict__ perm,
-                                                           const int *__restrict__ inv_perm,
-                                                           const int4 *__restrict__ pos4, const double4 *__restrict__ r64,
-                                                           const int *__restrict__ cell_start,
-                                                           const int *__restrict__ cell_count,
-                                                           const int *__restrict__ excl_off, const int *__restrict__ excl,
-                                                           int cap, int *__restrict__ list, int2 *__restrict__ nneb,
-                                                           int *__restrict__ flags) {
-    __shared__ int4 cand[kCandLds];
-    __shared__ int cand_slot[kCandLds];
-    __shared__ int cells[kMaxScanCells];
-    __shared__ int cbase[kMaxScanCells + 1];
-    __shared__ int ncells;
-    const int gc = blockIdx.x;
-    const int nhome = cell_count[gc];
-    if (nhome == 0) return;
-    const DevSys &S = sys[cell_sys[gc]];
+__global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
+                                                       int g, const int *__restrict__ perm,
+                                                       const int *__restrict__ inv_perm,
+                                                       const int4 *__restrict__ pos4, const double4 *__restrict__ r64,
+                                                       const int *__restrict__ cell_start,
+                                                       const int *__restrict__ cell_count,
+                                                       const int *__restrict__ excl_off, const int *__restrict__ excl,
+                                                       int cap, int *__restrict__ list, int2 *__restrict__ nneb,
+                                                       int *__restrict__ flags) {
+    __shared__ int rbase[kListAtoms][kListRanges];   // slot of candidate p is rbase[r] + p
+    __shared__ int rend[kListAtoms][kListRanges];    // candidates up to the end of range r
+    const int slot0 = blockIdx.x * kListAtoms;
+    const DevSys &S = sys[wg_sys[slot0 / kBlock]];
     const DevGroup &G = S.g[g];
-    // system constants in registers: the list stores below may not alias them
-    const int cell_off = G.cell_off, slot_off = G.slot_off;
+    const int sub = threadIdx.x & (kListLanes - 1), grp = threadIdx.x / kListLanes;
+    const int i = slot0 + grp;
+    const int I = perm[i];
+    const bool valid = I >= 0;
+    const int slot_off = G.slot_off, cell_off = G.cell_off;
     const int cs0 = cell_start[cell_off];
-    const int home0 = slot_off + cell_start[gc] - cs0;
-    const int lc = gc - cell_off;
     const double box0 = S.box[0], box1 = S.box[1], box2 = S.box[2];
     const double bi0 = S.boxinv[0], bi1 = S.boxinv[1], bi2 = S.boxinv[2];
     const double rl2 = G.rl2;
     const bool hybrid = S.hybrid != 0;
     const float rho0 = S.rho[0], rho1 = S.rho[1], rho2 = S.rho[2];
     const float lo = G.rl2u_lo, hi = G.rl2u_hi, rc2u = G.rc2u;
-    if (threadIdx.x == 0) {
-        int nc;
-        if (G.all_pairs) {
-            nc = min(G.ncell, kMaxScanCells);
-            if (G.ncell > kMaxScanCells) atomicMax(&flags[3], G.ncell);
-            for (int q = 0; q < nc; ++q) cells[q] = q;
-        } else {
-            const int cz = lc % G.nc[2], cy = (lc / G.nc[2]) % G.nc[1], cx = lc / (G.nc[2] * G.nc[1]);
-            for (int q = 0; q < 27; ++q) {
-                int x = cx + q / 9 - 1, y = cy + (q / 3) % 3 - 1, z = cz + q % 3 - 1;
-                x += (x < 0) ? G.nc[0] : 0; x -= (x >= G.nc[0]) ? G.nc[0] : 0;
-                y += (y < 0) ? G.nc[1] : 0; y -= (y >= G.nc[1]) ? G.nc[1] : 0;
-                z += (z < 0) ? G.nc[2] : 0; z -= (z >= G.nc[2]) ? G.nc[2] : 0;
-                cells[q] = (x * G.nc[1] + y) * G.nc[2] + z;
+
+    int4 pi = make_int4(0, 0, 0, 0);
+    double4 ri = make_double4(0, 0, 0, 0);
+    if (valid) {
+        pi = pos4[i];
+        ri = r64[i];
+    }
+    int T = 0;   // candidates of this member
+    if (G.all_pairs) {
+        if (sub == 0) {
+            rbase[grp][0] = slot_off;
+            rend[grp][0] = G.nmem;
+        }
+        T = valid ? G.nmem : 0;
+    } else {
+        const int nc0 = G.nc[0], nc1 = G.nc[1], nc2 = G.nc[2];
+        const double rv[3] = {ri.x, ri.y, ri.z}, bi[3] = {bi0, bi1, bi2};
+        const int ncv[3] = {nc0, nc1, nc2};
+        int c[3];
+        float f[3], a[3];
+        for (int k = 0; k < 3; ++k) {   // home cell as k_cell_count assigns it, and the position inside it
+            const double u = (rv[k] * bi[k] + 0.5) * ncv[k];
+            int ck = (int)u;
+            ck = ck < 0 ? 0 : (ck >= ncv[k] ? ncv[k] - 1 : ck);
+            c[k] = ck;
+            f[k] = fminf(fmaxf((float)(u - ck), 0.f), 1.f);
+            a[k] = (float)(S.box[k] / ncv[k]);
+        }
+        // reach: rlist with a margin, so a pruned cell is always beyond rlist
+        const float reach2 = (float)(rl2 * (1.0 + 1e-4));
+        int st[4], len[4], np = 0;   // pieces of this lane's two columns, in column order
+        int n_half[2] = {0, 0};
+        for (int h = 0; h < 2; ++h) {
+            const int q = sub + h * kListLanes;
+            if (!valid || q >= 25) continue;
+            const int dx = q / 5 - 2, dy = q % 5 - 2;
+            const float gx = cell_gap(dx, f[0], a[0]), gy = cell_gap(dy, f[1], a[1]);
+            const float gxy = gx * gx + gy * gy;
+            if (gxy >= reach2) continue;
+            int dz0 = 0, dz1 = 0;
+            for (int d = -2; d < 0; ++d) {
+                const float gz = cell_gap(d, f[2], a[2]);
+                if (gxy + gz * gz < reach2) { dz0 = d; break; }
             }
-            nc = 27;
+            for (int d = 2; d > 0; --d) {
+                const float gz = cell_gap(d, f[2], a[2]);
+                if (gxy + gz * gz < reach2) { dz1 = d; break; }
+            }
+            int x = c[0] + dx, y = c[1] + dy;
+            x += x < 0 ? nc0 : 0; x -= x >= nc0 ? nc0 : 0;
+            y += y < 0 ? nc1 : 0; y -= y >= nc1 ? nc1 : 0;
+            const int col = cell_off + (x * nc1 + y) * nc2;
+            const int z0 = c[2] + dz0, z1 = c[2] + dz1;
+            int za[2], zb[2], nz = 0;
+            if (z0 < 0) { za[0] = z0 + nc2; zb[0] = nc2 - 1; za[1] = 0; zb[1] = z1; nz = 2; }
+            else if (z1 >= nc2) { za[0] = z0; zb[0] = nc2 - 1; za[1] = 0; zb[1] = z1 - nc2; nz = 2; }
+            else { za[0] = z0; zb[0] = z1; nz = 1; }
+            for (int k = 0; k < nz; ++k) {
+                const int b = cell_start[col + za[k]];
+                const int e = cell_start[col + zb[k]] + cell_count[col + zb[k]];
+                st[np] = slot_off + b - cs0;
+                len[np] = e - b;
+                ++np;
+                ++n_half[h];
+            }
         }
-        int b = 0;
-        for (int q = 0; q < nc; ++q) {
-            cbase[q] = b;
-            b += cell_count[cell_off + cells[q]];
+        // place the pieces: first-half columns by lane, then second-half columns by lane
+        int cnt[2] = {n_half[0], n_half[1]}, lsum[2] = {0, 0};
+        for (int k = 0; k < np; ++k) lsum[k < n_half[0] ? 0 : 1] += len[k];
+        int pos[2], cum[2], tot_n[2], tot_c[2];
+        for (int h = 0; h < 2; ++h) {
+            int xn = cnt[h], xc = lsum[h];
+            for (int off = 1; off < kListLanes; off <<= 1) {
+                const int yn = __shfl_up(xn, off, kListLanes), yc = __shfl_up(xc, off, kListLanes);
+                if (sub >= off) { xn += yn; xc += yc; }
+            }
+            tot_n[h] = __shfl(xn, kListLanes - 1, kListLanes);
+            tot_c[h] = __shfl(xc, kListLanes - 1, kListLanes);
+            pos[h] = xn - cnt[h];
+            cum[h] = xc - lsum[h];
         }
-        cbase[nc] = b;
-        ncells = nc;
+        pos[1] += tot_n[0];
+        cum[1] += tot_c[0];
+        for (int k = 0; k < np; ++k) {
+            const int h = k < n_half[0] ? 0 : 1;
+            const int r = pos[h]++;
+            if (r < kListRanges) {
+                rbase[grp][r] = st[k] - cum[h];
+                cum[h] += len[k];
+                rend[grp][r] = cum[h];
+            }
+        }
+        T = tot_c[0] + tot_c[1];
+        if (tot_n[0] + tot_n[1] > kListRanges) T = 0;   // cannot happen with 25 columns x 2 pieces
     }
     __syncthreads();
-    const int nscan = ncells, total = cbase[nscan];
-    const int nchunk = (total + kCandLds - 1) / kCandLds;
-    // stage candidates [c0, c0 + kCandLds) of the scanned cells, concatenated in cell order
-    auto stage = [&](int c0) {
-        const int c1 = min(c0 + kCandLds, total);
-        for (int t = c0 + (int)threadIdx.x; t < c1; t += kListBlock) {
-            int a = 0, b = nscan;
-            while (b - a > 1) {
-                const int mid = (a + b) >> 1;
-                if (cbase[mid] <= t) a = mid; else b = mid;
-            }
-            const int j = slot_off + cell_start[cell_off + cells[a]] - cs0 + (t - cbase[a]);
-            cand[t - c0] = pos4[j];
-            cand_slot[t - c0] = j;
-        }
-        __syncthreads();
-    };
-    const int sub = threadIdx.x & (kListLanes - 1), grp = threadIdx.x / kListLanes;
+
+    int ex0 = 0, ex1 = 0, li = 0;
+    int ex[8] = {-1, -1, -1, -1, -1, -1, -1, -1};   // slots of the first exclusions
+    if (valid) {
+        li = pos_label(pi.w);
+        ex0 = excl_off[I];
+        ex1 = excl_off[I + 1];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) ex[e] = (ex0 + e < ex1) ? inv_perm[excl[ex0 + e]] : -1;
+    }
+    int tmax = T;   // the wave walks as far as its longest member
+    for (int off = kListLanes; off < kWave; off <<= 1) tmax = max(tmax, __shfl_xor(tmax, off, kWave));
     const int gshift = (threadIdx.x & (kWave - 1)) & ~(kListLanes - 1);   // this group's bits in a wave ballot
     const unsigned below = (1u << sub) - 1u;
-    if (nchunk == 1) stage(0);
-    for (int hb = 0; hb < nhome; hb += kListAtoms) {
-        const bool valid = hb + grp < nhome;
-        const int i = home0 + hb + grp;
-        int4 pi = make_int4(0, 0, 0, 0);
-        double4 ri = make_double4(0, 0, 0, 0);
-        int li = 0, ex0 = 0, ex1 = 0;
-        int ex[8] = {-1, -1, -1, -1, -1, -1, -1, -1};   // slots of the first exclusions
-        if (valid) {
-            pi = pos4[i];
-            ri = r64[i];
-            const int I = perm[i];
-            li = pos_label(pi.w);
-            ex0 = excl_off[I];
-            ex1 = excl_off[I + 1];
+    int n_in = 0, n_out = 0, r = 0;
+    for (int t0 = 0; t0 < tmax; t0 += kListLanes) {
+        const int p = t0 + sub;
+        bool hit = false, inner = false;
+        int j = -1;
+        if (p < T) {
+            while (p >= rend[grp][r]) ++r;
+            j = rbase[grp][r] + p;
+            const int4 pj = pos4[j];
+            float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
+            float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
+            float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
+            if (!kCubic) { dx *= rho0; dy *= rho1; dz *= rho2; }
+            const float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            bool cand = r2u < lo;
+            if (r2u >= lo && r2u <= hi) cand = exact_r2(ri, r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2;
+            bool skip = (j == i) || (hybrid && li == 3 && pos_label(pj.w) == 3);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) ex[e] = (ex0 + e < ex1) ? inv_perm[excl[ex0 + e]] : -1;
+            for (int e = 0; e < 8; ++e) skip |= (ex[e] == j);
+            if (cand && !skip && ex1 - ex0 > 8)
+                for (int e = ex0 + 8; e < ex1; ++e) skip |= (inv_perm[excl[e]] == j);
+            hit = cand && !skip;
+            inner = r2u < rc2u;
         }
-        int n_in = 0, n_out = 0;
-        for (int ch = 0; ch < nchunk; ++ch) {
-            if (nchunk > 1) {
-                __syncthreads();
-                stage(ch * kCandLds);
-            }
-            const int m = min(kCandLds, total - ch * kCandLds);
-            for (int t0 = 0; t0 < m; t0 += kListLanes) {
-                const int t = t0 + sub;
-                bool hit = false, inner = false;
-                int j = -1;
-                if (valid && t < m) {
-                    const int4 pj = cand[t];
-                    j = cand_slot[t];
-                    float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
-                    float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
-                    float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
-                    if (!kCubic) { dx *= rho0; dy *= rho1; dz *= rho2; }
-                    const float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                    if (r2u <= hi && j != i &&
-                        (r2u < lo || exact_r2(ri, r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2)) {
-                        bool skip = hybrid && li == 3 && pos_label(pj.w) == 3;
-#pragma unroll
-                        for (int e = 0; e < 8; ++e) skip |= (ex[e] == j);
-                        for (int e = ex0 + 8; e < ex1 && !skip; ++e) skip = (inv_perm[excl[e]] == j);
-                        hit = !skip;
-                        inner = r2u < rc2u;
-                    }
-                }
-                const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
-                const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
-                if (hit) {
-                    if (inner) {
-                        const int k = n_in + __popc(gi & below);
-                        if (k < cap) list[list_pos(i, k, cap)] = j;
-                    } else {
-                        const int k = n_out + __popc(go & below);
-                        if (k < cap) list[list_pos(i, cap - 1 - k, cap)] = j;
-                    }
-                }
-                n_in += __popc(gi);
-                n_out += __popc(go);
-            }
+        const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
+        const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
+        if (hit) {
+            const unsigned k = inner ? (unsigned)n_in + __popc(gi & below) : (unsigned)(cap - 1 - n_out - __popc(go & below));
+            const bool fits = inner ? (int)k < cap : n_out + __popc(go & below) < cap;
+            if (fits) list[list_pos(i, (int)k, cap)] = j;
         }
-        if (valid && sub == 0) {
-            nneb[i] = make_int2(n_in, n_out);
-            if (n_in + n_out > cap) atomicMax(&flags[0], n_in + n_out);
-        }
+        n_in += __popc(gi);
+        n_out += __popc(go);
+    }
+    if (valid && sub == 0) {
+        nneb[i] = make_int2(n_in, n_out);
+        if (n_in + n_out > cap) atomicMax(&flags[0], n_in + n_out);
     }
 }
 
@@ -1100,14 +1141,15 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
                        a.cell_count, a.tmp, a.mem, a.perm, a.slot_cell, a.flags);
     hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
                        a.itype, a.label, a.r64, a.pos4, a.inv_perm);
+    const int nwg_list = a.nwg * (kBlock / kListAtoms);
     if (a.cubic)
-        hipLaunchKernelGGL(k_build_list<true>, dim3(a.ncell_total), dim3(kListBlock), 0, st, a.sys, a.cell_sys, a.g,
-                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap,
-                           a.list, a.nneb, a.flags);
+        hipLaunchKernelGGL(k_build_list<true>, dim3(nwg_list), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm,
+                           a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap, a.list,
+                           a.nneb, a.flags);
     else
-        hipLaunchKernelGGL(k_build_list<false>, dim3(a.ncell_total), dim3(kListBlock), 0, st, a.sys, a.cell_sys, a.g,
-                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap,
-                           a.list, a.nneb, a.flags);
+        hipLaunchKernelGGL(k_build_list<false>, dim3(nwg_list), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm,
+                           a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap, a.list,
+                           a.nneb, a.flags);
 }
 
 void launch_pack_only(hipStream_t st, const RebuildArgs &a) {

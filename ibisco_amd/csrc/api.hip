@@ -72,7 +72,8 @@ struct ibx_ctx {
     GroupDev gr[2];
     DBuf<DevSys> sys;
     DBuf<int> wg_items, itype, label, excl_off, excl, vs_of, vs_center, vs_moff, vs_members;
-    DBuf<int> term_meta, term_slot, inc_off, inc_slot, wg_terms, nb_map, flags, step;
+    DBuf<int> term_meta, inc_off, wg_terms, nb_map, flags, step;
+    DBuf<int4> term_slot;           // contribution slot of each member of a term
     DBuf<int4> term_mem;
     DBuf<float> contrib;
     int nterms = 0, term_nwg = 0, ncontrib = 0;
@@ -143,6 +144,15 @@ void append_rows(std::vector<float4> &rows, const Table &t) {
 // NI = INT(r/bin) and alpha = (r - x_NI)/bin with the file's own abscissas;
 // where those are not exactly k*bin (abscissas rounded to single precision in
 // the file, for instance) the kernel subtracts the offset from alpha.
+// Nonbonded rows {F_k, V_k, F_k+1 - F_k, V_k+1 - V_k}: the differences are
+// formed in single precision, as a lerp between the two stored rows would.
+void append_nb_rows(std::vector<float4> &rows, const Table &t) {
+    for (int k = 0; k <= t.ndat; ++k) {
+        const float f0 = (float)t.F[k], v0 = (float)t.V[k], f1 = (float)t.F[k + 1], v1 = (float)t.V[k + 1];
+        rows.push_back(make_float4(f0, v0, f1 - f0, v1 - v0));
+    }
+}
+
 NbTab nb_header(const Table &t, double unit, int row_off, std::vector<float> &xoff) {
     NbTab nt;
     nt.row_off = row_off;
@@ -270,7 +280,7 @@ int build_device(ibx_ctx *c) {
         if (D.nb_ntab > kMaxNbTabs) return fail(c, IBX_E_INVALID, "more than %d nonbonded tables", kMaxNbTabs);
         for (auto &t : h.nt) {
             hntab.push_back(nb_header(t, unit, (int)hnrows.size(), hnxoff));
-            append_rows(hnrows, t);
+            append_nb_rows(hnrows, t);
         }
         D.nb_nrow = (int)hnrows.size() - D.nb_row_off;
         D.ntype = h.ntype;
@@ -339,12 +349,19 @@ int build_device(ibx_ctx *c) {
         }
         hvm_off[N] = (int)hvm.size();
     }
-    std::vector<int> hinc_off(N + 1, 0), hinc;
+    // contribution slots renumbered so each item's slots are one contiguous range, in term order
+    std::vector<int> hinc_off(N + 1, 0), renum((size_t)std::max(1, nslot), 0);
     for (int i = 0; i < N; ++i) {
-        hinc_off[i] = (int)hinc.size();
-        hinc.insert(hinc.end(), inc[i].begin(), inc[i].end());
+        hinc_off[i + 1] = hinc_off[i] + (int)inc[i].size();
+        for (size_t q = 0; q < inc[i].size(); ++q) renum[inc[i][q]] = hinc_off[i] + (int)q;
     }
-    hinc_off[N] = (int)hinc.size();
+    std::vector<int4> hslot4(hslot.size());
+    for (size_t t = 0; t < hslot.size(); ++t) {
+        const int4 m = htm[t];
+        const int b = hslot[t];
+        hslot4[t] = make_int4(m.x >= 0 ? renum[b] : 0, m.y >= 0 ? renum[b + 1] : 0, m.z >= 0 ? renum[b + 2] : 0,
+                              m.w >= 0 ? renum[b + 3] : 0);
+    }
     c->nterms = (int)hmeta.size();
     c->term_nwg = c->nterms / kBlock;
     c->ncontrib = std::max(1, nslot);
@@ -356,8 +373,8 @@ int build_device(ibx_ctx *c) {
         (rc = upload(c, c->vs_of, hvs)) || (rc = upload(c, c->vs_coeff, hvc)) || (rc = upload(c, c->vs_center, hvcen)) ||
         (rc = upload(c, c->vs_invmass, hvim)) || (rc = upload(c, c->vs_moff, hvm_off)) ||
         (rc = upload(c, c->vs_members, hvm)) || (rc = upload(c, c->inc_off, hinc_off)) ||
-        (rc = upload(c, c->inc_slot, hinc)) || (rc = upload(c, c->term_mem, htm)) || (rc = upload(c, c->term_meta, hmeta)) ||
-        (rc = upload(c, c->term_slot, hslot)) || (rc = upload(c, c->wg_terms, hwgt)) ||
+        (rc = upload(c, c->term_mem, htm)) || (rc = upload(c, c->term_meta, hmeta)) ||
+        (rc = upload(c, c->term_slot, hslot4)) || (rc = upload(c, c->wg_terms, hwgt)) ||
         (rc = upload(c, c->bd_tabs, hbt)) || (rc = upload(c, c->bd_rows, hbrows)) || (rc = upload(c, c->bd_x, hbx)) ||
         (rc = upload(c, c->nb_rows, hnrows)) || (rc = upload(c, c->nb_tabs, hntab)) || (rc = upload(c, c->nb_map, hnmap)) ||
         (rc = upload(c, c->nb_xoff, hnxoff)))
@@ -529,7 +546,7 @@ void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
     if (ev) (void)hipEventRecord((*ev)[2], c->st);
     MoveLaunch m;
     m.s = c->s.p; m.v = c->v.p; m.fnb = c->fnb.p; m.contrib = c->contrib.p; m.inc_off = c->inc_off.p;
-    m.inc_slot = c->inc_slot.p; m.ncontrib = c->ncontrib; m.ftot = c->ftot.p; m.itype = c->itype.p;
+    m.ncontrib = c->ncontrib; m.ftot = c->ftot.p; m.itype = c->itype.p;
     m.label = c->label.p; m.mass = c->mass.p; m.invmass = c->invmass.p; m.vs_of = c->vs_of.p;
     m.vs_coeff = c->vs_coeff.p;
     for (int g = 0; g < 2; ++g) {
@@ -544,8 +561,11 @@ void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
         launch_vs_place(c->st, c->item_nwg, c->sys.p, c->wg_items.p, c->s.p, c->n_items, c->label.p, c->itype.p,
                         c->vs_center.p, c->vs_moff.p, c->vs_members.p, c->mass.p, c->vs_invmass.p, c->gr[1].pos4.p,
                         c->gr[1].inv_perm.p);
-    launch_finalize(c->st, c->nsys, c->sys.p, c->gr[0].part.p, c->gr[1].present ? c->gr[1].part.p : c->gr[0].part.p,
-                    c->part_bd.p, c->part_mv.p, c->obs.p, c->avg.p, c->sysstate.p, c->step.p, mode);
+    constexpr int per = kBlock / kNbAtomsPerBlock;
+    const GroupDev &G1 = c->gr[1].present ? c->gr[1] : c->gr[0];
+    launch_finalize(c->st, c->nsys, c->sys.p, c->gr[0].part.p, G1.part.p, c->part_bd.p, c->part_mv.p,
+                    per * c->gr[0].nwg, per * G1.nwg, c->term_nwg, c->item_nwg, c->obs.p, c->avg.p, c->sysstate.p,
+                    c->step.p, mode);
 }
 
 int check_flags(ibx_ctx *c) {
@@ -870,7 +890,7 @@ int ibx_set_nb_table(ibx_ctx *c, int sys, int t, int nrow, const double *x, cons
     h.nt[t] = nt;
     std::vector<float4> rows;
     std::vector<float> xoff;
-    append_rows(rows, nt);
+    append_nb_rows(rows, nt);
     const NbTab &hdr = c->h_nb_tabs[c->dsys[sys].nb_tab_off + t];
     const NbTab nh = nb_header(nt, c->dsys[sys].unit, hdr.row_off, xoff);
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -978,7 +998,7 @@ int ibx_get_info(ibx_ctx *c, int sys, int64_t info[8]) {
     if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
     const HostSystem &h = c->hs[sys];
     size_t bytes = c->s.bytes() + c->v.bytes() + c->fnb.bytes() + c->contrib.bytes() + c->ftot.bytes() +
-                   c->mass.bytes() + c->invmass.bytes() + c->excl.bytes() + c->inc_slot.bytes() + c->term_mem.bytes();
+                   c->mass.bytes() + c->invmass.bytes() + c->excl.bytes() + c->term_slot.bytes() + c->term_mem.bytes();
     for (auto &G : c->gr)
         bytes += G.list.bytes() + G.pos4.bytes() + G.r64.bytes() + G.perm.bytes() + G.nneb.bytes() + G.mem.bytes();
     info[0] = h.natoms;

@@ -18,6 +18,7 @@ constexpr int kNbPart = 16;        // doubles per workgroup partial (nonbonded)
 constexpr int kBdPart = 24;        // doubles per workgroup partial (bonded: energies, overextension, virial)
 constexpr int kMvPart = 24;        // doubles per workgroup partial (integrate)
 constexpr int kObs = 32;
+constexpr int kFinBlock = 1024;    // threads of the per-system reduction
 constexpr int kMaxNbTabs = 64;     // nonbonded tables per system
 constexpr int kMaxNbTypes = 32;    // particle types per system with several nonbonded tables
 enum TermKind { KBOND = 0, KANGLE = 1, KTORS = 2, KOOP = 3 };

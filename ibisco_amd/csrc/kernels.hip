@@ -69,6 +69,29 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double *lds /* N*4 */)
     __syncthreads();
 }
 
+// The same for a block of W waves: the per-wave sums are added in wave order
+// by thread k for value k, and thread 0 gets the sums.
+template <int N, int W>
+__device__ __forceinline__ void block_sum_waves(double (&v)[N], double *lds /* N*W + N */) {
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        double x = wave_sum(v[k]);
+        if (lane == 0) lds[k * W + wid] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < N) {
+        double x = lds[threadIdx.x * W];
+        for (int w = 1; w < W; ++w) x += lds[threadIdx.x * W + w];
+        lds[N * W + threadIdx.x] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int k = 0; k < N; ++k) v[k] = lds[N * W + k];
+    }
+}
+
 constexpr int kMultiTabsLds(bool multi) { return multi ? kMaxNbTabs : 1; }
 
 __device__ __forceinline__ int pos_type(int w) { return w & 0xffff; }
@@ -408,14 +431,14 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
 // What a pair needs besides the two positions: the tables, the FP32 cutoff
 // band, and what the exact cutoff test reads.
 struct NbCtx {
-    const float4 *rows;            // {F_k, V_k, F_k+1, V_k+1}
+    const float4 *rows;            // {F_k, V_k, F_k+1 - F_k, V_k+1 - V_k}
     const float *xoff;             // x_k/bin - k of non-uniform tables (global rows)
     const NbTab *tabs;
     const int *map;
     int ntype, xoff_base;
     NbTab t0;
-    float rho[3];
-    double rhod[3];
+    float rho[3];                  // L_k / L_x
+    double rhod[3];                // the same in FP64 (non-cubic boxes)
     float band_lo, band_hi;        // FP32 r^2 below band_lo is inside rc, at or above band_hi outside
     // exact cutoff test
     const DevSys *S;
@@ -438,15 +461,17 @@ __device__ __noinline__ bool nb_inside_exact(const DevSys *__restrict__ S, int g
 }
 
 // Contribution of one listed neighbour (NONBONDED_FORCE.F90:55-98).  The
-// cutoff test and the force direction use FP32 displacements from the int32
-// fixed-point positions; pairs within FP32 rounding of rc are settled by the
-// FP64 test.  For pairs inside the cutoff the table coordinate r/bin is formed
-// in FP64 (one Newton step from the FP32 root on the exact integer r^2), so
-// alpha = (r - x_k)/bin is exact to FP32 and the interpolated energy is not
-// limited by the rounding of r.
+// displacement is the wrap-around difference of the int32 fixed-point
+// positions; the cutoff test and the force direction use it in FP32, and
+// pairs within FP32 rounding of rc are settled by the FP64 test.  For pairs
+// inside the cutoff the table coordinate r/bin is formed in FP64 from the
+// exact integer r^2 (one Newton step from the FP32 root), so alpha = (r -
+// x_k)/bin is exact to FP32: an FP32 r would leave a random error of ~1e-7
+// of the summed energy, dominated by pairs on the steep repulsive wall.
+// alpha uses the table's own abscissas (NONBONDED_FORCE.F90:66-68).
 template <bool kMulti, int kGroup, bool kCubic>
-__device__ __forceinline__ void nb_pair(NbCtx &C, const int4 pi, int j, const int4 pj, int li, float &fx, float &fy,
-                                        float &fz, float (&e)[3], float (&w)[6]) {
+__device__ __forceinline__ void nb_pair(const NbCtx &C, const int4 pi, int j, const int4 pj, int li, float &fx,
+                                        float &fy, float &fz, float (&e)[3], float (&w)[6]) {
     const int ix = (int)((uint32_t)pi.x - (uint32_t)pj.x);
     const int iy = (int)((uint32_t)pi.y - (uint32_t)pj.y);
     const int iz = (int)((uint32_t)pi.z - (uint32_t)pj.z);
@@ -460,17 +485,16 @@ __device__ __forceinline__ void nb_pair(NbCtx &C, const int4 pi, int j, const in
     double ddx = (double)ix, ddy = (double)iy, ddz = (double)iz;
     if (!kCubic) { ddx *= C.rhod[0]; ddy *= C.rhod[1]; ddz *= C.rhod[2]; }
     const double r2d = fma(ddx, ddx, fma(ddy, ddy, ddz * ddz));
-    const float rs = __builtin_amdgcn_rsqf(r2);
-    const double r0 = r2 > 0.f ? (double)(r2 * rs) : 0.0;      // FP32 root, then one FP64 Newton step
-    const double rd = fma(fma(-r0, r0, r2d), r2 > 0.f ? (double)(0.5f * rs) : 0.0, r0);
+    const float rs = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0f));   // r2 >= 1 for any two distinct positions
+    const double r0 = (double)(r2 * rs);
+    const double rd = fma(fma(-r0, r0, r2d), (double)(0.5f * rs), r0);
     const double tt = rd * tb.c;
-    int kr = (int)tt;
-    kr = kr > tb.ndat ? tb.ndat : kr;
+    const int kr = min((int)tt, tb.ndat);
     float a = (float)(tt - (double)kr);
     if (!tb.uniform) a -= C.xoff[C.xoff_base + tb.row_off + kr];
     const float4 row = C.rows[tb.row_off + kr];
-    const float F = fmaf(a, row.z - row.x, row.x);
-    const float V = fmaf(a, row.w - row.y, row.y);
+    const float F = fmaf(a, row.z, row.x);
+    const float V = fmaf(a, row.w, row.y);
     fx = fmaf(F, dx, fx);
     fy = fmaf(F, dy, fy);
     fz = fmaf(F, dz, fz);
@@ -568,14 +592,23 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
             const ptrdiff_t step = run == 0 ? kWave : -kWave;
             const int nm = n > q ? (n - q + kLanes - 1) / kLanes : 0;
             int m = 0;
+            // indices of the next four pairs are loaded one round ahead of their positions
+            int j0 = 0, j1 = 0, j2 = 0, j3 = 0;
+            if (nm >= 4) {
+                j0 = l0[0]; j1 = l0[step]; j2 = l0[2 * step]; j3 = l0[3 * step];
+            }
             for (; m + 4 <= nm; m += 4) {
-                const int j0 = l0[m * step], j1 = l0[(m + 1) * step];
-                const int j2 = l0[(m + 2) * step], j3 = l0[(m + 3) * step];
                 const int4 p0 = pos4[j0], p1 = pos4[j1], p2 = pos4[j2], p3 = pos4[j3];
+                int k0 = j0, k1 = j1, k2 = j2, k3 = j3;
+                if (m + 8 <= nm) {
+                    k0 = l0[(m + 4) * step]; k1 = l0[(m + 5) * step];
+                    k2 = l0[(m + 6) * step]; k3 = l0[(m + 7) * step];
+                }
                 nb_pair<kMulti, kGroup, kCubic>(C, pi, j0, p0, li, fx, fy, fz, e, w);
                 nb_pair<kMulti, kGroup, kCubic>(C, pi, j1, p1, li, fx, fy, fz, e, w);
                 nb_pair<kMulti, kGroup, kCubic>(C, pi, j2, p2, li, fx, fy, fz, e, w);
                 nb_pair<kMulti, kGroup, kCubic>(C, pi, j3, p3, li, fx, fy, fz, e, w);
+                j0 = k0; j1 = k1; j2 = k2; j3 = k3;
             }
             for (; m < nm; ++m) {
                 const int j = l0[m * step];
@@ -598,10 +631,12 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
     double acc[9] = {e[0], e[1], e[2], w[0], w[1], w[2], w[3], w[4], w[5]};
     block_sum<9>(acc, red);
     if (threadIdx.x == 0) {
-        double *p = part + (size_t)blockIdx.x * kNbPart;
+        // field-major partials: field k of workgroup w at part[k * gridDim.x + w]
+        double *p = part + blockIdx.x;
+        const size_t fs = gridDim.x;
         const double u2 = S.unit * S.unit;
-        for (int k = 0; k < 3; ++k) p[k] = 0.5 * acc[k];
-        for (int k = 3; k < 9; ++k) p[k] = 0.5 * acc[k] * u2;
+        for (int k = 0; k < 3; ++k) p[k * fs] = 0.5 * acc[k];
+        for (int k = 3; k < 9; ++k) p[k * fs] = 0.5 * acc[k] * u2;
     }
 }
 
@@ -616,7 +651,7 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
 struct TermArgs {
     const int4 *mem;      // members (unused -1)
     const int *meta;      // kind | table << 2, -1 for padding
-    const int *slot;      // first contribution slot of the term
+    const int4 *slot;     // contribution slot of each member
     const BdTab *tabs;    // all bonded table headers
     const float4 *rows;   // all bonded rows {F_k, V_k, F_k+1, V_k+1}
     const double *xs;     // abscissa of every bonded row
@@ -866,17 +901,25 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
             oop_term<double>(tb, A.rows, A.xs, m4, s, n, fm, V, flags);
             nm = 4;
         }
-        en[3 * kind + tb.label - 1] += V;
+        // static indices only, so en[] and fm[][] stay in registers
+        const int ek = 3 * kind + tb.label - 1;
+#pragma unroll
+        for (int k = 0; k < 12; ++k) en[k] += (k == ek) ? V : 0.0;
         const int mem[4] = {m4.x, m4.y, m4.z, m4.w};
-        const int sl = A.slot[t];
-        for (int r = 0; r < nm; ++r) {
-            contrib[sl + r] = (float)fm[r][0];
-            contrib[ncontrib + sl + r] = (float)fm[r][1];
-            contrib[2 * ncontrib + sl + r] = (float)fm[r][2];
+        const int4 s4 = A.slot[t];
+        const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (r >= nm) break;
+            contrib[sl[r]] = (float)fm[r][0];
+            contrib[ncontrib + sl[r]] = (float)fm[r][1];
+            contrib[2 * ncontrib + sl[r]] = (float)fm[r][2];
         }
-        for (int r = 1; r < nm; ++r) {
-            const double dx = s[mem[r]] - s[mem[0]], dy = s[n + mem[r]] - s[n + mem[0]],
-                         dz = s[2 * n + mem[r]] - s[2 * n + mem[0]];
+        const double s0x = s[mem[0]], s0y = s[n + mem[0]], s0z = s[2 * n + mem[0]];
+#pragma unroll
+        for (int r = 1; r < 4; ++r) {
+            if (r >= nm) break;
+            const double dx = s[mem[r]] - s0x, dy = s[n + mem[r]] - s0y, dz = s[2 * n + mem[r]] - s0z;
             en[13] += fm[r][0] * dx;
             en[14] += fm[r][1] * dy;
             en[15] += fm[r][2] * dz;
@@ -887,8 +930,8 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
     }
     block_sum<19>(en, red);
     if (threadIdx.x == 0) {
-        double *p = part + (size_t)blockIdx.x * kBdPart;
-        for (int k = 0; k < 19; ++k) p[k] = en[k];
+        double *p = part + blockIdx.x;   // field-major
+        for (int k = 0; k < 19; ++k) p[(size_t)k * gridDim.x] = en[k];
     }
 }
 
@@ -903,8 +946,8 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
 struct MoveArgs {
     double *s, *v;
     const float *fnb;
-    const float *contrib;   // bonded contribution slots (SoA x, y, z)
-    const int *inc_off, *inc_slot;
+    const float *contrib;   // bonded contribution slots (SoA x, y, z); item i owns [inc_off[i], inc_off[i+1])
+    const int *inc_off;
     int ncontrib;
     float *ftot;
     const int *itype, *label;
@@ -935,20 +978,16 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
         for (int k = 0; k < 3; ++k) A.ftot[k * n + it] = 0.0f;   // distributed (DISTRIBUTE_VSFORCE.f90:40)
     if (ty >= 0 && lab != 3) {
         double F[3], fbv[3] = {0, 0, 0};
-        for (int q = A.inc_off[it]; q < A.inc_off[it + 1]; ++q) {
-            const int sl = A.inc_slot[q];
-            for (int k = 0; k < 3; ++k) fbv[k] += (double)A.contrib[k * A.ncontrib + sl];
-        }
+        for (int q = A.inc_off[it]; q < A.inc_off[it + 1]; ++q)
+            for (int k = 0; k < 3; ++k) fbv[k] += (double)A.contrib[k * A.ncontrib + q];
         for (int k = 0; k < 3; ++k) F[k] = (double)A.fnb[k * n + it] + fbv[k];
         const int vs = A.vs_of[it];
         if (vs >= 0) {
             // the virtual site's nonbonded and bonded forces, spread by mass fraction
             const double c = A.vs_coeff[it];
             double fv[3] = {0, 0, 0};
-            for (int q = A.inc_off[vs]; q < A.inc_off[vs + 1]; ++q) {
-                const int sl = A.inc_slot[q];
-                for (int k = 0; k < 3; ++k) fv[k] += (double)A.contrib[k * A.ncontrib + sl];
-            }
+            for (int q = A.inc_off[vs]; q < A.inc_off[vs + 1]; ++q)
+                for (int k = 0; k < 3; ++k) fv[k] += (double)A.contrib[k * A.ncontrib + q];
             for (int k = 0; k < 3; ++k) F[k] += ((double)A.fnb[k * n + vs] + fv[k]) * c;
         }
         double sp[3] = {A.s[it], A.s[n + it], A.s[2 * n + it]};
@@ -992,8 +1031,8 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     }
     block_sum<21>(acc, red);
     if (threadIdx.x == 0) {
-        double *p = A.part + (size_t)blockIdx.x * kMvPart;
-        for (int k = 0; k < 21; ++k) p[k] = acc[k];
+        double *p = A.part + blockIdx.x;   // field-major
+        for (int k = 0; k < 21; ++k) p[(size_t)k * gridDim.x] = acc[k];
     }
 }
 
@@ -1031,35 +1070,46 @@ __global__ void __launch_bounds__(kBlock) k_vs_place(const DevSys *__restrict__ 
 
 // Per-system reduction of the workgroup partials into the observables
 // (AVERAGE.f90:10-47 window sums included).  One workgroup per system.
-__global__ void __launch_bounds__(kBlock) k_finalize(const DevSys *__restrict__ sys, const double *__restrict__ part_nb0,
-                                                     const double *__restrict__ part_nb1, const double *__restrict__ part_bd,
-                                                     const double *__restrict__ part_mv, double *__restrict__ obs,
-                                                     double *__restrict__ avg, double *__restrict__ sysstate,
-                                                     const int *__restrict__ step, int mode) {
-    __shared__ double red[40 * 4];
+__global__ void __launch_bounds__(kFinBlock) k_finalize(const DevSys *__restrict__ sys, const double *__restrict__ part_nb0,
+                                                        const double *__restrict__ part_nb1,
+                                                        const double *__restrict__ part_bd,
+                                                        const double *__restrict__ part_mv, int snb0, int snb1, int sbd,
+                                                        int smv, double *__restrict__ obs, double *__restrict__ avg,
+                                                        double *__restrict__ sysstate, const int *__restrict__ step,
+                                                        int mode) {
+    // fields: 0..8 nonbonded (both groups), 9..27 bonded, 28..48 integrate; one wave per field at a time,
+    // lanes stride over the workgroup partials, so every sum is in a fixed order
+    constexpr int kF = 49;
+    __shared__ double red[kF];
     const int s_id = blockIdx.x;
     const DevSys &S = sys[s_id];
-    double acc[40];
-    for (int k = 0; k < 40; ++k) acc[k] = 0.0;
-    for (int g = 0; g < S.ngroup; ++g) {
-        const double *pp = g == 0 ? part_nb0 : part_nb1;
-        constexpr int per = kBlock / kNbAtomsPerBlock;   // nonbonded workgroups per 256-slot block
-        for (int w = per * S.g[g].wg_off + threadIdx.x; w < per * (S.g[g].wg_off + S.g[g].nwg); w += kBlock)
-            for (int k = 0; k < 9; ++k) acc[k] += pp[(size_t)w * kNbPart + k];
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    constexpr int per = kBlock / kNbAtomsPerBlock;   // nonbonded workgroups per 256-slot block
+    for (int f = wid; f < kF; f += kFinBlock / kWave) {
+        double x = 0.0;
+        if (f < 9) {
+            for (int g = 0; g < S.ngroup; ++g) {
+                const double *pp = (g == 0 ? part_nb0 + (size_t)f * snb0 : part_nb1 + (size_t)f * snb1);
+                for (int w = per * S.g[g].wg_off + lane; w < per * (S.g[g].wg_off + S.g[g].nwg); w += kWave) x += pp[w];
+            }
+        } else if (f < 28) {
+            const double *pp = part_bd + (size_t)(f - 9) * sbd;
+            for (int w = S.term_wg_off + lane; w < S.term_wg_off + S.term_nwg; w += kWave) x += pp[w];
+        } else {
+            const double *pp = part_mv + (size_t)(f - 28) * smv;
+            for (int w = S.item_wg_off + lane; w < S.item_wg_off + S.item_nwg; w += kWave) x += pp[w];
+        }
+        x = wave_sum(x);
+        if (lane == 0) red[f] = x;
     }
-    for (int w = S.term_wg_off + threadIdx.x; w < S.term_wg_off + S.term_nwg; w += kBlock) {
-        for (int k = 0; k < 13; ++k) acc[9 + k] += part_bd[(size_t)w * kBdPart + k];
-        for (int k = 0; k < 6; ++k) acc[22 + k] += part_bd[(size_t)w * kBdPart + 13 + k];
-    }
-    // integrate partials: (unused 6), kinetic tensor of each population (6+6), momentum (3)
-    double mom[3] = {0, 0, 0};
-    for (int w = S.item_wg_off + threadIdx.x; w < S.item_wg_off + S.item_nwg; w += kBlock) {
-        for (int k = 0; k < 18; ++k) acc[22 + k] += part_mv[(size_t)w * kMvPart + k];
-        for (int k = 0; k < 3; ++k) mom[k] += part_mv[(size_t)w * kMvPart + 18 + k];
-    }
-    block_sum<40>(acc, red);
-    block_sum<3>(mom, red);
+    __syncthreads();
     if (threadIdx.x != 0) return;
+    // acc keeps the former layout: 0..8 nonbonded, 9..21 bonded energies and overextension,
+    // 22..27 bonded virial, 28..39 kinetic tensor of each population, 40..42 momentum
+    double acc[43];
+    for (int k = 0; k < 28; ++k) acc[k] = red[k];
+    for (int k = 0; k < 15; ++k) acc[28 + k] = red[34 + k];
+    const double mom[3] = {acc[40], acc[41], acc[42]};
     double *o = obs + (size_t)s_id * kObs;
     double *st = sysstate + (size_t)s_id * 8;
     for (int k = 0; k < 3; ++k) o[k] = acc[k];                 // V_NB
@@ -1197,7 +1247,7 @@ void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &
 
 void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const MoveLaunch &m) {
     MoveArgs A;
-    A.s = m.s; A.v = m.v; A.fnb = m.fnb; A.contrib = m.contrib; A.inc_off = m.inc_off; A.inc_slot = m.inc_slot;
+    A.s = m.s; A.v = m.v; A.fnb = m.fnb; A.contrib = m.contrib; A.inc_off = m.inc_off;
     A.ncontrib = m.ncontrib; A.ftot = m.ftot; A.itype = m.itype; A.label = m.label;
     A.mass = m.mass; A.invmass = m.invmass; A.vs_of = m.vs_of; A.vs_coeff = m.vs_coeff;
     A.pos4[0] = m.pos4[0]; A.pos4[1] = m.pos4[1]; A.inv_perm[0] = m.inv_perm[0]; A.inv_perm[1] = m.inv_perm[1];
@@ -1213,9 +1263,10 @@ void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_s
 }
 
 void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,
-                     const double *pbd, const double *pmv, double *obs, double *avg, double *sysstate, const int *step,
-                     int mode) {
-    hipLaunchKernelGGL(k_finalize, dim3(nsys), dim3(kBlock), 0, st, sys, p0, p1, pbd, pmv, obs, avg, sysstate, step, mode);
+                     const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *obs,
+                     double *avg, double *sysstate, const int *step, int mode) {
+    hipLaunchKernelGGL(k_finalize, dim3(nsys), dim3(kFinBlock), 0, st, sys, p0, p1, pbd, pmv, snb0, snb1, sbd, smv, obs,
+                       avg, sysstate, step, mode);
 }
 
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,

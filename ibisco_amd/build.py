@@ -37,6 +37,30 @@ def _compile(src):
     return obj
 
 
+FORTRAN = os.path.join(HERE, "fortran")
+DRIVER = os.path.join(HERE, "ibx_driver")
+
+
+def build_driver(verbose=False):
+    """The Fortran step driver (iso_c_binding over libibx.so), when amdflang is installed."""
+    fc = os.environ.get("FC", "amdflang")
+    if subprocess.run(["which", fc], capture_output=True).returncode != 0:
+        if verbose:
+            print("no Fortran compiler: ibx_driver not built")
+        return None
+    mod = os.path.join(OBJDIR, "ibx_mod.o")
+    cmds = [[fc, "-O2", "-c", os.path.join(FORTRAN, "ibx_mod.f90"), "-module-dir", OBJDIR, "-o", mod],
+            [fc, "-O2", "-I" + OBJDIR, os.path.join(FORTRAN, "ibx_driver.f90"), mod, "-L" + HERE, "-libx",
+             "-Wl,-rpath,$ORIGIN", "-o", DRIVER]]
+    for cmd in cmds:
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=OBJDIR)
+        if r.returncode != 0:
+            raise RuntimeError(f"Fortran build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
+
+
 def build(verbose=False):
     os.makedirs(OBJDIR, exist_ok=True)
     with ThreadPoolExecutor(max_workers=4) as ex:
@@ -49,6 +73,7 @@ def build(verbose=False):
     os.replace(tmp, OUT)
     if verbose:
         print("built", OUT)
+    build_driver(verbose)
     return OUT
 
 

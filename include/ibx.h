@@ -86,10 +86,11 @@ int ibx_load_dir(const char *dir, ibx_ctx **out);
 void ibx_destroy(ibx_ctx *ctx);
 const char *ibx_last_error(const ibx_ctx *ctx);   /* ctx may be NULL: last create error */
 
-/* nsteps of the step loop (NEW_LOOP.F90:19-156, or MTS_LOOP.f90:13-60 when
- * nmts>0), steps numbered step0+1..step0+nsteps: SHIFT, VIRTUAL_DEF,
- * UPDATE_NEIGHBOURLIST every nupdate, NEW_FORCE, MOVE, MOMENTUM every
- * halt_drift, AVERAGE.  All replicas of a batch advance together. */
+/* nsteps of the step loop (NEW_LOOP.F90:19-156), steps numbered
+ * step0+1..step0+nsteps: SHIFT, VIRTUAL_DEF, UPDATE_NEIGHBOURLIST every
+ * nupdate, NEW_FORCE, MOVE, MOMENTUM every halt_drift, AVERAGE.  All replicas
+ * of a batch advance together.  The MTS cycle (MTS_LOOP.f90, nmts>0) is not on
+ * the device: ibx_create rejects such systems with IBX_E_INVALID. */
 int ibx_step(ibx_ctx *ctx, int nsteps, int step0);
 
 /* The same steps as ibx_step, launched as a captured HIP graph replayed per
@@ -115,6 +116,8 @@ int ibx_get_averages(ibx_ctx *ctx, int sys, double sums[32], int reset);
 
 /* Per-item forces (natoms+nvs)*3 after the last force phase, reduced units. */
 int ibx_get_forces(ibx_ctx *ctx, int sys, double *f);
+/* State of system sys, xyz interleaved: s (natoms+nvs)*3 unwrapped positions
+ * (SXYZ), v natoms*3 reduced velocities (VXYZ).  Either pointer may be NULL. */
 int ibx_get_state(ibx_ctx *ctx, int sys, double *s, double *v);
 int ibx_set_state(ibx_ctx *ctx, int sys, const double *s, const double *v);
 /* Replaces nonbonded table t of system sys (raw rows as in ibx_tables). */

@@ -69,6 +69,8 @@ struct ibx_ctx {
     std::vector<int> item_off, slot_off[2];
     int n_items = 0, item_nwg = 0, nupdate = 1;
     hipStream_t st = nullptr;
+    hipStream_t st2 = nullptr;               // bonded terms run here, beside the nonbonded kernel
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
     GroupDev gr[2];
     DBuf<DevSys> sys;
     DBuf<int> wg_items, itype, label, excl_off, excl, vs_of, vs_center, vs_moff, vs_members;
@@ -77,7 +79,7 @@ struct ibx_ctx {
     DBuf<int4> term_mem;
     DBuf<float> contrib;
     int nterms = 0, term_nwg = 0, ncontrib = 0;
-    DBuf<double> s, v, mass, invmass, vs_coeff, vs_invmass, part_bd, part_mv, obs, avg, sysstate;
+    DBuf<double> s, v, mass, invmass, vs_coeff, vs_invmass, part_bd, part_mv, fsum, obs, avg, sysstate;
     DBuf<float> fnb, ftot;
     DBuf<float4> nb_rows, bd_rows;
     DBuf<double> bd_x;
@@ -101,6 +103,9 @@ struct ibx_ctx {
     ~ibx_ctx() {
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto e : evpool) (void)hipEventDestroy(e);
+        if (ev_fork) (void)hipEventDestroy(ev_fork);
+        if (ev_join) (void)hipEventDestroy(ev_join);
+        if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
 };
@@ -390,6 +395,7 @@ int build_device(ibx_ctx *c) {
     HIPCHK(c, c->part_mv.alloc((size_t)c->item_nwg * kMvPart));
     HIPCHK(c, hipMemset(c->part_bd.p, 0, c->part_bd.bytes()));
     HIPCHK(c, hipMemset(c->part_mv.p, 0, c->part_mv.bytes()));
+    HIPCHK(c, c->fsum.alloc((size_t)S * kFields));
     HIPCHK(c, c->obs.alloc((size_t)S * kObs));
     HIPCHK(c, c->avg.alloc((size_t)S * kObs));
     HIPCHK(c, c->sysstate.alloc((size_t)S * 8));
@@ -539,11 +545,26 @@ void enqueue_forces(ibx_ctx *c, int mode) {
     }
 }
 
-void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
+void enqueue_terms(ibx_ctx *c, hipStream_t st) {
     TermLaunch b{c->term_mem.p, c->term_meta.p, c->term_slot.p, c->bd_tabs.p, c->bd_rows.p, c->bd_x.p};
-    launch_terms(c->st, c->term_nwg, c->wg_terms.p, b, c->s.p, c->n_items, c->nterms, c->contrib.p, c->ncontrib,
+    launch_terms(st, c->term_nwg, c->wg_terms.p, b, c->s.p, c->n_items, c->nterms, c->contrib.p, c->ncontrib,
                  c->part_bd.p, c->flags.p);
-    if (ev) (void)hipEventRecord((*ev)[2], c->st);
+}
+
+// The bonded terms read only the unwrapped positions and write their own
+// buffers, so they run on a second stream beside the list rebuild and the
+// nonbonded kernel; the integrator waits for both (fork here, join_terms).
+void fork_terms(ibx_ctx *c) {
+    (void)hipEventRecord(c->ev_fork, c->st);
+    (void)hipStreamWaitEvent(c->st2, c->ev_fork, 0);
+    enqueue_terms(c, c->st2);
+    (void)hipEventRecord(c->ev_join, c->st2);
+}
+
+void join_terms(ibx_ctx *c) { (void)hipStreamWaitEvent(c->st, c->ev_join, 0); }
+
+// Integration and reductions after the forces (the bonded terms must be enqueued before).
+void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
     MoveLaunch m;
     m.s = c->s.p; m.v = c->v.p; m.fnb = c->fnb.p; m.contrib = c->contrib.p; m.inc_off = c->inc_off.p;
     m.ncontrib = c->ncontrib; m.ftot = c->ftot.p; m.itype = c->itype.p;
@@ -564,8 +585,8 @@ void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
     constexpr int per = kBlock / kNbAtomsPerBlock;
     const GroupDev &G1 = c->gr[1].present ? c->gr[1] : c->gr[0];
     launch_finalize(c->st, c->nsys, c->sys.p, c->gr[0].part.p, G1.part.p, c->part_bd.p, c->part_mv.p,
-                    per * c->gr[0].nwg, per * G1.nwg, c->term_nwg, c->item_nwg, c->obs.p, c->avg.p, c->sysstate.p,
-                    c->step.p, mode);
+                    per * c->gr[0].nwg, per * G1.nwg, c->term_nwg, c->item_nwg, c->fsum.p, c->obs.p, c->avg.p,
+                    c->sysstate.p, c->step.p, mode);
 }
 
 int check_flags(ibx_ctx *c) {
@@ -599,15 +620,25 @@ int rebuild_checked(ibx_ctx *c) {
     return fail(c, IBX_E_LIST_OVERFLOW, "LIST TOO SMALL after resizing");
 }
 
+// One step.  With timing events the kernels run in order on one stream, so
+// each event pair brackets one kernel class; otherwise the bonded terms
+// overlap the rebuild and the nonbonded kernel.
 void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
     if (ev) (void)hipEventRecord((*ev)[0], c->st);
+    if (!ev) fork_terms(c);
     if (step % c->nupdate == 0) {
         enqueue_rebuild(c);
         if (c->rdf_nbin && c->rdf_every > 0) enqueue_rdf(c);
     }
     if (ev) (void)hipEventRecord((*ev)[4], c->st);
     enqueue_forces(c, 0);
-    if (ev) (void)hipEventRecord((*ev)[1], c->st);
+    if (ev) {
+        (void)hipEventRecord((*ev)[1], c->st);
+        enqueue_terms(c, c->st);
+        (void)hipEventRecord((*ev)[2], c->st);
+    } else {
+        join_terms(c);
+    }
     enqueue_rest(c, 0, ev);
     if (ev) (void)hipEventRecord((*ev)[5], c->st);
 }
@@ -664,7 +695,10 @@ int ibx_create(int nsys, const ibx_system_desc *const *desc, ibx_ctx **out) {
         }
     }
     c->nupdate = c->hs[0].nupdate;
-    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
         return fail(nullptr, IBX_E_DEVICE, "stream creation failed");
     int rc = build_device(c.get());
     if (rc) { g_create_err = c->err; return rc; }
@@ -677,6 +711,7 @@ int ibx_create(int nsys, const ibx_system_desc *const *desc, ibx_ctx **out) {
 void ibx_destroy(ibx_ctx *ctx) {
     if (!ctx) return;
     (void)hipStreamSynchronize(ctx->st);
+    if (ctx->st2) (void)hipStreamSynchronize(ctx->st2);
     delete ctx;
 }
 
@@ -692,7 +727,9 @@ int ibx_compute_forces(ibx_ctx *c, int rebuild) {
         if (rc) return rc;
     }
     HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
+    fork_terms(c);
     enqueue_forces(c, 1);
+    join_terms(c);
     enqueue_rest(c, 1, nullptr);
     return check_flags(c);
 }

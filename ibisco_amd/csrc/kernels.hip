@@ -94,6 +94,17 @@ __device__ __forceinline__ void block_sum_waves(double (&v)[N], double *lds /* N
 
 constexpr int kMultiTabsLds(bool multi) { return multi ? kMaxNbTabs : 1; }
 
+// Workgroups are dispatched to the XCDs in turn, and each XCD has its own L2.
+// A kernel over n spatially ordered blocks is launched with xcd_grid(n)
+// workgroups and works on block xcd_block(n), so each XCD runs one contiguous
+// range of blocks (one region of the system) and finds its neighbours' data in
+// its own L2.  Blocks at or past n return at once.
+__device__ __forceinline__ int xcd_block(int n) {
+    const int per = (n + kXcd - 1) / kXcd;
+    return (int)(blockIdx.x % kXcd) * per + (int)(blockIdx.x / kXcd);
+}
+static inline int xcd_grid(int n) { return ((n + kXcd - 1) / kXcd) * kXcd; }
+
 __device__ __forceinline__ int pos_type(int w) { return w & 0xffff; }
 __device__ __forceinline__ int pos_label(int w) { return (w >> 16) & 0xff; }
 
@@ -252,10 +263,12 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
                                                        const int *__restrict__ cell_count,
                                                        const int *__restrict__ excl_off, const int *__restrict__ excl,
                                                        int cap, int *__restrict__ list, int2 *__restrict__ nneb,
-                                                       int *__restrict__ flags) {
+                                                       int *__restrict__ flags, int nblk) {
     __shared__ int rbase[kListAtoms][kListRanges];   // slot of candidate p is rbase[r] + p
     __shared__ int rend[kListAtoms][kListRanges];    // candidates up to the end of range r
-    const int slot0 = blockIdx.x * kListAtoms;
+    const int blk = xcd_block(nblk);
+    if (blk >= nblk) return;
+    const int slot0 = blk * kListAtoms;
     const DevSys &S = sys[wg_sys[slot0 / kBlock]];
     const DevGroup &G = S.g[g];
     const int sub = threadIdx.x & (kListLanes - 1), grp = threadIdx.x / kListLanes;
@@ -530,12 +543,14 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
                                                       const float *__restrict__ nb_xoff,
                                                       const NbTab *__restrict__ nb_tabs, const int *__restrict__ nb_map,
                                                       int nitems_total, const double *__restrict__ s,
-                                                      float *__restrict__ fnb, double *__restrict__ part) {
+                                                      float *__restrict__ fnb, double *__restrict__ part, int nblk) {
     extern __shared__ float4 lds_rows[];
     __shared__ NbTab tabs[kMultiTabsLds(kMulti)];
     __shared__ int map[kMulti ? kMaxNbTypes * kMaxNbTypes : 1];
     __shared__ double red[9 * 4];
-    const DevSys &S = sys[wg_sys[blockIdx.x / (kBlock / kNbAtomsPerBlock)]];
+    const int blk = xcd_block(nblk);
+    if (blk >= nblk) return;
+    const DevSys &S = sys[wg_sys[blk / (kBlock / kNbAtomsPerBlock)]];
     const DevGroup &G = S.g[kGroup];
     if (kLds)
         for (int r = threadIdx.x; r < S.nb_nrow; r += kBlock) lds_rows[r] = nb_rows[S.nb_row_off + r];
@@ -553,7 +568,7 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
 
     const int lane = threadIdx.x & (kWave - 1);
     const int a = lane / kLanes, q = lane % kLanes;
-    const int i = blockIdx.x * kNbAtomsPerBlock + (threadIdx.x / kWave) * kAtomsPerWave + a;
+    const int i = blk * kNbAtomsPerBlock + (threadIdx.x / kWave) * kAtomsPerWave + a;
     const int I = perm[i];
     float fx = 0.f, fy = 0.f, fz = 0.f;
     float e[3] = {0.f, 0.f, 0.f};
@@ -631,9 +646,9 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
     double acc[9] = {e[0], e[1], e[2], w[0], w[1], w[2], w[3], w[4], w[5]};
     block_sum<9>(acc, red);
     if (threadIdx.x == 0) {
-        // field-major partials: field k of workgroup w at part[k * gridDim.x + w]
-        double *p = part + blockIdx.x;
-        const size_t fs = gridDim.x;
+        // field-major partials: field k of block w at part[k * nblk + w]
+        double *p = part + blk;
+        const size_t fs = nblk;
         const double u2 = S.unit * S.unit;
         for (int k = 0; k < 3; ++k) p[k * fs] = 0.5 * acc[k];
         for (int k = 3; k < 9; ++k) p[k * fs] = 0.5 * acc[k] * u2;
@@ -963,6 +978,22 @@ struct MoveArgs {
     int mode;
 };
 
+// Adds the contribution slots [a, b) of one item in slot order, loading
+// twelve slots at a time so their latencies overlap.
+__device__ __forceinline__ void gather_contrib(const float *__restrict__ contrib, int nc, int a, int b, double f[3]) {
+    for (int q = a; q < b; q += 12) {
+        float c[3][12];
+#pragma unroll
+        for (int u = 0; u < 12; ++u)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) c[k][u] = (q + u < b) ? contrib[k * nc + q + u] : 0.f;
+#pragma unroll
+        for (int u = 0; u < 12; ++u)
+#pragma unroll
+            for (int k = 0; k < 3; ++k) f[k] += (double)c[k][u];
+    }
+}
+
 __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, MoveArgs A) {
     __shared__ double red[21 * 4];
     const int s_id = wg_sys[blockIdx.x];
@@ -972,25 +1003,30 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     if (A.mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.step, 1);
     double acc[21];
     for (int k = 0; k < 21; ++k) acc[k] = 0.0;
+    // every per-item load up front (the arrays are padded to whole blocks), so
+    // they are in flight together rather than one dependent round trip each
     const int ty = A.itype[it];
     const int lab = A.label[it];
+    const int q0 = A.inc_off[it], q1 = A.inc_off[it + 1];
+    const int vs = A.vs_of[it];
+    const float fn[3] = {A.fnb[it], A.fnb[n + it], A.fnb[2 * n + it]};
+    double sp[3] = {A.s[it], A.s[n + it], A.s[2 * n + it]};
+    const double vin[3] = {A.v[it], A.v[n + it], A.v[2 * n + it]};
+    const double m = A.mass[it], im = A.invmass[it];
+    const int slot_g[2] = {A.inv_perm[0][it], A.inv_perm[1][it]};
     if (A.mode == 1 && ty >= 0 && lab == 3)
         for (int k = 0; k < 3; ++k) A.ftot[k * n + it] = 0.0f;   // distributed (DISTRIBUTE_VSFORCE.f90:40)
     if (ty >= 0 && lab != 3) {
         double F[3], fbv[3] = {0, 0, 0};
-        for (int q = A.inc_off[it]; q < A.inc_off[it + 1]; ++q)
-            for (int k = 0; k < 3; ++k) fbv[k] += (double)A.contrib[k * A.ncontrib + q];
-        for (int k = 0; k < 3; ++k) F[k] = (double)A.fnb[k * n + it] + fbv[k];
-        const int vs = A.vs_of[it];
+        gather_contrib(A.contrib, A.ncontrib, q0, q1, fbv);
+        for (int k = 0; k < 3; ++k) F[k] = (double)fn[k] + fbv[k];
         if (vs >= 0) {
             // the virtual site's nonbonded and bonded forces, spread by mass fraction
             const double c = A.vs_coeff[it];
             double fv[3] = {0, 0, 0};
-            for (int q = A.inc_off[vs]; q < A.inc_off[vs + 1]; ++q)
-                for (int k = 0; k < 3; ++k) fv[k] += (double)A.contrib[k * A.ncontrib + q];
+            gather_contrib(A.contrib, A.ncontrib, A.inc_off[vs], A.inc_off[vs + 1], fv);
             for (int k = 0; k < 3; ++k) F[k] += ((double)A.fnb[k * n + vs] + fv[k]) * c;
         }
-        double sp[3] = {A.s[it], A.s[n + it], A.s[2 * n + it]};
         if (A.mode == 1) {
             for (int k = 0; k < 3; ++k) A.ftot[k * n + it] = (float)F[k];
         } else {
@@ -998,10 +1034,9 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
             const int pop = (lab == 2) ? 1 : 0;
             double lam = 1.0;
             if (S.ensemble == 1) lam = sqrt(1.0 + S.fac * ((S.temp_in / st[pop]) - 1.0));
-            const double m = A.mass[it], im = A.invmass[it];
             double vt[3];
             for (int k = 0; k < 3; ++k) {
-                const double v0 = A.v[k * n + it] - st[2 + k] * im;   // deferred MOMENTUM
+                const double v0 = vin[k] - st[2 + k] * im;   // deferred MOMENTUM
                 const double v1 = (v0 + S.dt * F[k] * im) * lam;
                 A.v[k * n + it] = v1;
                 sp[k] += S.dt * v1;
@@ -1018,7 +1053,7 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
             kk[5] = m * vt[1] * vt[2];
             // fixed-point copy for the next nonbonded evaluation
             const int g = (S.hybrid && lab == 2) ? 1 : 0;
-            const int slot = A.inv_perm[g][it];
+            const int slot = slot_g[g];
             if (slot >= 0) {
                 int4 p;
                 p.x = (int)to_fixed(sp[0], S.inv_box_frac[0]);
@@ -1070,40 +1105,63 @@ __global__ void __launch_bounds__(kBlock) k_vs_place(const DevSys *__restrict__ 
 
 // Per-system reduction of the workgroup partials into the observables
 // (AVERAGE.f90:10-47 window sums included).  One workgroup per system.
-__global__ void __launch_bounds__(kFinBlock) k_finalize(const DevSys *__restrict__ sys, const double *__restrict__ part_nb0,
-                                                        const double *__restrict__ part_nb1,
-                                                        const double *__restrict__ part_bd,
-                                                        const double *__restrict__ part_mv, int snb0, int snb1, int sbd,
-                                                        int smv, double *__restrict__ obs, double *__restrict__ avg,
-                                                        double *__restrict__ sysstate, const int *__restrict__ step,
-                                                        int mode) {
-    // fields: 0..8 nonbonded (both groups), 9..27 bonded, 28..48 integrate; one wave per field at a time,
-    // lanes stride over the workgroup partials, so every sum is in a fixed order
-    constexpr int kF = 49;
-    __shared__ double red[kF];
-    const int s_id = blockIdx.x;
-    const DevSys &S = sys[s_id];
-    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
-    constexpr int per = kBlock / kNbAtomsPerBlock;   // nonbonded workgroups per 256-slot block
-    for (int f = wid; f < kF; f += kFinBlock / kWave) {
-        double x = 0.0;
-        if (f < 9) {
-            for (int g = 0; g < S.ngroup; ++g) {
-                const double *pp = (g == 0 ? part_nb0 + (size_t)f * snb0 : part_nb1 + (size_t)f * snb1);
-                for (int w = per * S.g[g].wg_off + lane; w < per * (S.g[g].wg_off + S.g[g].nwg); w += kWave) x += pp[w];
-            }
-        } else if (f < 28) {
-            const double *pp = part_bd + (size_t)(f - 9) * sbd;
-            for (int w = S.term_wg_off + lane; w < S.term_wg_off + S.term_nwg; w += kWave) x += pp[w];
-        } else {
-            const double *pp = part_mv + (size_t)(f - 28) * smv;
-            for (int w = S.item_wg_off + lane; w < S.item_wg_off + S.item_nwg; w += kWave) x += pp[w];
-        }
-        x = wave_sum(x);
-        if (lane == 0) red[f] = x;
+// Sum of p[w] for w = a + t, a + t + stride, ... < b, eight loads in flight;
+// the order of the additions is fixed.
+__device__ __forceinline__ double strided_sum(const double *__restrict__ p, int a, int b, int t, int stride) {
+    double x[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    int w = a + t;
+    for (; w + 7 * stride < b; w += 8 * stride) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) x[u] += p[w + u * stride];
     }
+#pragma unroll
+    for (int u = 0; u < 8; ++u)
+        if (w + u * stride < b) x[u] += p[w + u * stride];
+    return ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
+}
+
+// Per-system sums of the workgroup partials: one workgroup per (system,
+// field), fields 0..8 nonbonded (both groups), 9..27 bonded, 28..48
+// integrate.  Each sum is taken in a fixed order, so results are deterministic.
+__global__ void __launch_bounds__(kBlock) k_field_sums(const DevSys *__restrict__ sys,
+                                                      const double *__restrict__ part_nb0,
+                                                      const double *__restrict__ part_nb1,
+                                                      const double *__restrict__ part_bd,
+                                                      const double *__restrict__ part_mv, int snb0, int snb1, int sbd,
+                                                      int smv, double *__restrict__ fsum) {
+    __shared__ double red[kBlock / kWave];
+    const int s_id = blockIdx.x / kFields, f = blockIdx.x % kFields;
+    const DevSys &S = sys[s_id];
+    const int t = threadIdx.x;
+    constexpr int per = kBlock / kNbAtomsPerBlock;   // nonbonded workgroups per 256-slot block
+    double x = 0.0;
+    if (f < 9) {
+        for (int g = 0; g < S.ngroup; ++g) {
+            const double *pp = (g == 0 ? part_nb0 + (size_t)f * snb0 : part_nb1 + (size_t)f * snb1);
+            x += strided_sum(pp, per * S.g[g].wg_off, per * (S.g[g].wg_off + S.g[g].nwg), t, kBlock);
+        }
+    } else if (f < 28) {
+        x = strided_sum(part_bd + (size_t)(f - 9) * sbd, S.term_wg_off, S.term_wg_off + S.term_nwg, t, kBlock);
+    } else {
+        x = strided_sum(part_mv + (size_t)(f - 28) * smv, S.item_wg_off, S.item_wg_off + S.item_nwg, t, kBlock);
+    }
+    x = wave_sum(x);
+    if ((t & (kWave - 1)) == 0) red[t / kWave] = x;
     __syncthreads();
-    if (threadIdx.x != 0) return;
+    if (t == 0) fsum[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+}
+
+// Observables of each system from its field sums (AVERAGE.f90:10-47 window
+// sums included), the temperatures the next MOVE uses, and the MOMENTUM
+// correction when the step is a HALT_DRIFT step.  One thread per system.
+__global__ void __launch_bounds__(kWave) k_finalize(const DevSys *__restrict__ sys, int nsys,
+                                                   const double *__restrict__ fsum, double *__restrict__ obs,
+                                                   double *__restrict__ avg, double *__restrict__ sysstate,
+                                                   const int *__restrict__ step, int mode) {
+    const int s_id = blockIdx.x * kWave + threadIdx.x;
+    if (s_id >= nsys) return;
+    const DevSys &S = sys[s_id];
+    const double *red = fsum + (size_t)s_id * kFields;
     // acc keeps the former layout: 0..8 nonbonded, 9..21 bonded energies and overextension,
     // 22..27 bonded virial, 28..39 kinetic tensor of each population, 40..42 momentum
     double acc[43];
@@ -1193,13 +1251,13 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
                        a.itype, a.label, a.r64, a.pos4, a.inv_perm);
     const int nwg_list = a.nwg * (kBlock / kListAtoms);
     if (a.cubic)
-        hipLaunchKernelGGL(k_build_list<true>, dim3(nwg_list), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm,
-                           a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap, a.list,
-                           a.nneb, a.flags);
+        hipLaunchKernelGGL(k_build_list<true>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
+                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap,
+                           a.list, a.nneb, a.flags, nwg_list);
     else
-        hipLaunchKernelGGL(k_build_list<false>, dim3(nwg_list), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm,
-                           a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap, a.list,
-                           a.nneb, a.flags);
+        hipLaunchKernelGGL(k_build_list<false>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
+                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap,
+                           a.list, a.nneb, a.flags, nwg_list);
 }
 
 void launch_pack_only(hipStream_t st, const RebuildArgs &a) {
@@ -1209,9 +1267,10 @@ void launch_pack_only(hipStream_t st, const RebuildArgs &a) {
 
 template <bool M, bool L, int G, bool K>
 static void nb_go(hipStream_t st, const NbArgs &a, size_t lds) {
-    hipLaunchKernelGGL((k_nonbonded<M, L, G, K>), dim3(a.nwg * (kBlock / kNbAtomsPerBlock)), dim3(kBlock), lds, st,
-                       a.sys, a.wg_sys, a.perm, a.pos4, a.list, a.nneb, a.cap, a.rows, a.xoff, a.tabs, a.map,
-                       a.nitems_total, a.s, a.fnb, a.part);
+    const int nblk = a.nwg * (kBlock / kNbAtomsPerBlock);
+    hipLaunchKernelGGL((k_nonbonded<M, L, G, K>), dim3(xcd_grid(nblk)), dim3(kBlock), lds, st, a.sys, a.wg_sys, a.perm,
+                       a.pos4, a.list, a.nneb, a.cap, a.rows, a.xoff, a.tabs, a.map, a.nitems_total, a.s, a.fnb,
+                       a.part, nblk);
 }
 
 template <bool M, bool L, int G>
@@ -1263,10 +1322,12 @@ void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_s
 }
 
 void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,
-                     const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *obs,
-                     double *avg, double *sysstate, const int *step, int mode) {
-    hipLaunchKernelGGL(k_finalize, dim3(nsys), dim3(kFinBlock), 0, st, sys, p0, p1, pbd, pmv, snb0, snb1, sbd, smv, obs,
-                       avg, sysstate, step, mode);
+                     const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *fsum,
+                     double *obs, double *avg, double *sysstate, const int *step, int mode) {
+    hipLaunchKernelGGL(k_field_sums, dim3(nsys * kFields), dim3(kBlock), 0, st, sys, p0, p1, pbd, pmv, snb0, snb1, sbd,
+                       smv, fsum);
+    hipLaunchKernelGGL(k_finalize, dim3((nsys + kWave - 1) / kWave), dim3(kWave), 0, st, sys, nsys, fsum, obs, avg,
+                       sysstate, step, mode);
 }
 
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,

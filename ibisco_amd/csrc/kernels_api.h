@@ -82,8 +82,8 @@ void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_s
                      const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,
                      const double *mass, const double *vs_invmass, int4 *pos4, const int *inv_perm);
 void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,
-                     const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *obs,
-                     double *avg, double *sysstate, const int *step, int mode);
+                     const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *fsum,
+                     double *obs, double *avg, double *sysstate, const int *step, int mode);
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
                 const int *list, const int2 *nneb, int cap, int nbin, float inv_dr_u, unsigned long long *hist);
 

@@ -64,6 +64,24 @@ def cpu_baseline(sysm, nsteps, threads):
         shutil.rmtree(d, ignore_errors=True)
 
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_nonbonded_ua100k_traffic.md")
+
+
+def measured_traffic(sites):
+    """HBM bytes per k_nonbonded launch from the committed rocprofv3 counter summary
+    (FETCH_SIZE and WRITE_SIZE scaled by a device copy), when it covers this workload."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if line.startswith("traffic_json:"):
+                    t = json.loads(line.split(":", 1)[1])
+                    if t.get("workload") == "ua_melt_100k_plain" and t.get("sites") == sites:
+                        return t["bytes"]
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -125,7 +143,7 @@ def main():
     b_nb = 4.0 * prl_n + 28.0                     # algorithmic bytes per site-step
     achieved = b_nb * n / (nb_ms * 1e-3) / 1e9    # GB/s
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
             "kernel": "k_nonbonded", "kernel_ms": round(nb_ms, 5), "bytes_per_site_step": round(b_nb, 1),
             "half_list_pairs_per_site": round(prl_n, 2)}
     cpu = None
@@ -143,6 +161,7 @@ def main():
         "data": "synthetic (seeded random-walk melt, equilibrated on device)",
         "config": {"workload": "ua_melt_100k_plain", "sites": n, "chains": args.nchains, "chain_length": args.nbeads,
                    "rc_nm": 1.0, "rlist_nm": 1.3, "nupdate": 10, "dt_fs": 1.0, "ensemble": "NVE",
+                   "time_stepping": "single dt (the reference has no multiple time step for non-hybrid systems)",
                    "path": "one system per launch, " + ("eager launches" if args.eager else
                                                          "HIP graph per list interval"),
                    "precision": "FP32 forces, FP64 integration and reductions"},

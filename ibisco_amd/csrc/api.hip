@@ -52,7 +52,7 @@ struct GroupDev {
     DBuf<int> mem, mem_cell, cell_count, cell_start, cell_fill, tmp, perm, slot_cell, inv_perm, list, wg_sys,
         cell_sys;
     DBuf<double4> r64;
-    DBuf<int4> pos4;
+    DBuf<int4> pos4, pos4_ref;
     DBuf<double> part;
 };
 
@@ -75,6 +75,7 @@ struct ibx_ctx {
     DBuf<DevSys> sys;
     DBuf<int> wg_items, itype, label, excl_off, excl, vs_of, vs_center, vs_moff, vs_members;
     DBuf<int> term_meta, inc_off, wg_terms, nb_map, flags, step;
+    DBuf<int> dmax;                  // [group][system] largest displacement^2 since the rebuild (float bits)
     DBuf<int4> term_slot;           // contribution slot of each member of a term
     DBuf<int4> term_mem;
     DBuf<float> contrib;
@@ -407,6 +408,11 @@ int build_device(ibx_ctx *c) {
         st[8 * s + 1] = c->hs[s].temp_bead0;
     }
     if ((rc = upload(c, c->sysstate, st))) return rc;
+    HIPCHK(c, c->dmax.alloc(2 * (size_t)S));
+    {
+        std::vector<int> inf(2 * (size_t)S, 0x7f800000);   // no bound until the first rebuild
+        if ((rc = upload(c, c->dmax, inf))) return rc;
+    }
     HIPCHK(c, c->flags.alloc(8));
     HIPCHK(c, hipMemset(c->flags.p, 0, c->flags.bytes()));
     HIPCHK(c, c->step.alloc(1));
@@ -458,6 +464,11 @@ int build_device(ibx_ctx *c) {
             dg.rc2 = rc * rc;
             dg.rc2u_lo = (float)((rc / unit) * (rc / unit) * (1.0 - band));
             dg.rc2u_hi = (float)((rc / unit) * (rc / unit) * (1.0 + band));
+            // front run: pairs closer than rc + delta at the rebuild; the back run is scanned only once a
+            // member has moved 0.49 delta (two members closing delta apart), so no pair is ever missed
+            const double delta = kInnerFrac * std::max(0.0, rl - rc);
+            dg.rin2u = (float)(((rc + delta) / unit) * ((rc + delta) / unit));
+            dg.dmax2u = (float)((0.49 * delta / unit) * (0.49 * delta / unit));
             if (nm > 0) {
                 any = true;
                 double vol = h.box[0] * h.box[1] * h.box[2];
@@ -488,6 +499,7 @@ int build_device(ibx_ctx *c) {
         HIPCHK(c, G.slot_cell.alloc(slot));
         HIPCHK(c, G.r64.alloc(slot));
         HIPCHK(c, G.pos4.alloc(slot));
+        HIPCHK(c, G.pos4_ref.alloc(slot));
         HIPCHK(c, G.nneb.alloc(slot));
         HIPCHK(c, G.part.alloc((size_t)G.nwg * (kBlock / kNbAtomsPerBlock) * kNbPart));
         HIPCHK(c, hipMemset(G.part.p, 0, G.part.bytes()));
@@ -509,7 +521,8 @@ RebuildArgs rebuild_args(ibx_ctx *c, int g) {
     a.mem = G.mem.p; a.s = c->s.p; a.itype = c->itype.p; a.label = c->label.p;
     a.mem_cell = G.mem_cell.p; a.cell_count = G.cell_count.p; a.cell_start = G.cell_start.p;
     a.cell_fill = G.cell_fill.p; a.tmp = G.tmp.p; a.perm = G.perm.p; a.slot_cell = G.slot_cell.p;
-    a.r64 = G.r64.p; a.pos4 = G.pos4.p; a.inv_perm = G.inv_perm.p;
+    a.r64 = G.r64.p; a.pos4 = G.pos4.p; a.inv_perm = G.inv_perm.p; a.pos4_ref = G.pos4_ref.p;
+    a.dmax = c->dmax.p + (size_t)g * c->nsys; a.nsys = c->nsys;
     a.excl_off = c->excl_off.p; a.excl = c->excl.p; a.list = G.list.p; a.nneb = G.nneb.p; a.flags = c->flags.p;
     return a;
 }
@@ -538,7 +551,7 @@ void enqueue_forces(ibx_ctx *c, int mode) {
         a.sys = c->sys.p; a.wg_sys = G.wg_sys.p; a.g = g; a.nwg = G.nwg; a.cap = G.cap; a.nitems_total = c->n_items;
         a.max_rows_lds = G.max_rows; a.multi = G.multi; a.perm = G.perm.p; a.pos4 = G.pos4.p; a.list = G.list.p;
         a.nneb = G.nneb.p; a.rows = c->nb_rows.p; a.xoff = c->nb_xoff.p; a.tabs = c->nb_tabs.p; a.map = c->nb_map.p;
-        a.fnb = c->fnb.p; a.part = G.part.p; a.s = c->s.p;
+        a.fnb = c->fnb.p; a.part = G.part.p; a.s = c->s.p; a.dmax = c->dmax.p; a.nsys = c->nsys;
         a.cubic = true;
         for (const DevSys &D : c->dsys) a.cubic = a.cubic && D.rho[1] == 1.0f && D.rho[2] == 1.0f;
         launch_nonbonded(c->st, a);
@@ -572,16 +585,18 @@ void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
     m.vs_coeff = c->vs_coeff.p;
     for (int g = 0; g < 2; ++g) {
         m.pos4[g] = c->gr[g].present ? c->gr[g].pos4.p : nullptr;
+        m.pos4_ref[g] = c->gr[g].present ? c->gr[g].pos4_ref.p : nullptr;
         m.inv_perm[g] = c->gr[g].present ? c->gr[g].inv_perm.p : nullptr;
     }
-    if (!m.pos4[1]) { m.pos4[1] = m.pos4[0]; m.inv_perm[1] = m.inv_perm[0]; }
+    if (!m.pos4[1]) { m.pos4[1] = m.pos4[0]; m.pos4_ref[1] = m.pos4_ref[0]; m.inv_perm[1] = m.inv_perm[0]; }
+    m.dmax = c->dmax.p; m.nsys = c->nsys;
     m.sysstate = c->sysstate.p; m.part = c->part_mv.p; m.step = c->step.p; m.nitems_total = c->n_items; m.mode = mode;
     launch_move(c->st, c->item_nwg, c->sys.p, c->wg_items.p, m);
     if (ev) (void)hipEventRecord((*ev)[3], c->st);
     if (mode == 0 && c->gr[1].present && c->hs[0].hybrid)
         launch_vs_place(c->st, c->item_nwg, c->sys.p, c->wg_items.p, c->s.p, c->n_items, c->label.p, c->itype.p,
                         c->vs_center.p, c->vs_moff.p, c->vs_members.p, c->mass.p, c->vs_invmass.p, c->gr[1].pos4.p,
-                        c->gr[1].inv_perm.p);
+                        c->gr[1].pos4_ref.p, c->gr[1].inv_perm.p, c->dmax.p + c->nsys);
     constexpr int per = kBlock / kNbAtomsPerBlock;
     const GroupDev &G1 = c->gr[1].present ? c->gr[1] : c->gr[0];
     launch_finalize(c->st, c->nsys, c->sys.p, c->gr[0].part.p, G1.part.p, c->part_bd.p, c->part_mv.p,

@@ -30,13 +30,14 @@ constexpr int kListLanes = 16;            // lanes testing the candidates of one
 constexpr int kListAtoms = kBlock / kListLanes;
 constexpr int kListRanges = 64;           // candidate slot ranges per member (25 columns, at most 2 each)
 constexpr int kMinCells = 5;              // cells per axis for the +-2 cell stencil
+constexpr double kInnerFrac = 1.0 / 3.0;  // delta = this fraction of the list skin rlist - rc
 
 // Position of entry k of sorted slot i in the Verlet list.  Entries of the
 // kAtomsPerWave slots of one wave are interleaved so that the kLanes lanes
 // serving a slot, and the slots of a wave, read consecutive words.
-// A slot's cap entries hold two runs: nneb.x pairs that were inside rc when
-// the list was built, from k = 0 up, and nneb.y pairs with rc <= r < rlist,
-// from k = cap - 1 down.
+// A slot's cap entries hold two runs: nneb.x pairs that were closer than
+// rc + delta when the list was built, from k = 0 up, and nneb.y pairs with
+// rc + delta <= r < rlist, from k = cap - 1 down.
 __host__ __device__ __forceinline__ size_t list_pos(int slot, int k, int cap) {
     return (size_t)(slot / kAtomsPerWave) * kAtomsPerWave * cap + (size_t)(k / kLanes) * kWave +
            (slot % kAtomsPerWave) * kLanes + (k % kLanes);
@@ -53,6 +54,9 @@ struct DevGroup {
     float rc2u;                    // rcut^2 in working units^2
     float rc2u_lo, rc2u_hi;        // band around rc2u where the FP64 cutoff test decides
     double rc2;                    // rcut^2 (FP64 cutoff criterion)
+    float rin2u;                   // (rc + delta)^2: pairs closer than this at the rebuild form the front run
+    float dmax2u;                  // (delta/2)^2 less a margin: while every member moved less, the back run
+                                   // cannot reach rc and is skipped
     int wg_off, nwg;               // workgroup range of this system in the group's grid
 };
 

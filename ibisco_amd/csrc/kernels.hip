@@ -111,9 +111,31 @@ __device__ __forceinline__ int pos_label(int w) { return (w >> 16) & 0xff; }
 // ------------------------------------------------------------------------------------------------
 // list rebuild (replaces LINKS.f90, MAPS.f90, NEW_NEIGHBOUR_WITHLIST.f90)
 
-__global__ void k_zero_int(int *p, int n) {
+__global__ void k_fill_int(int *p, int n, int v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
+    if (i < n) p[i] = v;
+}
+
+// Squared displacement, in working units^2, between two fixed-point positions.
+__device__ __forceinline__ float fixed_d2(const int4 a, const int4 b, const DevSys &S) {
+    const float dx = (float)(int)((uint32_t)a.x - (uint32_t)b.x);
+    const float dy = (float)(int)((uint32_t)a.y - (uint32_t)b.y) * S.rho[1];
+    const float dz = (float)(int)((uint32_t)a.z - (uint32_t)b.z) * S.rho[2];
+    return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+}
+
+// Largest value over a block of kBlock threads, into *dst (float bits, atomicMax on
+// non-negative floats orders like the values).  Every thread must call it.
+__device__ __forceinline__ void block_max_to(float x, int *dst, float *lds /* kBlock / kWave */) {
+    for (int off = kWave / 2; off > 0; off >>= 1) x = fmaxf(x, __shfl_xor(x, off, kWave));
+    if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = lds[0];
+        for (int w = 1; w < kBlock / kWave; ++w) m = fmaxf(m, lds[w]);
+        if (m > 0.f) atomicMax(dst, __float_as_int(m));
+    }
+    __syncthreads();
 }
 
 // Cell of every member from its folded FP64 position (LINKS.f90:16-18, clamped).
@@ -216,20 +238,24 @@ __global__ void __launch_bounds__(kBlock) k_pack(const DevSys *__restrict__ sys,
                                                  const int *__restrict__ perm, const double *__restrict__ s,
                                                  int nitems_total, const int *__restrict__ itype,
                                                  const int *__restrict__ label, double4 *__restrict__ r64,
-                                                 int4 *__restrict__ pos4, int *__restrict__ inv_perm) {
+                                                 int4 *__restrict__ pos4, int *__restrict__ inv_perm,
+                                                 int4 *__restrict__ pos4_ref) {
     const DevSys &S = sys[wg_sys[blockIdx.x]];
     int slot = blockIdx.x * kBlock + threadIdx.x;
     int it = perm[slot];
     if (it < 0) {
         pos4[slot] = make_int4(0, 0, 0, -1);
+        if (pos4_ref) pos4_ref[slot] = make_int4(0, 0, 0, -1);
         r64[slot] = make_double4(0, 0, 0, 0);
         return;
     }
     double sp[3] = {s[it], s[nitems_total + it], s[2 * nitems_total + it]}, r[3];
     fold3(sp, S.box2, r);
     r64[slot] = make_double4(r[0], r[1], r[2], 0.0);
-    pos4[slot] = make_int4((int)to_fixed(sp[0], S.inv_box_frac[0]), (int)to_fixed(sp[1], S.inv_box_frac[1]),
-                           (int)to_fixed(sp[2], S.inv_box_frac[2]), itype[it] | (label[it] << 16));
+    const int4 p = make_int4((int)to_fixed(sp[0], S.inv_box_frac[0]), (int)to_fixed(sp[1], S.inv_box_frac[1]),
+                             (int)to_fixed(sp[2], S.inv_box_frac[2]), itype[it] | (label[it] << 16));
+    pos4[slot] = p;
+    if (pos4_ref) pos4_ref[slot] = p;   // positions of the rebuild, for the displacement bound
     inv_perm[it] = slot;
 }
 
@@ -251,8 +277,8 @@ __device__ __forceinline__ float cell_gap(int d, float f, float a) {
 // each (the z extent reachable from i, split where it wraps), and kListLanes
 // lanes walk their concatenation, 16 candidates per step.  Hits are compacted
 // with a wave ballot, so each list is in candidate order and deterministic.
-// Pairs inside rc when the list is built fill the front run, the others the
-// back run.  With fewer than five cells along an axis every member scans the
+// Pairs closer than rc + delta when the list is built fill the front run, the
+// others the back run.  With fewer than five cells along an axis every member scans the
 // whole group once.
 template <bool kCubic>
 __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
@@ -282,7 +308,7 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
     const double rl2 = G.rl2;
     const bool hybrid = S.hybrid != 0;
     const float rho0 = S.rho[0], rho1 = S.rho[1], rho2 = S.rho[2];
-    const float lo = G.rl2u_lo, hi = G.rl2u_hi, rc2u = G.rc2u;
+    const float lo = G.rl2u_lo, hi = G.rl2u_hi, rin2u = G.rin2u;
 
     int4 pi = make_int4(0, 0, 0, 0);
     double4 ri = make_double4(0, 0, 0, 0);
@@ -415,7 +441,7 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
             if (cand && !skip && ex1 - ex0 > 8)
                 for (int e = ex0 + 8; e < ex1; ++e) skip |= (inv_perm[excl[e]] == j);
             hit = cand && !skip;
-            inner = r2u < rc2u;
+            inner = r2u < rin2u;
         }
         const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
         const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
@@ -528,9 +554,10 @@ __device__ __forceinline__ void nb_pair(const NbCtx &C, const int4 pi, int j, co
 
 // Nonbonded forces (replaces NONBONDED_FORCE.F90:11-109).
 //
-// kLanes lanes per sorted slot walk its full list, first the run of pairs
-// that were inside rc when the list was built, then the rest, so the lanes
-// of a wave mostly agree on the cutoff branch.  Positions are int32 fixed
+// kLanes lanes per sorted slot walk its list: the run of pairs that were
+// closer than rc + delta at the rebuild, and the rest only once some member has
+// moved delta/2 since (before that they cannot be inside rc: the result is the
+// reference's full list scan).  Positions are int32 fixed
 // point, so the minimum image is the wrap-around of the integer difference.
 // Tables are staged in LDS when they fit (kLds), else read through the cache.
 // Each pair is visited from both ends; energies and virial are halved.  The
@@ -539,7 +566,8 @@ template <bool kMulti, bool kLds, int kGroup, bool kCubic>
 __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
                                                       const int *__restrict__ perm, const int4 *__restrict__ pos4,
                                                       const int *__restrict__ list, const int2 *__restrict__ nneb,
-                                                      int cap, const float4 *__restrict__ nb_rows,
+                                                      const int *__restrict__ dmax, int nsys, int cap,
+                                                      const float4 *__restrict__ nb_rows,
                                                       const float *__restrict__ nb_xoff,
                                                       const NbTab *__restrict__ nb_tabs, const int *__restrict__ nb_map,
                                                       int nitems_total, const double *__restrict__ s,
@@ -597,7 +625,11 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
         const int4 pi = pos4[i];
         const int li = pos_label(pi.w);
         const int2 nn = nneb[i];
-        const int n_in = min(nn.x, cap), n_out = min(nn.y, cap - n_in);
+        const int n_in = min(nn.x, cap);
+        // the back run (rc + delta <= r < rlist at the rebuild) can only reach rc once some member of
+        // the system has moved delta/2 since then
+        const bool far_ok = __int_as_float(dmax[kGroup * nsys + wg_sys[blk / (kBlock / kNbAtomsPerBlock)]]) < G.dmax2u;
+        const int n_out = far_ok ? 0 : min(nn.y, cap - n_in);
         // front run: entry q + kLanes*m at lp[m*kWave]; back run: entry q + kLanes*m at lb[-m*kWave]
         const int *lp = list + list_pos(i, q, cap);
         const int *lb = list + list_pos(i, cap - 1 - q, cap);
@@ -970,7 +1002,10 @@ struct MoveArgs {
     const int *vs_of;
     const double *vs_coeff;
     int4 *pos4[2];
+    const int4 *pos4_ref[2];
     const int *inv_perm[2];
+    int *dmax;                // [group][system] largest displacement^2 since the rebuild (float bits)
+    int nsys;
     const double *sysstate;   // per system: temp_atom, temp_bead, pending momentum x,y,z
     double *part;
     int *step;
@@ -996,6 +1031,8 @@ __device__ __forceinline__ void gather_contrib(const float *__restrict__ contrib
 
 __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, MoveArgs A) {
     __shared__ double red[21 * 4];
+    __shared__ float mx[kBlock / kWave];
+    float disp[2] = {0.f, 0.f};   // squared displacement since the rebuild, per group
     const int s_id = wg_sys[blockIdx.x];
     const DevSys &S = sys[s_id];
     const int it = blockIdx.x * kBlock + threadIdx.x;
@@ -1061,8 +1098,13 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
                 p.z = (int)to_fixed(sp[2], S.inv_box_frac[2]);
                 p.w = ty | (lab << 16);
                 A.pos4[g][slot] = p;
+                disp[g] = fixed_d2(p, A.pos4_ref[g][slot], S);
             }
         }
+    }
+    if (A.mode == 0) {
+        block_max_to(disp[0], A.dmax + s_id, mx);
+        if (S.hybrid) block_max_to(disp[1], A.dmax + A.nsys + s_id, mx);
     }
     block_sum<21>(acc, red);
     if (threadIdx.x == 0) {
@@ -1078,33 +1120,40 @@ __global__ void __launch_bounds__(kBlock) k_vs_place(const DevSys *__restrict__ 
                                                      const int *__restrict__ itype, const int *__restrict__ vs_center,
                                                      const int *__restrict__ vs_moff, const int *__restrict__ vs_members,
                                                      const double *__restrict__ mass, const double *__restrict__ vs_invmass,
-                                                     int4 *__restrict__ pos4, const int *__restrict__ inv_perm) {
-    const DevSys &S = sys[wg_sys[blockIdx.x]];
+                                                     int4 *__restrict__ pos4, const int4 *__restrict__ pos4_ref,
+                                                     const int *__restrict__ inv_perm, int *__restrict__ dmax) {
+    __shared__ float mx[kBlock / kWave];
+    const int s_id = wg_sys[blockIdx.x];
+    const DevSys &S = sys[s_id];
     const int it = blockIdx.x * kBlock + threadIdx.x;
     const int n = nitems_total;
-    if (itype[it] < 0 || label[it] != 3) return;
-    double p[3];
-    const int c = vs_center[it];
-    if (c >= 0) {
-        for (int k = 0; k < 3; ++k) p[k] = s[k * n + c];
-    } else {
-        double sum[3] = {0, 0, 0};
-        for (int q = vs_moff[it]; q < vs_moff[it + 1]; ++q) {
-            const int a = vs_members[q];
-            const double m = mass[a];
-            for (int k = 0; k < 3; ++k) sum[k] += m * s[k * n + a];
+    float disp = 0.f;
+    if (itype[it] >= 0 && label[it] == 3) {
+        double p[3];
+        const int c = vs_center[it];
+        if (c >= 0) {
+            for (int k = 0; k < 3; ++k) p[k] = s[k * n + c];
+        } else {
+            double sum[3] = {0, 0, 0};
+            for (int q = vs_moff[it]; q < vs_moff[it + 1]; ++q) {
+                const int a = vs_members[q];
+                const double m = mass[a];
+                for (int k = 0; k < 3; ++k) sum[k] += m * s[k * n + a];
+            }
+            for (int k = 0; k < 3; ++k) p[k] = sum[k] * vs_invmass[it];
         }
-        for (int k = 0; k < 3; ++k) p[k] = sum[k] * vs_invmass[it];
+        for (int k = 0; k < 3; ++k) s[k * n + it] = p[k];
+        const int slot = inv_perm[it];
+        if (slot >= 0) {
+            const int4 q4 = make_int4((int)to_fixed(p[0], S.inv_box_frac[0]), (int)to_fixed(p[1], S.inv_box_frac[1]),
+                                      (int)to_fixed(p[2], S.inv_box_frac[2]), itype[it] | (3 << 16));
+            pos4[slot] = q4;
+            disp = fixed_d2(q4, pos4_ref[slot], S);
+        }
     }
-    for (int k = 0; k < 3; ++k) s[k * n + it] = p[k];
-    const int slot = inv_perm[it];
-    if (slot >= 0)
-        pos4[slot] = make_int4((int)to_fixed(p[0], S.inv_box_frac[0]), (int)to_fixed(p[1], S.inv_box_frac[1]),
-                               (int)to_fixed(p[2], S.inv_box_frac[2]), itype[it] | (3 << 16));
+    block_max_to(disp, dmax + s_id, mx);
 }
 
-// Per-system reduction of the workgroup partials into the observables
-// (AVERAGE.f90:10-47 window sums included).  One workgroup per system.
 // Sum of p[w] for w = a + t, a + t + stride, ... < b, eight loads in flight;
 // the order of the additions is fixed.
 __device__ __forceinline__ double strided_sum(const double *__restrict__ p, int a, int b, int t, int stride) {
@@ -1235,7 +1284,7 @@ __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, 
 // launch wrappers
 
 void launch_zero_int(hipStream_t st, int *p, int n) {
-    if (n > 0) hipLaunchKernelGGL(k_zero_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
+    if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, 0);
 }
 
 void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
@@ -1248,7 +1297,8 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
     hipLaunchKernelGGL(k_sort_cells, dim3(a.ncell_total), dim3(kBlock), 0, st, a.cell_sys, a.sys, a.g, a.cell_start,
                        a.cell_count, a.tmp, a.mem, a.perm, a.slot_cell, a.flags);
     hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
-                       a.itype, a.label, a.r64, a.pos4, a.inv_perm);
+                       a.itype, a.label, a.r64, a.pos4, a.inv_perm, a.pos4_ref);
+    hipLaunchKernelGGL(k_fill_int, dim3((a.nsys + 255) / 256), dim3(256), 0, st, a.dmax, a.nsys, 0);
     const int nwg_list = a.nwg * (kBlock / kListAtoms);
     if (a.cubic)
         hipLaunchKernelGGL(k_build_list<true>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
@@ -1260,17 +1310,20 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
                            a.list, a.nneb, a.flags, nwg_list);
 }
 
+// New positions without a rebuild: the displacement bound no longer holds, so
+// the nonbonded kernel scans whole lists until the next rebuild.
 void launch_pack_only(hipStream_t st, const RebuildArgs &a) {
     hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
-                       a.itype, a.label, a.r64, a.pos4, a.inv_perm);
+                       a.itype, a.label, a.r64, a.pos4, a.inv_perm, (int4 *)nullptr);
+    hipLaunchKernelGGL(k_fill_int, dim3((a.nsys + 255) / 256), dim3(256), 0, st, a.dmax, a.nsys, 0x7f800000);
 }
 
 template <bool M, bool L, int G, bool K>
 static void nb_go(hipStream_t st, const NbArgs &a, size_t lds) {
     const int nblk = a.nwg * (kBlock / kNbAtomsPerBlock);
     hipLaunchKernelGGL((k_nonbonded<M, L, G, K>), dim3(xcd_grid(nblk)), dim3(kBlock), lds, st, a.sys, a.wg_sys, a.perm,
-                       a.pos4, a.list, a.nneb, a.cap, a.rows, a.xoff, a.tabs, a.map, a.nitems_total, a.s, a.fnb,
-                       a.part, nblk);
+                       a.pos4, a.list, a.nneb, a.dmax, a.nsys, a.cap, a.rows, a.xoff, a.tabs, a.map, a.nitems_total,
+                       a.s, a.fnb, a.part, nblk);
 }
 
 template <bool M, bool L, int G>
@@ -1310,15 +1363,17 @@ void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, 
     A.ncontrib = m.ncontrib; A.ftot = m.ftot; A.itype = m.itype; A.label = m.label;
     A.mass = m.mass; A.invmass = m.invmass; A.vs_of = m.vs_of; A.vs_coeff = m.vs_coeff;
     A.pos4[0] = m.pos4[0]; A.pos4[1] = m.pos4[1]; A.inv_perm[0] = m.inv_perm[0]; A.inv_perm[1] = m.inv_perm[1];
+    A.pos4_ref[0] = m.pos4_ref[0]; A.pos4_ref[1] = m.pos4_ref[1]; A.dmax = m.dmax; A.nsys = m.nsys;
     A.sysstate = m.sysstate; A.part = m.part; A.step = m.step; A.nitems_total = m.nitems_total; A.mode = m.mode;
     hipLaunchKernelGGL(k_move, dim3(nwg), dim3(kBlock), 0, st, sys, wg_sys, A);
 }
 
 void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, double *s, int nitems_total,
                      const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,
-                     const double *mass, const double *vs_invmass, int4 *pos4, const int *inv_perm) {
+                     const double *mass, const double *vs_invmass, int4 *pos4, const int4 *pos4_ref,
+                     const int *inv_perm, int *dmax) {
     hipLaunchKernelGGL(k_vs_place, dim3(nwg), dim3(kBlock), 0, st, sys, wg_sys, s, nitems_total, label, itype, vs_center,
-                       vs_moff, vs_members, mass, vs_invmass, pos4, inv_perm);
+                       vs_moff, vs_members, mass, vs_invmass, pos4, pos4_ref, inv_perm, dmax);
 }
 
 void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,

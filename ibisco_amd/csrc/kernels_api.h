@@ -17,8 +17,10 @@ struct RebuildArgs {
     const int *itype, *label;
     int *mem_cell, *cell_count, *cell_start, *cell_fill, *tmp, *perm, *slot_cell;
     double4 *r64;
-    int4 *pos4;
+    int4 *pos4, *pos4_ref;
     int *inv_perm;
+    int *dmax;          // this group's per-system largest displacement^2 since the rebuild (float bits)
+    int nsys;
     const int *excl_off, *excl;
     int *list;
     int2 *nneb;
@@ -34,6 +36,8 @@ struct NbArgs {
     const int4 *pos4;
     const int *list;
     const int2 *nneb;
+    const int *dmax;
+    int nsys;
     const float4 *rows;
     const float *xoff;
     const NbTab *tabs;
@@ -64,7 +68,10 @@ struct MoveLaunch {
     const int *vs_of;
     const double *vs_coeff;
     int4 *pos4[2];
+    const int4 *pos4_ref[2];
     const int *inv_perm[2];
+    int *dmax;          // [group][system] largest displacement^2 since the rebuild (float bits)
+    int nsys;
     const double *sysstate;
     double *part;
     int *step;
@@ -80,7 +87,8 @@ void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &
 void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const MoveLaunch &m);
 void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, double *s, int nitems_total,
                      const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,
-                     const double *mass, const double *vs_invmass, int4 *pos4, const int *inv_perm);
+                     const double *mass, const double *vs_invmass, int4 *pos4, const int4 *pos4_ref,
+                     const int *inv_perm, int *dmax);
 void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,
                      const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *fsum,
                      double *obs, double *avg, double *sysstate, const int *step, int mode);

@@ -115,3 +115,36 @@ def test_graph_equals_eager():
     assert np.array_equal(sa, sb)
     assert np.array_equal(va, vb)
     assert np.array_equal(a.obs()["pt"], b.obs()["pt"])
+
+
+def test_back_run_after_large_displacement(oracle_mod):
+    """The force kernel skips the back run of each list (pairs beyond rc + delta at the rebuild) only while
+    every member has moved less than delta/2.  An equilibrated melt is run 30 steps without a rebuild, so
+    members move further and back-run pairs come inside rc: forces at the final positions must equal those
+    of a fresh list (the oracle), which holds as long as no pair beyond rlist came inside rc."""
+    import copy
+    from ibisco_amd import units, workloads
+    s = _sys_cg(n_chains=30, seed=21)
+    pre = Engine(s)
+    workloads.equilibrate(pre, s, stage_steps=100, final_steps=400)
+    s_eq, v_eq = pre.state()
+    pre.close()
+    t = copy.deepcopy(s)
+    t.s, t.v = s_eq.copy(), units.vel_from_reduced(v_eq)
+    t.control.nupdate = 10 ** 6
+    e = Engine(t)
+    e.step(30)
+    e.compute_forces(rebuild=False)
+    sg, vg = e.state()
+    fg = e.forces()
+    delta = (t.control.rlist - t.control.cutoff) / 3.0
+    moved = float(np.sqrt(((sg - s_eq) ** 2).sum(1)).max())
+    print("largest displacement", moved, "delta/2", delta / 2, flush=True)
+    assert delta / 2 < moved < 0.5 * (t.control.rlist - t.control.cutoff)
+    o = oracle_mod.Oracle(t)
+    o.set_state(sg, vg)
+    o.update_lists()
+    o.new_force()
+    rr = _rel_rms(fg, o.forces())
+    print("force rel rms after", moved, "nm:", rr)
+    assert rr <= FORCE_RTOL

@@ -64,6 +64,53 @@ def cpu_baseline(sysm, nsteps, threads):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def replicas_compare(nrep, steps, warmup, seed=3):
+    """Config-3 replicas (log-uniform 2k-50k beads, each with its own corrected table): atom-steps/s of all
+    replicas in one context (every kernel launch covers all of them, HIP graph per list interval) against
+    the plain path, each replica alone in its own context, one after the other, from the same equilibrated
+    states.  RDFs accumulate on the device at every list rebuild in the batched run."""
+    import copy
+    from ibisco_amd import units, workloads
+    from ibisco_amd.engine import Engine
+
+    systems = workloads.config3(nrep=nrep, seed=seed)
+    beads = [s.natoms for s in systems]
+    total = sum(beads)
+    print(f"[bench] {nrep} replicas, {total} beads (min {min(beads)}, max {max(beads)})", file=sys.stderr, flush=True)
+    eb = Engine(systems)
+    workloads.equilibrate(eb, systems, stage_steps=100, final_steps=400)
+    print("[bench] replicas equilibrated", file=sys.stderr, flush=True)
+    eb.rdf_enable(150, 0.01, every=1)
+    eb.step(warmup, graph=True)
+    eb.obs()
+    t0 = time.perf_counter()
+    eb.step(steps, graph=True)
+    eb.obs()
+    tb = time.perf_counter() - t0
+    states = [eb.state(k) for k in range(nrep)]
+    eb.close()
+    tp = 0.0
+    for k, sy in enumerate(systems):
+        one = copy.copy(sy)
+        one.s, one.v = states[k][0][:sy.natoms].copy(), units.vel_from_reduced(states[k][1])
+        e = Engine(one)
+        e.step(warmup, graph=True)
+        e.obs()
+        t0 = time.perf_counter()
+        e.step(steps, graph=True)
+        e.obs()
+        tp += time.perf_counter() - t0
+        e.close()
+        if (k + 1) % 32 == 0:
+            print(f"[bench] plain path: {k + 1}/{nrep} replicas", file=sys.stderr, flush=True)
+    vb, vp = total * steps / tb, total * steps / tp
+    print(f"[bench] replicas: batched {vb:.3e}, plain {vp:.3e} atom-steps/s", file=sys.stderr, flush=True)
+    return {"workload": "cg_replicas_corrected_tables", "replicas": nrep, "beads": total, "steps": steps,
+            "batched": {"value": vb, "ms_per_step": 1e3 * tb / steps},
+            "plain": {"value": vp, "ms_per_step": 1e3 * tp / steps, "path": "one context per replica, in turn"},
+            "ratio": vb / vp}
+
+
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_nonbonded_ua100k_traffic.md")
 
 
@@ -94,6 +141,9 @@ def main():
     ap.add_argument("--nbeads", type=int, default=250)
     ap.add_argument("--eager", action="store_true",
                     help="launch kernel by kernel instead of through HIP graphs (profiling runs)")
+    ap.add_argument("--replicas", type=int, default=0, metavar="NREP",
+                    help="also compare NREP config-3 replicas batched in one context against one context each")
+    ap.add_argument("--replica-steps", type=int, default=200)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -154,6 +204,7 @@ def main():
         sysm.v = units.vel_from_reduced(vel)
         print("[bench] timing the reference CPU path", file=sys.stderr, flush=True)
         cpu = cpu_baseline(sysm, args.cpu_steps, args.cpu_threads)
+    batched = replicas_compare(args.replicas, args.replica_steps, 50) if args.replicas > 0 else None
     out = {
         "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps, "higher_is_better": True,
@@ -170,6 +221,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
     }
+    if batched:
+        out["batched_vs_plain"] = batched
     print(json.dumps(out))
 
 

@@ -90,9 +90,9 @@ struct ibx_ctx {
     std::vector<NbTab> h_nb_tabs;
     // rdf
     DBuf<unsigned long long> rdf;
+    DBuf<int> rdf_samples;           // histograms accumulated per system (counted on the device)
     int rdf_nbin = 0, rdf_every = 0;
     double rdf_dr = 0;
-    std::vector<int> rdf_samples;
     // graph
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
@@ -535,12 +535,8 @@ void enqueue_rebuild(ibx_ctx *c) {
 void enqueue_rdf(ibx_ctx *c) {
     if (!c->rdf_nbin || !c->gr[0].present) return;
     GroupDev &G = c->gr[0];
-    // bins in working units of each system: identical boxes across a batch are not required,
-    // the unit differs per system, so the bin width is rescaled per system on the host side
-    float inv_dr_u = (float)(c->dsys[0].unit / c->rdf_dr);
     launch_rdf(c->st, G.nwg, c->sys.p, G.wg_sys.p, G.perm.p, G.pos4.p, G.list.p, G.nneb.p, G.cap, c->rdf_nbin,
-               inv_dr_u, c->rdf.p);
-    for (auto &x : c->rdf_samples) ++x;
+               (float)(1.0 / c->rdf_dr), c->rdf.p, c->rdf_samples.p);
 }
 
 void enqueue_forces(ibx_ctx *c, int mode) {
@@ -1003,16 +999,15 @@ int ibx_get_pairs(ibx_ctx *c, int sys, int group, int *out, int64_t cap, int64_t
 
 int ibx_rdf_enable(ibx_ctx *c, int nbin, double dr, int every) {
     if (!c || nbin < 0 || nbin > 8192 || (nbin > 0 && !(dr > 0))) return fail(c, IBX_E_INVALID, "invalid RDF settings");
+    HIPCHK(c, hipStreamSynchronize(c->st));
     c->rdf_nbin = nbin;
     c->rdf_dr = dr;
     c->rdf_every = every;
-    c->rdf_samples.assign(c->nsys, 0);
     if (nbin) {
-        for (int s = 1; s < c->nsys; ++s)
-            if (std::fabs(c->dsys[s].unit - c->dsys[0].unit) > 0)
-                return fail(c, IBX_E_INVALID, "the device RDF needs equal box lengths across a batch");
         HIPCHK(c, c->rdf.alloc((size_t)c->nsys * nbin));
         HIPCHK(c, hipMemset(c->rdf.p, 0, c->rdf.bytes()));
+        HIPCHK(c, c->rdf_samples.alloc(c->nsys));
+        HIPCHK(c, hipMemset(c->rdf_samples.p, 0, c->rdf_samples.bytes()));
     }
     if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
     return 0;
@@ -1024,10 +1019,12 @@ int ibx_rdf_get(ibx_ctx *c, int sys, double *g, int reset) {
     std::vector<unsigned long long> h(c->rdf_nbin);
     HIPCHK(c, hipMemcpy(h.data(), c->rdf.p + (size_t)sys * c->rdf_nbin, h.size() * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost));
+    int ns = 0;
+    HIPCHK(c, hipMemcpy(&ns, c->rdf_samples.p + sys, sizeof(int), hipMemcpyDeviceToHost));
     const HostSystem &hs = c->hs[sys];
     const double n = (double)hs.members[0].size();
     const double vol = hs.box[0] * hs.box[1] * hs.box[2];
-    const double samples = std::max(1, c->rdf_samples[sys]);
+    const double samples = std::max(1, ns);
     for (int b = 0; b < c->rdf_nbin; ++b) {
         const double r0 = b * c->rdf_dr, r1 = (b + 1) * c->rdf_dr;
         const double shell = 4.0 / 3.0 * M_PI * (r1 * r1 * r1 - r0 * r0 * r0);
@@ -1035,7 +1032,7 @@ int ibx_rdf_get(ibx_ctx *c, int sys, double *g, int reset) {
     }
     if (reset) {
         HIPCHK(c, hipMemset(c->rdf.p + (size_t)sys * c->rdf_nbin, 0, c->rdf_nbin * sizeof(unsigned long long)));
-        c->rdf_samples[sys] = 0;
+        HIPCHK(c, hipMemset(c->rdf_samples.p + sys, 0, sizeof(int)));
     }
     return 0;
 }

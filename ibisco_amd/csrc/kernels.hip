@@ -1255,10 +1255,13 @@ __global__ void __launch_bounds__(kWave) k_finalize(const DevSys *__restrict__ s
 __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
                                                 const int *__restrict__ perm, const int4 *__restrict__ pos4,
                                                 const int *__restrict__ list, const int2 *__restrict__ nneb, int cap,
-                                                int nbin, float inv_dr_u, unsigned long long *__restrict__ hist) {
+                                                int nbin, float inv_dr, unsigned long long *__restrict__ hist,
+                                                int *__restrict__ samples) {
     extern __shared__ unsigned int h[];
     const int s_id = wg_sys[blockIdx.x];
     const DevSys &S = sys[s_id];
+    const float inv_dr_u = (float)S.unit * inv_dr;   // bins in this system's working units
+    if (threadIdx.x == 0 && (int)blockIdx.x == S.g[0].wg_off) atomicAdd(&samples[s_id], 1);
     for (int b = threadIdx.x; b < nbin; b += kBlock) h[b] = 0;
     __syncthreads();
     const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -1386,9 +1389,10 @@ void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *
 }
 
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
-                const int *list, const int2 *nneb, int cap, int nbin, float inv_dr_u, unsigned long long *hist) {
+                const int *list, const int2 *nneb, int cap, int nbin, float inv_dr, unsigned long long *hist,
+                int *samples) {
     hipLaunchKernelGGL(k_rdf, dim3(nwg), dim3(kBlock), (size_t)nbin * sizeof(unsigned int), st, sys, wg_sys, perm, pos4,
-                       list, nneb, cap, nbin, inv_dr_u, hist);
+                       list, nneb, cap, nbin, inv_dr, hist, samples);
 }
 
 }  // namespace ibx

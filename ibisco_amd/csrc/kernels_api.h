@@ -93,6 +93,7 @@ void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *
                      const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *fsum,
                      double *obs, double *avg, double *sysstate, const int *step, int mode);
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
-                const int *list, const int2 *nneb, int cap, int nbin, float inv_dr_u, unsigned long long *hist);
+                const int *list, const int2 *nneb, int cap, int nbin, float inv_dr, unsigned long long *hist,
+                int *samples);
 
 }  // namespace ibx

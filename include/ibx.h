@@ -130,8 +130,12 @@ int ibx_set_thermostat(ibx_ctx *ctx, int ensemble, double temp_in, double fac, d
  * With out == NULL only *count is written. */
 int ibx_get_pairs(ibx_ctx *ctx, int sys, int group, int *out, int64_t cap, int64_t *count);
 
-/* Radial distribution function of system sys accumulated on the device every
- * `every` steps (0 disables) in nbin bins of width dr over the group-0 pairs. */
+/* Radial distribution function of each system, accumulated on the device at
+ * every list rebuild while every > 0 (0 pauses it), in nbin bins of width dr
+ * over the group-0 pairs of the list: nonbonded pairs only (exclusions are
+ * not in the list), and bins must stay below rlist.  Systems
+ * of a batch may have different boxes.  ibx_rdf_get normalises by the ideal
+ * gas shell counts and the number of accumulated rebuilds. */
 int ibx_rdf_enable(ibx_ctx *ctx, int nbin, double dr, int every);
 int ibx_rdf_get(ibx_ctx *ctx, int sys, double *g, int reset);
 

@@ -141,8 +141,9 @@ def main():
     ap.add_argument("--nbeads", type=int, default=250)
     ap.add_argument("--eager", action="store_true",
                     help="launch kernel by kernel instead of through HIP graphs (profiling runs)")
-    ap.add_argument("--replicas", type=int, default=0, metavar="NREP",
-                    help="also compare NREP config-3 replicas batched in one context against one context each")
+    ap.add_argument("--replicas", type=int, default=256, metavar="NREP",
+                    help="also compare NREP config-3 replicas batched in one context against one context each "
+                         "(0 skips it)")
     ap.add_argument("--replica-steps", type=int, default=200)
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB five times (HBM counter calibration)")

@@ -111,6 +111,105 @@ def replicas_compare(nrep, steps, warmup, seed=3):
             "ratio": vb / vp}
 
 
+def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=8, nsample=12, ncheck=4):
+    """Config-4 serving: Poisson arrivals of CG melt jobs (1k-100k beads, 500-5000 steps, each with its own
+    corrected table) submitted to the serving loop at their arrival times; every job returns its RDF and mean
+    pressure.  Job states are equilibrated (batched, untimed) before the stream starts.  A sample of the jobs
+    is then run alone, one context each (the plain path), for the throughput ratio, and the ``ncheck``
+    smallest jobs of that sample must equal their served results bit for bit."""
+    import queue
+    import threading
+    import numpy as np
+    from ibisco_amd import units, workloads
+    from ibisco_amd.engine import Engine, Server
+
+    stream = workloads.serving_stream(rate_hz, duration_s, seed)
+    systems = workloads.serving_jobs(stream)
+    nj = len(stream)
+    print(f"[bench] serving: {nj} jobs over {duration_s:.0f} s, {sum(s.natoms for s in systems)} beads",
+          file=sys.stderr, flush=True)
+    batch, nb = [], 0
+    for k, sy in enumerate(systems + [None]):
+        if sy is not None:
+            batch.append(sy)
+            nb += sy.natoms
+        if batch and (sy is None or nb > 2_000_000):
+            e = Engine(batch)
+            workloads.equilibrate(e, batch, stage_steps=100, final_steps=400)
+            for q, b in enumerate(batch):
+                s, v = e.state(q)
+                b.s, b.v = s[:b.natoms].copy(), units.vel_from_reduced(v)
+            e.close()
+            print(f"[bench] serving: {k}/{nj} job states equilibrated", file=sys.stderr, flush=True)
+            batch, nb = [], 0
+    sample = sorted(range(nj), key=lambda k: systems[k].natoms)[::max(1, nj // nsample)][:nsample]
+    check = set(sorted(sample, key=lambda k: systems[k].natoms)[:ncheck])
+    srv = Server(hbm_budget=budget, max_cohorts=max_cohorts, rdf_nbin=150, rdf_dr=0.01)
+    results = [None] * nj
+    ids = queue.Queue()
+
+    def collect():
+        for _ in range(nj):
+            k, j = ids.get()
+            results[k] = srv.wait(j)
+
+    th = threading.Thread(target=collect)
+    th.start()
+    t0 = time.perf_counter()
+    for k, (ta, _n, steps, _sd) in enumerate(stream):
+        lag = t0 + ta - time.perf_counter()
+        if lag > 0:
+            time.sleep(lag)
+        ids.put((k, srv.submit(systems[k], steps, keep_state=k in check)))
+        if (k + 1) % 100 == 0:
+            print(f"[bench] serving: {k + 1}/{nj} submitted, {srv.stats()['completed']:.0f} done",
+                  file=sys.stderr, flush=True)
+    while th.is_alive():
+        th.join(timeout=10.0)
+        print(f"[bench] serving: {srv.stats()['completed']:.0f}/{nj} done", file=sys.stderr, flush=True)
+    st = srv.stats()
+    srv.close()
+    lat = np.array([r["t_done"] - r["t_submit"] for r in results])
+    wait = np.array([r["t_admit"] - r["t_submit"] for r in results])
+    work = sum(r["natoms"] * r["nsteps"] for r in results)
+    span = max(r["t_done"] for r in results) - min(r["t_submit"] for r in results)
+    # the plain path on a sample of the same jobs, and the bit-for-bit check of the smallest of them
+    tp, wp, same = 0.0, 0, True
+    for k in sample:
+        sy, steps = systems[k], stream[k][2]
+        e = Engine(sy)
+        e.rdf_enable(150, 0.01)
+        e.obs()
+        ts = time.perf_counter()
+        e.step(steps, graph=True)
+        e.obs()
+        tp += time.perf_counter() - ts
+        wp += sy.natoms * steps
+        if k in check:
+            s, v = e.state()
+            r = results[k]
+            same = same and bool(np.array_equal(s, r["s"]) and np.array_equal(v, r["v"]) and
+                                 np.array_equal(e.rdf(0), r["rdf"]) and
+                                 np.array_equal(e.averages(0)[:29], r["sums"][:29]))
+        e.close()
+    vs, vp = work / span, wp / tp
+    print(f"[bench] serving: {vs:.3e} atom-steps/s, p50 {np.percentile(lat, 50):.2f} s, "
+          f"p99 {np.percentile(lat, 99):.2f} s; plain sample {vp:.3e}", file=sys.stderr, flush=True)
+    pres = [units.pressure_to_bar(r["mean_pressure"]) for r in results]
+    return {"workload": "cg_serving_poisson", "rate_jobs_per_s": rate_hz, "arrival_window_s": duration_s,
+            "jobs": nj, "atom_steps": work, "offered_atom_steps_per_s": work / duration_s,
+            "served": {"value": vs, "span_s": span, "latency_p50_s": float(np.percentile(lat, 50)),
+                       "latency_p99_s": float(np.percentile(lat, 99)),
+                       "admission_wait_p50_s": float(np.percentile(wait, 50)),
+                       "admission_wait_p99_s": float(np.percentile(wait, 99)),
+                       "batches": int(st["cohorts"]), "mean_jobs_per_batch": round(st["mean_cohort_jobs"], 2),
+                       "peak_concurrent_batches": int(st["peak_running"]),
+                       "peak_resident_bytes": int(st["peak_bytes"]), "budget_bytes": int(st["budget"])},
+            "plain_sample": {"value": vp, "jobs": len(sample), "path": "one context per job, in turn"},
+            "ratio": vs / vp, "alone_equals_served_bitwise": same, "jobs_checked_bitwise": len(check),
+            "mean_pressure_bar_p50": float(np.median(pres))}
+
+
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_nonbonded_ua100k_traffic.md")
 
 
@@ -145,6 +244,9 @@ def main():
                     help="also compare NREP config-3 replicas batched in one context against one context each "
                          "(0 skips it)")
     ap.add_argument("--replica-steps", type=int, default=200)
+    ap.add_argument("--serve-seconds", type=float, default=20.0,
+                    help="also run the serving loop on Poisson arrivals (20 jobs/s) for this many seconds "
+                         "(0 skips it)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -206,6 +308,7 @@ def main():
         print("[bench] timing the reference CPU path", file=sys.stderr, flush=True)
         cpu = cpu_baseline(sysm, args.cpu_steps, args.cpu_threads)
     batched = replicas_compare(args.replicas, args.replica_steps, 50) if args.replicas > 0 else None
+    served = serve_stream(20.0, args.serve_seconds) if args.serve_seconds > 0 else None
     out = {
         "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps, "higher_is_better": True,
@@ -224,6 +327,8 @@ def main():
     }
     if batched:
         out["batched_vs_plain"] = batched
+    if served:
+        out["served_vs_plain"] = served
     print(json.dumps(out))
 
 

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
+#include <cstddef>
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -317,6 +318,7 @@ int build_device(ibx_ctx *c) {
         D.volume = h.box[0] * h.box[1] * h.box[2];
         D.halt_drift = h.halt_drift < 0 ? -h.halt_drift : h.halt_drift;
         D.nmts = h.nmts; D.mts_quirk = h.mts_quirk; D.nupdate = h.nupdate;
+        D.live = 1;
     }
     for (int i = 0; i < N; ++i) {
         hex_off[i + 1] = (int)hex.size();   // placeholder, rebuilt below
@@ -511,6 +513,71 @@ int build_device(ibx_ctx *c) {
     return 0;
 }
 
+// Device bytes a context holds (every buffer it allocated).
+size_t device_bytes(const ibx_ctx *c) {
+    size_t b = c->sys.bytes() + c->wg_items.bytes() + c->itype.bytes() + c->label.bytes() + c->excl_off.bytes() +
+               c->excl.bytes() + c->vs_of.bytes() + c->vs_center.bytes() + c->vs_moff.bytes() + c->vs_members.bytes() +
+               c->term_meta.bytes() + c->inc_off.bytes() + c->wg_terms.bytes() + c->nb_map.bytes() + c->flags.bytes() +
+               c->step.bytes() + c->dmax.bytes() + c->term_slot.bytes() + c->term_mem.bytes() + c->contrib.bytes() +
+               c->s.bytes() + c->v.bytes() + c->mass.bytes() + c->invmass.bytes() + c->vs_coeff.bytes() +
+               c->vs_invmass.bytes() + c->part_bd.bytes() + c->part_mv.bytes() + c->fsum.bytes() + c->obs.bytes() +
+               c->avg.bytes() + c->sysstate.bytes() + c->fnb.bytes() + c->ftot.bytes() + c->nb_rows.bytes() +
+               c->bd_rows.bytes() + c->bd_x.bytes() + c->nb_tabs.bytes() + c->nb_xoff.bytes() + c->bd_tabs.bytes() +
+               c->rdf.bytes() + c->rdf_samples.bytes();
+    for (const GroupDev &G : c->gr)
+        b += G.nneb.bytes() + G.mem.bytes() + G.mem_cell.bytes() + G.cell_count.bytes() + G.cell_start.bytes() +
+             G.cell_fill.bytes() + G.tmp.bytes() + G.perm.bytes() + G.slot_cell.bytes() + G.inv_perm.bytes() +
+             G.list.bytes() + G.wg_sys.bytes() + G.cell_sys.bytes() + G.r64.bytes() + G.pos4.bytes() +
+             G.pos4_ref.bytes() + G.part.bytes();
+    return b;
+}
+
+}  // namespace
+
+// Device bytes a one-system context of h would allocate at creation (the sizes
+// build_device computes), plus an RDF of rdf_nbin bins.
+size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
+    auto at_least1 = [](size_t n) { return n ? n : (size_t)1; };
+    const size_t N = (size_t)pad_to(h.nitems, kBlock);
+    const size_t nterm = h.bonds.size() + h.angles.size() + h.tors.size() + h.oops.size();
+    const size_t members = 2 * h.bonds.size() + 3 * h.angles.size() + 4 * (h.tors.size() + h.oops.size());
+    const size_t nterm_pad = (size_t)pad_to(std::max<int>(1, (int)nterm), kBlock);
+    size_t bd_rows = 0, nb_rows = 0;
+    for (const auto *v : {&h.bt, &h.at, &h.tt, &h.ot})
+        for (const Table &tb : *v) bd_rows += (size_t)tb.ndat + 1;
+    for (const Table &tb : h.nt) nb_rows += (size_t)tb.ndat + 1;
+    size_t b = sizeof(DevSys) + (N / kBlock) * 4                 // sys, wg_items
+               + N * (4 + 4 + 4 + 4 + 8 + 8 + 8 + 8 + 24 + 24 + 12 + 12)   // itype label vs_of vs_center mass
+                                                                  // invmass vs_coeff vs_invmass s v fnb ftot
+               + 3 * (N + 1) * 4                                  // excl_off vs_moff inc_off
+               + at_least1(h.excl.size()) * 4 + at_least1(h.vs_atoms.size()) * 4
+               + nterm_pad * (4 + 16 + 16) + (nterm_pad / kBlock) * 4   // meta, members, slots, wg_terms
+               + at_least1(members) * 12                           // contributions
+               + (nterm_pad / kBlock) * kBdPart * 8 + (N / kBlock) * kMvPart * 8
+               + (size_t)(kFields + 2 * kObs + 8) * 8 + 2 * 4 + 8 * 4 + 4   // fsum obs avg sysstate dmax flags step
+               + at_least1(bd_rows) * (16 + 8) + at_least1(h.bt.size() + h.at.size() + h.tt.size() + h.ot.size()) *
+                                                     sizeof(BdTab)
+               + at_least1(nb_rows) * (16 + 4) + at_least1(h.nt.size()) * sizeof(NbTab) +
+               at_least1((size_t)h.ntype * h.ntype) * 4;
+    if (rdf_nbin > 0) b += (size_t)rdf_nbin * 8 + 4;
+    for (int g = 0; g < 2; ++g) {
+        const int nm = g < h.ngroup ? (int)h.members[g].size() : 0;
+        const size_t slots = (size_t)pad_to(nm, kBlock), nwg = slots / kBlock;
+        size_t ncell = 1;
+        for (int k = 0; k < 3; ++k) ncell *= (size_t)std::max(1, (int)(2.0 * h.box[k] / h.rlist[g]));
+        if (nm == 0) continue;   // an absent group allocates nothing
+        const double nbr = 4.0 / 3.0 * M_PI * h.rlist[g] * h.rlist[g] * h.rlist[g] * nm /
+                           (h.box[0] * h.box[1] * h.box[2]);
+        const size_t cap = (size_t)pad_to((int)(1.5 * nbr) + 64, 8);
+        b += slots * (4 * 5 + 32 + 16 + 16 + 8)                  // mem mem_cell tmp perm slot_cell r64 pos4 pos4_ref nneb
+             + N * 4 + nwg * 4 + ncell * 16                       // inv_perm, wg_sys, cell arrays
+             + nwg * (kBlock / kNbAtomsPerBlock) * kNbPart * 8 + slots * cap * 4;
+    }
+    return b;
+}
+
+namespace {
+
 RebuildArgs rebuild_args(ibx_ctx *c, int g) {
     GroupDev &G = c->gr[g];
     RebuildArgs a;
@@ -555,7 +622,7 @@ void enqueue_forces(ibx_ctx *c, int mode) {
 }
 
 void enqueue_terms(ibx_ctx *c, hipStream_t st) {
-    TermLaunch b{c->term_mem.p, c->term_meta.p, c->term_slot.p, c->bd_tabs.p, c->bd_rows.p, c->bd_x.p};
+    TermLaunch b{c->sys.p, c->term_mem.p, c->term_meta.p, c->term_slot.p, c->bd_tabs.p, c->bd_rows.p, c->bd_x.p};
     launch_terms(st, c->term_nwg, c->wg_terms.p, b, c->s.p, c->n_items, c->nterms, c->contrib.p, c->ncontrib,
                  c->part_bd.p, c->flags.p);
 }
@@ -665,13 +732,18 @@ int ensure_events(ibx_ctx *c, size_t n) {
 
 }  // namespace
 
-extern "C" {
+int ibx::device_free_bytes(size_t *free_bytes) {
+    size_t fr = 0, total = 0;
+    if (hipMemGetInfo(&fr, &total) != hipSuccess) return IBX_E_DEVICE;
+    *free_bytes = fr;
+    return IBX_OK;
+}
 
-const char *ibx_last_error(const ibx_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
-
-int ibx_create(int nsys, const ibx_system_desc *const *desc, ibx_ctx **out) {
-    if (!out || nsys < 1 || !desc) return fail(nullptr, IBX_E_INVALID, "invalid arguments");
+// Context of the host-side systems hs (moved in): device layout, first lists.
+int ibx::create_context(std::vector<HostSystem> &&hs, ibx_ctx **out) {
     *out = nullptr;
+    const int nsys = (int)hs.size();
+    if (nsys < 1) return fail(nullptr, IBX_E_INVALID, "no systems");
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess) return fail(nullptr, IBX_E_DEVICE, "no HIP device");
     hipDeviceProp_t prop;
@@ -680,11 +752,8 @@ int ibx_create(int nsys, const ibx_system_desc *const *desc, ibx_ctx **out) {
         return fail(nullptr, IBX_E_DEVICE, "device %s is not gfx950 (MI355X)", prop.gcnArchName);
     std::unique_ptr<ibx_ctx> c(new ibx_ctx());
     c->nsys = nsys;
-    c->hs.resize(nsys);
+    c->hs = std::move(hs);
     for (int s = 0; s < nsys; ++s) {
-        std::string err;
-        int rc = build_host_system(desc[s], c->hs[s], err);
-        if (rc) return fail(nullptr, rc, "system %d: %s", s, err.c_str());
         if (c->hs[s].nupdate != c->hs[0].nupdate)
             return fail(nullptr, IBX_E_INVALID, "all systems of a batch must share update_neighbour_list");
         if (c->hs[s].hybrid && nsys > 1) return fail(nullptr, IBX_E_INVALID, "hybrid systems run one per context");
@@ -717,6 +786,22 @@ int ibx_create(int nsys, const ibx_system_desc *const *desc, ibx_ctx **out) {
     if (rc) { g_create_err = c->err; return rc; }
     *out = c.release();
     return 0;
+}
+
+extern "C" {
+
+const char *ibx_last_error(const ibx_ctx *ctx) { return ctx ? ctx->err.c_str() : g_create_err.c_str(); }
+
+int ibx_create(int nsys, const ibx_system_desc *const *desc, ibx_ctx **out) {
+    if (!out || nsys < 1 || !desc) return fail(nullptr, IBX_E_INVALID, "invalid arguments");
+    *out = nullptr;
+    std::vector<HostSystem> hs(nsys);
+    for (int s = 0; s < nsys; ++s) {
+        std::string err;
+        int rc = build_host_system(desc[s], hs[s], err);
+        if (rc) return fail(nullptr, rc, "system %d: %s", s, err.c_str());
+    }
+    return create_context(std::move(hs), out);
 }
 
 void ibx_destroy(ibx_ctx *ctx) {
@@ -809,6 +894,16 @@ int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
         done += nblk * c->nupdate;
     }
     return ibx_step(c, nsteps - done, step0 + done);
+}
+
+int ibx_retire(ibx_ctx *c, int sys) {
+    if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
+    if (!c->dsys[sys].live) return 0;
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    c->dsys[sys].live = 0;
+    const int zero = 0;
+    HIPCHK(c, hipMemcpy((char *)(c->sys.p + sys) + offsetof(DevSys, live), &zero, sizeof zero, hipMemcpyHostToDevice));
+    return 0;
 }
 
 int ibx_get_observables(ibx_ctx *c, int sys, double obs[32]) {
@@ -1046,10 +1141,7 @@ int ibx_get_timing(ibx_ctx *c, double ms[8]) {
 int ibx_get_info(ibx_ctx *c, int sys, int64_t info[8]) {
     if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
     const HostSystem &h = c->hs[sys];
-    size_t bytes = c->s.bytes() + c->v.bytes() + c->fnb.bytes() + c->contrib.bytes() + c->ftot.bytes() +
-                   c->mass.bytes() + c->invmass.bytes() + c->excl.bytes() + c->term_slot.bytes() + c->term_mem.bytes();
-    for (auto &G : c->gr)
-        bytes += G.list.bytes() + G.pos4.bytes() + G.r64.bytes() + G.perm.bytes() + G.nneb.bytes() + G.mem.bytes();
+    const size_t bytes = device_bytes(c);
     info[0] = h.natoms;
     info[1] = h.nvs;
     info[2] = c->gr[0].cap;

@@ -1,6 +1,7 @@
 // Internal host-side data model of the ibx library (not part of the C ABI).
 #pragma once
 #include <array>
+#include <cstddef>
 #include <cstdint>
 #include <string>
 #include <vector>
@@ -72,5 +73,13 @@ struct DescStorage {
     int ntrj = 100, nstep = 0;
 };
 int read_input_dir(const char *dir, DescStorage &st, std::string &err);
+
+// Context of already built host systems (moved in); errors as ibx_create.
+int create_context(std::vector<HostSystem> &&hs, ibx_ctx **out);
+// Device bytes a one-system context of h allocates at creation, plus an RDF of
+// rdf_nbin bins (admission by footprint).
+size_t estimate_device_bytes(const HostSystem &h, int rdf_nbin);
+// Free device memory now (hipMemGetInfo).
+int device_free_bytes(size_t *free_bytes);
 
 }  // namespace ibx

@@ -84,6 +84,8 @@ struct DevSys {
     int vs_off;                    // offset into the VS arrays
     int bead_off;                  // offset into MTS bead buffers
     int nbead;                     // members of group 1 (beads+VS) for MTS
+    int live;                      // 0 once retired: the system's workgroups return at once and its
+                                   // state, observables, window sums and RDF keep their values
 };
 
 struct NbTab {                     // one nonbonded table as seen by a system

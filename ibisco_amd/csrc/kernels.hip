@@ -296,6 +296,7 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
     if (blk >= nblk) return;
     const int slot0 = blk * kListAtoms;
     const DevSys &S = sys[wg_sys[slot0 / kBlock]];
+    if (!S.live) return;
     const DevGroup &G = S.g[g];
     const int sub = threadIdx.x & (kListLanes - 1), grp = threadIdx.x / kListLanes;
     const int i = slot0 + grp;
@@ -579,6 +580,7 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
     const int blk = xcd_block(nblk);
     if (blk >= nblk) return;
     const DevSys &S = sys[wg_sys[blk / (kBlock / kNbAtomsPerBlock)]];
+    if (!S.live) return;
     const DevGroup &G = S.g[kGroup];
     if (kLds)
         for (int r = threadIdx.x; r < S.nb_nrow; r += kBlock) lds_rows[r] = nb_rows[S.nb_row_off + r];
@@ -696,6 +698,7 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
 // taken in FP64 from the unwrapped coordinates, the rest in FP32.
 
 struct TermArgs {
+    const DevSys *sys;    // per-system constants (retired systems are skipped)
     const int4 *mem;      // members (unused -1)
     const int *meta;      // kind | table << 2, -1 for padding
     const int4 *slot;     // contribution slot of each member
@@ -921,7 +924,7 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
                                                   int nitems_total, int nterms, float *__restrict__ contrib,
                                                   int ncontrib, double *__restrict__ part, int *__restrict__ flags) {
     __shared__ double red[19 * 4];
-    (void)wg_sys;
+    if (!A.sys[wg_sys[blockIdx.x]].live) return;
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const int n = nitems_total;
     double en[19];
@@ -1038,6 +1041,7 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     const int it = blockIdx.x * kBlock + threadIdx.x;
     const int n = A.nitems_total;
     if (A.mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.step, 1);
+    if (!S.live) return;
     double acc[21];
     for (int k = 0; k < 21; ++k) acc[k] = 0.0;
     // every per-item load up front (the arrays are padded to whole blocks), so
@@ -1125,6 +1129,7 @@ __global__ void __launch_bounds__(kBlock) k_vs_place(const DevSys *__restrict__ 
     __shared__ float mx[kBlock / kWave];
     const int s_id = wg_sys[blockIdx.x];
     const DevSys &S = sys[s_id];
+    if (!S.live) return;
     const int it = blockIdx.x * kBlock + threadIdx.x;
     const int n = nitems_total;
     float disp = 0.f;
@@ -1210,6 +1215,7 @@ __global__ void __launch_bounds__(kWave) k_finalize(const DevSys *__restrict__ s
     const int s_id = blockIdx.x * kWave + threadIdx.x;
     if (s_id >= nsys) return;
     const DevSys &S = sys[s_id];
+    if (!S.live) return;
     const double *red = fsum + (size_t)s_id * kFields;
     // acc keeps the former layout: 0..8 nonbonded, 9..21 bonded energies and overextension,
     // 22..27 bonded virial, 28..39 kinetic tensor of each population, 40..42 momentum
@@ -1260,6 +1266,7 @@ __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, 
     extern __shared__ unsigned int h[];
     const int s_id = wg_sys[blockIdx.x];
     const DevSys &S = sys[s_id];
+    if (!S.live) return;
     const float inv_dr_u = (float)S.unit * inv_dr;   // bins in this system's working units
     if (threadIdx.x == 0 && (int)blockIdx.x == S.g[0].wg_off) atomicAdd(&samples[s_id], 1);
     for (int b = threadIdx.x; b < nbin; b += kBlock) h[b] = 0;
@@ -1355,7 +1362,7 @@ void launch_nonbonded(hipStream_t st, const NbArgs &a) {
 
 void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &b, const double *s, int nitems_total,
                   int nterms, float *contrib, int ncontrib, double *part, int *flags) {
-    TermArgs A{b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs};
+    TermArgs A{b.sys, b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs};
     hipLaunchKernelGGL(k_terms, dim3(nwg), dim3(kBlock), 0, st, wg_sys, A, s, nitems_total, nterms, contrib, ncontrib,
                        part, flags);
 }

@@ -48,6 +48,7 @@ struct NbArgs {
 };
 
 struct TermLaunch {
+    const DevSys *sys;
     const int4 *mem;
     const int *meta;
     const int4 *slot;

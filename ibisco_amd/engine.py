@@ -38,11 +38,24 @@ class IbxSystemDesc(C.Structure):
     ]
 
 
+class IbxServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_cohorts", C.c_int), ("max_cohort_jobs", C.c_int),
+                ("chunk_steps", C.c_int), ("rdf_nbin", C.c_int), ("rdf_dr", C.c_double)]
+
+
+class IbxJobResult(C.Structure):
+    _fields_ = [("status", C.c_int), ("natoms", C.c_int), ("nsteps", C.c_int), ("cohort", C.c_int),
+                ("cohort_jobs", C.c_int), ("t_submit", C.c_double), ("t_admit", C.c_double),
+                ("t_done", C.c_double), ("mean_pressure", C.c_double), ("obs", C.c_double * 32),
+                ("sums", C.c_double * 32), ("error", C.c_char * 256)]
+
+
 EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_step", "ibx_step_graph",
            "ibx_compute_forces", "ibx_rebuild_lists", "ibx_get_observables", "ibx_get_averages",
            "ibx_get_forces", "ibx_get_state", "ibx_set_state", "ibx_set_nb_table", "ibx_set_thermostat",
            "ibx_get_pairs", "ibx_rdf_enable", "ibx_rdf_get", "ibx_get_timing", "ibx_set_timing",
-           "ibx_get_info")
+           "ibx_get_info", "ibx_retire", "ibx_server_create", "ibx_server_submit", "ibx_server_wait",
+           "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes")
 
 _lib = None
 
@@ -79,6 +92,16 @@ def load_library(path=LIB_PATH):
     L.ibx_get_timing.argtypes = [vp, _dp]
     L.ibx_set_timing.argtypes = [vp, C.c_int]
     L.ibx_get_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+    L.ibx_retire.argtypes = [vp, C.c_int]
+    L.ibx_server_create.argtypes = [C.POINTER(IbxServerConfig), C.POINTER(vp)]
+    L.ibx_server_submit.argtypes = [vp, C.POINTER(IbxSystemDesc), C.c_int, C.c_int, C.POINTER(C.c_int64)]
+    L.ibx_server_wait.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(IbxJobResult), _dp, _dp, _dp]
+    L.ibx_server_stats.argtypes = [vp, _dp]
+    L.ibx_server_destroy.argtypes = [vp]
+    L.ibx_server_destroy.restype = None
+    L.ibx_estimate_bytes.argtypes = [C.POINTER(IbxSystemDesc), C.c_int, C.POINTER(C.c_int64)]
+    L.ibx_server_last_error.argtypes = []
+    L.ibx_server_last_error.restype = C.c_char_p
     _lib = L
     return L
 
@@ -254,3 +277,88 @@ class Engine:
         i = (C.c_int64 * 8)()
         self._chk(self.L.ibx_get_info(self.h, sys, i))
         return list(i)
+
+    def retire(self, sys):
+        self._chk(self.L.ibx_retire(self.h, sys))
+
+
+IBX_E_PENDING = -7
+
+
+def estimate_bytes(sysm, rdf_nbin=0):
+    """Device bytes a one-system context of ``sysm`` allocates (ibx_estimate_bytes; no device work)."""
+    L = load_library()
+    keep = []
+    d, _ = make_desc(sysm, keep)
+    b = C.c_int64()
+    rc = L.ibx_estimate_bytes(C.byref(d), int(rdf_nbin), C.byref(b))
+    if rc != 0:
+        raise IbxError(rc, L.ibx_server_last_error().decode())
+    return b.value
+
+
+class Server:
+    """The serving loop of libibx (ibx_server_*): jobs are admitted by device
+    footprint into batched contexts that run concurrently; see include/ibx.h."""
+
+    def __init__(self, hbm_budget=200e9, max_cohorts=8, max_cohort_jobs=256, chunk_steps=200, rdf_nbin=150,
+                 rdf_dr=0.01):
+        self.L = load_library()
+        self.h = C.c_void_p()
+        cfg = IbxServerConfig(int(hbm_budget), max_cohorts, max_cohort_jobs, chunk_steps, rdf_nbin, rdf_dr)
+        rc = self.L.ibx_server_create(C.byref(cfg), C.byref(self.h))
+        if rc != 0:
+            raise IbxError(rc, self.L.ibx_server_last_error().decode())
+        self.rdf_nbin = rdf_nbin
+        self.shape = {}
+
+    def submit(self, sysm, nsteps, keep_state=False):
+        keep = []
+        d, a = make_desc(sysm, keep)
+        job = C.c_int64()
+        rc = self.L.ibx_server_submit(self.h, C.byref(d), int(nsteps), int(keep_state), C.byref(job))
+        if rc != 0:
+            raise IbxError(rc, self.L.ibx_server_last_error().decode())
+        self.shape[job.value] = (a["natoms"] + a["nvs"], a["natoms"], bool(keep_state))
+        return job.value
+
+    def wait(self, job, timeout=None):
+        """The job's result (dict), or None if it is still running after ``timeout`` seconds."""
+        nitems, natoms, ks = self.shape[job]
+        res = IbxJobResult()
+        g = np.zeros(max(1, self.rdf_nbin))
+        s = np.zeros((nitems, 3)) if ks else None
+        v = np.zeros((natoms, 3)) if ks else None
+        rc = self.L.ibx_server_wait(self.h, job, -1.0 if timeout is None else float(timeout), C.byref(res),
+                                    _p(g, C.c_double), None if s is None else _p(s, C.c_double),
+                                    None if v is None else _p(v, C.c_double))
+        if rc == IBX_E_PENDING:
+            return None
+        del self.shape[job]
+        if rc != 0:
+            raise IbxError(rc, res.error.decode() or self.L.ibx_server_last_error().decode())
+        return dict(job=job, natoms=res.natoms, nsteps=res.nsteps, cohort=res.cohort, cohort_jobs=res.cohort_jobs,
+                    t_submit=res.t_submit, t_admit=res.t_admit, t_done=res.t_done,
+                    mean_pressure=res.mean_pressure, obs=np.array(res.obs[:]), sums=np.array(res.sums[:]),
+                    rdf=g[:self.rdf_nbin].copy(), s=s, v=v)
+
+    def stats(self):
+        st = np.zeros(16)
+        rc = self.L.ibx_server_stats(self.h, _p(st, C.c_double))
+        if rc != 0:
+            raise IbxError(rc, self.L.ibx_server_last_error().decode())
+        keys = ("submitted", "completed", "failed", "queued", "running", "cohorts", "peak_running", "bytes_in_use",
+                "peak_bytes", "atom_steps", "clock_s", "largest_cohort", "mean_cohort_jobs", "budget")
+        return {k: float(st[i]) for i, k in enumerate(keys)}
+
+    def close(self):
+        """Runs every submitted job to its end, then frees the server."""
+        if getattr(self, "h", None) and self.h.value:
+            self.L.ibx_server_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

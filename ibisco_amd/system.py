@@ -334,6 +334,75 @@ def polymer_melt(nchains, nbeads, density, mass_kgmol, temp_k, seed, *, sigma, e
     return sysm
 
 
+def hybrid_melt(n_at_chains, n_cg_chains, chain_len=60, group=6, seed=0, temp_k=450.0, rc=1.4, skin=0.3,
+                dr_nb=0.002, dt=0.001, nupdate=10, atom_density=33.5, avoid=None, light=False):
+    """Hybrid atomistic/CG melt: atomistic chains of ``chain_len`` CH2-like atoms
+    (type A) carrying one centre-of-mass virtual site (type V) per ``group``
+    consecutive atoms, and CG chains of chain_len/group beads (type B, each the
+    mass of ``group`` atoms).  Three LJ tables with sigma drawn from 0.35-0.50 nm
+    and epsilon from 0.2-3.0 kJ/mol (atom-atom, bead-bead, bead-site; the
+    site-site table is the bead-bead one and never used, as virtual-site pairs
+    are always excluded), shifted force to rc at dr_nb.  The box holds the atoms
+    at ``atom_density`` per nm^3, a bead taking the volume of ``group`` atoms.
+    With ``light`` every second atom of an atomistic chain is a light type H
+    (1.008 g/mol, the same tables as A), so centres of mass differ from means."""
+    rng = np.random.default_rng(seed)
+    m_atom = 0.014027
+    nb_cg = chain_len // group
+    n_at = n_at_chains * chain_len
+    n_cg = n_cg_chains * nb_cg
+    L = ((n_at + group * n_cg) / atom_density) ** (1.0 / 3.0)
+    box = np.array([L, L, L])
+    if avoid:
+        sa = _grow_chains_avoiding(rng, n_at_chains, chain_len, box, 0.153, 112.0, avoid * 0.35)
+        sb = _grow_chains_avoiding(rng, n_cg_chains, nb_cg, box, 0.5, 130.0, avoid * 0.35)
+    else:
+        sa = _grow_chains(rng, n_at_chains, chain_len, box, 0.153, 112.0)
+        sb = _grow_chains(rng, n_cg_chains, nb_cg, box, 0.5, 130.0)
+    s = np.concatenate([sa.reshape(-1, 3), sb.reshape(-1, 3)])
+    itype = np.concatenate([np.zeros(n_at, np.int32), np.ones(n_cg, np.int32)])
+    masses = np.array([m_atom, group * m_atom, group * m_atom, 0.001008])
+    if light:
+        itype[1:n_at:2] = 3
+    v = maxwell_velocities(rng, masses[itype], temp_k)
+    jbond = []
+    for nch, clen, base in ((n_at_chains, chain_len, 0), (n_cg_chains, nb_cg, n_at)):
+        for c in range(nch):
+            for k in range(clen):
+                i = base + c * clen + k
+                jbond.append(([i - 1] if k > 0 else []) + ([i + 1] if k < clen - 1 else []))
+    vs_atoms = [np.arange(c * chain_len + g * group, c * chain_len + (g + 1) * group, dtype=np.int32)
+                for c in range(n_at_chains) for g in range(chain_len // group)]
+    pars = [(rng.uniform(0.35, 0.50), rng.uniform(0.2, 3.0)) for _ in range(3)]
+    tabs = [lj_sf_table(sg, ep, rc, dr_nb) for sg, ep in pars]
+    names, labels = ["A", "B", "V", "H"], ["A", "B", "B", "A"]
+    if not light:
+        names, labels, masses = names[:3], labels[:3], masses[:3]
+    sysm = System(box=box, type_names=names, type_mass=masses, type_label=labels,
+                  itype=itype, s=s, v=v, jbond=jbond,
+                  mol_sizes=[chain_len] * n_at_chains + [nb_cg] * n_cg_chains,
+                  mol_names=["atomistic"] * n_at_chains + ["coarse"] * n_cg_chains)
+    br, bv = harmonic_bond_table(0.153, 250000.0, 0.4, 0.0005)
+    cr, cv = harmonic_bond_table(0.5, 5000.0, 1.3, 0.001)
+    at, av = harmonic_angle_table(112.0, 520.0)
+    ct, cvv = harmonic_angle_table(130.0, 25.0)
+    sysm.bond_tables = [Table((0, 0), br, bv, "bondA"), Table((1, 1), cr, cv, "bondB")]
+    sysm.angle_tables = [Table((0, 0, 0), at, av, "angA"), Table((1, 1, 1), ct, cvv, "angB")]
+    sysm.nb_tables = [Table((0, 0), tabs[0][0], tabs[0][1], "nbAA"), Table((1, 1), tabs[1][0], tabs[1][1], "nbBB"),
+                      Table((1, 2), tabs[2][0], tabs[2][1], "nbBV"), Table((2, 2), tabs[1][0], tabs[1][1], "nbVV")]
+    if light:
+        sysm.bond_tables[0] = Table((0, 3), br, bv, "bondAH")
+        sysm.angle_tables[0] = Table((0, 3, 0), at, av, "angAHA")
+        sysm.angle_tables.append(Table((3, 0, 3), at, av, "angHAH"))
+        sysm.nb_tables += [Table((0, 3), tabs[0][0], tabs[0][1], "nbAH"), Table((3, 3), tabs[0][0], tabs[0][1], "nbHH")]
+    sysm.vs_type = np.full(len(vs_atoms), 2, np.int32)
+    sysm.vs_center = np.full(len(vs_atoms), -1, np.int32)
+    sysm.vs_atoms = vs_atoms
+    sysm.control = Control(ensemble="NVE", temperature=temp_k, nstep=1000, dt=dt, cutoff=rc, rlist=rc + skin,
+                           bead_cutoff=rc, bead_rlist=rc + skin, nupdate=nupdate, non_bonded=4, non_bonded_bead=4)
+    return sysm
+
+
 def cg_melt(nchains=200, nbeads=50, seed=0, density=8.0, temp_k=500.0, nb_table=None, **kw):
     """Coarse-grained melt: 72 amu beads, LJ(0.47 nm, 3.5 kJ/mol) to 1.5 nm at 0.002 nm."""
     args = dict(mass_kgmol=0.072, temp_k=temp_k, seed=seed, sigma=0.47, eps=3.5, rc=1.5,

@@ -52,3 +52,8 @@ def dt_to_reduced(dt_ps):
 
 def temp_to_reduced(t_k):
     return t_k / TEMPSCALE
+
+
+def pressure_to_bar(p):
+    """Reduced pressure (energy unit per nm^3) in bar (PSCALE is in kPa)."""
+    return p * PSCALE / 100.0

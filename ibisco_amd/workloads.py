@@ -5,6 +5,7 @@ config1  UA melt, 400 x 250 CH2 (100,000), 0.78 g/cm^3, 450 K, LJ(0.401, 0.39) t
 config3  replicas: CG melts of 2,000-50,000 beads (log-uniform), chain length 20-100, each with
          its own Gaussian-corrected table
 config4  serving: a Poisson stream of such jobs (sizes 1,000-100,000, 500-5,000 steps)
+         at 20 jobs/s
 
 Chains are random walks (fixed bond length, bond angle near theta0) that may
 overlap; :func:`equilibrate` removes the overlaps on the device with a capped
@@ -70,13 +71,13 @@ def equilibrate(engine, systems, stage_steps=100, final_steps=200, caps=CAP_STAG
     engine.set_thermostat(1, temp, 1.0, 0.5 * dt)
     for cap in caps:
         for k, sy in enumerate(systems):
-            t = sy.nb_tables[0]
-            sig = _sigma_guess(t)
-            engine.set_nb_table(t.x, units.kjmol_to_reduced(capped_table(t.x, t.v, cap * sig)), sys=k)
+            for q, t in enumerate(sy.nb_tables):
+                sig = _sigma_guess(t)
+                engine.set_nb_table(t.x, units.kjmol_to_reduced(capped_table(t.x, t.v, cap * sig)), sys=k, t=q)
         engine.step(stage_steps)
     for k, sy in enumerate(systems):
-        t = sy.nb_tables[0]
-        engine.set_nb_table(t.x, units.kjmol_to_reduced(t.v), sys=k)
+        for q, t in enumerate(sy.nb_tables):
+            engine.set_nb_table(t.x, units.kjmol_to_reduced(t.v), sys=k, t=q)
     engine.step(final_steps // 4)
     engine.set_thermostat(1, temp, 0.1, 0.5 * dt)
     engine.step(final_steps // 4)
@@ -90,6 +91,20 @@ def _sigma_guess(t):
     """Where the tabulated potential first turns negative (LJ sigma)."""
     neg = np.nonzero(np.asarray(t.v) < 0)[0]
     return float(t.x[neg[0]]) if len(neg) else float(t.x[-1]) * 0.3
+
+
+def serving_jobs(stream, cmin=20, cmax=100, rlist=1.8, nupdate=10, dt=0.005):
+    """The systems of a serving stream: CG melts of about n_beads beads (chain
+    length uniform in cmin..cmax), each with its own Gaussian-corrected table."""
+    out = []
+    for (_t, n, _steps, sd) in stream:
+        rng = np.random.default_rng(sd)
+        c = int(rng.integers(cmin, cmax + 1))
+        nch = max(2, int(round(n / c)))
+        r, v = system.corrected_cg_table(sd)
+        out.append(system.cg_melt(nchains=nch, nbeads=c, seed=sd, temp_k=500.0, dt=dt, rlist=rlist,
+                                  nupdate=nupdate, nb_table=(r, v)))
+    return out
 
 
 def serving_stream(rate_hz, duration_s, seed, nmin=1000, nmax=100000, smin=500, smax=5000):

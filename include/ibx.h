@@ -27,6 +27,7 @@ extern "C" {
 #define IBX_E_DEVICE (-4)        /* HIP runtime failure, or no usable gfx950 device */
 #define IBX_E_NOMEM (-5)         /* device allocation failed */
 #define IBX_E_IO (-6)            /* input file missing or unreadable */
+#define IBX_E_PENDING (-7)       /* ibx_server_wait: the job has not finished within the timeout */
 
 typedef struct ibx_ctx ibx_ctx;
 
@@ -146,8 +147,74 @@ int ibx_get_timing(ibx_ctx *ctx, double ms[8]);
 /* Enables (on != 0) or disables device timing of ibx_step and clears the totals. */
 int ibx_set_timing(ibx_ctx *ctx, int on);
 
-/* Sizes: natoms, nvs, list capacity, padded slots; and device bytes held. */
+/* Sizes: info[0] natoms, [1] nvs, [2]/[3] list capacity of groups 0/1,
+ * [4] padded slots of group 0, [5] device bytes the context holds,
+ * [6] systems in the context, [7] padded items of all systems. */
 int ibx_get_info(ibx_ctx *ctx, int sys, int64_t info[8]);
+
+/* Stops advancing system sys of a batch for the rest of the context's life:
+ * its workgroups return at once, and its state, observables, window sums and
+ * RDF keep the values of its last step.  The other systems are unaffected
+ * (bit for bit).  No counterpart in the reference (one system per run); the
+ * serving loop uses it when a job of a batch reaches its step count. */
+int ibx_retire(ibx_ctx *ctx, int sys);
+
+/* ---- Serving loop (no counterpart in the reference) ----------------------
+ * Jobs (one system and a step count each) are admitted in arrival order into
+ * batched contexts while the estimated device bytes of all resident jobs stay
+ * within hbm_budget and fewer than max_cohorts batches run; batches admitted
+ * at different times run concurrently on their own streams and host threads.
+ * A job that reaches its step count is retired from its batch, its results are
+ * copied out, and the batch's bytes return to the budget when its last job
+ * ends.  Each job's results equal those of the same system stepped alone in
+ * its own context (ibx_step_graph from step 0, RDF every rebuild). */
+typedef struct ibx_server ibx_server;
+
+typedef struct {
+    int64_t hbm_budget;   /* device bytes for all resident jobs (<= 0: 200e9; capped at 90% of free memory) */
+    int max_cohorts;      /* batches stepping at once (<= 0: 8) */
+    int max_cohort_jobs;  /* jobs per batch (<= 0: 256) */
+    int chunk_steps;      /* most steps between completion checks of a batch (<= 0: 200) */
+    int rdf_nbin;         /* RDF bins per job (0: none) */
+    double rdf_dr;        /* RDF bin width, nm */
+} ibx_server_config;
+
+typedef struct {
+    int status;           /* IBX_OK or the error of the job's batch */
+    int natoms, nsteps;
+    int cohort, cohort_jobs;      /* batch that ran the job and its number of jobs */
+    double t_submit, t_admit, t_done;   /* seconds since ibx_server_create */
+    double mean_pressure;         /* mean over the job's steps of (PT11+PT22+PT33)/3, reduced units */
+    double obs[32];               /* as ibx_get_observables after the last step */
+    double sums[32];              /* as ibx_get_averages over the job's steps */
+    char error[256];
+} ibx_job_result;
+
+/* Device bytes ibx_create allocates for this one system, plus an RDF of
+ * rdf_nbin bins: the footprint the serving loop admits by.  Errors are
+ * reported through ibx_server_last_error. */
+int ibx_estimate_bytes(const ibx_system_desc *desc, int rdf_nbin, int64_t *bytes);
+
+/* cfg may be NULL (defaults).  Starts the scheduler thread. */
+int ibx_server_create(const ibx_server_config *cfg, ibx_server **out);
+/* Queues a job; the descriptor's arrays are copied before the call returns.
+ * keep_state != 0 keeps the final state for ibx_server_wait. */
+int ibx_server_submit(ibx_server *srv, const ibx_system_desc *desc, int nsteps, int keep_state, int64_t *job);
+/* Waits up to timeout_s seconds (< 0: no limit) for a job, then hands over its
+ * result once: res, rdf (rdf_nbin values of g(r)), and for keep_state jobs s
+ * ((natoms+nvs)*3) and v (natoms*3) as ibx_get_state; any pointer may be NULL.
+ * Returns IBX_E_PENDING while the job runs, else the job's status. */
+int ibx_server_wait(ibx_server *srv, int64_t job, double timeout_s, ibx_job_result *res, double *rdf, double *s,
+                    double *v);
+/* [0] submitted, [1] completed, [2] failed, [3] queued, [4] running batches,
+ * [5] batches created, [6] peak running batches, [7] bytes in use (estimated),
+ * [8] peak bytes, [9] atom-steps completed, [10] seconds since creation,
+ * [11] largest batch, [12] mean jobs per batch, [13] effective budget. */
+int ibx_server_stats(ibx_server *srv, double st[16]);
+/* Runs every submitted job to its end, then stops the threads and frees. */
+void ibx_server_destroy(ibx_server *srv);
+/* Message of the last failed ibx_server_* call on the calling thread. */
+const char *ibx_server_last_error(void);
 
 #ifdef __cplusplus
 }

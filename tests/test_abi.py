@@ -34,3 +34,29 @@ def test_engine_binding_loads_without_device():
         pytest.skip("libibx.so not built")
     L = engine.load_library()
     assert L.ibx_last_error(None) is not None
+
+
+def test_server_without_device_fails_cleanly():
+    if not os.path.exists(engine.LIB_PATH):
+        pytest.skip("libibx.so not built")
+    try:
+        srv = engine.Server()
+    except engine.IbxError as e:
+        assert e.code == -4 and "device" in str(e)
+        return
+    srv.close()
+    pytest.skip("a HIP device is present: the serving loop is covered by the GPU tests")
+
+
+def test_footprint_estimate_without_device():
+    """ibx_estimate_bytes needs no device: the list dominates (cap ~ 1.5 x the neighbours within rlist)."""
+    if not os.path.exists(engine.LIB_PATH):
+        pytest.skip("libibx.so not built")
+    from ibisco_amd import system
+    s = system.cg_melt(nchains=20, nbeads=50, seed=4, avoid=0.8, dt=0.002)
+    b0, b1 = engine.estimate_bytes(s), engine.estimate_bytes(s, 150)
+    assert b1 - b0 == 150 * 8 + 4
+    import math
+    nbr = 4.0 / 3.0 * math.pi * 1.8 ** 3 * 8.0
+    cap = (int(1.5 * nbr) + 64 + 7) // 8 * 8
+    assert 1024 * cap * 4 < b0 < 1024 * (cap * 4 + 600)

@@ -25,7 +25,7 @@ class OracleEngine:
         self.o.set_state(s[:sy.natoms], v)
         self.o.update_lists()
 
-    def set_nb_table(self, x, v_reduced, sys=0):
+    def set_nb_table(self, x, v_reduced, sys=0, t=0):
         self.table = (np.asarray(x), units.reduced_to_kjmol(np.asarray(v_reduced)))
         self._rebuild()
 

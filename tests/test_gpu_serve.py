@@ -1,0 +1,109 @@
+"""Retirement of batch members and the serving loop (no reference counterpart).
+
+A job's results from the serving loop (final state, observables, window sums,
+mean pressure, RDF) must equal, bit for bit, those of the same system stepped
+alone in its own context from step 0 with the RDF on, whichever batch it
+shared, whenever it was admitted and however its batch was chunked.  Batching
+is bit-exact because every per-system reduction and list run is independent of
+the other systems; retirement only stops a system's workgroups.
+"""
+import os
+import time
+
+import numpy as np
+import pytest
+
+from ibisco_amd import system
+from ibisco_amd.engine import Engine, Server, estimate_bytes
+
+pytestmark = pytest.mark.gpu
+
+NBIN, DR = 150, 0.01
+HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _job(nchains, nbeads, seed):
+    r, v = system.corrected_cg_table(seed)
+    return system.cg_melt(nchains=nchains, nbeads=nbeads, seed=seed, avoid=0.8, dt=0.002, nb_table=(r, v))
+
+
+def _alone(sysm, nsteps):
+    e = Engine(sysm)
+    e.rdf_enable(NBIN, DR)
+    e.step(nsteps, graph=True)
+    s, v = e.state()
+    out = dict(s=s, v=v, rdf=e.rdf(0), sums=e.averages(0), obs=e.obs(0))
+    e.close()
+    return out
+
+
+def test_retired_member_freezes_and_others_continue_bitwise():
+    a, b = _job(12, 40, 21), _job(20, 30, 22)
+    e = Engine([a, b])
+    e.step(23, graph=True)
+    e.retire(0)
+    sa, va = e.state(0)
+    oa = e.obs(0)
+    e.step(40)   # steps 24..63 eager: a is frozen, b goes on
+    sa2, va2 = e.state(0)
+    sb, vb = e.state(1)
+    oa2 = e.obs(0)
+    e.close()
+    assert np.array_equal(sa, sa2) and np.array_equal(va, va2)
+    assert all(np.array_equal(oa[k], oa2[k]) for k in ("vnb", "pt", "ek"))
+    ea, eb = Engine(a), Engine(b)
+    ea.step(23, graph=True)
+    eb.step(63, graph=True)
+    ra, rb = ea.state(), eb.state()
+    ea.close()
+    eb.close()
+    assert np.array_equal(sa, ra[0]) and np.array_equal(va, ra[1])
+    assert np.array_equal(sb, rb[0]) and np.array_equal(vb, rb[1])
+
+
+@pytest.mark.parametrize("case", ["cg", "ua", "hybridPA"])
+def test_footprint_estimate_matches_allocation(case):
+    if case == "cg":
+        sysm = _job(20, 50, 5)
+    elif case == "ua":
+        sysm = system.ua_melt(nchains=16, nbeads=60, seed=2)
+    else:
+        z = np.load(os.path.join(HERE, "hybridPA.npz"))
+        sysm = system.from_arrays({k[4:]: z[k] for k in z.files if k.startswith("sys_")})
+        sysm.control.ensemble = "NVE"
+    est = estimate_bytes(sysm, NBIN)
+    e = Engine(sysm)
+    e.rdf_enable(NBIN, DR)
+    got = e.info(0)[5]
+    e.close()
+    print(case, "estimate", est, "allocated", got, flush=True)
+    assert got <= est <= got * 1.01
+
+
+def test_served_jobs_equal_alone():
+    specs = [(10, 40, 31, 180), (14, 30, 32, 23), (8, 50, 33, 57), (16, 25, 34, 40), (12, 35, 35, 75),
+             (6, 60, 36, 31), (9, 44, 37, 66)]
+    systems = [_job(nc, nb, sd) for nc, nb, sd, _ in specs]
+    srv = Server(max_cohorts=2, max_cohort_jobs=3, chunk_steps=16, rdf_nbin=NBIN, rdf_dr=DR)
+    ids = []
+    for k, (sy, sp) in enumerate(zip(systems, specs)):
+        ids.append(srv.submit(sy, sp[3], keep_state=True))
+        if k == 2:
+            time.sleep(0.05)   # later jobs arrive while the first batches run
+    res = [srv.wait(j, timeout=120) for j in ids]
+    st = srv.stats()
+    srv.close()
+    print("stats", st, flush=True)
+    assert all(r is not None for r in res)
+    assert st["completed"] == len(specs) and st["failed"] == 0
+    assert max(r["cohort_jobs"] for r in res) >= 2     # some jobs shared a batch
+    for sy, sp, r in zip(systems, specs, res):
+        ref = _alone(sy, sp[3])
+        print("job", sp, "cohort", r["cohort"], "of", r["cohort_jobs"], flush=True)
+        assert r["nsteps"] == sp[3] and r["sums"][28] == sp[3]
+        assert np.array_equal(r["s"], ref["s"]) and np.array_equal(r["v"], ref["v"])
+        assert np.array_equal(r["rdf"], ref["rdf"])
+        assert np.array_equal(r["sums"][:29], ref["sums"][:29])
+        p = (ref["sums"][15] + ref["sums"][16] + ref["sums"][17]) / (3 * ref["sums"][28])
+        assert r["mean_pressure"] == p
+        assert r["t_submit"] <= r["t_admit"] <= r["t_done"]

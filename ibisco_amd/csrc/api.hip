@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +24,11 @@ namespace {
 
 thread_local std::string g_create_err;
 
+// Graph capture on one context's stream, and device allocations, frees and
+// stream/event/graph destruction of every context, are kept apart: several
+// contexts may be driven from different host threads (the serving loop).
+std::mutex g_capture_mu;
+
 template <typename T>
 struct DBuf {
     T *p = nullptr;
@@ -32,13 +38,17 @@ struct DBuf {
     DBuf &operator=(const DBuf &) = delete;
     ~DBuf() { release(); }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            std::lock_guard<std::mutex> lk(g_capture_mu);
+            (void)hipFree(p);
+        }
         p = nullptr;
         n = 0;
     }
     hipError_t alloc(size_t count) {
         release();
         n = count ? count : 1;
+        std::lock_guard<std::mutex> lk(g_capture_mu);
         return hipMalloc((void **)&p, n * sizeof(T));
     }
     size_t bytes() const { return p ? n * sizeof(T) : 0; }
@@ -93,6 +103,7 @@ struct ibx_ctx {
     DBuf<unsigned long long> rdf;
     DBuf<int> rdf_samples;           // histograms accumulated per system (counted on the device)
     int rdf_nbin = 0, rdf_every = 0;
+    bool skin_tune = true;           // list-skin autotuner at every scheduled rebuild
     double rdf_dr = 0;
     // graph
     hipGraphExec_t gexec = nullptr;
@@ -103,6 +114,7 @@ struct ibx_ctx {
     std::vector<hipEvent_t> evpool;
     std::string err;
     ~ibx_ctx() {
+        std::lock_guard<std::mutex> lk(g_capture_mu);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto e : evpool) (void)hipEventDestroy(e);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
@@ -133,10 +145,31 @@ int fail(ibx_ctx *c, int code, const char *fmt, ...) {
                         hipGetErrorString(e_));                                                            \
     } while (0)
 
+void drop_graph(ibx_ctx *c) {
+    if (!c->gexec) return;
+    std::lock_guard<std::mutex> lk(g_capture_mu);
+    (void)hipGraphExecDestroy(c->gexec);
+    c->gexec = nullptr;
+}
+
+// Copies and fills go through the context's own (non-blocking) stream, never
+// the legacy null stream: another host thread may be capturing a graph on its
+// context's stream meanwhile (the serving loop), and a null-stream operation
+// would have to order itself against that capture.
+hipError_t memcpy_st(ibx_ctx *c, void *dst, const void *src, size_t n, hipMemcpyKind kind) {
+    hipError_t e = hipMemcpyAsync(dst, src, n, kind, c->st);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->st);
+}
+
+hipError_t memset_st(ibx_ctx *c, void *p, int v, size_t n) {
+    hipError_t e = hipMemsetAsync(p, v, n, c->st);
+    return e != hipSuccess ? e : hipStreamSynchronize(c->st);
+}
+
 template <typename T>
 int upload(ibx_ctx *c, DBuf<T> &b, const std::vector<T> &h) {
     HIPCHK(c, b.alloc(h.size()));
-    if (!h.empty()) HIPCHK(c, hipMemcpy(b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    if (!h.empty()) HIPCHK(c, memcpy_st(c, b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
     return 0;
 }
 
@@ -390,20 +423,20 @@ int build_device(ibx_ctx *c) {
     c->h_nb_tabs = hntab;
     HIPCHK(c, c->fnb.alloc(3 * (size_t)N));
     HIPCHK(c, c->contrib.alloc(3 * (size_t)c->ncontrib));
-    HIPCHK(c, hipMemset(c->contrib.p, 0, c->contrib.bytes()));
+    HIPCHK(c, memset_st(c, c->contrib.p, 0, c->contrib.bytes()));
     HIPCHK(c, c->ftot.alloc(3 * (size_t)N));
-    HIPCHK(c, hipMemset(c->fnb.p, 0, c->fnb.bytes()));
-    HIPCHK(c, hipMemset(c->ftot.p, 0, c->ftot.bytes()));
+    HIPCHK(c, memset_st(c, c->fnb.p, 0, c->fnb.bytes()));
+    HIPCHK(c, memset_st(c, c->ftot.p, 0, c->ftot.bytes()));
     HIPCHK(c, c->part_bd.alloc((size_t)c->term_nwg * kBdPart));
     HIPCHK(c, c->part_mv.alloc((size_t)c->item_nwg * kMvPart));
-    HIPCHK(c, hipMemset(c->part_bd.p, 0, c->part_bd.bytes()));
-    HIPCHK(c, hipMemset(c->part_mv.p, 0, c->part_mv.bytes()));
+    HIPCHK(c, memset_st(c, c->part_bd.p, 0, c->part_bd.bytes()));
+    HIPCHK(c, memset_st(c, c->part_mv.p, 0, c->part_mv.bytes()));
     HIPCHK(c, c->fsum.alloc((size_t)S * kFields));
     HIPCHK(c, c->obs.alloc((size_t)S * kObs));
     HIPCHK(c, c->avg.alloc((size_t)S * kObs));
     HIPCHK(c, c->sysstate.alloc((size_t)S * 8));
-    HIPCHK(c, hipMemset(c->obs.p, 0, c->obs.bytes()));
-    HIPCHK(c, hipMemset(c->avg.p, 0, c->avg.bytes()));
+    HIPCHK(c, memset_st(c, c->obs.p, 0, c->obs.bytes()));
+    HIPCHK(c, memset_st(c, c->avg.p, 0, c->avg.bytes()));
     std::vector<double> st(8 * (size_t)S, 0.0);
     for (int s = 0; s < S; ++s) {
         st[8 * s] = c->hs[s].temp_atom0;
@@ -416,9 +449,9 @@ int build_device(ibx_ctx *c) {
         if ((rc = upload(c, c->dmax, inf))) return rc;
     }
     HIPCHK(c, c->flags.alloc(8));
-    HIPCHK(c, hipMemset(c->flags.p, 0, c->flags.bytes()));
+    HIPCHK(c, memset_st(c, c->flags.p, 0, c->flags.bytes()));
     HIPCHK(c, c->step.alloc(1));
-    HIPCHK(c, hipMemset(c->step.p, 0, c->step.bytes()));
+    HIPCHK(c, memset_st(c, c->step.p, 0, c->step.bytes()));
 
     // groups
     for (int g = 0; g < 2; ++g) {
@@ -426,7 +459,7 @@ int build_device(ibx_ctx *c) {
         std::vector<int> mem, wg, cellsys;
         int slot = 0, cell = 0, maxrows = 0;
         bool any = false, multi = false;
-        double max_dens_vol = 0.0;
+        double max_dens_vol = 0.0, margin = 1.5;
         for (int s = 0; s < S; ++s) {
             HostSystem &h = c->hs[s];
             DevSys &D = c->dsys[s];
@@ -476,6 +509,7 @@ int build_device(ibx_ctx *c) {
                 double vol = h.box[0] * h.box[1] * h.box[2];
                 double nbr = 4.0 / 3.0 * M_PI * rl * rl * rl * nm / vol;
                 max_dens_vol = std::max(max_dens_vol, nbr);
+                margin = std::max(margin, h.list_margin);
                 maxrows = std::max(maxrows, D.nb_nrow);
                 multi = multi || D.nb_ntab > 1;
             }
@@ -488,7 +522,7 @@ int build_device(ibx_ctx *c) {
         G.max_rows = std::min(maxrows, kLdsRows);
         if (maxrows > kLdsRows) G.max_rows = 0;
         if (!any) continue;
-        G.cap = pad_to((int)(1.5 * max_dens_vol) + 64, 8);
+        G.cap = pad_to((int)(margin * max_dens_vol) + 64, 8);
         std::vector<int> perm_init(slot, -1);
         if ((rc = upload(c, G.mem, mem)) || (rc = upload(c, G.wg_sys, wg)) || (rc = upload(c, G.cell_sys, cellsys)) ||
             (rc = upload(c, G.perm, perm_init)))
@@ -504,7 +538,7 @@ int build_device(ibx_ctx *c) {
         HIPCHK(c, G.pos4_ref.alloc(slot));
         HIPCHK(c, G.nneb.alloc(slot));
         HIPCHK(c, G.part.alloc((size_t)G.nwg * (kBlock / kNbAtomsPerBlock) * kNbPart));
-        HIPCHK(c, hipMemset(G.part.p, 0, G.part.bytes()));
+        HIPCHK(c, memset_st(c, G.part.p, 0, G.part.bytes()));
         HIPCHK(c, G.list.alloc((size_t)slot * G.cap));
         std::vector<int> inv(N, -1);
         if ((rc = upload(c, G.inv_perm, inv))) return rc;
@@ -568,7 +602,7 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
         if (nm == 0) continue;   // an absent group allocates nothing
         const double nbr = 4.0 / 3.0 * M_PI * h.rlist[g] * h.rlist[g] * h.rlist[g] * nm /
                            (h.box[0] * h.box[1] * h.box[2]);
-        const size_t cap = (size_t)pad_to((int)(1.5 * nbr) + 64, 8);
+        const size_t cap = (size_t)pad_to((int)(std::max(1.5, h.list_margin) * nbr) + 64, 8);
         b += slots * (4 * 5 + 32 + 16 + 16 + 8)                  // mem mem_cell tmp perm slot_cell r64 pos4 pos4_ref nneb
              + N * 4 + nwg * 4 + ncell * 16                       // inv_perm, wg_sys, cell arrays
              + nwg * (kBlock / kNbAtomsPerBlock) * kNbPart * 8 + slots * cap * 4;
@@ -594,7 +628,10 @@ RebuildArgs rebuild_args(ibx_ctx *c, int g) {
     return a;
 }
 
-void enqueue_rebuild(ibx_ctx *c) {
+// tune: a rebuild of the step schedule, after a full interval of moves (the
+// list-skin autotuner reads the displacement that interval reached)
+void enqueue_rebuild(ibx_ctx *c, bool tune = false) {
+    if (tune) launch_tune_skin(c->st, c->sys.p, c->nsys, c->dmax.p, c->nupdate, c->skin_tune ? 1 : 0);
     for (int g = 0; g < 2; ++g)
         if (c->gr[g].present) launch_rebuild(c->st, rebuild_args(c, g));
 }
@@ -670,7 +707,7 @@ void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
 int check_flags(ibx_ctx *c) {
     int f[8];
     HIPCHK(c, hipStreamSynchronize(c->st));
-    HIPCHK(c, hipMemcpy(f, c->flags.p, sizeof f, hipMemcpyDeviceToHost));
+    HIPCHK(c, memcpy_st(c, f, c->flags.p, sizeof f, hipMemcpyDeviceToHost));
     if (f[0] > 0)
         return fail(c, IBX_E_LIST_OVERFLOW, "LIST TOO SMALL: %d neighbours exceed the list capacity", f[0]);
     if (f[1] & 1) return fail(c, IBX_E_TABLE, "torsion angle outside its table");
@@ -685,7 +722,7 @@ int rebuild_checked(ibx_ctx *c) {
         enqueue_rebuild(c);
         int f0 = 0;
         HIPCHK(c, hipStreamSynchronize(c->st));
-        HIPCHK(c, hipMemcpy(&f0, c->flags.p, sizeof f0, hipMemcpyDeviceToHost));
+        HIPCHK(c, memcpy_st(c, &f0, c->flags.p, sizeof f0, hipMemcpyDeviceToHost));
         if (f0 == 0) return 0;
         for (int g = 0; g < 2; ++g) {
             GroupDev &G = c->gr[g];
@@ -693,7 +730,7 @@ int rebuild_checked(ibx_ctx *c) {
             G.cap = pad_to((int)(f0 * 1.25) + 16, 8);
             HIPCHK(c, G.list.alloc((size_t)G.nslot_pad * G.cap));
         }
-        if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+        drop_graph(c);
     }
     return fail(c, IBX_E_LIST_OVERFLOW, "LIST TOO SMALL after resizing");
 }
@@ -705,7 +742,7 @@ void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
     if (ev) (void)hipEventRecord((*ev)[0], c->st);
     if (!ev) fork_terms(c);
     if (step % c->nupdate == 0) {
-        enqueue_rebuild(c);
+        enqueue_rebuild(c, true);
         if (c->rdf_nbin && c->rdf_every > 0) enqueue_rdf(c);
     }
     if (ev) (void)hipEventRecord((*ev)[4], c->st);
@@ -724,7 +761,10 @@ void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
 int ensure_events(ibx_ctx *c, size_t n) {
     while (c->evpool.size() < n) {
         hipEvent_t e;
-        HIPCHK(c, hipEventCreate(&e));
+        {
+            std::lock_guard<std::mutex> lk(g_capture_mu);
+            HIPCHK(c, hipEventCreate(&e));
+        }
         c->evpool.push_back(e);
     }
     return 0;
@@ -775,11 +815,13 @@ int ibx::create_context(std::vector<HostSystem> &&hs, ibx_ctx **out) {
         }
     }
     c->nupdate = c->hs[0].nupdate;
+    std::unique_lock<std::mutex> lk(g_capture_mu);
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
         return fail(nullptr, IBX_E_DEVICE, "stream creation failed");
+    lk.unlock();
     int rc = build_device(c.get());
     if (rc) { g_create_err = c->err; return rc; }
     rc = rebuild_checked(c.get());
@@ -875,9 +917,12 @@ int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
     int nblk = (nsteps - done) / c->nupdate;
     if (nblk > 0) {
         if (!c->gexec || c->graph_steps != c->nupdate) {
-            if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+            drop_graph(c);
+            std::lock_guard<std::mutex> lk(g_capture_mu);
             hipGraph_t graph;
-            HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeThreadLocal));
+            // relaxed: other host threads (other contexts, the serving loop) keep allocating and copying
+            // while this one captures; nothing captured here is a call that capture forbids
+            HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
             for (int k = 0; k < c->nupdate; ++k) enqueue_step(c, c->nupdate + k, nullptr);   // block starts at a rebuild
             HIPCHK(c, hipStreamEndCapture(c->st, &graph));
             hipError_t e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
@@ -896,22 +941,39 @@ int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
     return ibx_step(c, nsteps - done, step0 + done);
 }
 
+int ibx_set_skin_tuning(ibx_ctx *c, int on) {
+    if (!c) return fail(nullptr, IBX_E_INVALID, "null context");
+    c->skin_tune = on != 0;
+    drop_graph(c);
+    return 0;
+}
+
+int ibx_get_skin(ibx_ctx *c, int sys, double delta[2]) {
+    if (!c || sys < 0 || sys >= c->nsys || !delta) return fail(c, IBX_E_INVALID, "invalid system index");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    DevSys D;
+    HIPCHK(c, memcpy_st(c, &D, c->sys.p + sys, sizeof D, hipMemcpyDeviceToHost));
+    for (int g = 0; g < 2; ++g)
+        delta[g] = g < D.ngroup ? std::sqrt((double)D.g[g].rin2u) * D.unit - std::sqrt(D.g[g].rc2) : 0.0;
+    return 0;
+}
+
 int ibx_retire(ibx_ctx *c, int sys) {
     if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
     if (!c->dsys[sys].live) return 0;
     HIPCHK(c, hipStreamSynchronize(c->st));
     c->dsys[sys].live = 0;
     const int zero = 0;
-    HIPCHK(c, hipMemcpy((char *)(c->sys.p + sys) + offsetof(DevSys, live), &zero, sizeof zero, hipMemcpyHostToDevice));
+    HIPCHK(c, memcpy_st(c, (char *)(c->sys.p + sys) + offsetof(DevSys, live), &zero, sizeof zero, hipMemcpyHostToDevice));
     return 0;
 }
 
 int ibx_get_observables(ibx_ctx *c, int sys, double obs[32]) {
     if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
     HIPCHK(c, hipStreamSynchronize(c->st));
-    HIPCHK(c, hipMemcpy(obs, c->obs.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, memcpy_st(c, obs, c->obs.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
     double a[kObs];
-    HIPCHK(c, hipMemcpy(a, c->avg.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(c, memcpy_st(c, a, c->avg.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
     obs[26] = c->dsys[sys].volume;
     obs[28] = a[28];
     return 0;
@@ -920,8 +982,8 @@ int ibx_get_observables(ibx_ctx *c, int sys, double obs[32]) {
 int ibx_get_averages(ibx_ctx *c, int sys, double sums[32], int reset) {
     if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
     HIPCHK(c, hipStreamSynchronize(c->st));
-    HIPCHK(c, hipMemcpy(sums, c->avg.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
-    if (reset) HIPCHK(c, hipMemset(c->avg.p + (size_t)sys * kObs, 0, kObs * sizeof(double)));
+    HIPCHK(c, memcpy_st(c, sums, c->avg.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
+    if (reset) HIPCHK(c, memset_st(c, c->avg.p + (size_t)sys * kObs, 0, kObs * sizeof(double)));
     return 0;
 }
 
@@ -932,7 +994,7 @@ int ibx_get_forces(ibx_ctx *c, int sys, double *f) {
     const int o = c->item_off[sys], N = c->n_items;
     std::vector<float> tmp(h.nitems);
     for (int k = 0; k < 3; ++k) {
-        HIPCHK(c, hipMemcpy(tmp.data(), c->ftot.p + (size_t)k * N + o, h.nitems * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCHK(c, memcpy_st(c, tmp.data(), c->ftot.p + (size_t)k * N + o, h.nitems * sizeof(float), hipMemcpyDeviceToHost));
         for (int i = 0; i < h.nitems; ++i) f[3 * (size_t)i + k] = tmp[i];
     }
     return 0;
@@ -945,14 +1007,14 @@ int ibx_get_state(ibx_ctx *c, int sys, double *s, double *v) {
     const int o = c->item_off[sys], N = c->n_items;
     std::vector<double> tmp(h.nitems);
     double st[8];
-    HIPCHK(c, hipMemcpy(st, c->sysstate.p + 8 * (size_t)sys, sizeof st, hipMemcpyDeviceToHost));
+    HIPCHK(c, memcpy_st(c, st, c->sysstate.p + 8 * (size_t)sys, sizeof st, hipMemcpyDeviceToHost));
     for (int k = 0; k < 3; ++k) {
         if (s) {
-            HIPCHK(c, hipMemcpy(tmp.data(), c->s.p + (size_t)k * N + o, h.nitems * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(c, memcpy_st(c, tmp.data(), c->s.p + (size_t)k * N + o, h.nitems * sizeof(double), hipMemcpyDeviceToHost));
             for (int i = 0; i < h.nitems; ++i) s[3 * (size_t)i + k] = tmp[i];
         }
         if (v) {
-            HIPCHK(c, hipMemcpy(tmp.data(), c->v.p + (size_t)k * N + o, h.natoms * sizeof(double), hipMemcpyDeviceToHost));
+            HIPCHK(c, memcpy_st(c, tmp.data(), c->v.p + (size_t)k * N + o, h.natoms * sizeof(double), hipMemcpyDeviceToHost));
             // a MOMENTUM correction decided at the last step is applied lazily by the next move
             for (int i = 0; i < h.natoms; ++i) v[3 * (size_t)i + k] = tmp[i] - st[2 + k] * h.invmass[h.itype[i]];
         }
@@ -984,16 +1046,16 @@ int ibx_set_state(ibx_ctx *c, int sys, const double *s, const double *v) {
         }
         for (int k = 0; k < 3; ++k) {
             for (int i = 0; i < h.nitems; ++i) tmp[i] = h.s[3 * (size_t)i + k];
-            HIPCHK(c, hipMemcpy(c->s.p + (size_t)k * N + o, tmp.data(), h.nitems * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(c, memcpy_st(c, c->s.p + (size_t)k * N + o, tmp.data(), h.nitems * sizeof(double), hipMemcpyHostToDevice));
         }
     }
     if (v) {
         for (int k = 0; k < 3; ++k) {
             for (int i = 0; i < h.natoms; ++i) tmp[i] = v[3 * (size_t)i + k];
-            HIPCHK(c, hipMemcpy(c->v.p + (size_t)k * N + o, tmp.data(), h.natoms * sizeof(double), hipMemcpyHostToDevice));
+            HIPCHK(c, memcpy_st(c, c->v.p + (size_t)k * N + o, tmp.data(), h.natoms * sizeof(double), hipMemcpyHostToDevice));
         }
     }
-    HIPCHK(c, hipMemset(c->sysstate.p + 8 * (size_t)sys + 2, 0, 3 * sizeof(double)));
+    HIPCHK(c, memset_st(c, c->sysstate.p + 8 * (size_t)sys + 2, 0, 3 * sizeof(double)));
     // refresh the sorted copies with the current order (lists unchanged)
     for (int g = 0; g < 2; ++g) {
         if (!c->gr[g].present) continue;
@@ -1037,15 +1099,18 @@ int ibx_set_nb_table(ibx_ctx *c, int sys, int t, int nrow, const double *x, cons
     const NbTab &hdr = c->h_nb_tabs[c->dsys[sys].nb_tab_off + t];
     const NbTab nh = nb_header(nt, c->dsys[sys].unit, hdr.row_off, xoff);
     HIPCHK(c, hipStreamSynchronize(c->st));
-    HIPCHK(c, hipMemcpy(c->nb_rows.p + hdr.row_off, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice));
-    HIPCHK(c, hipMemcpy(c->nb_xoff.p + hdr.row_off, xoff.data(), xoff.size() * sizeof(float), hipMemcpyHostToDevice));
+    HIPCHK(c, memcpy_st(c, c->nb_rows.p + hdr.row_off, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice));
+    HIPCHK(c, memcpy_st(c, c->nb_xoff.p + hdr.row_off, xoff.data(), xoff.size() * sizeof(float), hipMemcpyHostToDevice));
     c->h_nb_tabs[c->dsys[sys].nb_tab_off + t] = nh;
-    HIPCHK(c, hipMemcpy(c->nb_tabs.p + c->dsys[sys].nb_tab_off + t, &nh, sizeof nh, hipMemcpyHostToDevice));
+    HIPCHK(c, memcpy_st(c, c->nb_tabs.p + c->dsys[sys].nb_tab_off + t, &nh, sizeof nh, hipMemcpyHostToDevice));
     return 0;
 }
 
 int ibx_set_thermostat(ibx_ctx *c, int ensemble, double temp_in, double fac, double dt) {
     if (!c || (ensemble != 0 && ensemble != 1)) return fail(c, IBX_E_INVALID, "invalid thermostat");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    // the device copy carries the tuned skins: start from it
+    HIPCHK(c, memcpy_st(c, c->dsys.data(), c->sys.p, c->dsys.size() * sizeof(DevSys), hipMemcpyDeviceToHost));
     for (auto &D : c->dsys) {
         D.ensemble = ensemble;
         D.temp_in = temp_in;
@@ -1053,8 +1118,8 @@ int ibx_set_thermostat(ibx_ctx *c, int ensemble, double temp_in, double fac, dou
         D.dt = dt;
     }
     HIPCHK(c, hipStreamSynchronize(c->st));
-    HIPCHK(c, hipMemcpy(c->sys.p, c->dsys.data(), c->dsys.size() * sizeof(DevSys), hipMemcpyHostToDevice));
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    HIPCHK(c, memcpy_st(c, c->sys.p, c->dsys.data(), c->dsys.size() * sizeof(DevSys), hipMemcpyHostToDevice));
+    drop_graph(c);
     return 0;
 }
 
@@ -1069,9 +1134,9 @@ int ibx_get_pairs(ibx_ctx *c, int sys, int group, int *out, int64_t cap, int64_t
     HIPCHK(c, hipStreamSynchronize(c->st));
     std::vector<int> perm(dg.nslot_pad), list((size_t)dg.nslot_pad * G.cap);
     std::vector<int2> nneb(dg.nslot_pad);
-    HIPCHK(c, hipMemcpy(perm.data(), G.perm.p + dg.slot_off, perm.size() * sizeof(int), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(nneb.data(), G.nneb.p + dg.slot_off, nneb.size() * sizeof(int2), hipMemcpyDeviceToHost));
-    HIPCHK(c, hipMemcpy(list.data(), G.list.p + (size_t)dg.slot_off * G.cap, list.size() * sizeof(int),
+    HIPCHK(c, memcpy_st(c, perm.data(), G.perm.p + dg.slot_off, perm.size() * sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, memcpy_st(c, nneb.data(), G.nneb.p + dg.slot_off, nneb.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    HIPCHK(c, memcpy_st(c, list.data(), G.list.p + (size_t)dg.slot_off * G.cap, list.size() * sizeof(int),
                         hipMemcpyDeviceToHost));
     const int o = c->item_off[sys];
     std::vector<std::pair<int, int>> pr;
@@ -1100,11 +1165,11 @@ int ibx_rdf_enable(ibx_ctx *c, int nbin, double dr, int every) {
     c->rdf_every = every;
     if (nbin) {
         HIPCHK(c, c->rdf.alloc((size_t)c->nsys * nbin));
-        HIPCHK(c, hipMemset(c->rdf.p, 0, c->rdf.bytes()));
+        HIPCHK(c, memset_st(c, c->rdf.p, 0, c->rdf.bytes()));
         HIPCHK(c, c->rdf_samples.alloc(c->nsys));
-        HIPCHK(c, hipMemset(c->rdf_samples.p, 0, c->rdf_samples.bytes()));
+        HIPCHK(c, memset_st(c, c->rdf_samples.p, 0, c->rdf_samples.bytes()));
     }
-    if (c->gexec) { (void)hipGraphExecDestroy(c->gexec); c->gexec = nullptr; }
+    drop_graph(c);
     return 0;
 }
 
@@ -1112,10 +1177,10 @@ int ibx_rdf_get(ibx_ctx *c, int sys, double *g, int reset) {
     if (!c || sys < 0 || sys >= c->nsys || !c->rdf_nbin) return fail(c, IBX_E_INVALID, "RDF not enabled");
     HIPCHK(c, hipStreamSynchronize(c->st));
     std::vector<unsigned long long> h(c->rdf_nbin);
-    HIPCHK(c, hipMemcpy(h.data(), c->rdf.p + (size_t)sys * c->rdf_nbin, h.size() * sizeof(unsigned long long),
+    HIPCHK(c, memcpy_st(c, h.data(), c->rdf.p + (size_t)sys * c->rdf_nbin, h.size() * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost));
     int ns = 0;
-    HIPCHK(c, hipMemcpy(&ns, c->rdf_samples.p + sys, sizeof(int), hipMemcpyDeviceToHost));
+    HIPCHK(c, memcpy_st(c, &ns, c->rdf_samples.p + sys, sizeof(int), hipMemcpyDeviceToHost));
     const HostSystem &hs = c->hs[sys];
     const double n = (double)hs.members[0].size();
     const double vol = hs.box[0] * hs.box[1] * hs.box[2];
@@ -1126,8 +1191,8 @@ int ibx_rdf_get(ibx_ctx *c, int sys, double *g, int reset) {
         g[b] = (double)h[b] / (samples * n * (n / vol) * shell);
     }
     if (reset) {
-        HIPCHK(c, hipMemset(c->rdf.p + (size_t)sys * c->rdf_nbin, 0, c->rdf_nbin * sizeof(unsigned long long)));
-        HIPCHK(c, hipMemset(c->rdf_samples.p + sys, 0, sizeof(int)));
+        HIPCHK(c, memset_st(c, c->rdf.p + (size_t)sys * c->rdf_nbin, 0, c->rdf_nbin * sizeof(unsigned long long)));
+        HIPCHK(c, memset_st(c, c->rdf_samples.p + sys, 0, sizeof(int)));
     }
     return 0;
 }

@@ -59,6 +59,9 @@ struct HostSystem {
     int numatoms = 0, numbeads = 0;
     double mktemp = 0, mktemp_atom = 0, mktemp_bead = 0;
     double temp_atom0 = 0, temp_bead0 = 0;
+    // list capacity per member as a multiple of the expected neighbours within rlist
+    // (a batch takes the largest of its systems')
+    double list_margin = 1.5;
 };
 
 // Set-up of main.F90:107-234 from a descriptor (no device work).

@@ -1290,8 +1290,61 @@ __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, 
         if (h[b]) atomicAdd(&hist[(size_t)s_id * nbin + b], (unsigned long long)h[b]);
 }
 
+// List-skin autotuner (no counterpart in the reference).  Before a scheduled
+// rebuild, each system's inner skin delta (the front run holds the pairs closer
+// than rc + delta at the build; the back run is scanned once a member has moved
+// 0.49 delta) is chosen to minimise the list entries the nonbonded kernel is
+// expected to read over the next interval: nmove front runs, plus a back run for
+// every step whose displacement, extrapolated linearly from the largest one the
+// system reached over the interval that ends (with a 25% margin), passes
+// 0.49 delta.  In a uniform fluid the run lengths scale as (rc + delta)^3 and
+// rlist^3 - (rc + delta)^3.  Which pairs are listed does not change, and the
+// back-run rule keeps every pair within rc whatever delta is.  One thread per
+// system and group: the choice depends only on the system's own trajectory, so
+// a system tunes the same alone and in a batch.  tune == 0 restores the fixed
+// delta = kInnerFrac (rlist - rc).  Runs only where a list is about to be built.
+__global__ void k_tune_skin(DevSys *__restrict__ sys, int nsys, const int *__restrict__ dmax, int nmove, int tune) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2 * nsys) return;
+    const int g = t / nsys, s_id = t - g * nsys;
+    DevSys &S = sys[s_id];
+    if (g >= S.ngroup || !S.live) return;
+    DevGroup &G = S.g[g];
+    const double rc = sqrt(G.rc2), rl = sqrt(G.rl2), skin = fmax(0.0, rl - rc);
+    const double u = S.unit;
+    double best_delta = kInnerFrac * skin;
+    const float d2u = __int_as_float(dmax[t]);
+    if (tune && skin > 0.0 && d2u < 3.0e38f) {           // +inf: no interval observed since the last build
+        const double d = 1.25 * sqrt((double)d2u) * u;   // nm, after nmove moves
+        const double rl3 = rl * rl * rl;
+        double best = 1e300;
+        for (int i = 2; i <= 32; ++i) {
+            const double delta = skin * i / 32.0;
+            const double a = (rc + delta) * (rc + delta) * (rc + delta);
+            // evaluation j = 0..nmove-1 of the interval sees j moves: scanned while j d / nmove >= 0.49 delta
+            int nscan = 0;
+            if (d > 0.0) {
+                nscan = nmove - (int)ceil(0.49 * delta * nmove / d);
+                nscan = nscan < 0 ? 0 : (nscan > nmove ? nmove : nscan);
+            }
+            const double cost = nmove * a + nscan * (rl3 - a);
+            if (cost < best) {
+                best = cost;
+                best_delta = delta;
+            }
+        }
+    }
+    G.rin2u = (float)(((rc + best_delta) / u) * ((rc + best_delta) / u));
+    G.dmax2u = (float)((0.49 * best_delta / u) * (0.49 * best_delta / u));
+}
+
 // ------------------------------------------------------------------------------------------------
 // launch wrappers
+
+void launch_tune_skin(hipStream_t st, DevSys *sys, int nsys, const int *dmax, int nmove, int tune) {
+    hipLaunchKernelGGL(k_tune_skin, dim3((2 * nsys + kWave - 1) / kWave), dim3(kWave), 0, st, sys, nsys, dmax, nmove,
+                       tune);
+}
 
 void launch_zero_int(hipStream_t st, int *p, int n) {
     if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, 0);

@@ -80,6 +80,7 @@ struct MoveLaunch {
 };
 
 void launch_zero_int(hipStream_t st, int *p, int n);
+void launch_tune_skin(hipStream_t st, DevSys *sys, int nsys, const int *dmax, int nmove, int tune);
 void launch_rebuild(hipStream_t st, const RebuildArgs &a);
 void launch_pack_only(hipStream_t st, const RebuildArgs &a);
 void launch_nonbonded(hipStream_t st, const NbArgs &a);

@@ -299,6 +299,9 @@ int ibx_server_submit(ibx_server *S, const ibx_system_desc *desc, int nsteps, in
     j->keep_state = keep_state != 0;
     j->nupdate = j->hs.nupdate;
     j->hybrid = j->hs.hybrid;
+    // served jobs run thousands of steps unattended: room for local density
+    // twice the mean before a list would overflow
+    j->hs.list_margin = 2.0;
     j->bytes = estimate_device_bytes(j->hs, S->cfg.rdf_nbin);
     std::lock_guard<std::mutex> lk(S->mu);
     if (S->stop) return serr(IBX_E_INVALID, "server is shutting down");

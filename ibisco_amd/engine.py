@@ -55,7 +55,8 @@ EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_s
            "ibx_get_forces", "ibx_get_state", "ibx_set_state", "ibx_set_nb_table", "ibx_set_thermostat",
            "ibx_get_pairs", "ibx_rdf_enable", "ibx_rdf_get", "ibx_get_timing", "ibx_set_timing",
            "ibx_get_info", "ibx_retire", "ibx_server_create", "ibx_server_submit", "ibx_server_wait",
-           "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes")
+           "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes",
+           "ibx_set_skin_tuning", "ibx_get_skin")
 
 _lib = None
 
@@ -93,6 +94,8 @@ def load_library(path=LIB_PATH):
     L.ibx_set_timing.argtypes = [vp, C.c_int]
     L.ibx_get_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.ibx_retire.argtypes = [vp, C.c_int]
+    L.ibx_set_skin_tuning.argtypes = [vp, C.c_int]
+    L.ibx_get_skin.argtypes = [vp, C.c_int, _dp]
     L.ibx_server_create.argtypes = [C.POINTER(IbxServerConfig), C.POINTER(vp)]
     L.ibx_server_submit.argtypes = [vp, C.POINTER(IbxSystemDesc), C.c_int, C.c_int, C.POINTER(C.c_int64)]
     L.ibx_server_wait.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(IbxJobResult), _dp, _dp, _dp]
@@ -280,6 +283,14 @@ class Engine:
 
     def retire(self, sys):
         self._chk(self.L.ibx_retire(self.h, sys))
+
+    def set_skin_tuning(self, on=True):
+        self._chk(self.L.ibx_set_skin_tuning(self.h, int(on)))
+
+    def skin(self, sys=0):
+        d = np.zeros(2)
+        self._chk(self.L.ibx_get_skin(self.h, sys, _p(d, C.c_double)))
+        return d
 
 
 IBX_E_PENDING = -7

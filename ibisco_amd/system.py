@@ -176,6 +176,13 @@ def rb_torsion_table(coeffs=RB_ALKANE, dphi=0.5):
     return phi, v
 
 
+def oop_table(k_kjmol=20.0, dphi=2.0):
+    """Out-of-plane potential K (1 - cos phi) over the signed angle phi in [-180, 180] degrees."""
+    n = int(round(360.0 / dphi)) + 1
+    phi = -180.0 + np.arange(n, dtype=np.float64) * dphi
+    return phi, k_kjmol * (1.0 - np.cos(phi * units.D2R))
+
+
 # ----------------------------------------------------------------------------------------------
 # generators
 
@@ -400,6 +407,67 @@ def hybrid_melt(n_at_chains, n_cg_chains, chain_len=60, group=6, seed=0, temp_k=
     sysm.vs_atoms = vs_atoms
     sysm.control = Control(ensemble="NVE", temperature=temp_k, nstep=1000, dt=dt, cutoff=rc, rlist=rc + skin,
                            bead_cutoff=rc, bead_rlist=rc + skin, nupdate=nupdate, non_bonded=4, non_bonded_bead=4)
+    return sysm
+
+
+def comb_melt(nchains=20, nbackbone=40, seed=0, temp_k=400.0, rc=1.0, rlist=1.3, dt=0.001, nupdate=10,
+              site_density=33.0, spread_deg=25.0, oop_only=False):
+    """Comb polymer melt with out-of-plane terms: backbone atoms C (type 0) carry a
+    side atom S (type 1) on every odd backbone position, so those atoms have three
+    bonds (previous, next, side, in that file order) and an improper C-C-C-S
+    (OUTOFPLANES.inc) with a K (1 - cos phi) table over [-180, 180] degrees.
+    Side atoms leave the backbone plane by up to ``spread_deg``.  Harmonic bonds
+    and angles, LJ between all sites, 1-2 and 1-3 exclusions.  ``oop_only`` zeroes
+    every table but the out-of-plane one (the exclusions and terms stay), so the
+    forces are the out-of-plane forces alone."""
+    rng = np.random.default_rng(seed)
+    nside = len(range(1, nbackbone - 1, 2))
+    per = nbackbone + nside
+    n = nchains * per
+    L = (n / site_density) ** (1.0 / 3.0)
+    box = np.array([L, L, L])
+    bb = _grow_chains_avoiding(rng, nchains, nbackbone, box, 0.153, 112.0, 0.3).reshape(nchains, nbackbone, 3)
+    s = np.empty((nchains, per, 3))
+    itype = np.zeros(n, np.int32)
+    jbond = [None] * n
+    for c in range(nchains):
+        base = c * per
+        s[c, :nbackbone] = bb[c]
+        for k in range(nbackbone):
+            jbond[base + k] = ([base + k - 1] if k > 0 else []) + ([base + k + 1] if k < nbackbone - 1 else [])
+        for q, k in enumerate(range(1, nbackbone - 1, 2)):
+            i = base + nbackbone + q
+            a, b = bb[c, k - 1] - bb[c, k], bb[c, k + 1] - bb[c, k]
+            out = -(a / np.linalg.norm(a) + b / np.linalg.norm(b))
+            out /= np.linalg.norm(out)
+            nrm = np.cross(a, b)
+            nrm /= np.linalg.norm(nrm)
+            tilt = np.deg2rad(rng.uniform(-spread_deg, spread_deg))
+            d = np.cos(tilt) * out + np.sin(tilt) * nrm
+            s[c, nbackbone + q] = bb[c, k] + 0.153 * d
+            itype[i] = 1
+            jbond[base + k].append(i)
+            jbond[i] = [base + k]
+    s = s.reshape(-1, 3)
+    masses = np.array([0.013019, 0.015035])
+    v = maxwell_velocities(rng, masses[itype], temp_k)
+    sysm = System(box=box, type_names=["C", "S"], type_mass=masses, type_label=["A", "A"], itype=itype, s=s,
+                  v=v, jbond=jbond, mol_sizes=[per] * nchains, mol_names=["comb"] * nchains)
+    br, bv = harmonic_bond_table(0.153, 250000.0, 0.4, 0.0005)
+    at, av = harmonic_angle_table(112.0, 520.0)
+    at2, av2 = harmonic_angle_table(110.0, 400.0)
+    pt, pv = oop_table()
+    nbr, nbv = lj_sf_table(0.38, 0.45, rc, 0.002)
+    sysm.bond_tables = [Table((0, 0), br, bv, "bondCC"), Table((0, 1), br, bv, "bondCS")]
+    sysm.angle_tables = [Table((0, 0, 0), at, av, "angCCC"), Table((0, 0, 1), at2, av2, "angCCS")]
+    sysm.oop_tables = [Table((0, 0, 0, 1), pt, pv, "oopCCCS")]
+    sysm.nb_tables = [Table((0, 0), nbr, nbv, "nbCC"), Table((0, 1), nbr, nbv, "nbCS"),
+                      Table((1, 1), nbr, nbv, "nbSS")]
+    if oop_only:
+        for tb in sysm.bond_tables + sysm.angle_tables + sysm.nb_tables:
+            tb.v = np.zeros_like(tb.v)
+    sysm.control = Control(ensemble="NVE", temperature=temp_k, nstep=100, dt=dt, cutoff=rc, rlist=rlist,
+                           nupdate=nupdate, non_bonded=4)
     return sysm
 
 

@@ -152,6 +152,20 @@ int ibx_set_timing(ibx_ctx *ctx, int on);
  * [6] systems in the context, [7] padded items of all systems. */
 int ibx_get_info(ibx_ctx *ctx, int sys, int64_t info[8]);
 
+/* List-skin autotuner (on by default; no counterpart in the reference).  At
+ * every scheduled rebuild each system's inner skin delta -- pairs closer than
+ * rc + delta at the build are read every step, the rest of the list (to
+ * rlist) only once a member has moved 0.49 delta since the build -- is chosen
+ * from the largest displacement that system reached over the interval just
+ * ended, to minimise the list entries read over the next one.  The listed
+ * pairs and the forces do not depend on delta beyond FP32 summation order;
+ * the choice depends only on the system's own trajectory, so batches stay
+ * bit-identical to their systems run alone.  Off: delta = (rlist - rc)/3
+ * from the next scheduled rebuild on (delta changes only where a list is built). */
+int ibx_set_skin_tuning(ibx_ctx *ctx, int on);
+/* Current inner skin delta (nm) of each group of system sys. */
+int ibx_get_skin(ibx_ctx *ctx, int sys, double delta[2]);
+
 /* Stops advancing system sys of a batch for the rest of the context's life:
  * its workgroups return at once, and its state, observables, window sums and
  * RDF keep the values of its last step.  The other systems are unaffected

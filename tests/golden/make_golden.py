@@ -8,12 +8,14 @@ subroutines driven by this repo's harness), and stores inputs and outputs as
 of the neighbour pair sets, exclusion lists and enumerated terms, and states
 after short runs.  The atomisticPA case also keeps, as numbers, the expected
 coordinates and velocities of the first three atoms that the reference's
-testsuite compares against (testsuite/tests.py:47-68).
+testsuite compares against (testsuite/tests.py:47-68).  comb_oop is a
+generated comb polymer whose branch atoms carry out-of-plane terms, and
+comb_oop_only the same with every other table zero (out-of-plane forces alone).
 
 Inputs are stored in single precision, which is what the reference's REAL*4
 readers see, so the oracle and the reference start from identical numbers.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [case ...]
 """
 from __future__ import annotations
 
@@ -179,8 +181,15 @@ def make_case(name, sysm, runs=()):
         shutil.rmtree(d, ignore_errors=True)
 
 
-def main():
+def main(only=()):
     subprocess.run(["make", "-s", "-j8", "-f", os.path.join(ROOT, "oracle", "Makefile.ref")], check=True)
+    if only:
+        for name in only:
+            if name == "comb_oop":
+                make_case("comb_oop", system.comb_melt(nchains=36, seed=1), runs=(10,))
+            if name == "comb_oop_only":
+                make_case("comb_oop_only", system.comb_melt(nchains=36, seed=1, oop_only=True), runs=(10,))
+        return
     make_case("cg2000", system.cg_melt(nchains=40, nbeads=50, seed=3, avoid=0.8), runs=(10,))
     ua = system.ua_melt(nchains=20, nbeads=100, seed=5, avoid=0.6)   # 3 cells per edge at rlist 1.3
     ua.control.non_bonded = 5
@@ -196,7 +205,9 @@ def main():
     exp = testsuite_expected()
     np.savez_compressed(os.path.join(HERE, "atomisticPA_expected.npz"), **exp)
     print("atomisticPA expected arrays:", {k: v.shape for k, v in exp.items()})
+    make_case("comb_oop", system.comb_melt(nchains=36, seed=1), runs=(10,))
+    make_case("comb_oop_only", system.comb_melt(nchains=36, seed=1, oop_only=True), runs=(10,))
 
 
 if __name__ == "__main__":
-    main()
+    main(sys.argv[1:])

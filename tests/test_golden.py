@@ -30,7 +30,8 @@ from ibisco_amd.arrays import flatten
 from ibisco_amd.units import vel_from_reduced
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
-CASES = ("cg2000", "ua2000_excl5", "cg1500_strained", "twoatombox", "hybridPA", "atomisticPA")
+CASES = ("cg2000", "ua2000_excl5", "cg1500_strained", "twoatombox", "hybridPA", "atomisticPA", "comb_oop",
+         "comb_oop_only")
 
 FP32_R2_TOL = 1e-5        # relative band around rlist^2 / rc^2 that FP32 folding and distances can cross
 FORCE_RTOL = 5e-5         # FP32 reference vs FP64: measured 3e-6 .. 1.8e-5 relative RMS
@@ -258,7 +259,8 @@ def test_energies_and_virial(case, oracle_mod):
     assert np.all(np.abs(pt - ptr) <= VIRIAL_RTOL * np.abs(ptr[:3]).sum() + vir_band), (pt, ptr)
 
 
-RUNS = [("cg2000", 10), ("twoatombox", 40), ("atomisticPA", 10), ("atomisticPA", 40)]
+RUNS = [("cg2000", 10), ("twoatombox", 40), ("atomisticPA", 10), ("atomisticPA", 40), ("comb_oop", 10),
+        ("comb_oop_only", 10)]
 
 
 @pytest.mark.parametrize("case,n", RUNS)

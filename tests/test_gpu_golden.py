@@ -31,7 +31,7 @@ def _load(name):
     return sysm
 
 
-@pytest.mark.parametrize("case", ["twoatombox", "ua2000_excl5", "hybridPA", "atomisticPA"])
+@pytest.mark.parametrize("case", ["twoatombox", "ua2000_excl5", "hybridPA", "atomisticPA", "comb_oop", "comb_oop_only"])
 def test_golden_inputs_forces(case, oracle_mod):
     sysm = _load(case)
     print("case", case, flush=True)

@@ -111,7 +111,7 @@ def replicas_compare(nrep, steps, warmup, seed=3):
             "ratio": vb / vp}
 
 
-def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=8, nsample=12, ncheck=4):
+def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=16, nsample=12, ncheck=4):
     """Config-4 serving: Poisson arrivals of CG melt jobs (1k-100k beads, 500-5000 steps, each with its own
     corrected table) submitted to the serving loop at their arrival times; every job returns its RDF and mean
     pressure.  Job states are equilibrated (batched, untimed) before the stream starts.  A sample of the jobs
@@ -215,7 +215,7 @@ def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=8, nsamp
                        "peak_concurrent_batches": int(st["peak_running"]),
                        "peak_resident_bytes": int(st["peak_bytes"]), "budget_bytes": int(st["budget"])},
             "plain_sample": {"value": vp, "jobs": len(sample), "path": "one context per job, in turn"},
-            "ratio": vs / vp, "alone_equals_served_bitwise": same, "jobs_checked_bitwise": len(check),
+            "max_concurrent_batches": max_cohorts, "ratio": vs / vp, "alone_equals_served_bitwise": same, "jobs_checked_bitwise": len(check),
             "mean_pressure_bar_p50": float(np.median(pres))}
 
 
@@ -253,6 +253,7 @@ def main():
                     help="also compare NREP config-3 replicas batched in one context against one context each "
                          "(0 skips it)")
     ap.add_argument("--replica-steps", type=int, default=200)
+    ap.add_argument("--serve-cohorts", type=int, default=16, help="batches the serving loop runs at once")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="also run the serving loop on Poisson arrivals (20 jobs/s) for this many seconds "
                          "(0 skips it)")
@@ -321,6 +322,20 @@ def main():
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
             "kernel": "k_nonbonded", "kernel_ms": round(nb_ms, 5), "bytes_per_site_step": round(b_nb, 1),
             "half_list_pairs_per_site": round(prl_n, 2)}
+    # config 1's multiple time step: bonded terms every 1 fs step, nonbonded every 4 (r-RESPA impulse form)
+    e.set_respa(4)
+    e.step(40, graph=not args.eager)
+    e.obs()
+    nr = max(400, (args.steps // 10) // 40 * 40)
+    t0 = time.perf_counter()
+    e.step(nr, graph=not args.eager)
+    e.obs()
+    tr = time.perf_counter() - t0
+    e.set_respa(1)
+    respa = {"inner_dt_fs": 1.0, "nonbonded_every": 4, "inner_steps": nr,
+             "atom_inner_steps_per_s": n * nr / tr, "atom_outer_steps_per_s": n * nr / 4 / tr,
+             "ms_per_outer_step": 1e3 * tr / (nr / 4), "ns_per_day": nr * 1e-6 / tr * 86400,
+             "plain_1fs_ns_per_day": args.steps * 1e-6 / secs * 86400}
     cpu = None
     if not args.no_cpu_baseline:
         from ibisco_amd import units
@@ -330,7 +345,7 @@ def main():
         print("[bench] timing the reference CPU path", file=sys.stderr, flush=True)
         cpu = cpu_baseline(sysm, args.cpu_steps, args.cpu_threads)
     batched = replicas_compare(args.replicas, args.replica_steps, 50) if args.replicas > 0 else None
-    served = serve_stream(20.0, args.serve_seconds) if args.serve_seconds > 0 else None
+    served = serve_stream(20.0, args.serve_seconds, max_cohorts=args.serve_cohorts) if args.serve_seconds > 0 else None
     out = {
         "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps, "higher_is_better": True,
@@ -345,6 +360,7 @@ def main():
         "step_breakdown_ms": {k: round(tm[k] / tm["nsteps_timed"], 5) for k in
                               ("nonbonded", "bonded", "integrate", "rebuild", "step")},
         "list_skin_autotuner": skin_tuning,
+        "multiple_time_step": respa,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

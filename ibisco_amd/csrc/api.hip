@@ -104,6 +104,10 @@ struct ibx_ctx {
     DBuf<int> rdf_samples;           // histograms accumulated per system (counted on the device)
     int rdf_nbin = 0, rdf_every = 0;
     bool skin_tune = true;           // list-skin autotuner at every scheduled rebuild
+    int respa = 1;                   // nonbonded force every respa steps, with weight respa
+    bool mts = false;                // MTS cycle of a hybrid system (NMTS > 0)
+    DBuf<float> mts_f;               // [3 slots][xyz][items] bead-group forces of the explicit steps
+    DBuf<double> mts_part;           // [3 slots][9 fields][workgroups] bead-group energy/virial partials
     double rdf_dr = 0;
     // graph
     hipGraphExec_t gexec = nullptr;
@@ -144,6 +148,18 @@ int fail(ibx_ctx *c, int code, const char *fmt, ...) {
             return fail(ctx, e_ == hipErrorOutOfMemory ? IBX_E_NOMEM : IBX_E_DEVICE, "%s: %s", #expr,      \
                         hipGetErrorString(e_));                                                            \
     } while (0)
+
+// Steps per captured graph: a whole number of list intervals and multiple-time-step cycles.
+int block_period(const ibx_ctx *c) {
+    auto lcm = [](int x, int y) {
+        int a = x, b = y;
+        while (b) { const int r = a % b; a = b; b = r; }
+        return x / a * y;
+    };
+    int p = lcm(c->nupdate, c->respa);
+    if (c->mts) p = lcm(p, 3 + c->hs[0].nmts);
+    return p;
+}
 
 void drop_graph(ibx_ctx *c) {
     if (!c->gexec) return;
@@ -557,7 +573,7 @@ size_t device_bytes(const ibx_ctx *c) {
                c->vs_invmass.bytes() + c->part_bd.bytes() + c->part_mv.bytes() + c->fsum.bytes() + c->obs.bytes() +
                c->avg.bytes() + c->sysstate.bytes() + c->fnb.bytes() + c->ftot.bytes() + c->nb_rows.bytes() +
                c->bd_rows.bytes() + c->bd_x.bytes() + c->nb_tabs.bytes() + c->nb_xoff.bytes() + c->bd_tabs.bytes() +
-               c->rdf.bytes() + c->rdf_samples.bytes();
+               c->rdf.bytes() + c->rdf_samples.bytes() + c->mts_f.bytes() + c->mts_part.bytes();
     for (const GroupDev &G : c->gr)
         b += G.nneb.bytes() + G.mem.bytes() + G.mem_cell.bytes() + G.cell_count.bytes() + G.cell_start.bytes() +
              G.cell_fill.bytes() + G.tmp.bytes() + G.perm.bytes() + G.slot_cell.bytes() + G.inv_perm.bytes() +
@@ -594,6 +610,8 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
                + at_least1(nb_rows) * (16 + 4) + at_least1(h.nt.size()) * sizeof(NbTab) +
                at_least1((size_t)h.ntype * h.ntype) * 4;
     if (rdf_nbin > 0) b += (size_t)rdf_nbin * 8 + 4;
+    if (h.hybrid && h.nmts > 0 && h.ngroup > 1 && !h.members[1].empty())   // MTS slots
+        b += 9 * N * 4 + 27 * (size_t)(pad_to((int)h.members[1].size(), kBlock) / kBlock) * (kBlock / kNbAtomsPerBlock) * 8;
     for (int g = 0; g < 2; ++g) {
         const int nm = g < h.ngroup ? (int)h.members[g].size() : 0;
         const size_t slots = (size_t)pad_to(nm, kBlock), nwg = slots / kBlock;
@@ -643,10 +661,24 @@ void enqueue_rdf(ibx_ctx *c) {
                (float)(1.0 / c->rdf_dr), c->rdf.p, c->rdf_samples.p);
 }
 
-void enqueue_forces(ibx_ctx *c, int mode) {
+// mode 0: the force phase of step `step`; with the MTS cycle (hybrid, NMTS > 0,
+// MTS_LOOP.f90:71-224) the bead group is evaluated and saved on explicit steps and
+// extrapolated on the others.  mode 1: an explicit evaluation outside the step loop.
+void enqueue_forces(ibx_ctx *c, int mode, int step = 0) {
+    const bool mts = mode == 0 && c->mts;
+    const int nmts = c->hs[0].nmts;
+    const int mod = mts ? step % (3 + nmts) : 1;
     for (int g = 0; g < 2; ++g) {
         GroupDev &G = c->gr[g];
         if (!G.present) continue;
+        const int npart = G.nwg * (kBlock / kNbAtomsPerBlock);
+        if (g == 1 && mts && (mod < 1 || mod > 3)) {
+            // MTS_APPROX: I = MOD(STEP, 3+NMTS), which is 0 on the last step of a cycle in the reference
+            const double I = (mod == 0 && !c->hs[0].mts_quirk) ? (double)(3 + nmts) : (double)mod;
+            launch_mts(c->st, false, c->fnb.p, c->label.p, c->n_items, G.part.p, npart, c->mts_f.p,
+                       c->mts_part.p, 0, I);
+            continue;
+        }
         NbArgs a;
         a.sys = c->sys.p; a.wg_sys = G.wg_sys.p; a.g = g; a.nwg = G.nwg; a.cap = G.cap; a.nitems_total = c->n_items;
         a.max_rows_lds = G.max_rows; a.multi = G.multi; a.perm = G.perm.p; a.pos4 = G.pos4.p; a.list = G.list.p;
@@ -655,6 +687,9 @@ void enqueue_forces(ibx_ctx *c, int mode) {
         a.cubic = true;
         for (const DevSys &D : c->dsys) a.cubic = a.cubic && D.rho[1] == 1.0f && D.rho[2] == 1.0f;
         launch_nonbonded(c->st, a);
+        if (g == 1 && mts)   // MTS_SAVEFORCE, MTS_SAVEAUXS into slot MOD - 1
+            launch_mts(c->st, true, c->fnb.p, c->label.p, c->n_items, G.part.p, npart, c->mts_f.p, c->mts_part.p,
+                       mod - 1, 0.0);
     }
 }
 
@@ -677,8 +712,9 @@ void fork_terms(ibx_ctx *c) {
 void join_terms(ibx_ctx *c) { (void)hipStreamWaitEvent(c->st, c->ev_join, 0); }
 
 // Integration and reductions after the forces (the bonded terms must be enqueued before).
-void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev) {
+void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev, float nb_weight = 1.0f) {
     MoveLaunch m;
+    m.nb_weight = mode == 1 ? 1.0f : nb_weight;
     m.s = c->s.p; m.v = c->v.p; m.fnb = c->fnb.p; m.contrib = c->contrib.p; m.inc_off = c->inc_off.p;
     m.ncontrib = c->ncontrib; m.ftot = c->ftot.p; m.itype = c->itype.p;
     m.label = c->label.p; m.mass = c->mass.p; m.invmass = c->invmass.p; m.vs_of = c->vs_of.p;
@@ -739,6 +775,8 @@ int rebuild_checked(ibx_ctx *c) {
 // each event pair brackets one kernel class; otherwise the bonded terms
 // overlap the rebuild and the nonbonded kernel.
 void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
+    // multiple time step: the nonbonded force on steps 1, k+1, 2k+1, ..., applied with weight k
+    const bool nb_step = c->respa == 1 || (step - 1) % c->respa == 0;
     if (ev) (void)hipEventRecord((*ev)[0], c->st);
     if (!ev) fork_terms(c);
     if (step % c->nupdate == 0) {
@@ -746,7 +784,7 @@ void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
         if (c->rdf_nbin && c->rdf_every > 0) enqueue_rdf(c);
     }
     if (ev) (void)hipEventRecord((*ev)[4], c->st);
-    enqueue_forces(c, 0);
+    if (nb_step) enqueue_forces(c, 0, step);
     if (ev) {
         (void)hipEventRecord((*ev)[1], c->st);
         enqueue_terms(c, c->st);
@@ -754,7 +792,7 @@ void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
     } else {
         join_terms(c);
     }
-    enqueue_rest(c, 0, ev);
+    enqueue_rest(c, 0, ev, nb_step ? (float)c->respa : 0.0f);
     if (ev) (void)hipEventRecord((*ev)[5], c->st);
 }
 
@@ -797,7 +835,6 @@ int ibx::create_context(std::vector<HostSystem> &&hs, ibx_ctx **out) {
         if (c->hs[s].nupdate != c->hs[0].nupdate)
             return fail(nullptr, IBX_E_INVALID, "all systems of a batch must share update_neighbour_list");
         if (c->hs[s].hybrid && nsys > 1) return fail(nullptr, IBX_E_INVALID, "hybrid systems run one per context");
-        if (c->hs[s].nmts > 0) return fail(nullptr, IBX_E_INVALID, "the MTS cycle runs on the oracle only in this build");
         // VIRTUAL_DEF at set-up (main.F90:148-150)
         HostSystem &h = c->hs[s];
         for (int v = 0; v < h.nvs; ++v) {
@@ -824,6 +861,16 @@ int ibx::create_context(std::vector<HostSystem> &&hs, ibx_ctx **out) {
     lk.unlock();
     int rc = build_device(c.get());
     if (rc) { g_create_err = c->err; return rc; }
+    if (nsys == 1 && c->hs[0].hybrid && c->hs[0].nmts > 0 && c->gr[1].present) {
+        // MTS_LOOP.f90:20-22 buffers (zeroed: the first cycle's explicit steps fill them)
+        ibx_ctx *cc = c.get();
+        const size_t npart = (size_t)cc->gr[1].nwg * (kBlock / kNbAtomsPerBlock);
+        if (cc->mts_f.alloc(9 * (size_t)cc->n_items) != hipSuccess || cc->mts_part.alloc(27 * npart) != hipSuccess ||
+            memset_st(cc, cc->mts_f.p, 0, cc->mts_f.bytes()) != hipSuccess ||
+            memset_st(cc, cc->mts_part.p, 0, cc->mts_part.bytes()) != hipSuccess)
+            return fail(nullptr, IBX_E_NOMEM, "MTS buffers");
+        cc->mts = true;
+    }
     rc = rebuild_checked(c.get());
     if (rc) { g_create_err = c->err; return rc; }
     *out = c.release();
@@ -908,27 +955,29 @@ int ibx_step(ibx_ctx *c, int nsteps, int step0) {
 
 int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
     if (!c || nsteps < 0) return fail(c, IBX_E_INVALID, "invalid arguments");
-    // eager head until the next rebuild step, graph blocks of nupdate steps, eager tail
+    // eager head until a step that starts a list interval (and a multiple-time-step cycle), graph
+    // blocks of that period, eager tail
+    const int period = block_period(c);
     int head = 0;
-    while (head < nsteps && (step0 + 1 + head) % c->nupdate != 0) ++head;
+    while (head < nsteps && (step0 + 1 + head) % period != 0) ++head;
     int rc = ibx_step(c, head, step0);
     if (rc) return rc;
     int done = head;
-    int nblk = (nsteps - done) / c->nupdate;
+    int nblk = (nsteps - done) / period;
     if (nblk > 0) {
-        if (!c->gexec || c->graph_steps != c->nupdate) {
+        if (!c->gexec || c->graph_steps != period) {
             drop_graph(c);
             std::lock_guard<std::mutex> lk(g_capture_mu);
             hipGraph_t graph;
             // relaxed: other host threads (other contexts, the serving loop) keep allocating and copying
             // while this one captures; nothing captured here is a call that capture forbids
             HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
-            for (int k = 0; k < c->nupdate; ++k) enqueue_step(c, c->nupdate + k, nullptr);   // block starts at a rebuild
+            for (int k = 0; k < period; ++k) enqueue_step(c, period + k, nullptr);   // block starts at a rebuild
             HIPCHK(c, hipStreamEndCapture(c->st, &graph));
             hipError_t e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (e != hipSuccess) return fail(c, IBX_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
-            c->graph_steps = c->nupdate;
+            c->graph_steps = period;
         }
         HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
         int cur = step0 + done;
@@ -936,9 +985,17 @@ int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
         for (int b = 0; b < nblk; ++b) HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
         rc = check_flags(c);
         if (rc) return rc;
-        done += nblk * c->nupdate;
+        done += nblk * period;
     }
     return ibx_step(c, nsteps - done, step0 + done);
+}
+
+int ibx_set_respa(ibx_ctx *c, int k) {
+    if (!c || k < 1 || k > 64) return fail(c, IBX_E_INVALID, "invalid multiple-time-step ratio");
+    if (c->mts && k > 1) return fail(c, IBX_E_INVALID, "the MTS cycle of a hybrid system excludes RESPA");
+    c->respa = k;
+    drop_graph(c);
+    return 0;
 }
 
 int ibx_set_skin_tuning(ibx_ctx *c, int on) {

@@ -1014,6 +1014,7 @@ struct MoveArgs {
     int *step;
     int nitems_total;
     int mode;
+    float nb_weight;          // multiple time step: k on the steps that evaluate the nonbonded force, else 0
 };
 
 // Adds the contribution slots [a, b) of one item in slot order, loading
@@ -1060,13 +1061,14 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     if (ty >= 0 && lab != 3) {
         double F[3], fbv[3] = {0, 0, 0};
         gather_contrib(A.contrib, A.ncontrib, q0, q1, fbv);
-        for (int k = 0; k < 3; ++k) F[k] = (double)fn[k] + fbv[k];
+        const double wnb = (double)A.nb_weight;
+        for (int k = 0; k < 3; ++k) F[k] = (double)fn[k] * wnb + fbv[k];
         if (vs >= 0) {
             // the virtual site's nonbonded and bonded forces, spread by mass fraction
             const double c = A.vs_coeff[it];
             double fv[3] = {0, 0, 0};
             gather_contrib(A.contrib, A.ncontrib, A.inc_off[vs], A.inc_off[vs + 1], fv);
-            for (int k = 0; k < 3; ++k) F[k] += ((double)A.fnb[k * n + vs] + fv[k]) * c;
+            for (int k = 0; k < 3; ++k) F[k] += ((double)A.fnb[k * n + vs] * wnb + fv[k]) * c;
         }
         if (A.mode == 1) {
             for (int k = 0; k < 3; ++k) A.ftot[k * n + it] = (float)F[k];
@@ -1338,8 +1340,58 @@ __global__ void k_tune_skin(DevSys *__restrict__ sys, int nsys, const int *__res
     G.dmax2u = (float)((0.49 * best_delta / u) * (0.49 * best_delta / u));
 }
 
+// MTS_LOOP.f90:71-224 on the device.  Explicit steps (MOD(STEP, 3+NMTS) = 1, 2, 3) keep the
+// nonbonded force of every bead-group item (beads and virtual sites, label 2 or 3) in slot m
+// (MTS_SAVEFORCE) and the group's per-workgroup energy and virial partials (MTS_SAVEAUXS, which
+// never stores PT23: its slot keeps zero).  The other steps replace both by F3 + (I-3)/2 (F3 - F1)
+// (MTS_APPROX, MTS_LOADAUXS).  One thread per item, and per partial.
+__global__ void __launch_bounds__(kBlock) k_mts_save(const float *__restrict__ fnb, const int *__restrict__ label,
+                                                     int n, const double *__restrict__ part, int npart,
+                                                     float *__restrict__ mts_f, double *__restrict__ mts_part, int m) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) {
+        const int lab = label[i];
+        if (lab == 2 || lab == 3)
+            for (int k = 0; k < 3; ++k) mts_f[((size_t)m * 3 + k) * n + i] = fnb[(size_t)k * n + i];
+    }
+    if (i < 9 * npart) {
+        const int field = i / npart;
+        mts_part[(size_t)m * 9 * npart + i] = field == 8 ? 0.0 : part[i];   // field 8 = PT23
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_mts_approx(float *__restrict__ fnb, const int *__restrict__ label, int n,
+                                                       double *__restrict__ part, int npart,
+                                                       const float *__restrict__ mts_f,
+                                                       const double *__restrict__ mts_part, double I) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    const double w = (I - 3.0) * 0.5;
+    if (i < n) {
+        const int lab = label[i];
+        if (lab == 2 || lab == 3)
+            for (int k = 0; k < 3; ++k) {
+                const double f1 = mts_f[(size_t)k * n + i], f3 = mts_f[((size_t)2 * 3 + k) * n + i];
+                fnb[(size_t)k * n + i] = (float)(f3 + w * (f3 - f1));
+            }
+    }
+    if (i < 9 * npart) {
+        const double p1 = mts_part[i], p3 = mts_part[(size_t)2 * 9 * npart + i];
+        part[i] = p3 + w * (p3 - p1);
+    }
+}
+
 // ------------------------------------------------------------------------------------------------
 // launch wrappers
+
+void launch_mts(hipStream_t st, bool save, float *fnb, const int *label, int n, double *part, int npart,
+                float *mts_f, double *mts_part, int m, double I) {
+    const int work = n > 9 * npart ? n : 9 * npart;
+    const dim3 grid((work + kBlock - 1) / kBlock);
+    if (save)
+        hipLaunchKernelGGL(k_mts_save, grid, dim3(kBlock), 0, st, fnb, label, n, part, npart, mts_f, mts_part, m);
+    else
+        hipLaunchKernelGGL(k_mts_approx, grid, dim3(kBlock), 0, st, fnb, label, n, part, npart, mts_f, mts_part, I);
+}
 
 void launch_tune_skin(hipStream_t st, DevSys *sys, int nsys, const int *dmax, int nmove, int tune) {
     hipLaunchKernelGGL(k_tune_skin, dim3((2 * nsys + kWave - 1) / kWave), dim3(kWave), 0, st, sys, nsys, dmax, nmove,
@@ -1428,6 +1480,7 @@ void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, 
     A.pos4[0] = m.pos4[0]; A.pos4[1] = m.pos4[1]; A.inv_perm[0] = m.inv_perm[0]; A.inv_perm[1] = m.inv_perm[1];
     A.pos4_ref[0] = m.pos4_ref[0]; A.pos4_ref[1] = m.pos4_ref[1]; A.dmax = m.dmax; A.nsys = m.nsys;
     A.sysstate = m.sysstate; A.part = m.part; A.step = m.step; A.nitems_total = m.nitems_total; A.mode = m.mode;
+    A.nb_weight = m.nb_weight;
     hipLaunchKernelGGL(k_move, dim3(nwg), dim3(kBlock), 0, st, sys, wg_sys, A);
 }
 

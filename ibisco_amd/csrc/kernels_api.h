@@ -77,9 +77,12 @@ struct MoveLaunch {
     double *part;
     int *step;
     int nitems_total, mode;
+    float nb_weight;    // weight of the nonbonded force in this move (multiple time step)
 };
 
 void launch_zero_int(hipStream_t st, int *p, int n);
+void launch_mts(hipStream_t st, bool save, float *fnb, const int *label, int n, double *part, int npart,
+                float *mts_f, double *mts_part, int m, double I);
 void launch_tune_skin(hipStream_t st, DevSys *sys, int nsys, const int *dmax, int nmove, int tune);
 void launch_rebuild(hipStream_t st, const RebuildArgs &a);
 void launch_pack_only(hipStream_t st, const RebuildArgs &a);

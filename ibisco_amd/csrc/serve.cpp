@@ -292,7 +292,6 @@ int ibx_server_submit(ibx_server *S, const ibx_system_desc *desc, int nsteps, in
     std::string err;
     int rc = build_host_system(desc, j->hs, err);
     if (rc) return serr(rc, err);
-    if (j->hs.nmts > 0) return serr(IBX_E_INVALID, "the MTS cycle runs on the oracle only in this build");
     j->natoms = j->hs.natoms;
     j->nitems = j->hs.nitems;
     j->nsteps = nsteps;

@@ -56,7 +56,7 @@ EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_s
            "ibx_get_pairs", "ibx_rdf_enable", "ibx_rdf_get", "ibx_get_timing", "ibx_set_timing",
            "ibx_get_info", "ibx_retire", "ibx_server_create", "ibx_server_submit", "ibx_server_wait",
            "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes",
-           "ibx_set_skin_tuning", "ibx_get_skin")
+           "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa")
 
 _lib = None
 
@@ -95,6 +95,7 @@ def load_library(path=LIB_PATH):
     L.ibx_get_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.ibx_retire.argtypes = [vp, C.c_int]
     L.ibx_set_skin_tuning.argtypes = [vp, C.c_int]
+    L.ibx_set_respa.argtypes = [vp, C.c_int]
     L.ibx_get_skin.argtypes = [vp, C.c_int, _dp]
     L.ibx_server_create.argtypes = [C.POINTER(IbxServerConfig), C.POINTER(vp)]
     L.ibx_server_submit.argtypes = [vp, C.POINTER(IbxSystemDesc), C.c_int, C.c_int, C.POINTER(C.c_int64)]
@@ -283,6 +284,10 @@ class Engine:
 
     def retire(self, sys):
         self._chk(self.L.ibx_retire(self.h, sys))
+
+    def set_respa(self, k):
+        """Nonbonded forces every k steps with weight k, bonded every step (ibx_set_respa)."""
+        self._chk(self.L.ibx_set_respa(self.h, int(k)))
 
     def set_skin_tuning(self, on=True):
         self._chk(self.L.ibx_set_skin_tuning(self.h, int(on)))

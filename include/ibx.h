@@ -90,8 +90,11 @@ const char *ibx_last_error(const ibx_ctx *ctx);   /* ctx may be NULL: last creat
 /* nsteps of the step loop (NEW_LOOP.F90:19-156), steps numbered
  * step0+1..step0+nsteps: SHIFT, VIRTUAL_DEF, UPDATE_NEIGHBOURLIST every
  * nupdate, NEW_FORCE, MOVE, MOMENTUM every halt_drift, AVERAGE.  All replicas
- * of a batch advance together.  The MTS cycle (MTS_LOOP.f90, nmts>0) is not on
- * the device: ibx_create rejects such systems with IBX_E_INVALID. */
+ * of a batch advance together.  A hybrid system with nmts > 0 runs the MTS
+ * cycle of MTS_LOOP.f90 instead of NEW_FORCE: the bead group's nonbonded forces,
+ * energies and virial are evaluated on steps with MOD(step, 3+nmts) = 1, 2, 3
+ * and extrapolated on the others, with I = 0 on the last step of a cycle as in
+ * the reference when mts_quirk is set (else I = 3+nmts). */
 int ibx_step(ibx_ctx *ctx, int nsteps, int step0);
 
 /* The same steps as ibx_step, launched as a captured HIP graph replayed per
@@ -151,6 +154,15 @@ int ibx_set_timing(ibx_ctx *ctx, int on);
  * [4] padded slots of group 0, [5] device bytes the context holds,
  * [6] systems in the context, [7] padded items of all systems. */
 int ibx_get_info(ibx_ctx *ctx, int sys, int64_t info[8]);
+
+/* Multiple time step (r-RESPA in leap-frog form; no counterpart in the
+ * reference, whose MTS_LOOP.f90 extrapolates the CG forces of hybrid models):
+ * every system of the context evaluates its nonbonded forces on steps 1,
+ * k+1, 2k+1, ... and applies them with weight k (an impulse of k inner
+ * steps), and its bonded terms on every step; dt stays the inner step.
+ * Observables on the other steps carry the last nonbonded energy and virial.
+ * k = 1 (the default) is the plain step loop, bit for bit. */
+int ibx_set_respa(ibx_ctx *ctx, int k);
 
 /* List-skin autotuner (on by default; no counterpart in the reference).  At
  * every scheduled rebuild each system's inner skin delta -- pairs closer than

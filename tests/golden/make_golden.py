@@ -10,7 +10,8 @@ after short runs.  The atomisticPA case also keeps, as numbers, the expected
 coordinates and velocities of the first three atoms that the reference's
 testsuite compares against (testsuite/tests.py:47-68).  comb_oop is a
 generated comb polymer whose branch atoms carry out-of-plane terms, and
-comb_oop_only the same with every other table zero (out-of-plane forces alone).
+comb_oop_only the same with every other table zero (out-of-plane forces alone);
+hybridPA_mts runs hybridPA through MTS_LOOP's force (MTS_FORCE, NMTS 2) for 12 steps.
 
 Inputs are stored in single precision, which is what the reference's REAL*4
 readers see, so the oracle and the reference start from identical numbers.
@@ -158,6 +159,16 @@ def testsuite_expected():
     return out
 
 
+def mts_case():
+    """hybridPA stepped by MTS_LOOP's force (NMTS 2: explicit steps 1-3 of every 5, extrapolated 4 and 5,
+    step 5 with the MOD = 0 backward extrapolation), NVE, list rebuilt every 5 steps."""
+    s = testsuite_case("hybridPA")
+    s.control.ensemble = "NVE"
+    s.control.mts = 2
+    s.control.nupdate = 5
+    return s
+
+
 def make_case(name, sysm, runs=()):
     sysm = single(sysm)
     d = tempfile.mkdtemp(prefix="ibx_gold_")
@@ -187,6 +198,8 @@ def main(only=()):
         for name in only:
             if name == "comb_oop":
                 make_case("comb_oop", system.comb_melt(nchains=36, seed=1), runs=(10,))
+            if name == "hybridPA_mts":
+                make_case("hybridPA_mts", mts_case(), runs=(12,))
             if name == "comb_oop_only":
                 make_case("comb_oop_only", system.comb_melt(nchains=36, seed=1, oop_only=True), runs=(10,))
         return
@@ -207,6 +220,7 @@ def main(only=()):
     print("atomisticPA expected arrays:", {k: v.shape for k, v in exp.items()})
     make_case("comb_oop", system.comb_melt(nchains=36, seed=1), runs=(10,))
     make_case("comb_oop_only", system.comb_melt(nchains=36, seed=1, oop_only=True), runs=(10,))
+    make_case("hybridPA_mts", mts_case(), runs=(12,))
 
 
 if __name__ == "__main__":

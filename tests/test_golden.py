@@ -302,3 +302,33 @@ def test_atomistic_testsuite_numbers(oracle_mod):
         print(key, "coords", dpos, "velocities", dvel)
         assert dpos <= 2e-3
         assert dvel <= 0.1
+
+
+def test_mts_cycle_matches_reference(oracle_mod):
+    """MTS_LOOP.f90:71-224 on hybridPA (NMTS 2, NVE, rebuild every 5): the oracle's MTS cycle against the
+    reference's MTS_FORCE step by step.  Steps with MOD(STEP, 3+NMTS) = 1, 2, 3 evaluate the CG forces;
+    MOD = 4 extrapolates forwards; MOD = 0 takes I = 0 (MTS_LOOP.f90:176) and so extrapolates backwards,
+    which the oracle reproduces with its quirk flag.  Without the flag the steps 5 and 10 differ."""
+    z, sysm = _load("hybridPA_mts")
+    ref = z["run12_obs"]
+    per_step = []
+    for quirk in (True, False):
+        o = oracle_mod.Oracle(sysm, mts_quirk=quirk)
+        rows = []
+        for k in range(12):
+            o.run(1, k)
+            ob = o.obs()
+            rows.append(np.concatenate([ob[q] for q in ("vnb", "vbond", "vangle", "vtors", "voop")]))
+        per_step.append(np.array(rows))
+        if quirk:
+            s, v, _ = o.state()
+    scale = np.abs(ref[:, :15]).sum(1)
+    err = np.abs(per_step[0] - ref[:, :15]).max(1) / scale
+    print("MTS per-step energy error (quirk)", err)
+    assert np.all(err <= OBS_RTOL)
+    err_plain = np.abs(per_step[1] - ref[:, :15]).max(1) / scale
+    print("without the quirk", err_plain)
+    assert err_plain[4] > 10 * OBS_RTOL and err_plain[9] > 10 * OBS_RTOL
+    sr, vr = z["run12_s"].astype(np.float64), z["run12_v"].astype(np.float64)
+    assert np.abs(s[:len(sr)] - sr).max() <= POS_ATOL
+    assert np.abs(v[:len(vr)] - vr).max() / np.abs(vr).max() <= VEL_RTOL

@@ -111,7 +111,7 @@ def replicas_compare(nrep, steps, warmup, seed=3):
             "ratio": vb / vp}
 
 
-def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=16, nsample=12, ncheck=4):
+def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=16, nsample=12, ncheck=4, step_band=0.0):
     """Config-4 serving: Poisson arrivals of CG melt jobs (1k-100k beads, 500-5000 steps, each with its own
     corrected table) submitted to the serving loop at their arrival times; every job returns its RDF and mean
     pressure.  Job states are equilibrated (batched, untimed) before the stream starts.  A sample of the jobs
@@ -144,7 +144,7 @@ def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=16, nsam
             batch, nb = [], 0
     sample = sorted(range(nj), key=lambda k: systems[k].natoms)[::max(1, nj // nsample)][:nsample]
     check = set(sorted(sample, key=lambda k: systems[k].natoms)[:ncheck])
-    srv = Server(hbm_budget=budget, max_cohorts=max_cohorts, rdf_nbin=150, rdf_dr=0.01)
+    srv = Server(hbm_budget=budget, max_cohorts=max_cohorts, rdf_nbin=150, rdf_dr=0.01, step_band=step_band)
     results = [None] * nj
     ids = queue.Queue()
 
@@ -215,8 +215,40 @@ def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=16, nsam
                        "peak_concurrent_batches": int(st["peak_running"]),
                        "peak_resident_bytes": int(st["peak_bytes"]), "budget_bytes": int(st["budget"])},
             "plain_sample": {"value": vp, "jobs": len(sample), "path": "one context per job, in turn"},
-            "max_concurrent_batches": max_cohorts, "ratio": vs / vp, "alone_equals_served_bitwise": same, "jobs_checked_bitwise": len(check),
+            "max_concurrent_batches": max_cohorts, "step_band": step_band, "ratio": vs / vp, "alone_equals_served_bitwise": same, "jobs_checked_bitwise": len(check),
             "mean_pressure_bar_p50": float(np.median(pres))}
+
+
+def hybrid_run(steps, warmup, seed=7):
+    """Config 2: 500,000 atoms' worth of chains of 60 (30 % atomistic, one centre-of-mass site per 6 atoms;
+    70 % coarse-grained, 10 beads each), three LJ tables with drawn sigma/epsilon, rc 1.4 nm, 1 fs."""
+    from ibisco_amd import system, workloads
+    from ibisco_amd.engine import Engine
+    nch = 500_000 // 60
+    nat = int(round(0.3 * nch))
+    sy = system.hybrid_melt(nat, nch - nat, chain_len=60, seed=seed, avoid=None, dt=0.001)
+    n = sy.natoms + sy.nvs
+    print(f"[bench] hybrid: {sy.natoms} atoms+beads, {sy.nvs} sites, box {sy.box[0]:.2f} nm", file=sys.stderr,
+          flush=True)
+    e = Engine(sy)
+    workloads.equilibrate(e, sy, stage_steps=100, final_steps=400)
+    e.step(warmup, graph=True)
+    e.obs()
+    t0 = time.perf_counter()
+    e.step(steps, graph=True)
+    ob = e.obs()
+    secs = time.perf_counter() - t0
+    e.set_timing(True)
+    e.step(50)
+    tm = e.timing()
+    e.close()
+    print(f"[bench] hybrid: {steps} steps in {secs:.3f} s", file=sys.stderr, flush=True)
+    return {"workload": "hybrid_com_sites_500k", "atoms": int((sy.itype == 0).sum()), "beads": int((sy.itype == 1).sum()),
+            "sites": sy.nvs, "items": n, "steps": steps, "value": sy.natoms * steps / secs,
+            "unit": "atom-steps/s (atoms and beads; sites are not integrated)", "ms_per_step": 1e3 * secs / steps,
+            "step_breakdown_ms": {k: round(tm[k] / tm["nsteps_timed"], 4) for k in
+                                  ("nonbonded", "bonded", "integrate", "rebuild", "step")},
+            "temperature_K": float(ob["temp"] * 300.0)}
 
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_nonbonded_ua100k_traffic.md")
@@ -254,6 +286,10 @@ def main():
                          "(0 skips it)")
     ap.add_argument("--replica-steps", type=int, default=200)
     ap.add_argument("--serve-cohorts", type=int, default=16, help="batches the serving loop runs at once")
+    ap.add_argument("--serve-band", type=float, default=0.0,
+                    help="serving: batch only jobs whose step counts are within this factor (<= 1: off)")
+    ap.add_argument("--hybrid-steps", type=int, default=500,
+                    help="also time the 500k-atom hybrid system for this many steps (0 skips it)")
     ap.add_argument("--serve-seconds", type=float, default=20.0,
                     help="also run the serving loop on Poisson arrivals (20 jobs/s) for this many seconds "
                          "(0 skips it)")
@@ -345,7 +381,9 @@ def main():
         print("[bench] timing the reference CPU path", file=sys.stderr, flush=True)
         cpu = cpu_baseline(sysm, args.cpu_steps, args.cpu_threads)
     batched = replicas_compare(args.replicas, args.replica_steps, 50) if args.replicas > 0 else None
-    served = serve_stream(20.0, args.serve_seconds, max_cohorts=args.serve_cohorts) if args.serve_seconds > 0 else None
+    served = serve_stream(20.0, args.serve_seconds, max_cohorts=args.serve_cohorts,
+                          step_band=args.serve_band) if args.serve_seconds > 0 else None
+    hybrid = hybrid_run(args.hybrid_steps, 50) if args.hybrid_steps > 0 else None
     out = {
         "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps, "higher_is_better": True,
@@ -368,6 +406,8 @@ def main():
         out["batched_vs_plain"] = batched
     if served:
         out["served_vs_plain"] = served
+    if hybrid:
+        out["hybrid"] = hybrid
     print(json.dumps(out))
 
 

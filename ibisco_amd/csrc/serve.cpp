@@ -178,15 +178,19 @@ void run_cohort(ibx_server *S, std::vector<JobPtr> jobs, size_t bytes, int cid) 
 }
 
 // Takes queued jobs, in arrival order, that fit the free bytes: jobs whose
-// update interval differs from the first one's wait for a later cohort, and a
-// hybrid system runs alone.  Called with the lock held.
+// update interval differs from the first one's (or, with a step band, whose
+// step count is too far from it) wait for a later cohort, and a hybrid system
+// runs alone.  Called with the lock held.
 std::vector<JobPtr> take_cohort(ibx_server *S, size_t &bytes) {
     std::vector<JobPtr> take;
     bytes = 0;
     const Job &head = *S->queue.front();
     for (auto it = S->queue.begin(); it != S->queue.end() && (int)take.size() < S->cfg.max_cohort_jobs;) {
         const Job &j = **it;
-        const bool fits_kind = j.nupdate == head.nupdate && (take.empty() || (!j.hybrid && !head.hybrid));
+        const double band = S->cfg.step_band;
+        const bool fits_steps = band <= 1.0 || (j.nsteps <= band * head.nsteps && head.nsteps <= band * j.nsteps);
+        const bool fits_kind =
+            j.nupdate == head.nupdate && fits_steps && (take.empty() || (!j.hybrid && !head.hybrid));
         if (!fits_kind) {
             ++it;
             continue;

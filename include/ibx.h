@@ -203,6 +203,8 @@ typedef struct {
     int chunk_steps;      /* most steps between completion checks of a batch (<= 0: 200) */
     int rdf_nbin;         /* RDF bins per job (0: none) */
     double rdf_dr;        /* RDF bin width, nm */
+    double step_band;     /* > 1: a batch takes, besides the oldest queued job, only jobs whose step counts
+                             are within this factor of its own, so its members end close together (<= 1: off) */
 } ibx_server_config;
 
 typedef struct {

@@ -530,7 +530,8 @@ __device__ __forceinline__ void nb_pair(const NbCtx &C, const int4 pi, int j, co
     const double rd = fma(fma(-r0, r0, r2d), (double)(0.5f * rs), r0);
     const double tt = rd * tb.c;
     const int kr = min((int)tt, tb.ndat);
-    float a = (float)(tt - (double)kr);
+    // tt - floor(tt), exact (kr = floor(tt) for every pair inside rc, so within the table)
+    float a = (float)__builtin_amdgcn_fract(tt);
     if (!tb.uniform) a -= C.xoff[C.xoff_base + tb.row_off + kr];
     const float4 row = C.rows[tb.row_off + kr];
     const float F = fmaf(a, row.z, row.x);
@@ -551,6 +552,43 @@ __device__ __forceinline__ void nb_pair(const NbCtx &C, const int4 pi, int j, co
     w[3] = fmaf(fdy, dx, w[3]);
     w[4] = fmaf(fdx, dz, w[4]);
     w[5] = fmaf(fdy, dz, w[5]);
+}
+
+// Position word of slot j, addressed by a 32-bit byte offset from the array
+// base so the gather keeps the base in scalar registers.
+__device__ __forceinline__ int4 ld_pos(const int4 *__restrict__ p, int j) {
+    return *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(p) + (uint32_t)j * 16u);
+}
+
+// One run of a slot's list: entries q, q + kLanes, ... at l[0], l[kStep], ...
+// (kStep = +kWave for the front run, -kWave for the back run), unrolled by four
+// with the next four indices loaded a round ahead of their positions.
+template <bool kMulti, int kGroup, bool kCubic, int kStep>
+__device__ __forceinline__ void nb_run(const NbCtx &C, const int4 *__restrict__ pos4, const int *__restrict__ l,
+                                       int nm, const int4 pi, int li, float &fx, float &fy, float &fz,
+                                       float (&e)[3], float (&w)[6]) {
+    int m = 0;
+    int j0 = 0, j1 = 0, j2 = 0, j3 = 0;
+    if (nm >= 4) {
+        j0 = l[0]; j1 = l[kStep]; j2 = l[2 * kStep]; j3 = l[3 * kStep];
+    }
+    for (; m + 4 <= nm; m += 4) {
+        const int4 p0 = ld_pos(pos4, j0), p1 = ld_pos(pos4, j1), p2 = ld_pos(pos4, j2), p3 = ld_pos(pos4, j3);
+        l += 4 * kStep;
+        int k0 = j0, k1 = j1, k2 = j2, k3 = j3;
+        if (m + 8 <= nm) {
+            k0 = l[0]; k1 = l[kStep]; k2 = l[2 * kStep]; k3 = l[3 * kStep];
+        }
+        nb_pair<kMulti, kGroup, kCubic>(C, pi, j0, p0, li, fx, fy, fz, e, w);
+        nb_pair<kMulti, kGroup, kCubic>(C, pi, j1, p1, li, fx, fy, fz, e, w);
+        nb_pair<kMulti, kGroup, kCubic>(C, pi, j2, p2, li, fx, fy, fz, e, w);
+        nb_pair<kMulti, kGroup, kCubic>(C, pi, j3, p3, li, fx, fy, fz, e, w);
+        j0 = k0; j1 = k1; j2 = k2; j3 = k3;
+    }
+    for (; m < nm; ++m, l += kStep) {
+        const int j = l[0];
+        nb_pair<kMulti, kGroup, kCubic>(C, pi, j, ld_pos(pos4, j), li, fx, fy, fz, e, w);
+    }
 }
 
 // Nonbonded forces (replaces NONBONDED_FORCE.F90:11-109).
@@ -635,35 +673,10 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
         // front run: entry q + kLanes*m at lp[m*kWave]; back run: entry q + kLanes*m at lb[-m*kWave]
         const int *lp = list + list_pos(i, q, cap);
         const int *lb = list + list_pos(i, cap - 1 - q, cap);
-        for (int run = 0; run < 2; ++run) {
-            const int n = run == 0 ? n_in : n_out;
-            const int *l0 = run == 0 ? lp : lb;
-            const ptrdiff_t step = run == 0 ? kWave : -kWave;
-            const int nm = n > q ? (n - q + kLanes - 1) / kLanes : 0;
-            int m = 0;
-            // indices of the next four pairs are loaded one round ahead of their positions
-            int j0 = 0, j1 = 0, j2 = 0, j3 = 0;
-            if (nm >= 4) {
-                j0 = l0[0]; j1 = l0[step]; j2 = l0[2 * step]; j3 = l0[3 * step];
-            }
-            for (; m + 4 <= nm; m += 4) {
-                const int4 p0 = pos4[j0], p1 = pos4[j1], p2 = pos4[j2], p3 = pos4[j3];
-                int k0 = j0, k1 = j1, k2 = j2, k3 = j3;
-                if (m + 8 <= nm) {
-                    k0 = l0[(m + 4) * step]; k1 = l0[(m + 5) * step];
-                    k2 = l0[(m + 6) * step]; k3 = l0[(m + 7) * step];
-                }
-                nb_pair<kMulti, kGroup, kCubic>(C, pi, j0, p0, li, fx, fy, fz, e, w);
-                nb_pair<kMulti, kGroup, kCubic>(C, pi, j1, p1, li, fx, fy, fz, e, w);
-                nb_pair<kMulti, kGroup, kCubic>(C, pi, j2, p2, li, fx, fy, fz, e, w);
-                nb_pair<kMulti, kGroup, kCubic>(C, pi, j3, p3, li, fx, fy, fz, e, w);
-                j0 = k0; j1 = k1; j2 = k2; j3 = k3;
-            }
-            for (; m < nm; ++m) {
-                const int j = l0[m * step];
-                nb_pair<kMulti, kGroup, kCubic>(C, pi, j, pos4[j], li, fx, fy, fz, e, w);
-            }
-        }
+        const int nm_in = n_in > q ? (n_in - q + kLanes - 1) / kLanes : 0;
+        const int nm_out = n_out > q ? (n_out - q + kLanes - 1) / kLanes : 0;
+        nb_run<kMulti, kGroup, kCubic, kWave>(C, pos4, lp, nm_in, pi, li, fx, fy, fz, e, w);
+        if (nm_out > 0) nb_run<kMulti, kGroup, kCubic, -kWave>(C, pos4, lb, nm_out, pi, li, fx, fy, fz, e, w);
     }
     // the kLanes lanes of a slot hold partial forces: butterfly-sum them
     for (int off = 1; off < kLanes; off <<= 1) {

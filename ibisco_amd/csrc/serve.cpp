@@ -275,7 +275,7 @@ int ibx_server_create(const ibx_server_config *cfg, ibx_server **out) {
     if (cfg) S->cfg = *cfg;
     ibx_server_config &c = S->cfg;
     if (c.hbm_budget <= 0) c.hbm_budget = (int64_t)200e9;
-    if (c.max_cohorts <= 0) c.max_cohorts = 8;
+    if (c.max_cohorts <= 0) c.max_cohorts = 3;
     if (c.max_cohort_jobs <= 0) c.max_cohort_jobs = 256;
     if (c.chunk_steps <= 0) c.chunk_steps = 200;
     if (c.rdf_nbin < 0 || c.rdf_nbin > 8192 || (c.rdf_nbin > 0 && !(c.rdf_dr > 0)))

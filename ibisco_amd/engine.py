@@ -317,8 +317,8 @@ class Server:
     """The serving loop of libibx (ibx_server_*): jobs are admitted by device
     footprint into batched contexts that run concurrently; see include/ibx.h."""
 
-    def __init__(self, hbm_budget=200e9, max_cohorts=8, max_cohort_jobs=256, chunk_steps=200, rdf_nbin=150,
-                 rdf_dr=0.01, step_band=0.0):
+    def __init__(self, hbm_budget=200e9, max_cohorts=3, max_cohort_jobs=256, chunk_steps=200, rdf_nbin=150,
+                 rdf_dr=0.01, step_band=1.5):
         self.L = load_library()
         self.h = C.c_void_p()
         cfg = IbxServerConfig(int(hbm_budget), max_cohorts, max_cohort_jobs, chunk_steps, rdf_nbin, rdf_dr,

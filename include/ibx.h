@@ -198,7 +198,7 @@ typedef struct ibx_server ibx_server;
 
 typedef struct {
     int64_t hbm_budget;   /* device bytes for all resident jobs (<= 0: 200e9; capped at 90% of free memory) */
-    int max_cohorts;      /* batches stepping at once (<= 0: 8) */
+    int max_cohorts;      /* batches stepping at once (<= 0: 3) */
     int max_cohort_jobs;  /* jobs per batch (<= 0: 256) */
     int chunk_steps;      /* most steps between completion checks of a batch (<= 0: 200) */
     int rdf_nbin;         /* RDF bins per job (0: none) */

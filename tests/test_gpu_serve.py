@@ -96,7 +96,7 @@ def test_served_jobs_equal_alone():
     specs = [(10, 40, 31, 180), (14, 30, 32, 23), (8, 50, 33, 57), (16, 25, 34, 40), (12, 35, 35, 75),
              (6, 60, 36, 31), (9, 44, 37, 66)]
     systems = _equilibrated([_job(nc, nb, sd) for nc, nb, sd, _ in specs])
-    srv = Server(max_cohorts=2, max_cohort_jobs=3, chunk_steps=16, rdf_nbin=NBIN, rdf_dr=DR)
+    srv = Server(max_cohorts=2, max_cohort_jobs=3, chunk_steps=16, rdf_nbin=NBIN, rdf_dr=DR, step_band=0.0)
     ids = []
     for k, (sy, sp) in enumerate(zip(systems, specs)):
         ids.append(srv.submit(sy, sp[3], keep_state=True))
@@ -119,3 +119,23 @@ def test_served_jobs_equal_alone():
         p = (ref["sums"][15] + ref["sums"][16] + ref["sums"][17]) / (3 * ref["sums"][28])
         assert r["mean_pressure"] == p
         assert r["t_submit"] <= r["t_admit"] <= r["t_done"]
+
+
+def test_step_band_groups_similar_jobs():
+    """With a step band, a batch holds the oldest queued job and only jobs whose step counts are within the
+    band of its own; every result still equals the job alone."""
+    specs = [(8, 40, 41, 200), (8, 40, 42, 30), (8, 40, 43, 190), (8, 40, 44, 35), (8, 40, 45, 210)]
+    *systems, slow = _equilibrated([_job(nc, nb, sd) for nc, nb, sd, _ in specs] + [_job(4, 40, 49)])
+    srv = Server(max_cohorts=1, max_cohort_jobs=8, chunk_steps=64, rdf_nbin=NBIN, rdf_dr=DR, step_band=1.5)
+    blocker = srv.submit(slow, 400)     # keeps the only batch slot busy while the rest queue up
+    ids = [srv.submit(sy, sp[3], keep_state=True) for sy, sp in zip(systems, specs)]
+    res = [srv.wait(j, timeout=120) for j in ids]
+    srv.wait(blocker, timeout=120)
+    srv.close()
+    cohort = {sp[3]: r["cohort"] for sp, r in zip(specs, res)}
+    print("cohorts by step count", cohort, flush=True)
+    assert cohort[200] == cohort[190] == cohort[210]
+    assert cohort[30] == cohort[35] != cohort[200]
+    for sy, sp, r in zip(systems, specs, res):
+        ref = _alone(sy, sp[3])
+        assert np.array_equal(r["s"], ref["s"]) and np.array_equal(r["rdf"], ref["rdf"])

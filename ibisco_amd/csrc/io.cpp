@@ -116,7 +116,9 @@ int read_input_dir(const char *dir, DescStorage &st, std::string &err) {
     D.dt = dt_ps * 1.0e-12 / U.timescale;
     D.temp_in = temp_k / U.tempscale;
     D.fac = D.dt / (taut * 1.0e-12 / U.timescale);
-    D.mts_quirk = 0;
+    // the reference's MTS_LOOP takes I = MOD(STEP, 3+NMTS) = 0 on the last step of each cycle
+    // (MTS_LOOP.f90:100, :176): an input directory runs the same cycle
+    D.mts_quirk = 1;
 
     // interaction
     std::vector<std::string> il;

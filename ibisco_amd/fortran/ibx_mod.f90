@@ -84,6 +84,20 @@ MODULE IBX_MOD
        INTEGER(C_INT), VALUE :: sys
        INTEGER(C_INT64_T), INTENT(OUT) :: info(8)
      END FUNCTION ibx_get_info
+
+     ! multiple time step: nonbonded forces every k steps with weight k, bonded every step
+     INTEGER(C_INT) FUNCTION ibx_set_respa(ctx, k) BIND(C, NAME='ibx_set_respa')
+       IMPORT :: C_INT, C_PTR
+       TYPE(C_PTR), VALUE :: ctx
+       INTEGER(C_INT), VALUE :: k
+     END FUNCTION ibx_set_respa
+
+     ! list-skin autotuner on (1) or off (0)
+     INTEGER(C_INT) FUNCTION ibx_set_skin_tuning(ctx, on) BIND(C, NAME='ibx_set_skin_tuning')
+       IMPORT :: C_INT, C_PTR
+       TYPE(C_PTR), VALUE :: ctx
+       INTEGER(C_INT), VALUE :: on
+     END FUNCTION ibx_set_skin_tuning
   END INTERFACE
 
 CONTAINS

@@ -421,38 +421,53 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
     const int gshift = (threadIdx.x & (kWave - 1)) & ~(kListLanes - 1);   // this group's bits in a wave ballot
     const unsigned below = (1u << sub) - 1u;
     int n_in = 0, n_out = 0, r = 0;
-    for (int t0 = 0; t0 < tmax; t0 += kListLanes) {
-        const int p = t0 + sub;
-        bool hit = false, inner = false;
-        int j = -1;
-        if (p < T) {
-            while (p >= rend[grp][r]) ++r;
-            j = rbase[grp][r] + p;
-            const int4 pj = pos4[j];
-            float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
-            float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
-            float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
-            if (!kCubic) { dx *= rho0; dy *= rho1; dz *= rho2; }
-            const float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-            bool cand = r2u < lo;
-            if (r2u >= lo && r2u <= hi) cand = exact_r2(ri, r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2;
-            bool skip = (j == i) || (hybrid && li == 3 && pos_label(pj.w) == 3);
+    // two candidates per lane and round: both position loads are in flight together; hits are
+    // written in candidate order, as one candidate per round would
+    for (int t0 = 0; t0 < tmax; t0 += 2 * kListLanes) {
+        int jj[2] = {-1, -1};
+        int4 pjj[2] = {make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
 #pragma unroll
-            for (int e = 0; e < 8; ++e) skip |= (ex[e] == j);
-            if (cand && !skip && ex1 - ex0 > 8)
-                for (int e = ex0 + 8; e < ex1; ++e) skip |= (inv_perm[excl[e]] == j);
-            hit = cand && !skip;
-            inner = r2u < rin2u;
+        for (int u = 0; u < 2; ++u) {
+            const int p = t0 + u * kListLanes + sub;
+            if (p < T) {
+                while (p >= rend[grp][r]) ++r;
+                jj[u] = rbase[grp][r] + p;
+            }
         }
-        const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
-        const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
-        if (hit) {
-            const unsigned k = inner ? (unsigned)n_in + __popc(gi & below) : (unsigned)(cap - 1 - n_out - __popc(go & below));
-            const bool fits = inner ? (int)k < cap : n_out + __popc(go & below) < cap;
-            if (fits) list[list_pos(i, (int)k, cap)] = j;
+#pragma unroll
+        for (int u = 0; u < 2; ++u)
+            if (jj[u] >= 0) pjj[u] = pos4[jj[u]];
+#pragma unroll
+        for (int u = 0; u < 2; ++u) {
+            bool hit = false, inner = false;
+            const int j = jj[u];
+            if (j >= 0) {
+                const int4 pj = pjj[u];
+                float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
+                float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
+                float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
+                if (!kCubic) { dx *= rho0; dy *= rho1; dz *= rho2; }
+                const float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+                bool cand = r2u < lo;
+                if (r2u >= lo && r2u <= hi) cand = exact_r2(ri, r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2;
+                bool skip = (j == i) || (hybrid && li == 3 && pos_label(pj.w) == 3);
+#pragma unroll
+                for (int e = 0; e < 8; ++e) skip |= (ex[e] == j);
+                if (cand && !skip && ex1 - ex0 > 8)
+                    for (int e = ex0 + 8; e < ex1; ++e) skip |= (inv_perm[excl[e]] == j);
+                hit = cand && !skip;
+                inner = r2u < rin2u;
+            }
+            const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
+            const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
+            if (hit) {
+                const unsigned k = inner ? (unsigned)n_in + __popc(gi & below) : (unsigned)(cap - 1 - n_out - __popc(go & below));
+                const bool fits = inner ? (int)k < cap : n_out + __popc(go & below) < cap;
+                if (fits) list[list_pos(i, (int)k, cap)] = j;
+            }
+            n_in += __popc(gi);
+            n_out += __popc(go);
         }
-        n_in += __popc(gi);
-        n_out += __popc(go);
     }
     if (valid && sub == 0) {
         nneb[i] = make_int2(n_in, n_out);

@@ -87,9 +87,10 @@ struct ibx_ctx {
     DBuf<int> wg_items, itype, label, excl_off, excl, vs_of, vs_center, vs_moff, vs_members;
     DBuf<int> term_meta, inc_off, wg_terms, nb_map, flags, step;
     DBuf<int> dmax;                  // [group][system] largest displacement^2 since the rebuild (float bits)
+    DBuf<int> fin_done;              // per system: field-sum workgroups finished in this launch
     DBuf<int4> term_slot;           // contribution slot of each member of a term
     DBuf<int4> term_mem;
-    DBuf<float> contrib;
+    DBuf<float4> contrib;           // bonded contribution slots {x, y, z, 0}
     int nterms = 0, term_nwg = 0, ncontrib = 0;
     DBuf<double> s, v, mass, invmass, vs_coeff, vs_invmass, part_bd, part_mv, fsum, obs, avg, sysstate;
     DBuf<float> fnb, ftot;
@@ -438,7 +439,7 @@ int build_device(ibx_ctx *c) {
         return rc;
     c->h_nb_tabs = hntab;
     HIPCHK(c, c->fnb.alloc(3 * (size_t)N));
-    HIPCHK(c, c->contrib.alloc(3 * (size_t)c->ncontrib));
+    HIPCHK(c, c->contrib.alloc((size_t)c->ncontrib));
     HIPCHK(c, memset_st(c, c->contrib.p, 0, c->contrib.bytes()));
     HIPCHK(c, c->ftot.alloc(3 * (size_t)N));
     HIPCHK(c, memset_st(c, c->fnb.p, 0, c->fnb.bytes()));
@@ -448,6 +449,8 @@ int build_device(ibx_ctx *c) {
     HIPCHK(c, memset_st(c, c->part_bd.p, 0, c->part_bd.bytes()));
     HIPCHK(c, memset_st(c, c->part_mv.p, 0, c->part_mv.bytes()));
     HIPCHK(c, c->fsum.alloc((size_t)S * kFields));
+    HIPCHK(c, c->fin_done.alloc((size_t)S));
+    HIPCHK(c, memset_st(c, c->fin_done.p, 0, c->fin_done.bytes()));
     HIPCHK(c, c->obs.alloc((size_t)S * kObs));
     HIPCHK(c, c->avg.alloc((size_t)S * kObs));
     HIPCHK(c, c->sysstate.alloc((size_t)S * 8));
@@ -568,7 +571,7 @@ size_t device_bytes(const ibx_ctx *c) {
     size_t b = c->sys.bytes() + c->wg_items.bytes() + c->itype.bytes() + c->label.bytes() + c->excl_off.bytes() +
                c->excl.bytes() + c->vs_of.bytes() + c->vs_center.bytes() + c->vs_moff.bytes() + c->vs_members.bytes() +
                c->term_meta.bytes() + c->inc_off.bytes() + c->wg_terms.bytes() + c->nb_map.bytes() + c->flags.bytes() +
-               c->step.bytes() + c->dmax.bytes() + c->term_slot.bytes() + c->term_mem.bytes() + c->contrib.bytes() +
+               c->step.bytes() + c->dmax.bytes() + c->fin_done.bytes() + c->term_slot.bytes() + c->term_mem.bytes() + c->contrib.bytes() +
                c->s.bytes() + c->v.bytes() + c->mass.bytes() + c->invmass.bytes() + c->vs_coeff.bytes() +
                c->vs_invmass.bytes() + c->part_bd.bytes() + c->part_mv.bytes() + c->fsum.bytes() + c->obs.bytes() +
                c->avg.bytes() + c->sysstate.bytes() + c->fnb.bytes() + c->ftot.bytes() + c->nb_rows.bytes() +
@@ -602,9 +605,10 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
                + 3 * (N + 1) * 4                                  // excl_off vs_moff inc_off
                + at_least1(h.excl.size()) * 4 + at_least1(h.vs_atoms.size()) * 4
                + nterm_pad * (4 + 16 + 16) + (nterm_pad / kBlock) * 4   // meta, members, slots, wg_terms
-               + at_least1(members) * 12                           // contributions
+               + at_least1(members) * 16                           // contributions
                + (nterm_pad / kBlock) * kBdPart * 8 + (N / kBlock) * kMvPart * 8
-               + (size_t)(kFields + 2 * kObs + 8) * 8 + 2 * 4 + 8 * 4 + 4   // fsum obs avg sysstate dmax flags step
+               + (size_t)(kFields + 2 * kObs + 8) * 8 + 2 * 4 + 8 * 4 + 4 + 4   // fsum obs avg sysstate dmax flags
+                                                                           // step fin_done
                + at_least1(bd_rows) * (16 + 8) + at_least1(h.bt.size() + h.at.size() + h.tt.size() + h.ot.size()) *
                                                      sizeof(BdTab)
                + at_least1(nb_rows) * (16 + 4) + at_least1(h.nt.size()) * sizeof(NbTab) +
@@ -736,8 +740,8 @@ void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev, float nb_we
     constexpr int per = kBlock / kNbAtomsPerBlock;
     const GroupDev &G1 = c->gr[1].present ? c->gr[1] : c->gr[0];
     launch_finalize(c->st, c->nsys, c->sys.p, c->gr[0].part.p, G1.part.p, c->part_bd.p, c->part_mv.p,
-                    per * c->gr[0].nwg, per * G1.nwg, c->term_nwg, c->item_nwg, c->fsum.p, c->obs.p, c->avg.p,
-                    c->sysstate.p, c->step.p, mode);
+                    per * c->gr[0].nwg, per * G1.nwg, c->term_nwg, c->item_nwg, c->fsum.p, c->fin_done.p, c->obs.p,
+                    c->avg.p, c->sysstate.p, c->step.p, mode);
 }
 
 int check_flags(ibx_ctx *c) {

@@ -421,53 +421,38 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
     const int gshift = (threadIdx.x & (kWave - 1)) & ~(kListLanes - 1);   // this group's bits in a wave ballot
     const unsigned below = (1u << sub) - 1u;
     int n_in = 0, n_out = 0, r = 0;
-    // two candidates per lane and round: both position loads are in flight together; hits are
-    // written in candidate order, as one candidate per round would
-    for (int t0 = 0; t0 < tmax; t0 += 2 * kListLanes) {
-        int jj[2] = {-1, -1};
-        int4 pjj[2] = {make_int4(0, 0, 0, 0), make_int4(0, 0, 0, 0)};
+    for (int t0 = 0; t0 < tmax; t0 += kListLanes) {
+        const int p = t0 + sub;
+        bool hit = false, inner = false;
+        int j = -1;
+        if (p < T) {
+            while (p >= rend[grp][r]) ++r;
+            j = rbase[grp][r] + p;
+            const int4 pj = pos4[j];
+            float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
+            float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
+            float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
+            if (!kCubic) { dx *= rho0; dy *= rho1; dz *= rho2; }
+            const float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            bool cand = r2u < lo;
+            if (r2u >= lo && r2u <= hi) cand = exact_r2(ri, r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2;
+            bool skip = (j == i) || (hybrid && li == 3 && pos_label(pj.w) == 3);
 #pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            const int p = t0 + u * kListLanes + sub;
-            if (p < T) {
-                while (p >= rend[grp][r]) ++r;
-                jj[u] = rbase[grp][r] + p;
-            }
+            for (int e = 0; e < 8; ++e) skip |= (ex[e] == j);
+            if (cand && !skip && ex1 - ex0 > 8)
+                for (int e = ex0 + 8; e < ex1; ++e) skip |= (inv_perm[excl[e]] == j);
+            hit = cand && !skip;
+            inner = r2u < rin2u;
         }
-#pragma unroll
-        for (int u = 0; u < 2; ++u)
-            if (jj[u] >= 0) pjj[u] = pos4[jj[u]];
-#pragma unroll
-        for (int u = 0; u < 2; ++u) {
-            bool hit = false, inner = false;
-            const int j = jj[u];
-            if (j >= 0) {
-                const int4 pj = pjj[u];
-                float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
-                float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
-                float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
-                if (!kCubic) { dx *= rho0; dy *= rho1; dz *= rho2; }
-                const float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
-                bool cand = r2u < lo;
-                if (r2u >= lo && r2u <= hi) cand = exact_r2(ri, r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2;
-                bool skip = (j == i) || (hybrid && li == 3 && pos_label(pj.w) == 3);
-#pragma unroll
-                for (int e = 0; e < 8; ++e) skip |= (ex[e] == j);
-                if (cand && !skip && ex1 - ex0 > 8)
-                    for (int e = ex0 + 8; e < ex1; ++e) skip |= (inv_perm[excl[e]] == j);
-                hit = cand && !skip;
-                inner = r2u < rin2u;
-            }
-            const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
-            const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
-            if (hit) {
-                const unsigned k = inner ? (unsigned)n_in + __popc(gi & below) : (unsigned)(cap - 1 - n_out - __popc(go & below));
-                const bool fits = inner ? (int)k < cap : n_out + __popc(go & below) < cap;
-                if (fits) list[list_pos(i, (int)k, cap)] = j;
-            }
-            n_in += __popc(gi);
-            n_out += __popc(go);
+        const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
+        const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
+        if (hit) {
+            const unsigned k = inner ? (unsigned)n_in + __popc(gi & below) : (unsigned)(cap - 1 - n_out - __popc(go & below));
+            const bool fits = inner ? (int)k < cap : n_out + __popc(go & below) < cap;
+            if (fits) list[list_pos(i, (int)k, cap)] = j;
         }
+        n_in += __popc(gi);
+        n_out += __popc(go);
     }
     if (valid && sub == 0) {
         nneb[i] = make_int2(n_in, n_out);
@@ -949,7 +934,7 @@ __device__ __forceinline__ void oop_term(const BdTab &t, const float4 *rows, con
 // lose precision to the size of the unwrapped coordinates.  Component order as
 // the reference: 11 22 33, then F_y s_x, F_z s_x, F_z s_y.
 __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys, TermArgs A, const double *__restrict__ s,
-                                                  int nitems_total, int nterms, float *__restrict__ contrib,
+                                                  int nitems_total, int nterms, float4 *__restrict__ contrib,
                                                   int ncontrib, double *__restrict__ part, int *__restrict__ flags) {
     __shared__ double red[19 * 4];
     if (!A.sys[wg_sys[blockIdx.x]].live) return;
@@ -989,9 +974,7 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             if (r >= nm) break;
-            contrib[sl[r]] = (float)fm[r][0];
-            contrib[ncontrib + sl[r]] = (float)fm[r][1];
-            contrib[2 * ncontrib + sl[r]] = (float)fm[r][2];
+            contrib[sl[r]] = make_float4((float)fm[r][0], (float)fm[r][1], (float)fm[r][2], 0.f);
         }
         const double s0x = s[mem[0]], s0y = s[n + mem[0]], s0z = s[2 * n + mem[0]];
 #pragma unroll
@@ -1024,7 +1007,7 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
 struct MoveArgs {
     double *s, *v;
     const float *fnb;
-    const float *contrib;   // bonded contribution slots (SoA x, y, z); item i owns [inc_off[i], inc_off[i+1])
+    const float4 *contrib;  // bonded contribution slots {x, y, z, 0}; item i owns [inc_off[i], inc_off[i+1])
     const int *inc_off;
     int ncontrib;
     float *ftot;
@@ -1047,17 +1030,18 @@ struct MoveArgs {
 
 // Adds the contribution slots [a, b) of one item in slot order, loading
 // twelve slots at a time so their latencies overlap.
-__device__ __forceinline__ void gather_contrib(const float *__restrict__ contrib, int nc, int a, int b, double f[3]) {
+__device__ __forceinline__ void gather_contrib(const float4 *__restrict__ contrib, int nc, int a, int b, double f[3]) {
+    (void)nc;
     for (int q = a; q < b; q += 12) {
-        float c[3][12];
+        float4 c[12];
 #pragma unroll
-        for (int u = 0; u < 12; ++u)
+        for (int u = 0; u < 12; ++u) c[u] = (q + u < b) ? contrib[q + u] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
-            for (int k = 0; k < 3; ++k) c[k][u] = (q + u < b) ? contrib[k * nc + q + u] : 0.f;
-#pragma unroll
-        for (int u = 0; u < 12; ++u)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) f[k] += (double)c[k][u];
+        for (int u = 0; u < 12; ++u) {
+            f[0] += (double)c[u].x;
+            f[1] += (double)c[u].y;
+            f[2] += (double)c[u].z;
+        }
     }
 }
 
@@ -1204,49 +1188,13 @@ __device__ __forceinline__ double strided_sum(const double *__restrict__ p, int 
     return ((x[0] + x[1]) + (x[2] + x[3])) + ((x[4] + x[5]) + (x[6] + x[7]));
 }
 
-// Per-system sums of the workgroup partials: one workgroup per (system,
-// field), fields 0..8 nonbonded (both groups), 9..27 bonded, 28..48
-// integrate.  Each sum is taken in a fixed order, so results are deterministic.
-__global__ void __launch_bounds__(kBlock) k_field_sums(const DevSys *__restrict__ sys,
-                                                      const double *__restrict__ part_nb0,
-                                                      const double *__restrict__ part_nb1,
-                                                      const double *__restrict__ part_bd,
-                                                      const double *__restrict__ part_mv, int snb0, int snb1, int sbd,
-                                                      int smv, double *__restrict__ fsum) {
-    __shared__ double red[kBlock / kWave];
-    const int s_id = blockIdx.x / kFields, f = blockIdx.x % kFields;
-    const DevSys &S = sys[s_id];
-    const int t = threadIdx.x;
-    constexpr int per = kBlock / kNbAtomsPerBlock;   // nonbonded workgroups per 256-slot block
-    double x = 0.0;
-    if (f < 9) {
-        for (int g = 0; g < S.ngroup; ++g) {
-            const double *pp = (g == 0 ? part_nb0 + (size_t)f * snb0 : part_nb1 + (size_t)f * snb1);
-            x += strided_sum(pp, per * S.g[g].wg_off, per * (S.g[g].wg_off + S.g[g].nwg), t, kBlock);
-        }
-    } else if (f < 28) {
-        x = strided_sum(part_bd + (size_t)(f - 9) * sbd, S.term_wg_off, S.term_wg_off + S.term_nwg, t, kBlock);
-    } else {
-        x = strided_sum(part_mv + (size_t)(f - 28) * smv, S.item_wg_off, S.item_wg_off + S.item_nwg, t, kBlock);
-    }
-    x = wave_sum(x);
-    if ((t & (kWave - 1)) == 0) red[t / kWave] = x;
-    __syncthreads();
-    if (t == 0) fsum[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
-}
-
-// Observables of each system from its field sums (AVERAGE.f90:10-47 window
+// Observables of one system from its field sums (AVERAGE.f90:10-47 window
 // sums included), the temperatures the next MOVE uses, and the MOMENTUM
-// correction when the step is a HALT_DRIFT step.  One thread per system.
-__global__ void __launch_bounds__(kWave) k_finalize(const DevSys *__restrict__ sys, int nsys,
-                                                   const double *__restrict__ fsum, double *__restrict__ obs,
-                                                   double *__restrict__ avg, double *__restrict__ sysstate,
-                                                   const int *__restrict__ step, int mode) {
-    const int s_id = blockIdx.x * kWave + threadIdx.x;
-    if (s_id >= nsys) return;
-    const DevSys &S = sys[s_id];
-    if (!S.live) return;
-    const double *red = fsum + (size_t)s_id * kFields;
+// correction when the step is a HALT_DRIFT step.  red is read past the
+// caches: other workgroups of the launch wrote it.
+__device__ void finalize_system(const DevSys &S, int s_id, const volatile double *red, double *__restrict__ obs,
+                                double *__restrict__ avg, double *__restrict__ sysstate, const int *__restrict__ step,
+                                int mode) {
     // acc keeps the former layout: 0..8 nonbonded, 9..21 bonded energies and overextension,
     // 22..27 bonded virial, 28..39 kinetic tensor of each population, 40..42 momentum
     double acc[43];
@@ -1285,6 +1233,51 @@ __global__ void __launch_bounds__(kWave) k_finalize(const DevSys *__restrict__ s
     for (int k = 0; k < 26; ++k) a[k] += o[k];
     a[27] += o[27];
     a[28] += 1.0;
+}
+
+// Per-system sums of the workgroup partials: one workgroup per (system,
+// field), fields 0..8 nonbonded (both groups), 9..27 bonded, 28..48
+// integrate.  Each sum is taken in a fixed order, so results are deterministic.
+// The last workgroup of a system to finish (a per-system counter, reset by that
+// workgroup for the next launch) forms the system's observables.
+__global__ void __launch_bounds__(kBlock) k_field_sums(const DevSys *__restrict__ sys,
+                                                      const double *__restrict__ part_nb0,
+                                                      const double *__restrict__ part_nb1,
+                                                      const double *__restrict__ part_bd,
+                                                      const double *__restrict__ part_mv, int snb0, int snb1, int sbd,
+                                                      int smv, double *__restrict__ fsum, int *__restrict__ done,
+                                                      double *__restrict__ obs, double *__restrict__ avg,
+                                                      double *__restrict__ sysstate, const int *__restrict__ step,
+                                                      int mode) {
+    __shared__ double red[kBlock / kWave];
+    const int s_id = blockIdx.x / kFields, f = blockIdx.x % kFields;
+    const DevSys &S = sys[s_id];
+    const int t = threadIdx.x;
+    constexpr int per = kBlock / kNbAtomsPerBlock;   // nonbonded workgroups per 256-slot block
+    double x = 0.0;
+    if (f < 9) {
+        for (int g = 0; g < S.ngroup; ++g) {
+            const double *pp = (g == 0 ? part_nb0 + (size_t)f * snb0 : part_nb1 + (size_t)f * snb1);
+            x += strided_sum(pp, per * S.g[g].wg_off, per * (S.g[g].wg_off + S.g[g].nwg), t, kBlock);
+        }
+    } else if (f < 28) {
+        x = strided_sum(part_bd + (size_t)(f - 9) * sbd, S.term_wg_off, S.term_wg_off + S.term_nwg, t, kBlock);
+    } else {
+        x = strided_sum(part_mv + (size_t)(f - 28) * smv, S.item_wg_off, S.item_wg_off + S.item_nwg, t, kBlock);
+    }
+    x = wave_sum(x);
+    if ((t & (kWave - 1)) == 0) red[t / kWave] = x;
+    __syncthreads();
+    if (t == 0) {
+        fsum[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+        __threadfence();
+        // the last of the system's kFields workgroups turns the sums into observables
+        if (atomicAdd(&done[s_id], 1) == kFields - 1) {
+            done[s_id] = 0;
+            __threadfence();
+            if (S.live) finalize_system(S, s_id, fsum + (size_t)s_id * kFields, obs, avg, sysstate, step, mode);
+        }
+    }
 }
 
 // Radial distribution function histogram over the current list (group 0).
@@ -1494,7 +1487,7 @@ void launch_nonbonded(hipStream_t st, const NbArgs &a) {
 }
 
 void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &b, const double *s, int nitems_total,
-                  int nterms, float *contrib, int ncontrib, double *part, int *flags) {
+                  int nterms, float4 *contrib, int ncontrib, double *part, int *flags) {
     TermArgs A{b.sys, b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs};
     hipLaunchKernelGGL(k_terms, dim3(nwg), dim3(kBlock), 0, st, wg_sys, A, s, nitems_total, nterms, contrib, ncontrib,
                        part, flags);
@@ -1522,11 +1515,9 @@ void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_s
 
 void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,
                      const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *fsum,
-                     double *obs, double *avg, double *sysstate, const int *step, int mode) {
+                     int *done, double *obs, double *avg, double *sysstate, const int *step, int mode) {
     hipLaunchKernelGGL(k_field_sums, dim3(nsys * kFields), dim3(kBlock), 0, st, sys, p0, p1, pbd, pmv, snb0, snb1, sbd,
-                       smv, fsum);
-    hipLaunchKernelGGL(k_finalize, dim3((nsys + kWave - 1) / kWave), dim3(kWave), 0, st, sys, nsys, fsum, obs, avg,
-                       sysstate, step, mode);
+                       smv, fsum, done, obs, avg, sysstate, step, mode);
 }
 
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,

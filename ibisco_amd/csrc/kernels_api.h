@@ -60,7 +60,7 @@ struct TermLaunch {
 struct MoveLaunch {
     double *s, *v;
     const float *fnb;
-    const float *contrib;
+    const float4 *contrib;
     const int *inc_off;
     int ncontrib;
     float *ftot;
@@ -88,7 +88,7 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a);
 void launch_pack_only(hipStream_t st, const RebuildArgs &a);
 void launch_nonbonded(hipStream_t st, const NbArgs &a);
 void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &b, const double *s, int nitems_total,
-                  int nterms, float *contrib, int ncontrib, double *part, int *flags);
+                  int nterms, float4 *contrib, int ncontrib, double *part, int *flags);
 void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const MoveLaunch &m);
 void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, double *s, int nitems_total,
                      const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,
@@ -96,7 +96,7 @@ void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_s
                      const int *inv_perm, int *dmax);
 void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,
                      const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *fsum,
-                     double *obs, double *avg, double *sysstate, const int *step, int mode);
+                     int *done, double *obs, double *avg, double *sysstate, const int *step, int mode);
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
                 const int *list, const int2 *nneb, int cap, int nbin, float inv_dr, unsigned long long *hist,
                 int *samples);

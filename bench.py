@@ -231,7 +231,7 @@ def hybrid_run(steps, warmup, seed=7):
     print(f"[bench] hybrid: {sy.natoms} atoms+beads, {sy.nvs} sites, box {sy.box[0]:.2f} nm", file=sys.stderr,
           flush=True)
     e = Engine(sy)
-    workloads.equilibrate(e, sy, stage_steps=100, final_steps=400)
+    workloads.equilibrate(e, sy, stage_steps=200, final_steps=2000)
     e.step(warmup, graph=True)
     e.obs()
     t0 = time.perf_counter()

@@ -525,7 +525,9 @@ __device__ __forceinline__ void nb_pair(const NbCtx &C, const int4 pi, int j, co
     double ddx = (double)ix, ddy = (double)iy, ddz = (double)iz;
     if (!kCubic) { ddx *= C.rhod[0]; ddy *= C.rhod[1]; ddz *= C.rhod[2]; }
     const double r2d = fma(ddx, ddx, fma(ddy, ddy, ddz * ddz));
-    const float rs = __builtin_amdgcn_rsqf(fmaxf(r2, 1.0f));   // r2 >= 1 for any two distinct positions
+    // r2 >= 1 for any two distinct positions; a coincident pair gets r = 0 (zero force, as the
+    // reference's F/r table times a zero displacement).  Integer max of the bits of a non-negative float.
+    const float rs = __builtin_amdgcn_rsqf(__int_as_float(max(__float_as_int(r2), 0x3f800000)));
     const double r0 = (double)(r2 * rs);
     const double rd = fma(fma(-r0, r0, r2d), (double)(0.5f * rs), r0);
     const double tt = rd * tb.c;
@@ -1190,9 +1192,9 @@ __device__ __forceinline__ double strided_sum(const double *__restrict__ p, int 
 
 // Observables of one system from its field sums (AVERAGE.f90:10-47 window
 // sums included), the temperatures the next MOVE uses, and the MOMENTUM
-// correction when the step is a HALT_DRIFT step.  red is read past the
-// caches: other workgroups of the launch wrote it.
-__device__ void finalize_system(const DevSys &S, int s_id, const volatile double *red, double *__restrict__ obs,
+// correction when the step is a HALT_DRIFT step.  red was written by other
+// workgroups of the launch: the caller's device-scope fence comes first.
+__device__ void finalize_system(const DevSys &S, int s_id, const double *red, double *__restrict__ obs,
                                 double *__restrict__ avg, double *__restrict__ sysstate, const int *__restrict__ step,
                                 int mode) {
     // acc keeps the former layout: 0..8 nonbonded, 9..21 bonded energies and overextension,

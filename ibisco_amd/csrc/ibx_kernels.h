@@ -17,9 +17,9 @@ constexpr int kXcd = 8;            // accelerator complex dies of an MI355X, eac
 constexpr int kLdsRows = 3072;     // float4 rows of nonbonded table staged in LDS (48 KiB)
 constexpr int kNbPart = 16;        // doubles per workgroup partial (nonbonded)
 constexpr int kBdPart = 24;        // doubles per workgroup partial (bonded: energies, overextension, virial)
-constexpr int kMvPart = 24;        // doubles per workgroup partial (integrate)
+constexpr int kMvPart = 16;        // doubles per workgroup partial (integrate: 15 fields)
 constexpr int kObs = 32;
-constexpr int kFields = 49;        // reduced fields per system: 9 nonbonded, 19 bonded, 21 integrate
+constexpr int kFields = 43;        // reduced fields per system: 9 nonbonded, 19 bonded, 15 integrate
 constexpr int kMaxNbTabs = 64;     // nonbonded tables per system
 constexpr int kMaxNbTypes = 32;    // particle types per system with several nonbonded tables
 enum TermKind { KBOND = 0, KANGLE = 1, KTORS = 2, KOOP = 3 };

@@ -1048,7 +1048,9 @@ __device__ __forceinline__ void gather_contrib(const float4 *__restrict__ contri
 }
 
 __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, MoveArgs A) {
-    __shared__ double red[21 * 4];
+    // partials: kinetic tensor (xx yy zz xy xz yz) of the atom population, of the bead population, momentum
+    constexpr int kNf = 15;
+    __shared__ double red[kNf * 4];
     __shared__ float mx[kBlock / kWave];
     float disp[2] = {0.f, 0.f};   // squared displacement since the rebuild, per group
     const int s_id = wg_sys[blockIdx.x];
@@ -1057,8 +1059,8 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     const int n = A.nitems_total;
     if (A.mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.step, 1);
     if (!S.live) return;
-    double acc[21];
-    for (int k = 0; k < 21; ++k) acc[k] = 0.0;
+    double acc[kNf];
+    for (int k = 0; k < kNf; ++k) acc[k] = 0.0;
     // every per-item load up front (the arrays are padded to whole blocks), so
     // they are in flight together rather than one dependent round trip each
     const int ty = A.itype[it];
@@ -1099,9 +1101,9 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
                 sp[k] += S.dt * v1;
                 A.s[k * n + it] = sp[k];
                 vt[k] = 0.5 * (v1 + v0);
-                acc[18 + k] = m * v1;
+                acc[12 + k] = m * v1;
             }
-            double *kk = acc + 6 + 6 * pop;
+            double *kk = acc + 6 * pop;
             kk[0] = m * vt[0] * vt[0];
             kk[1] = m * vt[1] * vt[1];
             kk[2] = m * vt[2] * vt[2];
@@ -1126,10 +1128,10 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
         block_max_to(disp[0], A.dmax + s_id, mx);
         if (S.hybrid) block_max_to(disp[1], A.dmax + A.nsys + s_id, mx);
     }
-    block_sum<21>(acc, red);
+    block_sum<kNf>(acc, red);
     if (threadIdx.x == 0) {
         double *p = A.part + blockIdx.x;   // field-major
-        for (int k = 0; k < 21; ++k) p[(size_t)k * gridDim.x] = acc[k];
+        for (int k = 0; k < kNf; ++k) p[(size_t)k * gridDim.x] = acc[k];
     }
 }
 
@@ -1197,11 +1199,10 @@ __device__ __forceinline__ double strided_sum(const double *__restrict__ p, int 
 __device__ void finalize_system(const DevSys &S, int s_id, const double *red, double *__restrict__ obs,
                                 double *__restrict__ avg, double *__restrict__ sysstate, const int *__restrict__ step,
                                 int mode) {
-    // acc keeps the former layout: 0..8 nonbonded, 9..21 bonded energies and overextension,
-    // 22..27 bonded virial, 28..39 kinetic tensor of each population, 40..42 momentum
-    double acc[43];
-    for (int k = 0; k < 28; ++k) acc[k] = red[k];
-    for (int k = 0; k < 15; ++k) acc[28 + k] = red[34 + k];
+    // the field sums: 0..8 nonbonded, 9..21 bonded energies and overextension, 22..27 bonded
+    // virial, 28..39 kinetic tensor of each population, 40..42 momentum
+    double acc[kFields];
+    for (int k = 0; k < kFields; ++k) acc[k] = red[k];
     const double mom[3] = {acc[40], acc[41], acc[42]};
     double *o = obs + (size_t)s_id * kObs;
     double *st = sysstate + (size_t)s_id * 8;
@@ -1238,7 +1239,7 @@ __device__ void finalize_system(const DevSys &S, int s_id, const double *red, do
 }
 
 // Per-system sums of the workgroup partials: one workgroup per (system,
-// field), fields 0..8 nonbonded (both groups), 9..27 bonded, 28..48
+// field), fields 0..8 nonbonded (both groups), 9..27 bonded, 28..42
 // integrate.  Each sum is taken in a fixed order, so results are deterministic.
 // The last workgroup of a system to finish (a per-system counter, reset by that
 // workgroup for the next launch) forms the system's observables.

@@ -292,19 +292,23 @@ def maxwell_velocities(rng, mass_kgmol, temp_k):
 def polymer_melt(nchains, nbeads, density, mass_kgmol, temp_k, seed, *, sigma, eps, rc, dr_nb,
                  bond_r0, bond_k, angle_theta0, angle_k, bond_rmax=None, bond_dr=None,
                  torsion=False, label="B", nb_table=None, dt=0.005, rlist=None, nupdate=10,
-                 nstep=1000, non_bonded=4, name="melt", avoid=None):
+                 nstep=1000, non_bonded=4, name="melt", avoid=None, box=None):
     """Linear homopolymer melt of one particle type in a periodic cube.
 
     ``density`` is in particles per nm^3.  ``avoid`` (a fraction of sigma) grows
     the chains with a minimum distance between non-bonded beads; without it the
-    walks overlap freely.  Bonds and angles are harmonic
+    walks overlap freely.  ``box`` (three edges, nm) replaces the cube of the
+    density's volume by an orthorhombic box.  Bonds and angles are harmonic
     (V = k/2 (x-x0)^2, k in kJ/mol/nm^2 and kJ/mol/rad^2), the nonbonded
     table is shifted-force LJ unless ``nb_table`` (r, V) is given.
     """
     rng = np.random.default_rng(seed)
     n = nchains * nbeads
-    L = (n / density) ** (1.0 / 3.0)
-    box = np.array([L, L, L])
+    if box is None:
+        L = (n / density) ** (1.0 / 3.0)
+        box = np.array([L, L, L])
+    else:
+        box = np.asarray(box, dtype=np.float64)
     if avoid:
         s = _grow_chains_avoiding(rng, nchains, nbeads, box, bond_r0, angle_theta0, avoid * sigma)
     else:

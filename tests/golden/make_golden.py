@@ -11,7 +11,8 @@ coordinates and velocities of the first three atoms that the reference's
 testsuite compares against (testsuite/tests.py:47-68).  comb_oop is a
 generated comb polymer whose branch atoms carry out-of-plane terms, and
 comb_oop_only the same with every other table zero (out-of-plane forces alone);
-hybridPA_mts runs hybridPA through MTS_LOOP's force (MTS_FORCE, NMTS 2) for 12 steps.
+hybridPA_mts runs hybridPA through MTS_LOOP's force (MTS_FORCE, NMTS 2) for 12 steps;
+cg_ortho is a melt in an orthorhombic box.
 
 Inputs are stored in single precision, which is what the reference's REAL*4
 readers see, so the oracle and the reference start from identical numbers.
@@ -159,6 +160,11 @@ def testsuite_expected():
     return out
 
 
+def ortho_case():
+    """A CG melt in an orthorhombic box (8.0 x 7.5 x 6.25 nm, 8 beads/nm^3): 4 x 4 x 3 reference cells."""
+    return system.cg_melt(nchains=60, nbeads=50, seed=13, avoid=0.8, box=(8.0, 7.5, 6.25))
+
+
 def mts_case():
     """hybridPA stepped by MTS_LOOP's force (NMTS 2: explicit steps 1-3 of every 5, extrapolated 4 and 5,
     step 5 with the MOD = 0 backward extrapolation), NVE, list rebuilt every 5 steps."""
@@ -198,6 +204,8 @@ def main(only=()):
         for name in only:
             if name == "comb_oop":
                 make_case("comb_oop", system.comb_melt(nchains=36, seed=1), runs=(10,))
+            if name == "cg_ortho":
+                make_case("cg_ortho", ortho_case(), runs=(10,))
             if name == "hybridPA_mts":
                 make_case("hybridPA_mts", mts_case(), runs=(12,))
             if name == "comb_oop_only":
@@ -221,6 +229,7 @@ def main(only=()):
     make_case("comb_oop", system.comb_melt(nchains=36, seed=1), runs=(10,))
     make_case("comb_oop_only", system.comb_melt(nchains=36, seed=1, oop_only=True), runs=(10,))
     make_case("hybridPA_mts", mts_case(), runs=(12,))
+    make_case("cg_ortho", ortho_case(), runs=(10,))
 
 
 if __name__ == "__main__":

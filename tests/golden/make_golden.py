@@ -162,7 +162,7 @@ def testsuite_expected():
 
 def ortho_case():
     """A CG melt in an orthorhombic box (8.0 x 7.5 x 6.25 nm, 8 beads/nm^3): 4 x 4 x 3 reference cells."""
-    return system.cg_melt(nchains=60, nbeads=50, seed=13, avoid=0.8, box=(8.0, 7.5, 6.25))
+    return system.cg_melt(nchains=60, nbeads=50, seed=13, avoid=0.8, dt=0.002, box=(8.0, 7.5, 6.25))
 
 
 def mts_case():

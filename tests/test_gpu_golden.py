@@ -2,7 +2,8 @@
 
 The fixtures hold the reference's own testsuite inputs (hybridPA with 960
 virtual sites and two neighbour groups, atomisticPA with torsions and NVT,
-twoatombox) and a torsion melt with ``non_bonded 5``.  The oracle is pinned to
+twoatombox), a torsion melt with ``non_bonded 5``, comb polymers with
+out-of-plane terms and a melt in an orthorhombic box.  The oracle is pinned to
 the reference's outputs on these inputs in test_golden.py; here the device path
 must match the oracle to the same gates as test_gpu_parity.py.  Forces do not
 depend on the ensemble, so the NPT input (hybridPA, which the device path does
@@ -31,7 +32,8 @@ def _load(name):
     return sysm
 
 
-@pytest.mark.parametrize("case", ["twoatombox", "ua2000_excl5", "hybridPA", "atomisticPA", "comb_oop", "comb_oop_only"])
+@pytest.mark.parametrize("case", ["twoatombox", "ua2000_excl5", "hybridPA", "atomisticPA", "comb_oop", "comb_oop_only",
+                                  "cg_ortho"])
 def test_golden_inputs_forces(case, oracle_mod):
     sysm = _load(case)
     print("case", case, flush=True)

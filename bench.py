@@ -65,7 +65,7 @@ def cpu_baseline(sysm, nsteps, threads):
 
 
 def replicas_compare(nrep, steps, warmup, seed=3):
-    """Config-3 replicas (log-uniform 2k-50k beads, each with its own corrected table): atom-steps/s of all
+    """CG melt replicas (log-uniform 2k-50k beads, each with its own corrected table): atom-steps/s of all
     replicas in one context (every kernel launch covers all of them, HIP graph per list interval) against
     the plain path, each replica alone in its own context, one after the other, from the same equilibrated
     states.  RDFs accumulate on the device at every list rebuild in the batched run."""
@@ -73,7 +73,7 @@ def replicas_compare(nrep, steps, warmup, seed=3):
     from ibisco_amd import units, workloads
     from ibisco_amd.engine import Engine
 
-    systems = workloads.config3(nrep=nrep, seed=seed)
+    systems = workloads.cg_replicas(nrep=nrep, seed=seed)
     beads = [s.natoms for s in systems]
     total = sum(beads)
     print(f"[bench] {nrep} replicas, {total} beads (min {min(beads)}, max {max(beads)})", file=sys.stderr, flush=True)
@@ -112,7 +112,7 @@ def replicas_compare(nrep, steps, warmup, seed=3):
 
 
 def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=3, nsample=12, ncheck=4, step_band=1.5):
-    """Config-4 serving: Poisson arrivals of CG melt jobs (1k-100k beads, 500-5000 steps, each with its own
+    """Serving: Poisson arrivals of CG melt jobs (1k-100k beads, 500-5000 steps, each with its own
     corrected table) submitted to the serving loop at their arrival times; every job returns its RDF and mean
     pressure.  Job states are equilibrated (batched, untimed) before the stream starts.  A sample of the jobs
     is then run alone, one context each (the plain path), for the throughput ratio, and the ``ncheck``
@@ -220,7 +220,7 @@ def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=3, nsamp
 
 
 def hybrid_run(steps, warmup, seed=7):
-    """Config 2: 500,000 atoms' worth of chains of 60 (30 % atomistic, one centre-of-mass site per 6 atoms;
+    """Hybrid melt: 500,000 atoms' worth of chains of 60 (30 % atomistic, one centre-of-mass site per 6 atoms;
     70 % coarse-grained, 10 beads each), three LJ tables with drawn sigma/epsilon, rc 1.4 nm, 1 fs."""
     from ibisco_amd import system, workloads
     from ibisco_amd.engine import Engine
@@ -282,7 +282,7 @@ def main():
     ap.add_argument("--eager", action="store_true",
                     help="launch kernel by kernel instead of through HIP graphs (profiling runs)")
     ap.add_argument("--replicas", type=int, default=256, metavar="NREP",
-                    help="also compare NREP config-3 replicas batched in one context against one context each "
+                    help="also compare NREP CG melt replicas batched in one context against one context each "
                          "(0 skips it)")
     ap.add_argument("--replica-steps", type=int, default=200)
     ap.add_argument("--serve-cohorts", type=int, default=3, help="batches the serving loop runs at once")
@@ -308,13 +308,13 @@ def main():
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); replicas and jobs shard across GPUs "
-              "without communication, see DESIGN.md", file=sys.stderr)
+              "without communication (one serving loop or batch per GPU)", file=sys.stderr)
         sys.exit(2)
 
     from ibisco_amd import workloads
     from ibisco_amd.engine import Engine
 
-    sysm = workloads.config1(nchains=args.nchains, nbeads=args.nbeads)
+    sysm = workloads.ua_melt_100k(nchains=args.nchains, nbeads=args.nbeads)
     n = sysm.natoms
     print(f"[bench] {n} sites, box {sysm.box[0]:.3f} nm", file=sys.stderr, flush=True)
     e = Engine(sysm)
@@ -330,7 +330,7 @@ def main():
     value = n * args.steps / secs
     print(f"[bench] {args.steps} steps in {secs:.3f} s, T = {ob['temp'] * 300:.1f} K", file=sys.stderr, flush=True)
 
-    # config 1's multiple time step: bonded terms every 1 fs step, nonbonded every 4 (r-RESPA impulse form)
+    # multiple time step: bonded terms every 1 fs step, nonbonded every 4 (r-RESPA impulse form)
     e.set_respa(4)
     e.step(40, graph=not args.eager)
     e.obs()

@@ -496,7 +496,7 @@ def ua_melt(nchains=400, nbeads=250, seed=1, temp_k=450.0, **kw):
 
 
 def corrected_cg_table(seed, rc=1.5, dr=0.002):
-    """Config-0 LJ plus a seeded sum of 8 Gaussians (|A|<=0.5 kJ/mol, w in 0.05-0.2 nm)."""
+    """The CG melt's LJ (0.47 nm, 3.5 kJ/mol) plus a seeded sum of 8 Gaussians (|A|<=0.5 kJ/mol, w in 0.05-0.2 nm)."""
     r, v = lj_sf_table(0.47, 3.5, rc, dr)
     rng = np.random.default_rng(10_000 + seed)
     return r, v + gaussian_correction(r, rng)

@@ -1,11 +1,10 @@
 """The benchmark workloads as synthetic systems, and their equilibration.
 
-config0  CG melt, 200 chains x 50 beads (10,000), 8 beads/nm^3, 500 K, LJ(0.47, 3.5) to 1.5 nm
-config1  UA melt, 400 x 250 CH2 (100,000), 0.78 g/cm^3, 450 K, LJ(0.401, 0.39) to 1.0 nm, RB torsion
-config3  replicas: CG melts of 2,000-50,000 beads (log-uniform), chain length 20-100, each with
-         its own Gaussian-corrected table
-config4  serving: a Poisson stream of such jobs (sizes 1,000-100,000, 500-5,000 steps)
-         at 20 jobs/s
+cg_melt_10k    CG melt, 200 chains x 50 beads (10,000), 8 beads/nm^3, 500 K, LJ(0.47, 3.5) to 1.5 nm
+ua_melt_100k   UA melt, 400 x 250 CH2 (100,000), 0.78 g/cm^3, 450 K, LJ(0.401, 0.39) to 1.0 nm, RB torsion
+cg_replicas    CG melts of 2,000-50,000 beads (log-uniform), chain length 20-100, each with its own
+               Gaussian-corrected table
+serving_jobs   a Poisson stream of such jobs (sizes 1,000-100,000, 500-5,000 steps) at 20 jobs/s
 
 Chains are random walks (fixed bond length, bond angle near theta0) that may
 overlap; :func:`equilibrate` removes the overlaps on the device with a capped
@@ -21,17 +20,17 @@ import numpy as np
 from . import system, units
 
 
-def config0(seed=0, nchains=200, nbeads=50):
+def cg_melt_10k(seed=0, nchains=200, nbeads=50):
     return system.cg_melt(nchains=nchains, nbeads=nbeads, seed=seed, temp_k=500.0, dt=0.005, nupdate=10)
 
 
-def config1(seed=1, nchains=400, nbeads=250, rlist=1.3, nupdate=10):
+def ua_melt_100k(seed=1, nchains=400, nbeads=250, rlist=1.3, nupdate=10):
     return system.ua_melt(nchains=nchains, nbeads=nbeads, seed=seed, temp_k=450.0, dt=0.001, rlist=rlist,
                           nupdate=nupdate)
 
 
-def config3(nrep=256, seed=3, rlist=1.8, nupdate=10, dt=0.005, scale=1.0):
-    """Replicas of config3; ``scale`` shrinks every replica (for tests)."""
+def cg_replicas(nrep=256, seed=3, rlist=1.8, nupdate=10, dt=0.005, scale=1.0):
+    """nrep CG melt replicas of log-uniform size, each with its own table; ``scale`` shrinks every replica."""
     out = []
     for k, (nch, c) in enumerate(system.replica_sizes(nrep, seed)):
         nch = max(2, int(round(nch * scale)))

@@ -1,6 +1,6 @@
 """HIP path against the FP64 oracle on identical inputs (needs a gfx950 device).
 
-Tolerances (north-star gates): pair sets and exclusions bit-exact; per-item
+Tolerances: pair sets and exclusions bit-exact; per-item
 forces within 1e-5 relative RMS; energies and virial within 1e-7 relative to
 the magnitude of their totals.
 """

@@ -51,3 +51,23 @@ def test_fortran_driver_matches_python_binding(oracle_mod):
         oo = o.obs()
         eo = sum(float(oo[k].sum()) for k in ("vnb", "vbond", "vangle", "vtors", "voop")) * CONV
         assert abs(rows[0, 2] - eo) <= 1e-6 * abs(eo)
+
+
+def test_fortran_driver_runs_the_mts_cycle():
+    """hybridPA with `mts 2` in its control file: the driver's blocks end on the steps that take the
+    reference's backward extrapolation (MOD(step, 5) = 0); their nonbonded energy must match the
+    reference's own MTS_FORCE run of the same input (fixture hybridPA_mts)."""
+    z = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "hybridPA_mts.npz"))
+    sysm = system.from_arrays({k[4:]: z[k] for k in z.files if k.startswith("sys_")})
+    ref = z["run12_obs"]
+    with tempfile.TemporaryDirectory() as d:
+        system.write_dir(sysm, d)
+        print("driver on", d, flush=True)
+        rows = _run_driver(d, 10, 5)
+    assert rows.shape == (2, 9) and list(rows[:, 0]) == [5.0, 10.0]
+    for k, step in enumerate((5, 10)):
+        vnb_ref = ref[step - 1, 0:3].sum() * CONV
+        epot_ref = ref[step - 1, 0:15].sum() * CONV
+        print("step", step, "V_NB driver", rows[k, 5], "reference", vnb_ref, flush=True)
+        assert abs(rows[k, 5] - vnb_ref) <= 1e-4 * abs(epot_ref)
+        assert abs(rows[k, 2] - epot_ref) <= 1e-4 * abs(epot_ref)

@@ -64,7 +64,8 @@ struct GroupDev {
         cell_sys;
     DBuf<double4> r64;
     DBuf<int4> pos4, pos4_ref;
-    DBuf<double> part;
+    DBuf<double> part;   // two buffers of [kNbPart fields][workgroups] nonbonded partials
+    double *part_buf(int par) { return part.p + (size_t)par * (part.n / 2); }
 };
 
 struct Timing {
@@ -80,8 +81,9 @@ struct ibx_ctx {
     std::vector<int> item_off, slot_off[2];
     int n_items = 0, item_nwg = 0, nupdate = 1;
     hipStream_t st = nullptr;
-    hipStream_t st2 = nullptr;               // bonded terms run here, beside the nonbonded kernel
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+    hipStream_t st2 = nullptr;               // bonded terms and field sums run here, beside the nonbonded kernel
+    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fin = nullptr;
+    int nb_par = 0;                          // nonbonded partial buffer of the latest evaluation
     GroupDev gr[2];
     DBuf<DevSys> sys;
     DBuf<int> wg_items, itype, label, excl_off, excl, vs_of, vs_center, vs_moff, vs_members;
@@ -113,6 +115,7 @@ struct ibx_ctx {
     // graph
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
+    int graph_par = 0;               // nb_par when the graph was captured (and after each replay)
     // timing
     bool timing = false;
     Timing tm;
@@ -124,6 +127,7 @@ struct ibx_ctx {
         for (auto e : evpool) (void)hipEventDestroy(e);
         if (ev_fork) (void)hipEventDestroy(ev_fork);
         if (ev_join) (void)hipEventDestroy(ev_join);
+        if (ev_fin) (void)hipEventDestroy(ev_fin);
         if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -159,6 +163,9 @@ int block_period(const ibx_ctx *c) {
     };
     int p = lcm(c->nupdate, c->respa);
     if (c->mts) p = lcm(p, 3 + c->hs[0].nmts);
+    // the nonbonded partials alternate between two buffers at each evaluation,
+    // so a block holds an even number of evaluations
+    if ((p / c->respa) % 2) p *= 2;
     return p;
 }
 
@@ -556,7 +563,7 @@ int build_device(ibx_ctx *c) {
         HIPCHK(c, G.pos4.alloc(slot));
         HIPCHK(c, G.pos4_ref.alloc(slot));
         HIPCHK(c, G.nneb.alloc(slot));
-        HIPCHK(c, G.part.alloc((size_t)G.nwg * (kBlock / kNbAtomsPerBlock) * kNbPart));
+        HIPCHK(c, G.part.alloc(2 * (size_t)G.nwg * (kBlock / kNbAtomsPerBlock) * kNbPart));
         HIPCHK(c, memset_st(c, G.part.p, 0, G.part.bytes()));
         HIPCHK(c, G.list.alloc((size_t)slot * G.cap));
         std::vector<int> inv(N, -1);
@@ -627,7 +634,7 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
         const size_t cap = (size_t)pad_to((int)(std::max(1.5, h.list_margin) * nbr) + 64, 8);
         b += slots * (4 * 5 + 32 + 16 + 16 + 8)                  // mem mem_cell tmp perm slot_cell r64 pos4 pos4_ref nneb
              + N * 4 + nwg * 4 + ncell * 16                       // inv_perm, wg_sys, cell arrays
-             + nwg * (kBlock / kNbAtomsPerBlock) * kNbPart * 8 + slots * cap * 4;
+             + 2 * nwg * (kBlock / kNbAtomsPerBlock) * kNbPart * 8 + slots * cap * 4;
     }
     return b;
 }
@@ -668,7 +675,11 @@ void enqueue_rdf(ibx_ctx *c) {
 // mode 0: the force phase of step `step`; with the MTS cycle (hybrid, NMTS > 0,
 // MTS_LOOP.f90:71-224) the bead group is evaluated and saved on explicit steps and
 // extrapolated on the others.  mode 1: an explicit evaluation outside the step loop.
+//
+// Each evaluation writes the other of the two partial buffers, so the field
+// sums of the previous step (second stream) can still read theirs meanwhile.
 void enqueue_forces(ibx_ctx *c, int mode, int step = 0) {
+    c->nb_par ^= 1;
     const bool mts = mode == 0 && c->mts;
     const int nmts = c->hs[0].nmts;
     const int mod = mts ? step % (3 + nmts) : 1;
@@ -679,7 +690,7 @@ void enqueue_forces(ibx_ctx *c, int mode, int step = 0) {
         if (g == 1 && mts && (mod < 1 || mod > 3)) {
             // MTS_APPROX: I = MOD(STEP, 3+NMTS), which is 0 on the last step of a cycle in the reference
             const double I = (mod == 0 && !c->hs[0].mts_quirk) ? (double)(3 + nmts) : (double)mod;
-            launch_mts(c->st, false, c->fnb.p, c->label.p, c->n_items, G.part.p, npart, c->mts_f.p,
+            launch_mts(c->st, false, c->fnb.p, c->label.p, c->n_items, G.part_buf(c->nb_par), npart, c->mts_f.p,
                        c->mts_part.p, 0, I);
             continue;
         }
@@ -687,13 +698,13 @@ void enqueue_forces(ibx_ctx *c, int mode, int step = 0) {
         a.sys = c->sys.p; a.wg_sys = G.wg_sys.p; a.g = g; a.nwg = G.nwg; a.cap = G.cap; a.nitems_total = c->n_items;
         a.max_rows_lds = G.max_rows; a.multi = G.multi; a.perm = G.perm.p; a.pos4 = G.pos4.p; a.list = G.list.p;
         a.nneb = G.nneb.p; a.rows = c->nb_rows.p; a.xoff = c->nb_xoff.p; a.tabs = c->nb_tabs.p; a.map = c->nb_map.p;
-        a.fnb = c->fnb.p; a.part = G.part.p; a.s = c->s.p; a.dmax = c->dmax.p; a.nsys = c->nsys;
+        a.fnb = c->fnb.p; a.part = G.part_buf(c->nb_par); a.s = c->s.p; a.dmax = c->dmax.p; a.nsys = c->nsys;
         a.cubic = true;
         for (const DevSys &D : c->dsys) a.cubic = a.cubic && D.rho[1] == 1.0f && D.rho[2] == 1.0f;
         launch_nonbonded(c->st, a);
         if (g == 1 && mts)   // MTS_SAVEFORCE, MTS_SAVEAUXS into slot MOD - 1
-            launch_mts(c->st, true, c->fnb.p, c->label.p, c->n_items, G.part.p, npart, c->mts_f.p, c->mts_part.p,
-                       mod - 1, 0.0);
+            launch_mts(c->st, true, c->fnb.p, c->label.p, c->n_items, G.part_buf(c->nb_par), npart, c->mts_f.p,
+                       c->mts_part.p, mod - 1, 0.0);
     }
 }
 
@@ -715,8 +726,19 @@ void fork_terms(ibx_ctx *c) {
 
 void join_terms(ibx_ctx *c) { (void)hipStreamWaitEvent(c->st, c->ev_join, 0); }
 
+// Everything queued on the second stream so far finishes before what is queued
+// on the first from now on (the end of a run of steps, of a captured block).
+void join_side(ibx_ctx *c) {
+    (void)hipEventRecord(c->ev_join, c->st2);
+    (void)hipStreamWaitEvent(c->st, c->ev_join, 0);
+}
+
 // Integration and reductions after the forces (the bonded terms must be enqueued before).
-void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev, float nb_weight = 1.0f) {
+// side: the field sums and observables go to the second stream, where they run
+// beside the next step's list rebuild and nonbonded kernel (which read neither
+// the partials they sum nor the system state they write); the next step's
+// integrator waits for them through join_terms.
+void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev, float nb_weight = 1.0f, bool side = false) {
     MoveLaunch m;
     m.nb_weight = mode == 1 ? 1.0f : nb_weight;
     m.s = c->s.p; m.v = c->v.p; m.fnb = c->fnb.p; m.contrib = c->contrib.p; m.inc_off = c->inc_off.p;
@@ -738,8 +760,15 @@ void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev, float nb_we
                         c->vs_center.p, c->vs_moff.p, c->vs_members.p, c->mass.p, c->vs_invmass.p, c->gr[1].pos4.p,
                         c->gr[1].pos4_ref.p, c->gr[1].inv_perm.p, c->dmax.p + c->nsys);
     constexpr int per = kBlock / kNbAtomsPerBlock;
-    const GroupDev &G1 = c->gr[1].present ? c->gr[1] : c->gr[0];
-    launch_finalize(c->st, c->nsys, c->sys.p, c->gr[0].part.p, G1.part.p, c->part_bd.p, c->part_mv.p,
+    GroupDev &G1 = c->gr[1].present ? c->gr[1] : c->gr[0];
+    hipStream_t fst = c->st;
+    if (side) {
+        (void)hipEventRecord(c->ev_fin, c->st);
+        (void)hipStreamWaitEvent(c->st2, c->ev_fin, 0);
+        fst = c->st2;
+    }
+    launch_finalize(fst, c->nsys, c->sys.p, c->gr[0].part_buf(c->nb_par), G1.part_buf(c->nb_par), c->part_bd.p,
+                    c->part_mv.p,
                     per * c->gr[0].nwg, per * G1.nwg, c->term_nwg, c->item_nwg, c->fsum.p, c->fin_done.p, c->obs.p,
                     c->avg.p, c->sysstate.p, c->step.p, mode);
 }
@@ -796,7 +825,7 @@ void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
     } else {
         join_terms(c);
     }
-    enqueue_rest(c, 0, ev, nb_step ? (float)c->respa : 0.0f);
+    enqueue_rest(c, 0, ev, nb_step ? (float)c->respa : 0.0f, !ev);
     if (ev) (void)hipEventRecord((*ev)[5], c->st);
 }
 
@@ -860,7 +889,8 @@ int ibx::create_context(std::vector<HostSystem> &&hs, ibx_ctx **out) {
     if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c->ev_fin, hipEventDisableTiming) != hipSuccess)
         return fail(nullptr, IBX_E_DEVICE, "stream creation failed");
     lk.unlock();
     int rc = build_device(c.get());
@@ -939,6 +969,7 @@ int ibx_step(ibx_ctx *c, int nsteps, int step0) {
         if (c->timing) e.assign(c->evpool.begin() + per * k, c->evpool.begin() + per * (k + 1));
         enqueue_step(c, step0 + 1 + k, c->timing ? &e : nullptr);
     }
+    join_side(c);
     int rc = check_flags(c);
     if (c->timing) {
         for (int k = 0; k < nsteps; ++k) {
@@ -969,7 +1000,8 @@ int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
     int done = head;
     int nblk = (nsteps - done) / period;
     if (nblk > 0) {
-        if (!c->gexec || c->graph_steps != period) {
+        // a block starts by reading the partial buffer the capture found latest
+        if (!c->gexec || c->graph_steps != period || c->graph_par != c->nb_par) {
             drop_graph(c);
             std::lock_guard<std::mutex> lk(g_capture_mu);
             hipGraph_t graph;
@@ -977,11 +1009,13 @@ int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
             // while this one captures; nothing captured here is a call that capture forbids
             HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
             for (int k = 0; k < period; ++k) enqueue_step(c, period + k, nullptr);   // block starts at a rebuild
+            join_side(c);
             HIPCHK(c, hipStreamEndCapture(c->st, &graph));
             hipError_t e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (e != hipSuccess) return fail(c, IBX_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
             c->graph_steps = period;
+            c->graph_par = c->nb_par;
         }
         HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
         int cur = step0 + done;

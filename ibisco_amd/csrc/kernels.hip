@@ -47,8 +47,71 @@ __device__ __forceinline__ double exact_r2(const double4 a, const double4 b, con
     return exact_r2(a, b, S.box[0], S.box[1], S.box[2], S.boxinv[0], S.boxinv[1], S.boxinv[2]);
 }
 
+// Compiler-only barrier: loads above it stay above it.  Without it the
+// compiler sinks a load below the next branch, into the block that uses it,
+// and the kernel waits for it there instead of while it checks other things.
+// The pointers a load reads through are passed in (they are uniform), so that
+// restrict-qualified arguments do not exempt their loads.
+__device__ __forceinline__ void issue_here() { asm volatile("" ::: "memory"); }
+__device__ __forceinline__ void issue_here(const void *a) { asm volatile("" ::"s"(a) : "memory"); }
+__device__ __forceinline__ void issue_here(const void *a, const void *b, const void *c) {
+    asm volatile("" ::"s"(a), "s"(b), "s"(c) : "memory");
+}
+
+// Cross-lane moves inside each row of 16 lanes through DPP (VALU operand
+// modifiers, no LDS traffic): the two quad permutations exchange lanes i and
+// i^1, i and i^2; the mirrors take lane 7-i of each half row and 15-i of each
+// row.  After the first two, each quad holds one value, so the mirrors pair
+// every lane with the other quad (half row) and the other half row, as i^4 and
+// i^8 would.  Every lane of the wave must be active.
+constexpr int kDppXor1 = 0xb1;         // quad_perm [1,0,3,2]
+constexpr int kDppXor2 = 0x4e;         // quad_perm [2,3,0,1]
+constexpr int kDppHalfMirror = 0x141;  // row_half_mirror
+constexpr int kDppMirror = 0x140;      // row_mirror
+
+template <int kCtrl>
+__device__ __forceinline__ float dpp_f(float x) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), kCtrl, 0xf, 0xf, false));
+}
+
+template <int kCtrl>
+__device__ __forceinline__ double dpp_d(double x) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)b, kCtrl, 0xf, 0xf, false);
+    const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp(0, (int)(unsigned)(b >> 32), kCtrl, 0xf, 0xf, false);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ float lane_f(float x, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), l));
+}
+
+__device__ __forceinline__ double lane_d(double x, int l) {
+    const unsigned long long b = (unsigned long long)__double_as_longlong(x);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)b, l);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), l);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// Sum over the 64 lanes of a wave in a fixed order (pairs, quads, half rows,
+// rows, then the four rows in order); every lane gets the same result.
 __device__ __forceinline__ double wave_sum(double x) {
-    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    x += dpp_d<kDppXor1>(x);
+    x += dpp_d<kDppXor2>(x);
+    x += dpp_d<kDppHalfMirror>(x);
+    x += dpp_d<kDppMirror>(x);
+    return (lane_d(x, 0) + lane_d(x, 16)) + (lane_d(x, 32) + lane_d(x, 48));
+}
+
+// Sum over aligned groups of kN lanes (kN = 1, 2, 4, 8 or 16), the same
+// butterfly order as i^1, i^2, i^4, i^8; every lane of a group gets the sum.
+template <int kN>
+__device__ __forceinline__ float group_sum(float x) {
+    static_assert(kN == 1 || kN == 2 || kN == 4 || kN == 8 || kN == 16, "group of 1 to 16 lanes");
+    if (kN > 1) x += dpp_f<kDppXor1>(x);
+    if (kN > 2) x += dpp_f<kDppXor2>(x);
+    if (kN > 4) x += dpp_f<kDppHalfMirror>(x);
+    if (kN > 8) x += dpp_f<kDppMirror>(x);
     return x;
 }
 
@@ -93,6 +156,7 @@ __device__ __forceinline__ void block_sum_waves(double (&v)[N], double *lds /* N
 }
 
 constexpr int kMultiTabsLds(bool multi) { return multi ? kMaxNbTabs : 1; }
+constexpr int kNbFields = 9;   // nonbonded partial fields: V_NB of three classes, virial xx yy zz xy xz yz
 
 // Workgroups are dispatched to the XCDs in turn, and each XCD has its own L2.
 // A kernel over n spatially ordered blocks is launched with xcd_grid(n)
@@ -127,7 +191,11 @@ __device__ __forceinline__ float fixed_d2(const int4 a, const int4 b, const DevS
 // Largest value over a block of kBlock threads, into *dst (float bits, atomicMax on
 // non-negative floats orders like the values).  Every thread must call it.
 __device__ __forceinline__ void block_max_to(float x, int *dst, float *lds /* kBlock / kWave */) {
-    for (int off = kWave / 2; off > 0; off >>= 1) x = fmaxf(x, __shfl_xor(x, off, kWave));
+    x = fmaxf(x, dpp_f<kDppXor1>(x));
+    x = fmaxf(x, dpp_f<kDppXor2>(x));
+    x = fmaxf(x, dpp_f<kDppHalfMirror>(x));
+    x = fmaxf(x, dpp_f<kDppMirror>(x));
+    x = fmaxf(fmaxf(lane_f(x, 0), lane_f(x, 16)), fmaxf(lane_f(x, 32), lane_f(x, 48)));
     if ((threadIdx.x & (kWave - 1)) == 0) lds[threadIdx.x / kWave] = x;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -616,14 +684,34 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
     extern __shared__ float4 lds_rows[];
     __shared__ NbTab tabs[kMultiTabsLds(kMulti)];
     __shared__ int map[kMulti ? kMaxNbTypes * kMaxNbTypes : 1];
-    __shared__ double red[9 * 4];
+    __shared__ double red[kNbFields * 16];
     const int blk = xcd_block(nblk);
     if (blk >= nblk) return;
-    const DevSys &S = sys[wg_sys[blk / (kBlock / kNbAtomsPerBlock)]];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int a = lane / kLanes, q = lane % kLanes;
+    const int i = blk * kNbAtomsPerBlock + (threadIdx.x / kWave) * kAtomsPerWave + a;
+    // the slot's own loads are in flight while the tables are staged
+    const int I = perm[i];
+    const int4 pi = pos4[i];
+    const int2 nn = nneb[i];
+    issue_here(perm, pos4, nneb);
+    const int sid = wg_sys[blk / (kBlock / kNbAtomsPerBlock)];
+    const DevSys &S = sys[sid];
     if (!S.live) return;
     const DevGroup &G = S.g[kGroup];
-    if (kLds)
-        for (int r = threadIdx.x; r < S.nb_nrow; r += kBlock) lds_rows[r] = nb_rows[S.nb_row_off + r];
+    if (kLds) {
+        // four rows per thread in flight at once (the last row is read again past the end)
+        constexpr int kU = 4;
+        const int nrow = S.nb_nrow;
+        for (int r0 = threadIdx.x; r0 < nrow; r0 += kU * kBlock) {
+            float4 v[kU];
+#pragma unroll
+            for (int u = 0; u < kU; ++u) v[u] = nb_rows[S.nb_row_off + min(r0 + u * kBlock, nrow - 1)];
+            issue_here(nb_rows);
+#pragma unroll
+            for (int u = 0; u < kU; ++u) lds_rows[min(r0 + u * kBlock, nrow - 1)] = v[u];   // same value twice at most
+        }
+    }
     for (int t = threadIdx.x; t < S.nb_ntab && t < kMultiTabsLds(kMulti); t += kBlock) {
         NbTab tb = nb_tabs[S.nb_tab_off + t];
         tb.row_off -= S.nb_row_off;
@@ -636,10 +724,6 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
         }
     __syncthreads();
 
-    const int lane = threadIdx.x & (kWave - 1);
-    const int a = lane / kLanes, q = lane % kLanes;
-    const int i = blk * kNbAtomsPerBlock + (threadIdx.x / kWave) * kAtomsPerWave + a;
-    const int I = perm[i];
     float fx = 0.f, fy = 0.f, fz = 0.f;
     float e[3] = {0.f, 0.f, 0.f};
     float w[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
@@ -664,13 +748,11 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
         C.n = nitems_total;
         C.I = I;
         C.g = kGroup;
-        const int4 pi = pos4[i];
         const int li = pos_label(pi.w);
-        const int2 nn = nneb[i];
         const int n_in = min(nn.x, cap);
         // the back run (rc + delta <= r < rlist at the rebuild) can only reach rc once some member of
         // the system has moved delta/2 since then
-        const bool far_ok = __int_as_float(dmax[kGroup * nsys + wg_sys[blk / (kBlock / kNbAtomsPerBlock)]]) < G.dmax2u;
+        const bool far_ok = __int_as_float(dmax[kGroup * nsys + sid]) < G.dmax2u;
         const int n_out = far_ok ? 0 : min(nn.y, cap - n_in);
         // front run: entry q + kLanes*m at lp[m*kWave]; back run: entry q + kLanes*m at lb[-m*kWave]
         const int *lp = list + list_pos(i, q, cap);
@@ -681,36 +763,49 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
         if (nm_out > 0) nb_run<kMulti, kGroup, kCubic, -kWave>(C, pos4, lb, nm_out, pi, li, fx, fy, fz, e, w);
     }
     // the kLanes lanes of a slot hold partial forces: butterfly-sum them
-    for (int off = 1; off < kLanes; off <<= 1) {
-        fx += __shfl_xor(fx, off, kWave);
-        fy += __shfl_xor(fy, off, kWave);
-        fz += __shfl_xor(fz, off, kWave);
-    }
+    fx = group_sum<kLanes>(fx);
+    fy = group_sum<kLanes>(fy);
+    fz = group_sum<kLanes>(fz);
     if (I >= 0 && q == 0) {
         const float u = (float)S.unit;
         fnb[I] = fx * u;
         fnb[nitems_total + I] = fy * u;
         fnb[2 * nitems_total + I] = fz * u;
     }
-    double acc[9] = {e[0], e[1], e[2], w[0], w[1], w[2], w[3], w[4], w[5]};
-    block_sum<9>(acc, red);
-    if (threadIdx.x == 0) {
+    // Block sums of the nine energy and virial accumulators through LDS (this
+    // kernel is VALU-bound, so no cross-lane arithmetic): the lane values go to
+    // the staging area, the table being no longer needed; sixteen threads per
+    // field add every sixteenth value, then one thread per field adds those
+    // sixteen sums, in FP64 and in a fixed order.
+    const float acc[kNbFields] = {e[0], e[1], e[2], w[0], w[1], w[2], w[3], w[4], w[5]};
+    float *acc_lds = reinterpret_cast<float *>(lds_rows);
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < kNbFields; ++k) acc_lds[k * kBlock + threadIdx.x] = acc[k];
+    __syncthreads();
+    if (threadIdx.x < kNbFields * 16) {
+        const float *pf = acc_lds + (threadIdx.x >> 4) * kBlock + (threadIdx.x & 15);
+        double x = 0.0;
+#pragma unroll
+        for (int u = 0; u < kBlock / 16; ++u) x += (double)pf[16 * u];
+        red[threadIdx.x] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < kNbFields) {
+        double x = 0.0;
+        for (int u = 0; u < 16; ++u) x += red[threadIdx.x * 16 + u];
         // field-major partials: field k of block w at part[k * nblk + w]
-        double *p = part + blk;
-        const size_t fs = nblk;
         const double u2 = S.unit * S.unit;
-        for (int k = 0; k < 3; ++k) p[k * fs] = 0.5 * acc[k];
-        for (int k = 3; k < 9; ++k) p[k * fs] = 0.5 * acc[k] * u2;
+        part[(size_t)threadIdx.x * nblk + blk] = threadIdx.x < 3 ? 0.5 * x : 0.5 * x * u2;
     }
 }
 
 // ------------------------------------------------------------------------------------------------
 // bonded forces (replaces BONDED_FORCE.F90 with BONDS/ANGLES/TORSIONS/OUTOFPLANES.inc)
 //
-// Gather form: each item evaluates every term it takes part in and keeps its
-// own share, so no atomics are needed and the sums are deterministic.  The
-// energy of a term is counted by its first member.  Geometry differences are
-// taken in FP64 from the unwrapped coordinates, the rest in FP32.
+// One thread per term, in FP64 from the unwrapped coordinates; each member's
+// force goes to a contribution slot of its own, which the integrator gathers
+// (no atomics, deterministic sums).
 
 struct TermArgs {
     const DevSys *sys;    // per-system constants (retired systems are skipped)
@@ -741,11 +836,12 @@ __device__ __forceinline__ void lerp_row(const float4 *rows, const BdTab &t, int
     V = (T)r.y * ((T)1 - alpha) + alpha * (T)r.w;
 }
 
+// Difference of two members' unwrapped positions (P holds the term's members).
 template <typename T>
-__device__ __forceinline__ void dvec(const double *s, int n, int a, int b, T d[3]) {
-    d[0] = (T)(s[a] - s[b]);
-    d[1] = (T)(s[n + a] - s[n + b]);
-    d[2] = (T)(s[2 * n + a] - s[2 * n + b]);
+__device__ __forceinline__ void dvec(const double (&P)[4][3], int a, int b, T d[3]) {
+    d[0] = (T)(P[a][0] - P[b][0]);
+    d[1] = (T)(P[a][1] - P[b][1]);
+    d[2] = (T)(P[a][2] - P[b][2]);
 }
 
 template <typename T>
@@ -760,10 +856,10 @@ __device__ __forceinline__ void cross(const T a[3], const T b[3], T c[3]) {
 
 // BONDS.inc:8-56.  Returns the force on member `role` and the energy.
 template <typename T>
-__device__ __forceinline__ void bond_term(const BdTab &t, const float4 *rows, const double *xs, const int4 b,
-                                          const double *s, int n, T fm[4][3], T &V, bool &over) {
+__device__ __forceinline__ void bond_term(const BdTab &t, const float4 *rows, const double *xs,
+                                          const double (&P)[4][3], T fm[4][3], T &V, bool &over) {
     T d[3];
-    dvec(s, n, b.x, b.y, d);
+    dvec(P, 0, 1, d);
     const T rij = nrm(d);
     const T xr = (rij - (T)t.x0) * (T)t.inv_bin;
     over = xr >= (T)(t.ndat + 1);              // BONDS.inc:21-31: warn, clamp, continue
@@ -779,12 +875,12 @@ __device__ __forceinline__ void bond_term(const BdTab &t, const float4 *rows, co
 
 // ANGLES.inc:2-75
 template <typename T>
-__device__ __forceinline__ void angle_term(const BdTab &t, const float4 *rows, const double *xs, const int4 a4,
-                                           const double *s, int n, T fm[4][3], T &V) {
+__device__ __forceinline__ void angle_term(const BdTab &t, const float4 *rows, const double *xs,
+                                           const double (&P)[4][3], T fm[4][3], T &V) {
     const T R2D = (T)180 / (T)3.14159265358;
     T ij[3], kj[3], M[3], N[3], Sv[3];
-    dvec(s, n, a4.x, a4.y, ij);
-    dvec(s, n, a4.z, a4.y, kj);
+    dvec(P, 0, 1, ij);
+    dvec(P, 2, 1, kj);
     const T rij = nrm(ij), rkj = nrm(kj);
     T ct = (ij[0] * kj[0] + ij[1] * kj[1] + ij[2] * kj[2]) / rij / rkj;
     ct = ct > (T)1 ? (T)1 : (ct < (T)-1 ? (T)-1 : ct);
@@ -820,20 +916,20 @@ __device__ __forceinline__ void angle_term(const BdTab &t, const float4 *rows, c
 
 // TORSIONS.inc:1-152
 template <typename T>
-__device__ __forceinline__ void torsion_term(const BdTab &t, const float4 *rows, const double *xs, const int4 q4,
-                                             const double *s, int n, T fm[4][3], T &V, int *flags) {
+__device__ __forceinline__ void torsion_term(const BdTab &t, const float4 *rows, const double *xs,
+                                             const double (&P)[4][3], T fm[4][3], T &V, int *flags) {
     const T R2D = (T)180 / (T)3.14159265358;
     T ij[3], jk[3], kl[3], M[3], N[3];
-    dvec(s, n, q4.x, q4.y, ij);
-    dvec(s, n, q4.y, q4.z, jk);
-    dvec(s, n, q4.z, q4.w, kl);
+    dvec(P, 0, 1, ij);
+    dvec(P, 1, 2, jk);
+    dvec(P, 2, 3, kl);
     T rij = nrm(ij);
     const T rjk = nrm(jk);
     T rkl = nrm(kl);
     cross(ij, jk, M);
     T rm = nrm(M);
     if (rm == (T)0) {           // TORSIONS.inc:47-59: perturb x of atom i
-        ij[0] = (T)(s[q4.x] * 0.99 - s[q4.y]);
+        ij[0] = (T)(P[0][0] * 0.99 - P[1][0]);
         rij = nrm(ij);
         cross(ij, jk, M);
         rm = nrm(M);
@@ -845,7 +941,7 @@ __device__ __forceinline__ void torsion_term(const BdTab &t, const float4 *rows,
     cross(jk, kl, N);
     T rn = nrm(N);
     if (rn == (T)0) {           // TORSIONS.inc:75-87: perturb x of atom l
-        kl[0] = (T)(s[q4.z] - s[q4.w] * 0.99);
+        kl[0] = (T)(P[2][0] - P[3][0] * 0.99);
         rkl = nrm(kl);
         cross(jk, kl, N);
         rn = nrm(N);
@@ -881,13 +977,13 @@ __device__ __forceinline__ void torsion_term(const BdTab &t, const float4 *rows,
 
 // OUTOFPLANES.inc:1-141
 template <typename T>
-__device__ __forceinline__ void oop_term(const BdTab &t, const float4 *rows, const double *xs, const int4 q4,
-                                         const double *s, int n, T fm[4][3], T &V, int *flags) {
+__device__ __forceinline__ void oop_term(const BdTab &t, const float4 *rows, const double *xs,
+                                         const double (&P)[4][3], T fm[4][3], T &V, int *flags) {
     const T R2D = (T)180 / (T)3.14159265358;
     T ij[3], jk[3], kl[3], M[3], N[3], mN[3];
-    dvec(s, n, q4.x, q4.y, ij);
-    dvec(s, n, q4.z, q4.y, jk);
-    dvec(s, n, q4.z, q4.w, kl);
+    dvec(P, 0, 1, ij);
+    dvec(P, 2, 1, jk);
+    dvec(P, 2, 3, kl);
     const T rij = nrm(ij), rjk = nrm(jk), rkl = nrm(kl);
     cross(ij, jk, M);
     const T rm = nrm(M);
@@ -939,50 +1035,58 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
                                                   int nitems_total, int nterms, float4 *__restrict__ contrib,
                                                   int ncontrib, double *__restrict__ part, int *__restrict__ flags) {
     __shared__ double red[19 * 4];
-    if (!A.sys[wg_sys[blockIdx.x]].live) return;
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const int n = nitems_total;
+    // the term and its members' positions are loaded before the system record
+    // (the term arrays are padded to whole blocks), so those round trips overlap
+    const int meta = t < nterms ? A.meta[t] : -1;
+    const int4 m4 = A.mem[t];
+    const int4 s4 = A.slot[t];
+    const int mem[4] = {m4.x, m4.y, m4.z, m4.w};
+    double P[4][3];   // unused members (and padding terms) read item 0, never used
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        const int mr = (meta >= 0 && mem[r] >= 0) ? mem[r] : 0;
+        for (int k = 0; k < 3; ++k) P[r][k] = s[k * n + mr];
+    }
+    const BdTab tb = A.tabs[max(meta, 0) >> 2];
+    issue_here(s, A.tabs, A.mem);
+    if (!A.sys[wg_sys[blockIdx.x]].live) return;
     double en[19];
     for (int k = 0; k < 19; ++k) en[k] = 0.0;
-    const int meta = t < nterms ? A.meta[t] : -1;
     if (meta >= 0) {
         const int kind = meta & 3;
-        const BdTab tb = A.tabs[meta >> 2];
-        const int4 m4 = A.mem[t];
         double fm[4][3], V;
         int nm;
         if (kind == KBOND) {
             bool over;
-            bond_term<double>(tb, A.rows, A.xs, m4, s, n, fm, V, over);
+            bond_term<double>(tb, A.rows, A.xs, P, fm, V, over);
             if (over) en[12] += 1.0;
             nm = 2;
         } else if (kind == KANGLE) {
-            angle_term<double>(tb, A.rows, A.xs, m4, s, n, fm, V);
+            angle_term<double>(tb, A.rows, A.xs, P, fm, V);
             nm = 3;
         } else if (kind == KTORS) {
-            torsion_term<double>(tb, A.rows, A.xs, m4, s, n, fm, V, flags);
+            torsion_term<double>(tb, A.rows, A.xs, P, fm, V, flags);
             nm = 4;
         } else {
-            oop_term<double>(tb, A.rows, A.xs, m4, s, n, fm, V, flags);
+            oop_term<double>(tb, A.rows, A.xs, P, fm, V, flags);
             nm = 4;
         }
         // static indices only, so en[] and fm[][] stay in registers
         const int ek = 3 * kind + tb.label - 1;
 #pragma unroll
         for (int k = 0; k < 12; ++k) en[k] += (k == ek) ? V : 0.0;
-        const int mem[4] = {m4.x, m4.y, m4.z, m4.w};
-        const int4 s4 = A.slot[t];
         const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             if (r >= nm) break;
             contrib[sl[r]] = make_float4((float)fm[r][0], (float)fm[r][1], (float)fm[r][2], 0.f);
         }
-        const double s0x = s[mem[0]], s0y = s[n + mem[0]], s0z = s[2 * n + mem[0]];
 #pragma unroll
         for (int r = 1; r < 4; ++r) {
             if (r >= nm) break;
-            const double dx = s[mem[r]] - s0x, dy = s[n + mem[r]] - s0y, dz = s[2 * n + mem[r]] - s0z;
+            const double dx = P[r][0] - P[0][0], dy = P[r][1] - P[0][1], dz = P[r][2] - P[0][2];
             en[13] += fm[r][0] * dx;
             en[14] += fm[r][1] * dy;
             en[15] += fm[r][2] * dz;
@@ -1031,18 +1135,22 @@ struct MoveArgs {
 };
 
 // Adds the contribution slots [a, b) of one item in slot order, loading
-// twelve slots at a time so their latencies overlap.
+// twelve slots at a time so their latencies overlap.  The loads are
+// unconditional (slots past b read slot 0 and are dropped): a load under a
+// per-lane condition would make the compiler wait for it before the next one.
 __device__ __forceinline__ void gather_contrib(const float4 *__restrict__ contrib, int nc, int a, int b, double f[3]) {
     (void)nc;
     for (int q = a; q < b; q += 12) {
         float4 c[12];
 #pragma unroll
-        for (int u = 0; u < 12; ++u) c[u] = (q + u < b) ? contrib[q + u] : make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int u = 0; u < 12; ++u) c[u] = contrib[q + u < b ? q + u : 0];
 #pragma unroll
         for (int u = 0; u < 12; ++u) {
-            f[0] += (double)c[u].x;
-            f[1] += (double)c[u].y;
-            f[2] += (double)c[u].z;
+            if (q + u < b) {
+                f[0] += (double)c[u].x;
+                f[1] += (double)c[u].y;
+                f[2] += (double)c[u].z;
+            }
         }
     }
 }
@@ -1053,16 +1161,13 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     __shared__ double red[kNf * 4];
     __shared__ float mx[kBlock / kWave];
     float disp[2] = {0.f, 0.f};   // squared displacement since the rebuild, per group
-    const int s_id = wg_sys[blockIdx.x];
-    const DevSys &S = sys[s_id];
     const int it = blockIdx.x * kBlock + threadIdx.x;
     const int n = A.nitems_total;
-    if (A.mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.step, 1);
-    if (!S.live) return;
-    double acc[kNf];
-    for (int k = 0; k < kNf; ++k) acc[k] = 0.0;
-    // every per-item load up front (the arrays are padded to whole blocks), so
-    // they are in flight together rather than one dependent round trip each
+    // Every load that does not need the system record is issued before it (the
+    // arrays are padded to whole blocks): the item's own data, then what that
+    // data points at (its contribution slots, its virtual site's, the reference
+    // positions of its list slots), so the kernel waits for two round trips
+    // rather than one per dependent load.
     const int ty = A.itype[it];
     const int lab = A.label[it];
     const int q0 = A.inc_off[it], q1 = A.inc_off[it + 1];
@@ -1072,24 +1177,37 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     const double vin[3] = {A.v[it], A.v[n + it], A.v[2 * n + it]};
     const double m = A.mass[it], im = A.invmass[it];
     const int slot_g[2] = {A.inv_perm[0][it], A.inv_perm[1][it]};
+    const bool own = ty >= 0 && lab != 3;   // an atom or bead (virtual sites are distributed, not moved)
+    double fbv[3] = {0, 0, 0}, fvs[3] = {0, 0, 0}, fvs_nb[3] = {0, 0, 0};
+    gather_contrib(A.contrib, A.ncontrib, q0, own ? q1 : q0, fbv);
+    double vs_c = 0.0;
+    if (own && vs >= 0) {
+        // the virtual site's nonbonded and bonded forces, spread by mass fraction
+        vs_c = A.vs_coeff[it];
+        for (int k = 0; k < 3; ++k) fvs_nb[k] = (double)A.fnb[k * n + vs];
+        gather_contrib(A.contrib, A.ncontrib, A.inc_off[vs], A.inc_off[vs + 1], fvs);
+    }
+    int4 ref[2];
+    for (int g = 0; g < 2; ++g) ref[g] = A.pos4_ref[g][max(slot_g[g], 0)];   // used only where slot_g[g] >= 0
+    issue_here();
+    if (A.mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.step, 1);
+    const int s_id = wg_sys[blockIdx.x];
+    const DevSys &S = sys[s_id];
+    if (!S.live) return;
+    const double *st = A.sysstate + (size_t)s_id * 8;
+    double acc[kNf];
+    for (int k = 0; k < kNf; ++k) acc[k] = 0.0;
     if (A.mode == 1 && ty >= 0 && lab == 3)
         for (int k = 0; k < 3; ++k) A.ftot[k * n + it] = 0.0f;   // distributed (DISTRIBUTE_VSFORCE.f90:40)
-    if (ty >= 0 && lab != 3) {
-        double F[3], fbv[3] = {0, 0, 0};
-        gather_contrib(A.contrib, A.ncontrib, q0, q1, fbv);
+    if (own) {
+        double F[3];
         const double wnb = (double)A.nb_weight;
         for (int k = 0; k < 3; ++k) F[k] = (double)fn[k] * wnb + fbv[k];
-        if (vs >= 0) {
-            // the virtual site's nonbonded and bonded forces, spread by mass fraction
-            const double c = A.vs_coeff[it];
-            double fv[3] = {0, 0, 0};
-            gather_contrib(A.contrib, A.ncontrib, A.inc_off[vs], A.inc_off[vs + 1], fv);
-            for (int k = 0; k < 3; ++k) F[k] += ((double)A.fnb[k * n + vs] * wnb + fv[k]) * c;
-        }
+        if (vs >= 0)
+            for (int k = 0; k < 3; ++k) F[k] += (fvs_nb[k] * wnb + fvs[k]) * vs_c;
         if (A.mode == 1) {
             for (int k = 0; k < 3; ++k) A.ftot[k * n + it] = (float)F[k];
         } else {
-            const double *st = A.sysstate + (size_t)s_id * 8;
             const int pop = (lab == 2) ? 1 : 0;
             double lam = 1.0;
             if (S.ensemble == 1) lam = sqrt(1.0 + S.fac * ((S.temp_in / st[pop]) - 1.0));
@@ -1120,7 +1238,7 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
                 p.z = (int)to_fixed(sp[2], S.inv_box_frac[2]);
                 p.w = ty | (lab << 16);
                 A.pos4[g][slot] = p;
-                disp[g] = fixed_d2(p, A.pos4_ref[g][slot], S);
+                disp[g] = fixed_d2(p, ref[g], S);
             }
         }
     }
@@ -1474,7 +1592,8 @@ static void nb_cubic(hipStream_t st, const NbArgs &a, size_t lds) {
 template <int G>
 static void nb_dispatch(hipStream_t st, const NbArgs &a) {
     const bool lds = a.max_rows_lds > 0;
-    const size_t bytes = lds ? (size_t)a.max_rows_lds * sizeof(float4) : 0;
+    // the staging area also holds the block's energy/virial accumulators at the end
+    const size_t bytes = std::max(lds ? (size_t)a.max_rows_lds * sizeof(float4) : 0, (size_t)kNbFields * kBlock * sizeof(float));
     if (a.multi) {
         if (lds) nb_cubic<true, true, G>(st, a, bytes);
         else nb_cubic<true, false, G>(st, a, bytes);

@@ -190,10 +190,12 @@ hipError_t memset_st(ibx_ctx *c, void *p, int v, size_t n) {
     return e != hipSuccess ? e : hipStreamSynchronize(c->st);
 }
 
+// An empty h still allocates one element, zeroed, so kernels may read it.
 template <typename T>
 int upload(ibx_ctx *c, DBuf<T> &b, const std::vector<T> &h) {
     HIPCHK(c, b.alloc(h.size()));
     if (!h.empty()) HIPCHK(c, memcpy_st(c, b.p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    else HIPCHK(c, memset_st(c, b.p, 0, b.bytes()));
     return 0;
 }
 

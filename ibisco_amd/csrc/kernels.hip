@@ -181,11 +181,15 @@ __global__ void k_fill_int(int *p, int n, int v) {
 }
 
 // Squared displacement, in working units^2, between two fixed-point positions.
-__device__ __forceinline__ float fixed_d2(const int4 a, const int4 b, const DevSys &S) {
+__device__ __forceinline__ float fixed_d2(const int4 a, const int4 b, float rho1, float rho2) {
     const float dx = (float)(int)((uint32_t)a.x - (uint32_t)b.x);
-    const float dy = (float)(int)((uint32_t)a.y - (uint32_t)b.y) * S.rho[1];
-    const float dz = (float)(int)((uint32_t)a.z - (uint32_t)b.z) * S.rho[2];
+    const float dy = (float)(int)((uint32_t)a.y - (uint32_t)b.y) * rho1;
+    const float dz = (float)(int)((uint32_t)a.z - (uint32_t)b.z) * rho2;
     return fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+}
+
+__device__ __forceinline__ float fixed_d2(const int4 a, const int4 b, const DevSys &S) {
+    return fixed_d2(a, b, S.rho[1], S.rho[2]);
 }
 
 // Largest value over a block of kBlock threads, into *dst (float bits, atomicMax on
@@ -307,9 +311,11 @@ __global__ void __launch_bounds__(kBlock) k_pack(const DevSys *__restrict__ sys,
                                                  int nitems_total, const int *__restrict__ itype,
                                                  const int *__restrict__ label, double4 *__restrict__ r64,
                                                  int4 *__restrict__ pos4, int *__restrict__ inv_perm,
-                                                 int4 *__restrict__ pos4_ref) {
+                                                 int4 *__restrict__ pos4_ref, int *__restrict__ dmax, int nsys,
+                                                 int dmax_value) {
     const DevSys &S = sys[wg_sys[blockIdx.x]];
     int slot = blockIdx.x * kBlock + threadIdx.x;
+    if (slot < nsys) dmax[slot] = dmax_value;   // the group's displacement bound restarts (every system has a block)
     int it = perm[slot];
     if (it < 0) {
         pos4[slot] = make_int4(0, 0, 0, -1);
@@ -384,6 +390,24 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
     if (valid) {
         pi = pos4[i];
         ri = r64[i];
+    }
+    int ex0 = 0, ex1 = 0, li = 0;
+    int ex[8] = {-1, -1, -1, -1, -1, -1, -1, -1};   // slots of the first exclusions
+    if (valid) {
+        li = pos_label(pi.w);
+        ex0 = excl_off[I];
+        ex1 = excl_off[I + 1];
+        // unconditional loads, eight in flight at a time, while the candidate ranges are worked
+        // out: entries past the member's read entry 0 of the array (which is allocated but holds
+        // nothing when no item has exclusions), and what they read is replaced by item 0 before
+        // it indexes inv_perm
+        int xi[8];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xi[e] = excl[ex0 + e < ex1 ? ex0 + e : 0];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) xi[e] = inv_perm[ex0 + e < ex1 ? xi[e] : 0];
+#pragma unroll
+        for (int e = 0; e < 8; ++e) ex[e] = ex0 + e < ex1 ? xi[e] : -1;
     }
     int T = 0;   // candidates of this member
     if (G.all_pairs) {
@@ -475,15 +499,6 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
     }
     __syncthreads();
 
-    int ex0 = 0, ex1 = 0, li = 0;
-    int ex[8] = {-1, -1, -1, -1, -1, -1, -1, -1};   // slots of the first exclusions
-    if (valid) {
-        li = pos_label(pi.w);
-        ex0 = excl_off[I];
-        ex1 = excl_off[I + 1];
-#pragma unroll
-        for (int e = 0; e < 8; ++e) ex[e] = (ex0 + e < ex1) ? inv_perm[excl[ex0 + e]] : -1;
-    }
     int tmax = T;   // the wave walks as far as its longest member
     for (int off = kListLanes; off < kWave; off <<= 1) tmax = max(tmax, __shfl_xor(tmax, off, kWave));
     const int gshift = (threadIdx.x & (kWave - 1)) & ~(kListLanes - 1);   // this group's bits in a wave ballot
@@ -829,9 +844,10 @@ __device__ __forceinline__ int row_index(T x, int ndat, int *flag, int bit) {
     return min((int)x, ndat);
 }
 
+// Interpolation in one table row {F_k, V_k, F_k+1, V_k+1}.  The callers load
+// the row and its abscissa together, before either is used.
 template <typename T>
-__device__ __forceinline__ void lerp_row(const float4 *rows, const BdTab &t, int ni, T alpha, T &F, T &V) {
-    const float4 r = rows[t.row_off + ni];
+__device__ __forceinline__ void lerp_row(const float4 r, T alpha, T &F, T &V) {
     F = (T)r.x * ((T)1 - alpha) + alpha * (T)r.z;
     V = (T)r.y * ((T)1 - alpha) + alpha * (T)r.w;
 }
@@ -864,9 +880,10 @@ __device__ __forceinline__ void bond_term(const BdTab &t, const float4 *rows, co
     const T xr = (rij - (T)t.x0) * (T)t.inv_bin;
     over = xr >= (T)(t.ndat + 1);              // BONDS.inc:21-31: warn, clamp, continue
     const int ni = row_index(xr, t.ndat, (int *)nullptr, 0);
+    const float4 r = rows[t.row_off + ni];
     const T alpha = (rij - (T)xs[t.row_off + ni]) * (T)t.inv_bin;   // RBOND(TIJ,NI), the table's own abscissa
     T F;
-    lerp_row(rows, t, ni, alpha, F, V);
+    lerp_row(r, alpha, F, V);
     for (int k = 0; k < 3; ++k) {
         fm[0][k] = F * d[k];
         fm[1][k] = -F * d[k];
@@ -898,13 +915,14 @@ __device__ __forceinline__ void angle_term(const BdTab &t, const float4 *rows, c
     Sv[2] = -(M[0] * kj[1] - M[1] * kj[0]) / rkj;
     const T xr = theta * (T)t.inv_bin;
     const int ni = row_index(xr, t.ndat, (int *)nullptr, 0);
+    const float4 r = rows[t.row_off + ni];
+    const T x = (T)xs[t.row_off + ni];
     T F;
     if (theta == (T)180) {
-        const float4 r = rows[t.row_off + ni];
         F = (T)r.x;
         V = (T)r.y;
     } else {
-        lerp_row(rows, t, ni, (theta - (T)xs[t.row_off + ni]) * (T)t.inv_bin, F, V);
+        lerp_row(r, (theta - x) * (T)t.inv_bin, F, V);
     }
     for (int k = 0; k < 3; ++k) {
         const T fa = F * N[k] / rij, fk = F * Sv[k] / rkj;
@@ -956,13 +974,14 @@ __device__ __forceinline__ void torsion_term(const BdTab &t, const float4 *rows,
     T phi = acos(cost) * R2D;
     if (signt < (T)0) phi = (T)360 - phi;
     const int ni = row_index(phi * (T)t.inv_bin, t.ndat, flags, 1);
+    const float4 r = rows[t.row_off + ni];
+    const T x = (T)xs[t.row_off + ni];
     T F;
     if (phi == (T)360) {
-        const float4 r = rows[t.row_off + ni];
         F = (T)r.x;
         V = (T)r.y;
     } else {
-        lerp_row(rows, t, ni, (phi - (T)xs[t.row_off + ni]) * (T)t.inv_bin, F, V);
+        lerp_row(r, (phi - x) * (T)t.inv_bin, F, V);
     }
     for (int k = 0; k < 3; ++k) {
         const T f1 = -F * M[k] / sinm / rij;
@@ -1001,13 +1020,15 @@ __device__ __forceinline__ void oop_term(const BdTab &t, const float4 *rows, con
     cost = cost > (T)1 ? (T)1 : (cost < (T)-1 ? (T)-1 : cost);
     const T phi = signt * acos(cost) * R2D;
     int ni = row_index(rint(((T)180 + phi) * (T)t.inv_bin), t.ndat, flags, 2);
+    float4 r = rows[t.row_off + ni];
     T alpha = fabs(fabs(phi) - fabs((T)xs[t.row_off + ni])) * (T)t.inv_bin;
     if (alpha > (T)1) {
         ni = min(ni + 1, t.ndat);
+        r = rows[t.row_off + ni];
         alpha = fabs(fabs(phi) - fabs((T)xs[t.row_off + ni])) * (T)t.inv_bin;
     }
     T F;
-    lerp_row(rows, t, ni, alpha, F, V);
+    lerp_row(r, alpha, F, V);
     for (int k = 0; k < 3; ++k) {
         const T f1 = F * M[k] / sinm / rij;
         const T f4 = -F * N[k] / sinn / rkl;
@@ -1035,6 +1056,7 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
                                                   int nitems_total, int nterms, float4 *__restrict__ contrib,
                                                   int ncontrib, double *__restrict__ part, int *__restrict__ flags) {
     __shared__ double red[19 * 4];
+    const int live = A.sys[wg_sys[blockIdx.x]].live;   // first, through the scalar cache
     const int t = blockIdx.x * kBlock + threadIdx.x;
     const int n = nitems_total;
     // the term and its members' positions are loaded before the system record
@@ -1051,7 +1073,7 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
     }
     const BdTab tb = A.tabs[max(meta, 0) >> 2];
     issue_here(s, A.tabs, A.mem);
-    if (!A.sys[wg_sys[blockIdx.x]].live) return;
+    if (!live) return;
     double en[19];
     for (int k = 0; k < 19; ++k) en[k] = 0.0;
     if (meta >= 0) {
@@ -1163,7 +1185,18 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     float disp[2] = {0.f, 0.f};   // squared displacement since the rebuild, per group
     const int it = blockIdx.x * kBlock + threadIdx.x;
     const int n = A.nitems_total;
-    // Every load that does not need the system record is issued before it (the
+    // The system's constants and state (temperatures of the two populations,
+    // pending momentum) are read first, before any store (so through the scalar
+    // cache, and not again after each store that might alias them).
+    const int s_id = wg_sys[blockIdx.x];
+    const DevSys &S = sys[s_id];
+    const int live = S.live, ensemble = S.ensemble, hybrid = S.hybrid;
+    const double fac = S.fac, temp_in = S.temp_in, dt = S.dt;
+    const double ibf[3] = {S.inv_box_frac[0], S.inv_box_frac[1], S.inv_box_frac[2]};
+    const float rho1 = S.rho[1], rho2 = S.rho[2];
+    const double *stp = A.sysstate + (size_t)s_id * 8;
+    const double st[5] = {stp[0], stp[1], stp[2], stp[3], stp[4]};
+    // Every per-item load is issued before the system record is checked (the
     // arrays are padded to whole blocks): the item's own data, then what that
     // data points at (its contribution slots, its virtual site's, the reference
     // positions of its list slots), so the kernel waits for two round trips
@@ -1191,10 +1224,7 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     for (int g = 0; g < 2; ++g) ref[g] = A.pos4_ref[g][max(slot_g[g], 0)];   // used only where slot_g[g] >= 0
     issue_here();
     if (A.mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.step, 1);
-    const int s_id = wg_sys[blockIdx.x];
-    const DevSys &S = sys[s_id];
-    if (!S.live) return;
-    const double *st = A.sysstate + (size_t)s_id * 8;
+    if (!live) return;
     double acc[kNf];
     for (int k = 0; k < kNf; ++k) acc[k] = 0.0;
     if (A.mode == 1 && ty >= 0 && lab == 3)
@@ -1210,13 +1240,13 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
         } else {
             const int pop = (lab == 2) ? 1 : 0;
             double lam = 1.0;
-            if (S.ensemble == 1) lam = sqrt(1.0 + S.fac * ((S.temp_in / st[pop]) - 1.0));
+            if (ensemble == 1) lam = sqrt(1.0 + fac * ((temp_in / (pop ? st[1] : st[0])) - 1.0));
             double vt[3];
             for (int k = 0; k < 3; ++k) {
                 const double v0 = vin[k] - st[2 + k] * im;   // deferred MOMENTUM
-                const double v1 = (v0 + S.dt * F[k] * im) * lam;
+                const double v1 = (v0 + dt * F[k] * im) * lam;
                 A.v[k * n + it] = v1;
-                sp[k] += S.dt * v1;
+                sp[k] += dt * v1;
                 A.s[k * n + it] = sp[k];
                 vt[k] = 0.5 * (v1 + v0);
                 acc[12 + k] = m * v1;
@@ -1229,22 +1259,22 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
             kk[4] = m * vt[0] * vt[2];
             kk[5] = m * vt[1] * vt[2];
             // fixed-point copy for the next nonbonded evaluation
-            const int g = (S.hybrid && lab == 2) ? 1 : 0;
+            const int g = (hybrid && lab == 2) ? 1 : 0;
             const int slot = slot_g[g];
             if (slot >= 0) {
                 int4 p;
-                p.x = (int)to_fixed(sp[0], S.inv_box_frac[0]);
-                p.y = (int)to_fixed(sp[1], S.inv_box_frac[1]);
-                p.z = (int)to_fixed(sp[2], S.inv_box_frac[2]);
+                p.x = (int)to_fixed(sp[0], ibf[0]);
+                p.y = (int)to_fixed(sp[1], ibf[1]);
+                p.z = (int)to_fixed(sp[2], ibf[2]);
                 p.w = ty | (lab << 16);
                 A.pos4[g][slot] = p;
-                disp[g] = fixed_d2(p, ref[g], S);
+                disp[g] = fixed_d2(p, ref[g], rho1, rho2);
             }
         }
     }
     if (A.mode == 0) {
         block_max_to(disp[0], A.dmax + s_id, mx);
-        if (S.hybrid) block_max_to(disp[1], A.dmax + A.nsys + s_id, mx);
+        if (hybrid) block_max_to(disp[1], A.dmax + A.nsys + s_id, mx);
     }
     block_sum<kNf>(acc, red);
     if (threadIdx.x == 0) {
@@ -1554,8 +1584,7 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
     hipLaunchKernelGGL(k_sort_cells, dim3(a.ncell_total), dim3(kBlock), 0, st, a.cell_sys, a.sys, a.g, a.cell_start,
                        a.cell_count, a.tmp, a.mem, a.perm, a.slot_cell, a.flags);
     hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
-                       a.itype, a.label, a.r64, a.pos4, a.inv_perm, a.pos4_ref);
-    hipLaunchKernelGGL(k_fill_int, dim3((a.nsys + 255) / 256), dim3(256), 0, st, a.dmax, a.nsys, 0);
+                       a.itype, a.label, a.r64, a.pos4, a.inv_perm, a.pos4_ref, a.dmax, a.nsys, 0);
     const int nwg_list = a.nwg * (kBlock / kListAtoms);
     if (a.cubic)
         hipLaunchKernelGGL(k_build_list<true>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
@@ -1571,8 +1600,7 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
 // the nonbonded kernel scans whole lists until the next rebuild.
 void launch_pack_only(hipStream_t st, const RebuildArgs &a) {
     hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
-                       a.itype, a.label, a.r64, a.pos4, a.inv_perm, (int4 *)nullptr);
-    hipLaunchKernelGGL(k_fill_int, dim3((a.nsys + 255) / 256), dim3(256), 0, st, a.dmax, a.nsys, 0x7f800000);
+                       a.itype, a.label, a.r64, a.pos4, a.inv_perm, (int4 *)nullptr, a.dmax, a.nsys, 0x7f800000);
 }
 
 template <bool M, bool L, int G, bool K>

@@ -646,29 +646,40 @@ __device__ __forceinline__ int4 ld_pos(const int4 *__restrict__ p, int j) {
 }
 
 // One run of a slot's list: entries q, q + kLanes, ... at l[0], l[kStep], ...
-// (kStep = +kWave for the front run, -kWave for the back run), unrolled by four
-// with the next four indices loaded a round ahead of their positions.
+// (kStep = +kWave for the front run, -kWave for the back run), kNbUnroll
+// entries a round, with the next round's indices loaded ahead of their
+// positions.  The pairs are added in entry order whatever the unroll.
+constexpr int kNbUnroll = 4;
+
 template <bool kMulti, int kGroup, bool kCubic, int kStep>
 __device__ __forceinline__ void nb_run(const NbCtx &C, const int4 *__restrict__ pos4, const int *__restrict__ l,
                                        int nm, const int4 pi, int li, float &fx, float &fy, float &fz,
                                        float (&e)[3], float (&w)[6]) {
+    constexpr int U = kNbUnroll;
     int m = 0;
-    int j0 = 0, j1 = 0, j2 = 0, j3 = 0;
-    if (nm >= 4) {
-        j0 = l[0]; j1 = l[kStep]; j2 = l[2 * kStep]; j3 = l[3 * kStep];
+    int jj[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) jj[u] = 0;
+    if (nm >= U) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = l[u * kStep];
     }
-    for (; m + 4 <= nm; m += 4) {
-        const int4 p0 = ld_pos(pos4, j0), p1 = ld_pos(pos4, j1), p2 = ld_pos(pos4, j2), p3 = ld_pos(pos4, j3);
-        l += 4 * kStep;
-        int k0 = j0, k1 = j1, k2 = j2, k3 = j3;
-        if (m + 8 <= nm) {
-            k0 = l[0]; k1 = l[kStep]; k2 = l[2 * kStep]; k3 = l[3 * kStep];
+    for (; m + U <= nm; m += U) {
+        int4 pp[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) pp[u] = ld_pos(pos4, jj[u]);
+        l += U * kStep;
+        int kk[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) kk[u] = jj[u];
+        if (m + 2 * U <= nm) {
+#pragma unroll
+            for (int u = 0; u < U; ++u) kk[u] = l[u * kStep];
         }
-        nb_pair<kMulti, kGroup, kCubic>(C, pi, j0, p0, li, fx, fy, fz, e, w);
-        nb_pair<kMulti, kGroup, kCubic>(C, pi, j1, p1, li, fx, fy, fz, e, w);
-        nb_pair<kMulti, kGroup, kCubic>(C, pi, j2, p2, li, fx, fy, fz, e, w);
-        nb_pair<kMulti, kGroup, kCubic>(C, pi, j3, p3, li, fx, fy, fz, e, w);
-        j0 = k0; j1 = k1; j2 = k2; j3 = k3;
+#pragma unroll
+        for (int u = 0; u < U; ++u) nb_pair<kMulti, kGroup, kCubic>(C, pi, jj[u], pp[u], li, fx, fy, fz, e, w);
+#pragma unroll
+        for (int u = 0; u < U; ++u) jj[u] = kk[u];
     }
     for (; m < nm; ++m, l += kStep) {
         const int j = l[0];

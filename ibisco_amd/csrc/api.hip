@@ -81,9 +81,9 @@ struct ibx_ctx {
     std::vector<int> item_off, slot_off[2];
     int n_items = 0, item_nwg = 0, nupdate = 1;
     hipStream_t st = nullptr;
-    hipStream_t st2 = nullptr;               // bonded terms and field sums run here, beside the nonbonded kernel
-    hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_fin = nullptr;
-    int nb_par = 0;                          // nonbonded partial buffer of the latest evaluation
+    int nb_par = 0, bd_par = 0;              // nonbonded / bonded partial buffer of the latest evaluation
+    FinLaunch fin{};                         // field sums left for the next force launch
+    bool fin_pending = false;
     GroupDev gr[2];
     DBuf<DevSys> sys;
     DBuf<int> wg_items, itype, label, excl_off, excl, vs_of, vs_center, vs_moff, vs_members;
@@ -95,6 +95,7 @@ struct ibx_ctx {
     DBuf<float4> contrib;           // bonded contribution slots {x, y, z, 0}
     int nterms = 0, term_nwg = 0, ncontrib = 0;
     DBuf<double> s, v, mass, invmass, vs_coeff, vs_invmass, part_bd, part_mv, fsum, obs, avg, sysstate;
+    double *part_bd_buf(int par) { return part_bd.p + (size_t)par * (part_bd.n / 2); }   // two buffers
     DBuf<float> fnb, ftot;
     DBuf<float4> nb_rows, bd_rows;
     DBuf<double> bd_x;
@@ -115,7 +116,7 @@ struct ibx_ctx {
     // graph
     hipGraphExec_t gexec = nullptr;
     int graph_steps = 0;
-    int graph_par = 0;               // nb_par when the graph was captured (and after each replay)
+    int graph_par = 0, graph_bd_par = 0;   // nb_par, bd_par when the graph was captured (and after each replay)
     // timing
     bool timing = false;
     Timing tm;
@@ -125,10 +126,6 @@ struct ibx_ctx {
         std::lock_guard<std::mutex> lk(g_capture_mu);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         for (auto e : evpool) (void)hipEventDestroy(e);
-        if (ev_fork) (void)hipEventDestroy(ev_fork);
-        if (ev_join) (void)hipEventDestroy(ev_join);
-        if (ev_fin) (void)hipEventDestroy(ev_fin);
-        if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
 };
@@ -453,7 +450,7 @@ int build_device(ibx_ctx *c) {
     HIPCHK(c, c->ftot.alloc(3 * (size_t)N));
     HIPCHK(c, memset_st(c, c->fnb.p, 0, c->fnb.bytes()));
     HIPCHK(c, memset_st(c, c->ftot.p, 0, c->ftot.bytes()));
-    HIPCHK(c, c->part_bd.alloc((size_t)c->term_nwg * kBdPart));
+    HIPCHK(c, c->part_bd.alloc(2 * (size_t)c->term_nwg * kBdPart));
     HIPCHK(c, c->part_mv.alloc((size_t)c->item_nwg * kMvPart));
     HIPCHK(c, memset_st(c, c->part_bd.p, 0, c->part_bd.bytes()));
     HIPCHK(c, memset_st(c, c->part_mv.p, 0, c->part_mv.bytes()));
@@ -615,7 +612,7 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
                + at_least1(h.excl.size()) * 4 + at_least1(h.vs_atoms.size()) * 4
                + nterm_pad * (4 + 16 + 16) + (nterm_pad / kBlock) * 4   // meta, members, slots, wg_terms
                + at_least1(members) * 16                           // contributions
-               + (nterm_pad / kBlock) * kBdPart * 8 + (N / kBlock) * kMvPart * 8
+               + 2 * (nterm_pad / kBlock) * kBdPart * 8 + (N / kBlock) * kMvPart * 8
                + (size_t)(kFields + 2 * kObs + 8) * 8 + 2 * 4 + 8 * 4 + 4 + 4   // fsum obs avg sysstate dmax flags
                                                                            // step fin_done
                + at_least1(bd_rows) * (16 + 8) + at_least1(h.bt.size() + h.at.size() + h.tt.size() + h.ot.size()) *
@@ -674,18 +671,28 @@ void enqueue_rdf(ibx_ctx *c) {
                (float)(1.0 / c->rdf_dr), c->rdf.p, c->rdf_samples.p);
 }
 
-// mode 0: the force phase of step `step`; with the MTS cycle (hybrid, NMTS > 0,
-// MTS_LOOP.f90:71-224) the bead group is evaluated and saved on explicit steps and
-// extrapolated on the others.  mode 1: an explicit evaluation outside the step loop.
-//
-// Each evaluation writes the other of the two partial buffers, so the field
-// sums of the previous step (second stream) can still read theirs meanwhile.
-void enqueue_forces(ibx_ctx *c, int mode, int step = 0) {
-    c->nb_par ^= 1;
+NbArgs nb_args(ibx_ctx *c, int g) {
+    GroupDev &G = c->gr[g];
+    NbArgs a;
+    a.sys = c->sys.p; a.wg_sys = G.wg_sys.p; a.g = g; a.nwg = G.nwg; a.cap = G.cap; a.nitems_total = c->n_items;
+    a.max_rows_lds = G.max_rows; a.multi = G.multi; a.perm = G.perm.p; a.pos4 = G.pos4.p; a.list = G.list.p;
+    a.nneb = G.nneb.p; a.rows = c->nb_rows.p; a.xoff = c->nb_xoff.p; a.tabs = c->nb_tabs.p; a.map = c->nb_map.p;
+    a.fnb = c->fnb.p; a.part = G.part_buf(c->nb_par); a.s = c->s.p; a.dmax = c->dmax.p; a.nsys = c->nsys;
+    a.cubic = true;
+    for (const DevSys &D : c->dsys) a.cubic = a.cubic && D.rho[1] == 1.0f && D.rho[2] == 1.0f;
+    return a;
+}
+
+// The nonbonded force of groups g0..1, into the partial buffer nb_par (the
+// caller has switched it).  mode 0: the force phase of step `step`; with the
+// MTS cycle (hybrid, NMTS > 0, MTS_LOOP.f90:71-224) the bead group is
+// evaluated and saved on explicit steps and extrapolated on the others.
+// mode 1: an explicit evaluation outside the step loop.
+void enqueue_nb_groups(ibx_ctx *c, int mode, int step, int g0) {
     const bool mts = mode == 0 && c->mts;
     const int nmts = c->hs[0].nmts;
     const int mod = mts ? step % (3 + nmts) : 1;
-    for (int g = 0; g < 2; ++g) {
+    for (int g = g0; g < 2; ++g) {
         GroupDev &G = c->gr[g];
         if (!G.present) continue;
         const int npart = G.nwg * (kBlock / kNbAtomsPerBlock);
@@ -696,51 +703,55 @@ void enqueue_forces(ibx_ctx *c, int mode, int step = 0) {
                        c->mts_part.p, 0, I);
             continue;
         }
-        NbArgs a;
-        a.sys = c->sys.p; a.wg_sys = G.wg_sys.p; a.g = g; a.nwg = G.nwg; a.cap = G.cap; a.nitems_total = c->n_items;
-        a.max_rows_lds = G.max_rows; a.multi = G.multi; a.perm = G.perm.p; a.pos4 = G.pos4.p; a.list = G.list.p;
-        a.nneb = G.nneb.p; a.rows = c->nb_rows.p; a.xoff = c->nb_xoff.p; a.tabs = c->nb_tabs.p; a.map = c->nb_map.p;
-        a.fnb = c->fnb.p; a.part = G.part_buf(c->nb_par); a.s = c->s.p; a.dmax = c->dmax.p; a.nsys = c->nsys;
-        a.cubic = true;
-        for (const DevSys &D : c->dsys) a.cubic = a.cubic && D.rho[1] == 1.0f && D.rho[2] == 1.0f;
-        launch_nonbonded(c->st, a);
+        launch_nonbonded(c->st, nb_args(c, g));
         if (g == 1 && mts)   // MTS_SAVEFORCE, MTS_SAVEAUXS into slot MOD - 1
             launch_mts(c->st, true, c->fnb.p, c->label.p, c->n_items, G.part_buf(c->nb_par), npart, c->mts_f.p,
                        c->mts_part.p, mod - 1, 0.0);
     }
 }
 
-void enqueue_terms(ibx_ctx *c, hipStream_t st) {
-    TermLaunch b{c->sys.p, c->term_mem.p, c->term_meta.p, c->term_slot.p, c->bd_tabs.p, c->bd_rows.p, c->bd_x.p};
-    launch_terms(st, c->term_nwg, c->wg_terms.p, b, c->s.p, c->n_items, c->nterms, c->contrib.p, c->ncontrib,
-                 c->part_bd.p, c->flags.p);
+// Each evaluation writes the other of the two nonbonded partial buffers (and
+// each bonded evaluation the other of the two bonded ones), so the field sums
+// of the previous step can read theirs while the next step's forces run.
+void enqueue_forces(ibx_ctx *c, int mode, int step = 0) {
+    c->nb_par ^= 1;
+    enqueue_nb_groups(c, mode, step, 0);
 }
 
-// The bonded terms read only the unwrapped positions and write their own
-// buffers, so they run on a second stream beside the list rebuild and the
-// nonbonded kernel; the integrator waits for both (fork here, join_terms).
-void fork_terms(ibx_ctx *c) {
-    (void)hipEventRecord(c->ev_fork, c->st);
-    (void)hipStreamWaitEvent(c->st2, c->ev_fork, 0);
-    enqueue_terms(c, c->st2);
-    (void)hipEventRecord(c->ev_join, c->st2);
+TermLaunch term_launch(const ibx_ctx *c) {
+    return TermLaunch{c->sys.p, c->term_mem.p, c->term_meta.p, c->term_slot.p, c->bd_tabs.p, c->bd_rows.p, c->bd_x.p};
 }
 
-void join_terms(ibx_ctx *c) { (void)hipStreamWaitEvent(c->st, c->ev_join, 0); }
+void enqueue_terms(ibx_ctx *c) {
+    c->bd_par ^= 1;
+    launch_terms(c->st, c->term_nwg, c->wg_terms.p, term_launch(c), c->s.p, c->n_items, c->nterms, c->contrib.p,
+                 c->ncontrib, c->part_bd_buf(c->bd_par), c->flags.p);
+}
 
-// Everything queued on the second stream so far finishes before what is queued
-// on the first from now on (the end of a run of steps, of a captured block).
-void join_side(ibx_ctx *c) {
-    (void)hipEventRecord(c->ev_join, c->st2);
-    (void)hipStreamWaitEvent(c->st, c->ev_join, 0);
+// The field sums of the latest evaluation (nonbonded, bonded and integrator partials).
+FinLaunch fin_launch(ibx_ctx *c, int mode) {
+    constexpr int per = kBlock / kNbAtomsPerBlock;
+    GroupDev &G1 = c->gr[1].present ? c->gr[1] : c->gr[0];
+    FinLaunch f;
+    f.sys = c->sys.p; f.part_nb0 = c->gr[0].part_buf(c->nb_par); f.part_nb1 = G1.part_buf(c->nb_par);
+    f.part_bd = c->part_bd_buf(c->bd_par); f.part_mv = c->part_mv.p;
+    f.snb0 = per * c->gr[0].nwg; f.snb1 = per * G1.nwg; f.sbd = c->term_nwg; f.smv = c->item_nwg;
+    f.fsum = c->fsum.p; f.done = c->fin_done.p; f.obs = c->obs.p; f.avg = c->avg.p; f.sysstate = c->sysstate.p;
+    f.step = c->step.p; f.mode = mode; f.nsys = c->nsys;
+    return f;
+}
+
+// The field sums a step left for the next step's force launch, enqueued on their own.
+void flush_fin(ibx_ctx *c) {
+    if (!c->fin_pending) return;
+    launch_finalize(c->st, c->fin);
+    c->fin_pending = false;
 }
 
 // Integration and reductions after the forces (the bonded terms must be enqueued before).
-// side: the field sums and observables go to the second stream, where they run
-// beside the next step's list rebuild and nonbonded kernel (which read neither
-// the partials they sum nor the system state they write); the next step's
-// integrator waits for them through join_terms.
-void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev, float nb_weight = 1.0f, bool side = false) {
+// defer: the field sums are left to the next step's force launch (flush_fin
+// enqueues them when no step follows).
+void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev, float nb_weight = 1.0f, bool defer = false) {
     MoveLaunch m;
     m.nb_weight = mode == 1 ? 1.0f : nb_weight;
     m.s = c->s.p; m.v = c->v.p; m.fnb = c->fnb.p; m.contrib = c->contrib.p; m.inc_off = c->inc_off.p;
@@ -761,18 +772,9 @@ void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev, float nb_we
         launch_vs_place(c->st, c->item_nwg, c->sys.p, c->wg_items.p, c->s.p, c->n_items, c->label.p, c->itype.p,
                         c->vs_center.p, c->vs_moff.p, c->vs_members.p, c->mass.p, c->vs_invmass.p, c->gr[1].pos4.p,
                         c->gr[1].pos4_ref.p, c->gr[1].inv_perm.p, c->dmax.p + c->nsys);
-    constexpr int per = kBlock / kNbAtomsPerBlock;
-    GroupDev &G1 = c->gr[1].present ? c->gr[1] : c->gr[0];
-    hipStream_t fst = c->st;
-    if (side) {
-        (void)hipEventRecord(c->ev_fin, c->st);
-        (void)hipStreamWaitEvent(c->st2, c->ev_fin, 0);
-        fst = c->st2;
-    }
-    launch_finalize(fst, c->nsys, c->sys.p, c->gr[0].part_buf(c->nb_par), G1.part_buf(c->nb_par), c->part_bd.p,
-                    c->part_mv.p,
-                    per * c->gr[0].nwg, per * G1.nwg, c->term_nwg, c->item_nwg, c->fsum.p, c->fin_done.p, c->obs.p,
-                    c->avg.p, c->sysstate.p, c->step.p, mode);
+    c->fin = fin_launch(c, mode);
+    c->fin_pending = true;
+    if (!defer) flush_fin(c);
 }
 
 int check_flags(ibx_ctx *c) {
@@ -806,29 +808,49 @@ int rebuild_checked(ibx_ctx *c) {
     return fail(c, IBX_E_LIST_OVERFLOW, "LIST TOO SMALL after resizing");
 }
 
-// One step.  With timing events the kernels run in order on one stream, so
-// each event pair brackets one kernel class; otherwise the bonded terms
-// overlap the rebuild and the nonbonded kernel.
+// One step.  With timing events each kernel class has its own launches, in
+// order, so each event pair brackets one class.  Otherwise the force phase is
+// one launch (group-0 nonbonded, bonded terms, and the previous step's field
+// sums, which the previous step left pending), then the bead group's
+// nonbonded force of a hybrid system, then the integrator; every launch
+// follows the one before on the context's stream.
 void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
     // multiple time step: the nonbonded force on steps 1, k+1, 2k+1, ..., applied with weight k
     const bool nb_step = c->respa == 1 || (step - 1) % c->respa == 0;
-    if (ev) (void)hipEventRecord((*ev)[0], c->st);
-    if (!ev) fork_terms(c);
+    const float w = nb_step ? (float)c->respa : 0.0f;
+    if (ev) {
+        flush_fin(c);
+        (void)hipEventRecord((*ev)[0], c->st);
+        if (step % c->nupdate == 0) {
+            enqueue_rebuild(c, true);
+            if (c->rdf_nbin && c->rdf_every > 0) enqueue_rdf(c);
+        }
+        (void)hipEventRecord((*ev)[4], c->st);
+        if (nb_step) enqueue_forces(c, 0, step);
+        (void)hipEventRecord((*ev)[1], c->st);
+        enqueue_terms(c);
+        (void)hipEventRecord((*ev)[2], c->st);
+        enqueue_rest(c, 0, ev, w);
+        (void)hipEventRecord((*ev)[5], c->st);
+        return;
+    }
     if (step % c->nupdate == 0) {
         enqueue_rebuild(c, true);
         if (c->rdf_nbin && c->rdf_every > 0) enqueue_rdf(c);
     }
-    if (ev) (void)hipEventRecord((*ev)[4], c->st);
-    if (nb_step) enqueue_forces(c, 0, step);
-    if (ev) {
-        (void)hipEventRecord((*ev)[1], c->st);
-        enqueue_terms(c, c->st);
-        (void)hipEventRecord((*ev)[2], c->st);
-    } else {
-        join_terms(c);
-    }
-    enqueue_rest(c, 0, ev, nb_step ? (float)c->respa : 0.0f, !ev);
-    if (ev) (void)hipEventRecord((*ev)[5], c->st);
+    if (nb_step) c->nb_par ^= 1;
+    c->bd_par ^= 1;
+    NbArgs a0;
+    const bool g0 = nb_step && c->gr[0].present;
+    if (g0) a0 = nb_args(c, 0);
+    FinLaunch f = c->fin;
+    const bool fin = c->fin_pending;
+    c->fin_pending = false;
+    launch_step_forces(c->st, g0 ? &a0 : nullptr, c->term_nwg, c->wg_terms.p, term_launch(c), c->s.p, c->n_items,
+                       c->nterms, c->contrib.p, c->ncontrib, c->part_bd_buf(c->bd_par), c->flags.p,
+                       fin ? &f : nullptr);
+    if (nb_step) enqueue_nb_groups(c, 0, step, 1);
+    enqueue_rest(c, 0, nullptr, w, true);
 }
 
 int ensure_events(ibx_ctx *c, size_t n) {
@@ -888,11 +910,7 @@ int ibx::create_context(std::vector<HostSystem> &&hs, ibx_ctx **out) {
     }
     c->nupdate = c->hs[0].nupdate;
     std::unique_lock<std::mutex> lk(g_capture_mu);
-    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess ||
-        hipStreamCreateWithFlags(&c->st2, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c->ev_fin, hipEventDisableTiming) != hipSuccess)
+    if (hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking) != hipSuccess)
         return fail(nullptr, IBX_E_DEVICE, "stream creation failed");
     lk.unlock();
     int rc = build_device(c.get());
@@ -932,7 +950,6 @@ int ibx_create(int nsys, const ibx_system_desc *const *desc, ibx_ctx **out) {
 void ibx_destroy(ibx_ctx *ctx) {
     if (!ctx) return;
     (void)hipStreamSynchronize(ctx->st);
-    if (ctx->st2) (void)hipStreamSynchronize(ctx->st2);
     delete ctx;
 }
 
@@ -948,9 +965,8 @@ int ibx_compute_forces(ibx_ctx *c, int rebuild) {
         if (rc) return rc;
     }
     HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
-    fork_terms(c);
     enqueue_forces(c, 1);
-    join_terms(c);
+    enqueue_terms(c);
     enqueue_rest(c, 1, nullptr);
     return check_flags(c);
 }
@@ -971,7 +987,7 @@ int ibx_step(ibx_ctx *c, int nsteps, int step0) {
         if (c->timing) e.assign(c->evpool.begin() + per * k, c->evpool.begin() + per * (k + 1));
         enqueue_step(c, step0 + 1 + k, c->timing ? &e : nullptr);
     }
-    join_side(c);
+    flush_fin(c);
     int rc = check_flags(c);
     if (c->timing) {
         for (int k = 0; k < nsteps; ++k) {
@@ -1003,7 +1019,7 @@ int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
     int nblk = (nsteps - done) / period;
     if (nblk > 0) {
         // a block starts by reading the partial buffer the capture found latest
-        if (!c->gexec || c->graph_steps != period || c->graph_par != c->nb_par) {
+        if (!c->gexec || c->graph_steps != period || c->graph_par != c->nb_par || c->graph_bd_par != c->bd_par) {
             drop_graph(c);
             std::lock_guard<std::mutex> lk(g_capture_mu);
             hipGraph_t graph;
@@ -1011,13 +1027,14 @@ int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
             // while this one captures; nothing captured here is a call that capture forbids
             HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
             for (int k = 0; k < period; ++k) enqueue_step(c, period + k, nullptr);   // block starts at a rebuild
-            join_side(c);
+            flush_fin(c);
             HIPCHK(c, hipStreamEndCapture(c->st, &graph));
             hipError_t e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
             if (e != hipSuccess) return fail(c, IBX_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
             c->graph_steps = period;
             c->graph_par = c->nb_par;
+            c->graph_bd_par = c->bd_par;
         }
         HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
         int cur = step0 + done;

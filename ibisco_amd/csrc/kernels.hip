@@ -163,10 +163,11 @@ constexpr int kNbFields = 9;   // nonbonded partial fields: V_NB of three classe
 // workgroups and works on block xcd_block(n), so each XCD runs one contiguous
 // range of blocks (one region of the system) and finds its neighbours' data in
 // its own L2.  Blocks at or past n return at once.
-__device__ __forceinline__ int xcd_block(int n) {
+__device__ __forceinline__ int xcd_block(int n, int b) {
     const int per = (n + kXcd - 1) / kXcd;
-    return (int)(blockIdx.x % kXcd) * per + (int)(blockIdx.x / kXcd);
+    return (b % kXcd) * per + b / kXcd;
 }
+__device__ __forceinline__ int xcd_block(int n) { return xcd_block(n, (int)blockIdx.x); }
 static inline int xcd_grid(int n) { return ((n + kXcd - 1) / kXcd) * kXcd; }
 
 __device__ __forceinline__ int pos_type(int w) { return w & 0xffff; }
@@ -697,8 +698,18 @@ __device__ __forceinline__ void nb_run(const NbCtx &C, const int4 *__restrict__ 
 // Tables are staged in LDS when they fit (kLds), else read through the cache.
 // Each pair is visited from both ends; energies and virial are halved.  The
 // loops are unrolled by four so four index and position loads are in flight.
+#define IBX_NB_PARAMS                                                                                           \
+    const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, const int *__restrict__ perm,                  \
+        const int4 *__restrict__ pos4, const int *__restrict__ list, const int2 *__restrict__ nneb,               \
+        const int *__restrict__ dmax, int nsys, int cap, const float4 *__restrict__ nb_rows,                      \
+        const float *__restrict__ nb_xoff, const NbTab *__restrict__ nb_tabs, const int *__restrict__ nb_map,     \
+        int nitems_total, const double *__restrict__ s, float *__restrict__ fnb, double *__restrict__ part, int nblk
+#define IBX_NB_ARGS sys, wg_sys, perm, pos4, list, nneb, dmax, nsys, cap, nb_rows, nb_xoff, nb_tabs, nb_map, \
+                    nitems_total, s, fnb, part, nblk
+
+// The work of workgroup `bidx` of a nonbonded launch (also one role of k_step_forces).
 template <bool kMulti, bool kLds, int kGroup, bool kCubic>
-__global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
+__device__ __forceinline__ void nb_block(const int bidx, const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
                                                       const int *__restrict__ perm, const int4 *__restrict__ pos4,
                                                       const int *__restrict__ list, const int2 *__restrict__ nneb,
                                                       const int *__restrict__ dmax, int nsys, int cap,
@@ -711,7 +722,7 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
     __shared__ NbTab tabs[kMultiTabsLds(kMulti)];
     __shared__ int map[kMulti ? kMaxNbTypes * kMaxNbTypes : 1];
     __shared__ double red[kNbFields * 16];
-    const int blk = xcd_block(nblk);
+    const int blk = xcd_block(nblk, bidx);
     if (blk >= nblk) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int a = lane / kLanes, q = lane % kLanes;
@@ -824,6 +835,11 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(const DevSys *__restrict__
         const double u2 = S.unit * S.unit;
         part[(size_t)threadIdx.x * nblk + blk] = threadIdx.x < 3 ? 0.5 * x : 0.5 * x * u2;
     }
+}
+
+template <bool kMulti, bool kLds, int kGroup, bool kCubic>
+__global__ void __launch_bounds__(kBlock) k_nonbonded(IBX_NB_PARAMS) {
+    nb_block<kMulti, kLds, kGroup, kCubic>((int)blockIdx.x, IBX_NB_ARGS);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1063,12 +1079,14 @@ __device__ __forceinline__ void oop_term(const BdTab &t, const float4 *rows, con
 // is accumulated per term as sum_m f_m (s_m - s_first) in FP64, which does not
 // lose precision to the size of the unwrapped coordinates.  Component order as
 // the reference: 11 22 33, then F_y s_x, F_z s_x, F_z s_y.
-__global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys, TermArgs A, const double *__restrict__ s,
-                                                  int nitems_total, int nterms, float4 *__restrict__ contrib,
-                                                  int ncontrib, double *__restrict__ part, int *__restrict__ flags) {
+// Workgroup b of nwg (also one role of k_step_forces).
+__device__ __forceinline__ void terms_block(const int b, const int nwg, const int *__restrict__ wg_sys, const TermArgs &A,
+                                            const double *__restrict__ s, int nitems_total, int nterms,
+                                            float4 *__restrict__ contrib, int ncontrib, double *__restrict__ part,
+                                            int *__restrict__ flags) {
     __shared__ double red[19 * 4];
-    const int live = A.sys[wg_sys[blockIdx.x]].live;   // first, through the scalar cache
-    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int live = A.sys[wg_sys[b]].live;   // first, through the scalar cache
+    const int t = b * kBlock + threadIdx.x;
     const int n = nitems_total;
     // the term and its members' positions are loaded before the system record
     // (the term arrays are padded to whole blocks), so those round trips overlap
@@ -1130,9 +1148,15 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
     }
     block_sum<19>(en, red);
     if (threadIdx.x == 0) {
-        double *p = part + blockIdx.x;   // field-major
-        for (int k = 0; k < 19; ++k) p[(size_t)k * gridDim.x] = en[k];
+        double *p = part + b;   // field-major
+        for (int k = 0; k < 19; ++k) p[(size_t)k * nwg] = en[k];
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys, TermArgs A, const double *__restrict__ s,
+                                                  int nitems_total, int nterms, float4 *__restrict__ contrib,
+                                                  int ncontrib, double *__restrict__ part, int *__restrict__ flags) {
+    terms_block((int)blockIdx.x, (int)gridDim.x, wg_sys, A, s, nitems_total, nterms, contrib, ncontrib, part, flags);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1402,17 +1426,18 @@ __device__ void finalize_system(const DevSys &S, int s_id, const double *red, do
 // integrate.  Each sum is taken in a fixed order, so results are deterministic.
 // The last workgroup of a system to finish (a per-system counter, reset by that
 // workgroup for the next launch) forms the system's observables.
-__global__ void __launch_bounds__(kBlock) k_field_sums(const DevSys *__restrict__ sys,
-                                                      const double *__restrict__ part_nb0,
-                                                      const double *__restrict__ part_nb1,
-                                                      const double *__restrict__ part_bd,
-                                                      const double *__restrict__ part_mv, int snb0, int snb1, int sbd,
-                                                      int smv, double *__restrict__ fsum, int *__restrict__ done,
-                                                      double *__restrict__ obs, double *__restrict__ avg,
-                                                      double *__restrict__ sysstate, const int *__restrict__ step,
-                                                      int mode) {
+#define IBX_FIN_PARAMS                                                                                          \
+    const DevSys *__restrict__ sys, const double *__restrict__ part_nb0, const double *__restrict__ part_nb1,      \
+        const double *__restrict__ part_bd, const double *__restrict__ part_mv, int snb0, int snb1, int sbd,      \
+        int smv, double *__restrict__ fsum, int *__restrict__ done, double *__restrict__ obs,                     \
+        double *__restrict__ avg, double *__restrict__ sysstate, const int *__restrict__ step, int mode
+#define IBX_FIN_ARGS sys, part_nb0, part_nb1, part_bd, part_mv, snb0, snb1, sbd, smv, fsum, done, obs, avg, sysstate, \
+                     step, mode
+
+// Workgroup b of the field sums (also one role of k_step_forces).
+__device__ __forceinline__ void field_sums_block(const int b, IBX_FIN_PARAMS) {
     __shared__ double red[kBlock / kWave];
-    const int s_id = blockIdx.x / kFields, f = blockIdx.x % kFields;
+    const int s_id = b / kFields, f = b % kFields;
     const DevSys &S = sys[s_id];
     const int t = threadIdx.x;
     constexpr int per = kBlock / kNbAtomsPerBlock;   // nonbonded workgroups per 256-slot block
@@ -1431,7 +1456,7 @@ __global__ void __launch_bounds__(kBlock) k_field_sums(const DevSys *__restrict_
     if ((t & (kWave - 1)) == 0) red[t / kWave] = x;
     __syncthreads();
     if (t == 0) {
-        fsum[blockIdx.x] = ((red[0] + red[1]) + red[2]) + red[3];
+        fsum[b] = ((red[0] + red[1]) + red[2]) + red[3];
         __threadfence();
         // the last of the system's kFields workgroups turns the sums into observables
         if (atomicAdd(&done[s_id], 1) == kFields - 1) {
@@ -1440,6 +1465,67 @@ __global__ void __launch_bounds__(kBlock) k_field_sums(const DevSys *__restrict_
             if (S.live) finalize_system(S, s_id, fsum + (size_t)s_id * kFields, obs, avg, sysstate, step, mode);
         }
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_field_sums(IBX_FIN_PARAMS) { field_sums_block((int)blockIdx.x, IBX_FIN_ARGS); }
+
+struct NbLaunchK {   // the arguments of k_nonbonded
+    const DevSys *sys;
+    const int *wg_sys, *perm;
+    const int4 *pos4;
+    const int *list;
+    const int2 *nneb;
+    const int *dmax;
+    int nsys, cap;
+    const float4 *rows;
+    const float *xoff;
+    const NbTab *tabs;
+    const int *map;
+    int nitems_total;
+    const double *s;
+    float *fnb;
+    double *part;
+    int nblk;
+};
+
+struct TermLaunchK {   // the arguments of k_terms
+    const int *wg_sys;
+    TermArgs A;
+    const double *s;
+    int nitems_total, nterms;
+    float4 *contrib;
+    int ncontrib;
+    double *part;
+    int *flags;
+    int nwg;
+};
+
+// One launch for the force phase of a step: the group-0 nonbonded workgroups
+// (none on a step that skips the nonbonded force), then the bonded terms, then
+// the field sums of the previous step (none on the first step of a run).  The
+// roles share the device with one another the way kernels on two streams
+// would, but with no dependency between streams to wait on; the field sums
+// read the other of the two nonbonded and bonded partial buffers.
+template <bool kMulti, bool kLds, bool kCubic>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_step_forces(NbLaunchK nb, int nb_grid, TermLaunchK tk, FinLaunch fk,
+                                                        int fin_blocks) {
+    int b = (int)blockIdx.x;
+    if (b < nb_grid) {
+        nb_block<kMulti, kLds, 0, kCubic>(b, nb.sys, nb.wg_sys, nb.perm, nb.pos4, nb.list, nb.nneb, nb.dmax, nb.nsys,
+                                          nb.cap, nb.rows, nb.xoff, nb.tabs, nb.map, nb.nitems_total, nb.s, nb.fnb,
+                                          nb.part, nb.nblk);
+        return;
+    }
+    b -= nb_grid;
+    if (b < tk.nwg) {
+        terms_block(b, tk.nwg, tk.wg_sys, tk.A, tk.s, tk.nitems_total, tk.nterms, tk.contrib, tk.ncontrib, tk.part,
+                    tk.flags);
+        return;
+    }
+    b -= tk.nwg;
+    if (b < fin_blocks)
+        field_sums_block(b, fk.sys, fk.part_nb0, fk.part_nb1, fk.part_bd, fk.part_mv, fk.snb0, fk.snb1, fk.sbd, fk.smv,
+                         fk.fsum, fk.done, fk.obs, fk.avg, fk.sysstate, fk.step, fk.mode);
 }
 
 // Radial distribution function histogram over the current list (group 0).
@@ -1674,11 +1760,52 @@ void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_s
                        vs_moff, vs_members, mass, vs_invmass, pos4, pos4_ref, inv_perm, dmax);
 }
 
-void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,
-                     const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *fsum,
-                     int *done, double *obs, double *avg, double *sysstate, const int *step, int mode) {
-    hipLaunchKernelGGL(k_field_sums, dim3(nsys * kFields), dim3(kBlock), 0, st, sys, p0, p1, pbd, pmv, snb0, snb1, sbd,
-                       smv, fsum, done, obs, avg, sysstate, step, mode);
+void launch_finalize(hipStream_t st, const FinLaunch &f) {
+    hipLaunchKernelGGL(k_field_sums, dim3(f.nsys * kFields), dim3(kBlock), 0, st, f.sys, f.part_nb0, f.part_nb1,
+                       f.part_bd, f.part_mv, f.snb0, f.snb1, f.sbd, f.smv, f.fsum, f.done, f.obs, f.avg, f.sysstate,
+                       f.step, f.mode);
+}
+
+template <bool M, bool L, bool K>
+static void step_forces_go(hipStream_t st, dim3 grid, size_t lds, const NbLaunchK &nb, int nb_grid,
+                           const TermLaunchK &tk, const FinLaunch &fk, int fin_blocks) {
+    hipLaunchKernelGGL((k_step_forces<M, L, K>), grid, dim3(kBlock), lds, st, nb, nb_grid, tk, fk, fin_blocks);
+}
+
+void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const int *wg_terms, const TermLaunch &b,
+                        const double *s, int nitems_total, int nterms, float4 *contrib, int ncontrib, double *part_bd,
+                        int *flags, const FinLaunch *fin) {
+    NbLaunchK k{};
+    int nb_grid = 0;
+    bool multi = false, lds = false, cubic = true;
+    size_t bytes = 0;
+    if (nb && nb->g == 0) {   // the fused launch carries group 0 only
+        k.sys = nb->sys; k.wg_sys = nb->wg_sys; k.perm = nb->perm; k.pos4 = nb->pos4; k.list = nb->list;
+        k.nneb = nb->nneb; k.dmax = nb->dmax; k.nsys = nb->nsys; k.cap = nb->cap;
+        k.rows = nb->rows; k.xoff = nb->xoff; k.tabs = nb->tabs; k.map = nb->map; k.nitems_total = nb->nitems_total;
+        k.s = nb->s; k.fnb = nb->fnb; k.part = nb->part; k.nblk = nb->nwg * (kBlock / kNbAtomsPerBlock);
+        nb_grid = xcd_grid(k.nblk);
+        multi = nb->multi; lds = nb->max_rows_lds > 0; cubic = nb->cubic;
+        bytes = std::max(lds ? (size_t)nb->max_rows_lds * sizeof(float4) : 0, (size_t)kNbFields * kBlock * sizeof(float));
+    }
+    TermLaunchK t{wg_terms, TermArgs{b.sys, b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs}, s, nitems_total, nterms,
+                  contrib, ncontrib, part_bd, flags, term_nwg};
+    FinLaunch f{};
+    const int fin_blocks = fin ? fin->nsys * kFields : 0;
+    if (fin) f = *fin;
+    const dim3 grid(nb_grid + term_nwg + fin_blocks);
+    if (grid.x == 0) return;
+    if (multi) {
+        if (lds) cubic ? step_forces_go<true, true, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks)
+                       : step_forces_go<true, true, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks);
+        else cubic ? step_forces_go<true, false, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks)
+                   : step_forces_go<true, false, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks);
+    } else {
+        if (lds) cubic ? step_forces_go<false, true, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks)
+                       : step_forces_go<false, true, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks);
+        else cubic ? step_forces_go<false, false, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks)
+                   : step_forces_go<false, false, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks);
+    }
 }
 
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,

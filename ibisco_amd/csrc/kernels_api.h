@@ -80,6 +80,18 @@ struct MoveLaunch {
     float nb_weight;    // weight of the nonbonded force in this move (multiple time step)
 };
 
+// The field sums and observables of one step (k_field_sums).
+struct FinLaunch {
+    const DevSys *sys;
+    const double *part_nb0, *part_nb1, *part_bd, *part_mv;
+    int snb0, snb1, sbd, smv;   // workgroups (field strides) of each partial buffer
+    double *fsum;
+    int *done;
+    double *obs, *avg, *sysstate;
+    const int *step;
+    int mode, nsys;
+};
+
 void launch_zero_int(hipStream_t st, int *p, int n);
 void launch_mts(hipStream_t st, bool save, float *fnb, const int *label, int n, double *part, int npart,
                 float *mts_f, double *mts_part, int m, double I);
@@ -94,9 +106,12 @@ void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_s
                      const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,
                      const double *mass, const double *vs_invmass, int4 *pos4, const int4 *pos4_ref,
                      const int *inv_perm, int *dmax);
-void launch_finalize(hipStream_t st, int nsys, const DevSys *sys, const double *p0, const double *p1,
-                     const double *pbd, const double *pmv, int snb0, int snb1, int sbd, int smv, double *fsum,
-                     int *done, double *obs, double *avg, double *sysstate, const int *step, int mode);
+void launch_finalize(hipStream_t st, const FinLaunch &f);
+// The force phase of a step in one launch: the group-0 nonbonded force (nb null: none this step), the bonded
+// terms, and the field sums of the previous step (fin null: none).
+void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const int *wg_terms, const TermLaunch &b,
+                        const double *s, int nitems_total, int nterms, float4 *contrib, int ncontrib, double *part_bd,
+                        int *flags, const FinLaunch *fin);
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
                 const int *list, const int2 *nneb, int cap, int nbin, float inv_dr, unsigned long long *hist,
                 int *samples);

@@ -60,7 +60,7 @@ struct GroupDev {
     int nslot_pad = 0, nwg = 0, ncell_total = 0, cap = 0, max_rows = 0;
     bool multi = false, present = false;
     DBuf<int2> nneb;
-    DBuf<int> mem, mem_cell, cell_count, cell_start, cell_fill, tmp, perm, slot_cell, inv_perm, list, wg_sys,
+    DBuf<int> mem, mem_cell, cell_count, cell_start, cell_fill, tile_sum, tmp, perm, slot_cell, inv_perm, list, wg_sys,
         cell_sys;
     DBuf<double4> r64;
     DBuf<int4> pos4, pos4_ref;
@@ -502,7 +502,7 @@ int build_device(ibx_ctx *c) {
             const double rc = h.rcut[g], rl = h.rlist[g];
             int nc[3];
             for (int k = 0; k < 3; ++k) {
-                nc[k] = (int)(2.0 * h.box[k] / rl);     // cells of edge >= rlist/2 (the reference: rlist)
+                nc[k] = (int)(kCellDiv * h.box[k] / rl);   // cells of edge >= rlist/kCellDiv (the reference: rlist)
                 if (nc[k] < 1) nc[k] = 1;
                 dg.nc[k] = nc[k];
             }
@@ -556,6 +556,7 @@ int build_device(ibx_ctx *c) {
         HIPCHK(c, G.cell_count.alloc(cell));
         HIPCHK(c, G.cell_start.alloc(cell));
         HIPCHK(c, G.cell_fill.alloc(cell));
+        HIPCHK(c, G.tile_sum.alloc((cell + kBlock - 1) / kBlock));
         HIPCHK(c, G.tmp.alloc(slot));
         HIPCHK(c, G.slot_cell.alloc(slot));
         HIPCHK(c, G.r64.alloc(slot));
@@ -585,7 +586,7 @@ size_t device_bytes(const ibx_ctx *c) {
                c->rdf.bytes() + c->rdf_samples.bytes() + c->mts_f.bytes() + c->mts_part.bytes();
     for (const GroupDev &G : c->gr)
         b += G.nneb.bytes() + G.mem.bytes() + G.mem_cell.bytes() + G.cell_count.bytes() + G.cell_start.bytes() +
-             G.cell_fill.bytes() + G.tmp.bytes() + G.perm.bytes() + G.slot_cell.bytes() + G.inv_perm.bytes() +
+             G.cell_fill.bytes() + G.tile_sum.bytes() + G.tmp.bytes() + G.perm.bytes() + G.slot_cell.bytes() + G.inv_perm.bytes() +
              G.list.bytes() + G.wg_sys.bytes() + G.cell_sys.bytes() + G.r64.bytes() + G.pos4.bytes() +
              G.pos4_ref.bytes() + G.part.bytes();
     return b;
@@ -626,13 +627,14 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
         const int nm = g < h.ngroup ? (int)h.members[g].size() : 0;
         const size_t slots = (size_t)pad_to(nm, kBlock), nwg = slots / kBlock;
         size_t ncell = 1;
-        for (int k = 0; k < 3; ++k) ncell *= (size_t)std::max(1, (int)(2.0 * h.box[k] / h.rlist[g]));
+        for (int k = 0; k < 3; ++k) ncell *= (size_t)std::max(1, (int)(kCellDiv * h.box[k] / h.rlist[g]));
         if (nm == 0) continue;   // an absent group allocates nothing
         const double nbr = 4.0 / 3.0 * M_PI * h.rlist[g] * h.rlist[g] * h.rlist[g] * nm /
                            (h.box[0] * h.box[1] * h.box[2]);
         const size_t cap = (size_t)pad_to((int)(std::max(1.5, h.list_margin) * nbr) + 64, 8);
         b += slots * (4 * 5 + 32 + 16 + 16 + 8)                  // mem mem_cell tmp perm slot_cell r64 pos4 pos4_ref nneb
              + N * 4 + nwg * 4 + ncell * 16                       // inv_perm, wg_sys, cell arrays
+             + (ncell + kBlock - 1) / kBlock * 4                  // scan tile sums
              + 2 * nwg * (kBlock / kNbAtomsPerBlock) * kNbPart * 8 + slots * cap * 4;
     }
     return b;
@@ -649,7 +651,7 @@ RebuildArgs rebuild_args(ibx_ctx *c, int g) {
     for (const DevSys &D : c->dsys) a.cubic = a.cubic && D.rho[1] == 1.0f && D.rho[2] == 1.0f;
     a.mem = G.mem.p; a.s = c->s.p; a.itype = c->itype.p; a.label = c->label.p;
     a.mem_cell = G.mem_cell.p; a.cell_count = G.cell_count.p; a.cell_start = G.cell_start.p;
-    a.cell_fill = G.cell_fill.p; a.tmp = G.tmp.p; a.perm = G.perm.p; a.slot_cell = G.slot_cell.p;
+    a.cell_fill = G.cell_fill.p; a.tile_sum = G.tile_sum.p; a.tmp = G.tmp.p; a.perm = G.perm.p; a.slot_cell = G.slot_cell.p;
     a.r64 = G.r64.p; a.pos4 = G.pos4.p; a.inv_perm = G.inv_perm.p; a.pos4_ref = G.pos4_ref.p;
     a.dmax = c->dmax.p + (size_t)g * c->nsys; a.nsys = c->nsys;
     a.excl_off = c->excl_off.p; a.excl = c->excl.p; a.list = G.list.p; a.nneb = G.nneb.p; a.flags = c->flags.p;

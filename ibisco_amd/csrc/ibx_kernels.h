@@ -28,8 +28,11 @@ constexpr int kAtomsPerWave = kWave / kLanes;
 constexpr int kNbAtomsPerBlock = kBlock / kLanes;
 constexpr int kListLanes = 16;            // lanes testing the candidates of one member in the list build
 constexpr int kListAtoms = kBlock / kListLanes;
-constexpr int kListRanges = 64;           // candidate slot ranges per member (25 columns, at most 2 each)
-constexpr int kMinCells = 5;              // cells per axis for the +-2 cell stencil
+constexpr int kCellDiv = 3;               // cells per rlist along each axis (cell edge >= rlist/3)
+constexpr int kReach = kCellDiv;          // the list stencil: +-kReach cells around the member's cell
+constexpr int kStencil = 2 * kReach + 1;
+constexpr int kListRanges = 2 * kStencil * kStencil;   // candidate slot ranges per member (columns, at most 2 each)
+constexpr int kMinCells = kStencil;       // cells per axis for the stencil (fewer: every member scans the group)
 constexpr double kInnerFrac = 1.0 / 3.0;  // delta = this fraction of the list skin rlist - rc
 
 // Position of entry k of sorted slot i in the Verlet list.  Entries of the

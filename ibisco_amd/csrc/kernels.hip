@@ -233,29 +233,45 @@ __global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *__restrict_
     atomicAdd(&cell_count[G.cell_off + cell], 1);
 }
 
-// Exclusive scan of the cell counts of the whole group grid (one workgroup).
-__global__ void __launch_bounds__(1024) k_scan_cells(const int *__restrict__ count, int *__restrict__ start,
-                                                     int *__restrict__ fill, int n) {
-    __shared__ int part[1024];
-    const int t = threadIdx.x;
-    const int per = (n + 1023) / 1024;
-    const int a = t * per, b = min(n, a + per);
-    int sum = 0;
-    for (int i = a; i < b; ++i) sum += count[i];
-    part[t] = sum;
+// Exclusive scan of the cell counts into cell starts in two passes over tiles
+// of kBlock cells, so that every pass is spread over many workgroups (the
+// counts of a large grid take one workgroup most of its time just to read):
+// k_scan_tiles scans each tile and writes its total, k_scan_add adds the
+// totals of the tiles before each one.  Integer sums, so the order is free.
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(const int *__restrict__ count, int *__restrict__ start,
+                                                      int *__restrict__ fill, int *__restrict__ tile_sum, int n) {
+    __shared__ int wtot[kBlock / kWave];
+    const int i = blockIdx.x * kBlock + threadIdx.x, lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const int c = i < n ? count[i] : 0;
+    int x = c;   // inclusive scan over the wave
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) wtot[wid] = x;
     __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {
-        int x = t >= off ? part[t - off] : 0;
-        __syncthreads();
-        part[t] += x;
-        __syncthreads();
-    }
-    int run = part[t] - sum;
-    for (int i = a; i < b; ++i) {
-        start[i] = run;
+    int pre = 0;
+    for (int w = 0; w < wid; ++w) pre += wtot[w];
+    if (i < n) {
+        start[i] = pre + x - c;
         fill[i] = 0;
-        run += count[i];
     }
+    if (threadIdx.x == kBlock - 1) tile_sum[blockIdx.x] = pre + x;
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_add(int *__restrict__ start, const int *__restrict__ tile_sum, int n) {
+    __shared__ int wtot[kBlock / kWave];
+    const int b = blockIdx.x;
+    if (b == 0) return;
+    int acc = 0;
+    for (int t = threadIdx.x; t < b; t += kBlock) acc += tile_sum[t];
+    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
+    if ((threadIdx.x & (kWave - 1)) == 0) wtot[threadIdx.x / kWave] = acc;
+    __syncthreads();
+    int off = 0;
+    for (int w = 0; w < kBlock / kWave; ++w) off += wtot[w];
+    const int i = b * kBlock + threadIdx.x;
+    if (i < n) start[i] += off;
 }
 
 __global__ void __launch_bounds__(kBlock) k_place(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, int g,
@@ -278,28 +294,34 @@ __global__ void __launch_bounds__(kBlock) k_sort_cells(const int *__restrict__ c
                                                        const int *__restrict__ cell_start, const int *__restrict__ cell_count,
                                                        const int *__restrict__ tmp, const int *__restrict__ mem,
                                                        int *__restrict__ perm, int *__restrict__ slot_cell,
-                                                       int *__restrict__ flags) {
-    __shared__ int vals[1024];
-    const int gc = blockIdx.x;
-    const DevSys &S = sys[cell_sys[gc]];
-    const DevGroup &G = S.g[g];
-    const int n = cell_count[gc];
-    if (n == 0) return;
-    const int base = G.slot_off + cell_start[gc] - cell_start[G.cell_off];
-    const int lc = gc - G.cell_off;
-    if (n > 1024) {   // too crowded to sort in LDS: keep arrival order, flag it
-        if (threadIdx.x == 0) atomicMax(&flags[2], n);
-        for (int i = threadIdx.x; i < n; i += kBlock) {
+                                                       int *__restrict__ flags, int ncell) {
+    // one wave per cell (cells hold a few members each), kBlock / kWave cells per workgroup
+    constexpr int kCap = 1024;
+    __shared__ int vals[kBlock / kWave][kCap];
+    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
+    const int gc = blockIdx.x * (kBlock / kWave) + w;
+    int n = 0, base = 0, lc = 0;
+    if (gc < ncell) {
+        const DevSys &S = sys[cell_sys[gc]];
+        const DevGroup &G = S.g[g];
+        n = cell_count[gc];
+        base = G.slot_off + cell_start[gc] - cell_start[G.cell_off];
+        lc = gc - G.cell_off;
+    }
+    if (n > kCap) {   // too crowded to sort in LDS: keep arrival order, flag it
+        if (lane == 0) atomicMax(&flags[2], n);
+        for (int i = lane; i < n; i += kWave) {
             perm[base + i] = mem[tmp[base + i]];
             slot_cell[base + i] = lc;
         }
-        return;
+        n = 0;
     }
-    for (int i = threadIdx.x; i < n; i += kBlock) vals[i] = tmp[base + i];
+    for (int i = lane; i < n; i += kWave) vals[w][i] = tmp[base + i];
     __syncthreads();
-    for (int i = threadIdx.x; i < n; i += kBlock) {
-        int v = vals[i], rank = 0;
-        for (int k = 0; k < n; ++k) rank += vals[k] < v;
+    for (int i = lane; i < n; i += kWave) {
+        const int v = vals[w][i];
+        int rank = 0;
+        for (int k = 0; k < n; ++k) rank += vals[w][k] < v;
         perm[base + rank] = mem[v];
         slot_cell[base + rank] = lc;
     }
@@ -433,21 +455,26 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
         }
         // reach: rlist with a margin, so a pruned cell is always beyond rlist
         const float reach2 = (float)(rl2 * (1.0 + 1e-4));
-        int st[4], len[4], np = 0;   // pieces of this lane's two columns, in column order
-        int n_half[2] = {0, 0};
-        for (int h = 0; h < 2; ++h) {
+        // the kStencil x kStencil columns around the member's cell, column q in group q / kListLanes handled by
+        // lane q % kListLanes; each column is one z range (two where it wraps)
+        constexpr int kCols = kStencil * kStencil, kGroups = (kCols + kListLanes - 1) / kListLanes;
+        int st[2 * kGroups], len[2 * kGroups], np = 0;   // pieces of this lane's columns, in column order
+        int n_grp[kGroups];
+#pragma unroll
+        for (int h = 0; h < kGroups; ++h) {
+            n_grp[h] = 0;
             const int q = sub + h * kListLanes;
-            if (!valid || q >= 25) continue;
-            const int dx = q / 5 - 2, dy = q % 5 - 2;
+            if (!valid || q >= kCols) continue;
+            const int dx = q / kStencil - kReach, dy = q % kStencil - kReach;
             const float gx = cell_gap(dx, f[0], a[0]), gy = cell_gap(dy, f[1], a[1]);
             const float gxy = gx * gx + gy * gy;
             if (gxy >= reach2) continue;
             int dz0 = 0, dz1 = 0;
-            for (int d = -2; d < 0; ++d) {
+            for (int d = -kReach; d < 0; ++d) {
                 const float gz = cell_gap(d, f[2], a[2]);
                 if (gxy + gz * gz < reach2) { dz0 = d; break; }
             }
-            for (int d = 2; d > 0; --d) {
+            for (int d = kReach; d > 0; --d) {
                 const float gz = cell_gap(d, f[2], a[2]);
                 if (gxy + gz * gz < reach2) { dz1 = d; break; }
             }
@@ -466,37 +493,47 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
                 st[np] = slot_off + b - cs0;
                 len[np] = e - b;
                 ++np;
-                ++n_half[h];
+                ++n_grp[h];
             }
         }
-        // place the pieces: first-half columns by lane, then second-half columns by lane
-        int cnt[2] = {n_half[0], n_half[1]}, lsum[2] = {0, 0};
-        for (int k = 0; k < np; ++k) lsum[k < n_half[0] ? 0 : 1] += len[k];
-        int pos[2], cum[2], tot_n[2], tot_c[2];
-        for (int h = 0; h < 2; ++h) {
-            int xn = cnt[h], xc = lsum[h];
+        // place the pieces: group-0 columns by lane, then group-1 columns by lane, ...
+        int lsum[kGroups], pos[kGroups], cum[kGroups], tot_n = 0, tot_c = 0;
+        {
+            int k = 0;
+#pragma unroll
+            for (int h = 0; h < kGroups; ++h) {
+                lsum[h] = 0;
+                for (int u = 0; u < n_grp[h]; ++u, ++k) lsum[h] += len[k];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < kGroups; ++h) {
+            int xn = n_grp[h], xc = lsum[h];
             for (int off = 1; off < kListLanes; off <<= 1) {
                 const int yn = __shfl_up(xn, off, kListLanes), yc = __shfl_up(xc, off, kListLanes);
                 if (sub >= off) { xn += yn; xc += yc; }
             }
-            tot_n[h] = __shfl(xn, kListLanes - 1, kListLanes);
-            tot_c[h] = __shfl(xc, kListLanes - 1, kListLanes);
-            pos[h] = xn - cnt[h];
-            cum[h] = xc - lsum[h];
+            pos[h] = tot_n + xn - n_grp[h];
+            cum[h] = tot_c + xc - lsum[h];
+            tot_n += __shfl(xn, kListLanes - 1, kListLanes);
+            tot_c += __shfl(xc, kListLanes - 1, kListLanes);
         }
-        pos[1] += tot_n[0];
-        cum[1] += tot_c[0];
-        for (int k = 0; k < np; ++k) {
-            const int h = k < n_half[0] ? 0 : 1;
-            const int r = pos[h]++;
-            if (r < kListRanges) {
-                rbase[grp][r] = st[k] - cum[h];
-                cum[h] += len[k];
-                rend[grp][r] = cum[h];
+        {
+            int k = 0;
+#pragma unroll
+            for (int h = 0; h < kGroups; ++h) {
+                for (int u = 0; u < n_grp[h]; ++u, ++k) {
+                    const int r = pos[h]++;
+                    if (r < kListRanges) {
+                        rbase[grp][r] = st[k] - cum[h];
+                        cum[h] += len[k];
+                        rend[grp][r] = cum[h];
+                    }
+                }
             }
         }
-        T = tot_c[0] + tot_c[1];
-        if (tot_n[0] + tot_n[1] > kListRanges) T = 0;   // cannot happen with 25 columns x 2 pieces
+        T = tot_c;
+        if (tot_n > kListRanges) T = 0;   // cannot happen: kListRanges holds every column split in two
     }
     __syncthreads();
 
@@ -1675,11 +1712,15 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
     launch_zero_int(st, a.cell_count, a.ncell_total);
     hipLaunchKernelGGL(k_cell_count, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.mem, a.s, a.nitems_total,
                        a.mem_cell, a.cell_count);
-    hipLaunchKernelGGL(k_scan_cells, dim3(1), dim3(1024), 0, st, a.cell_count, a.cell_start, a.cell_fill, a.ncell_total);
+    const int ntile = (a.ncell_total + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(kBlock), 0, st, a.cell_count, a.cell_start, a.cell_fill,
+                       a.tile_sum, a.ncell_total);
+    hipLaunchKernelGGL(k_scan_add, dim3(ntile), dim3(kBlock), 0, st, a.cell_start, a.tile_sum, a.ncell_total);
     hipLaunchKernelGGL(k_place, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.mem, a.mem_cell, a.cell_start,
                        a.cell_fill, a.tmp);
-    hipLaunchKernelGGL(k_sort_cells, dim3(a.ncell_total), dim3(kBlock), 0, st, a.cell_sys, a.sys, a.g, a.cell_start,
-                       a.cell_count, a.tmp, a.mem, a.perm, a.slot_cell, a.flags);
+    hipLaunchKernelGGL(k_sort_cells, dim3((a.ncell_total + kBlock / kWave - 1) / (kBlock / kWave)), dim3(kBlock), 0, st,
+                       a.cell_sys, a.sys, a.g, a.cell_start,
+                       a.cell_count, a.tmp, a.mem, a.perm, a.slot_cell, a.flags, a.ncell_total);
     hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
                        a.itype, a.label, a.r64, a.pos4, a.inv_perm, a.pos4_ref, a.dmax, a.nsys, 0);
     const int nwg_list = a.nwg * (kBlock / kListAtoms);

@@ -16,6 +16,7 @@ struct RebuildArgs {
     const double *s;
     const int *itype, *label;
     int *mem_cell, *cell_count, *cell_start, *cell_fill, *tmp, *perm, *slot_cell;
+    int *tile_sum;      // one per kBlock cells (the cell-start scan)
     double4 *r64;
     int4 *pos4, *pos4_ref;
     int *inv_perm;

@@ -1140,8 +1140,10 @@ __device__ __forceinline__ void terms_block(const int b, const int nwg, const in
     const BdTab tb = A.tabs[max(meta, 0) >> 2];
     issue_here(s, A.tabs, A.mem);
     if (!live) return;
-    double en[19];
+    double en[19];   // 0..11 unused here: a term's energy is vt in class ek (see the sums below)
     for (int k = 0; k < 19; ++k) en[k] = 0.0;
+    double vt = 0.0;
+    int ek = -1;
     if (meta >= 0) {
         const int kind = meta & 3;
         double fm[4][3], V;
@@ -1161,10 +1163,8 @@ __device__ __forceinline__ void terms_block(const int b, const int nwg, const in
             oop_term<double>(tb, A.rows, A.xs, P, fm, V, flags);
             nm = 4;
         }
-        // static indices only, so en[] and fm[][] stay in registers
-        const int ek = 3 * kind + tb.label - 1;
-#pragma unroll
-        for (int k = 0; k < 12; ++k) en[k] += (k == ek) ? V : 0.0;
+        ek = 3 * kind + tb.label - 1;
+        vt = V;
         const int sl[4] = {s4.x, s4.y, s4.z, s4.w};
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
@@ -1183,10 +1183,31 @@ __device__ __forceinline__ void terms_block(const int b, const int nwg, const in
             en[18] += fm[r][2] * dy;
         }
     }
-    block_sum<19>(en, red);
+    // Workgroup sums of the twelve energy classes (kind x label), the overextension
+    // count and the virial.  The terms of a wave usually share one class (they are
+    // grouped by kind and table), and such a wave sums its energies once instead
+    // of once per class; the sums are the same either way.
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    const unsigned long long valid = __ballot(ek >= 0);
+    const int ek_ref = valid ? __builtin_amdgcn_readlane(ek, __ffsll((long long)valid) - 1) : 0;
+    double w[19];
+    if (__ballot(ek >= 0 && ek != ek_ref) == 0) {
+        const double v = wave_sum(vt);
+#pragma unroll
+        for (int k = 0; k < 12; ++k) w[k] = k == ek_ref ? v : 0.0;
+    } else {
+#pragma unroll
+        for (int k = 0; k < 12; ++k) w[k] = wave_sum(ek == k ? vt : 0.0);
+    }
+#pragma unroll
+    for (int k = 12; k < 19; ++k) w[k] = wave_sum(en[k]);
+    if (lane == 0)
+        for (int k = 0; k < 19; ++k) red[k * 4 + wid] = w[k];
+    __syncthreads();
     if (threadIdx.x == 0) {
         double *p = part + b;   // field-major
-        for (int k = 0; k < 19; ++k) p[(size_t)k * nwg] = en[k];
+        for (int k = 0; k < 19; ++k)
+            p[(size_t)k * nwg] = ((red[k * 4] + red[k * 4 + 1]) + red[k * 4 + 2]) + red[k * 4 + 3];
     }
 }
 

@@ -387,8 +387,8 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
                                                        const int *__restrict__ excl_off, const int *__restrict__ excl,
                                                        int cap, int *__restrict__ list, int2 *__restrict__ nneb,
                                                        int *__restrict__ flags, int nblk) {
-    __shared__ int rbase[kListAtoms][kListRanges];   // slot of candidate p is rbase[r] + p
-    __shared__ int rend[kListAtoms][kListRanges];    // candidates up to the end of range r
+    // range r of a member: {candidates up to its end, slot of candidate p in it minus p} (one LDS read)
+    __shared__ int2 rng[kListAtoms][kListRanges];
     const int blk = xcd_block(nblk);
     if (blk >= nblk) return;
     const int slot0 = blk * kListAtoms;
@@ -435,8 +435,7 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
     int T = 0;   // candidates of this member
     if (G.all_pairs) {
         if (sub == 0) {
-            rbase[grp][0] = slot_off;
-            rend[grp][0] = G.nmem;
+            rng[grp][0] = make_int2(G.nmem, slot_off);
         }
         T = valid ? G.nmem : 0;
     } else {
@@ -525,9 +524,9 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
                 for (int u = 0; u < n_grp[h]; ++u, ++k) {
                     const int r = pos[h]++;
                     if (r < kListRanges) {
-                        rbase[grp][r] = st[k] - cum[h];
+                        const int rb = st[k] - cum[h];
                         cum[h] += len[k];
-                        rend[grp][r] = cum[h];
+                        rng[grp][r] = make_int2(cum[h], rb);
                     }
                 }
             }
@@ -547,8 +546,9 @@ __global__ void __launch_bounds__(kBlock) k_build_list(const DevSys *__restrict_
         bool hit = false, inner = false;
         int j = -1;
         if (p < T) {
-            while (p >= rend[grp][r]) ++r;
-            j = rbase[grp][r] + p;
+            int2 e = rng[grp][r];
+            while (p >= e.x) e = rng[grp][++r];
+            j = e.y + p;
             const int4 pj = pos4[j];
             float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
             float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);

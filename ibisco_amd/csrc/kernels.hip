@@ -781,7 +781,6 @@ __device__ __forceinline__ void nb_block(const int bidx, const DevSys *__restric
             float4 v[kU];
 #pragma unroll
             for (int u = 0; u < kU; ++u) v[u] = nb_rows[S.nb_row_off + min(r0 + u * kBlock, nrow - 1)];
-            issue_here(nb_rows);
 #pragma unroll
             for (int u = 0; u < kU; ++u) lds_rows[min(r0 + u * kBlock, nrow - 1)] = v[u];   // same value twice at most
         }

@@ -117,6 +117,25 @@ def test_graph_equals_eager():
     assert np.array_equal(a.obs()["pt"], b.obs()["pt"])
 
 
+def test_one_force_launch_equals_separate_launches():
+    """The step loop's single force launch (nonbonded, bonded and the previous
+    step's field sums as roles of one kernel) against the same step with each
+    kernel class launched on its own (the timed path): the same bits."""
+    s = _sys_cg(dt=0.002)
+    a, b = Engine(s), Engine(s)
+    b.set_timing(True)
+    for e in (a, b):
+        e.step(13)
+        e.step(22, graph=e is a)
+    sa, va = a.state()
+    sb, vb = b.state()
+    assert np.array_equal(sa, sb) and np.array_equal(va, vb)
+    oa, ob = a.obs(), b.obs()
+    for k in ("vnb", "vbond", "vangle", "vtors", "pt"):
+        assert np.array_equal(oa[k], ob[k]), k
+    assert np.array_equal(a.averages(), b.averages())
+
+
 def test_back_run_after_large_displacement(oracle_mod):
     """The force kernel skips the back run of each list (pairs beyond rc + delta at the rebuild) only while
     every member has moved less than delta/2.  An equilibrated melt is run 30 steps without a rebuild, so

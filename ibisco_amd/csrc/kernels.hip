@@ -53,7 +53,6 @@ __device__ __forceinline__ double exact_r2(const double4 a, const double4 b, con
 // The pointers a load reads through are passed in (they are uniform), so that
 // restrict-qualified arguments do not exempt their loads.
 __device__ __forceinline__ void issue_here() { asm volatile("" ::: "memory"); }
-__device__ __forceinline__ void issue_here(const void *a) { asm volatile("" ::"s"(a) : "memory"); }
 __device__ __forceinline__ void issue_here(const void *a, const void *b, const void *c) {
     asm volatile("" ::"s"(a), "s"(b), "s"(c) : "memory");
 }
@@ -130,29 +129,6 @@ __device__ __forceinline__ void block_sum(double (&v)[N], double *lds /* N*4 */)
         for (int k = 0; k < N; ++k) v[k] = ((lds[k * 4] + lds[k * 4 + 1]) + lds[k * 4 + 2]) + lds[k * 4 + 3];
     }
     __syncthreads();
-}
-
-// The same for a block of W waves: the per-wave sums are added in wave order
-// by thread k for value k, and thread 0 gets the sums.
-template <int N, int W>
-__device__ __forceinline__ void block_sum_waves(double (&v)[N], double *lds /* N*W + N */) {
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-#pragma unroll
-    for (int k = 0; k < N; ++k) {
-        double x = wave_sum(v[k]);
-        if (lane == 0) lds[k * W + wid] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x < N) {
-        double x = lds[threadIdx.x * W];
-        for (int w = 1; w < W; ++w) x += lds[threadIdx.x * W + w];
-        lds[N * W + threadIdx.x] = x;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int k = 0; k < N; ++k) v[k] = lds[N * W + k];
-    }
 }
 
 constexpr int kMultiTabsLds(bool multi) { return multi ? kMaxNbTabs : 1; }

@@ -554,9 +554,10 @@ int build_device(ibx_ctx *c) {
             return rc;
         HIPCHK(c, G.mem_cell.alloc(slot));
         HIPCHK(c, G.cell_count.alloc(cell));
-        HIPCHK(c, G.cell_start.alloc(cell));
+        HIPCHK(c, memset_st(c, G.cell_count.p, 0, G.cell_count.bytes()));   // each scan clears it for the next rebuild
+        HIPCHK(c, G.cell_start.alloc(cell + 1));
         HIPCHK(c, G.cell_fill.alloc(cell));
-        HIPCHK(c, G.tile_sum.alloc((cell + kBlock - 1) / kBlock));
+        HIPCHK(c, G.tile_sum.alloc((cell + 1 + kBlock - 1) / kBlock));
         HIPCHK(c, G.tmp.alloc(slot));
         HIPCHK(c, G.slot_cell.alloc(slot));
         HIPCHK(c, G.r64.alloc(slot));
@@ -633,8 +634,8 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
                            (h.box[0] * h.box[1] * h.box[2]);
         const size_t cap = (size_t)pad_to((int)(std::max(1.5, h.list_margin) * nbr) + 64, 8);
         b += slots * (4 * 5 + 32 + 16 + 16 + 8)                  // mem mem_cell tmp perm slot_cell r64 pos4 pos4_ref nneb
-             + N * 4 + nwg * 4 + ncell * 16                       // inv_perm, wg_sys, cell arrays
-             + (ncell + kBlock - 1) / kBlock * 4                  // scan tile sums
+             + N * 4 + nwg * 4 + ncell * 16 + 4                   // inv_perm, wg_sys, cell arrays (one more start)
+             + (ncell + 1 + kBlock - 1) / kBlock * 4              // scan tile sums
              + 2 * nwg * (kBlock / kNbAtomsPerBlock) * kNbPart * 8 + slots * cap * 4;
     }
     return b;

@@ -152,11 +152,6 @@ __device__ __forceinline__ int pos_label(int w) { return (w >> 16) & 0xff; }
 // ------------------------------------------------------------------------------------------------
 // list rebuild (replaces LINKS.f90, MAPS.f90, NEW_NEIGHBOUR_WITHLIST.f90)
 
-__global__ void k_fill_int(int *p, int n, int v) {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
 // Squared displacement, in working units^2, between two fixed-point positions.
 __device__ __forceinline__ float fixed_d2(const int4 a, const int4 b, float rho1, float rho2) {
     const float dx = (float)(int)((uint32_t)a.x - (uint32_t)b.x);
@@ -209,16 +204,18 @@ __global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *__restrict_
     atomicAdd(&cell_count[G.cell_off + cell], 1);
 }
 
-// Exclusive scan of the cell counts into cell starts in two passes over tiles
-// of kBlock cells, so that every pass is spread over many workgroups (the
-// counts of a large grid take one workgroup most of its time just to read):
-// k_scan_tiles scans each tile and writes its total, k_scan_add adds the
-// totals of the tiles before each one.  Integer sums, so the order is free.
-__global__ void __launch_bounds__(kBlock) k_scan_tiles(const int *__restrict__ count, int *__restrict__ start,
+// Exclusive scan of the cell counts into cell starts (n + 1 of them: start[c + 1]
+// is where cell c ends) in two passes over tiles of kBlock cells, so that every
+// pass is spread over many workgroups (the counts of a large grid take one
+// workgroup most of its time just to read): k_scan_tiles scans each tile,
+// writes its total and clears the counts for the next rebuild, k_scan_add adds
+// the totals of the tiles before each one.  Integer sums, so the order is free.
+__global__ void __launch_bounds__(kBlock) k_scan_tiles(int *__restrict__ count, int *__restrict__ start,
                                                       int *__restrict__ fill, int *__restrict__ tile_sum, int n) {
     __shared__ int wtot[kBlock / kWave];
     const int i = blockIdx.x * kBlock + threadIdx.x, lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const int c = i < n ? count[i] : 0;
+    if (i < n) count[i] = 0;
     int x = c;   // inclusive scan over the wave
     for (int off = 1; off < kWave; off <<= 1) {
         const int y = __shfl_up(x, off, kWave);
@@ -228,10 +225,8 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(const int *__restrict__ c
     __syncthreads();
     int pre = 0;
     for (int w = 0; w < wid; ++w) pre += wtot[w];
-    if (i < n) {
-        start[i] = pre + x - c;
-        fill[i] = 0;
-    }
+    if (i <= n) start[i] = pre + x - c;
+    if (i < n) fill[i] = 0;
     if (threadIdx.x == kBlock - 1) tile_sum[blockIdx.x] = pre + x;
 }
 
@@ -247,7 +242,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_add(int *__restrict__ start, co
     int off = 0;
     for (int w = 0; w < kBlock / kWave; ++w) off += wtot[w];
     const int i = b * kBlock + threadIdx.x;
-    if (i < n) start[i] += off;
+    if (i <= n) start[i] += off;
 }
 
 __global__ void __launch_bounds__(kBlock) k_place(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, int g,
@@ -266,35 +261,37 @@ __global__ void __launch_bounds__(kBlock) k_place(const DevSys *__restrict__ sys
 
 // Orders each cell by member index so the sorted layout (and every sum over
 // it) is deterministic.  One workgroup per cell.
+constexpr int kSortLanes = 16;   // lanes per cell in the cell sort
+
 __global__ void __launch_bounds__(kBlock) k_sort_cells(const int *__restrict__ cell_sys, const DevSys *__restrict__ sys, int g,
-                                                       const int *__restrict__ cell_start, const int *__restrict__ cell_count,
+                                                       const int *__restrict__ cell_start,
                                                        const int *__restrict__ tmp, const int *__restrict__ mem,
                                                        int *__restrict__ perm, int *__restrict__ slot_cell,
                                                        int *__restrict__ flags, int ncell) {
-    // one wave per cell (cells hold a few members each), kBlock / kWave cells per workgroup
-    constexpr int kCap = 1024;
-    __shared__ int vals[kBlock / kWave][kCap];
-    const int w = threadIdx.x / kWave, lane = threadIdx.x & (kWave - 1);
-    const int gc = blockIdx.x * (kBlock / kWave) + w;
+    // kSortLanes lanes per cell (cells hold a few members each), kBlock / kSortLanes cells per workgroup
+    constexpr int kCap = 256;
+    __shared__ int vals[kBlock / kSortLanes][kCap];
+    const int w = threadIdx.x / kSortLanes, lane = threadIdx.x & (kSortLanes - 1);
+    const int gc = blockIdx.x * (kBlock / kSortLanes) + w;
     int n = 0, base = 0, lc = 0;
     if (gc < ncell) {
         const DevSys &S = sys[cell_sys[gc]];
         const DevGroup &G = S.g[g];
-        n = cell_count[gc];
+        n = cell_start[gc + 1] - cell_start[gc];
         base = G.slot_off + cell_start[gc] - cell_start[G.cell_off];
         lc = gc - G.cell_off;
     }
     if (n > kCap) {   // too crowded to sort in LDS: keep arrival order, flag it
         if (lane == 0) atomicMax(&flags[2], n);
-        for (int i = lane; i < n; i += kWave) {
+        for (int i = lane; i < n; i += kSortLanes) {
             perm[base + i] = mem[tmp[base + i]];
             slot_cell[base + i] = lc;
         }
         n = 0;
     }
-    for (int i = lane; i < n; i += kWave) vals[w][i] = tmp[base + i];
+    for (int i = lane; i < n; i += kSortLanes) vals[w][i] = tmp[base + i];
     __syncthreads();
-    for (int i = lane; i < n; i += kWave) {
+    for (int i = lane; i < n; i += kSortLanes) {
         const int v = vals[w][i];
         int rank = 0;
         for (int k = 0; k < n; ++k) rank += vals[w][k] < v;
@@ -362,7 +359,6 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
                                                        const int *__restrict__ inv_perm,
                                                        const int4 *__restrict__ pos4, const double4 *__restrict__ r64,
                                                        const int *__restrict__ cell_start,
-                                                       const int *__restrict__ cell_count,
                                                        const int *__restrict__ excl_off, const int *__restrict__ excl,
                                                        int cap, int *__restrict__ list, int2 *__restrict__ nneb,
                                                        int *__restrict__ flags, int nblk) {
@@ -467,7 +463,7 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
             else { za[0] = z0; zb[0] = z1; nz = 1; }
             for (int k = 0; k < nz; ++k) {
                 const int b = cell_start[col + za[k]];
-                const int e = cell_start[col + zb[k]] + cell_count[col + zb[k]];
+                const int e = cell_start[col + zb[k] + 1];
                 st[np] = slot_off + b - cs0;
                 len[np] = e - b;
                 ++np;
@@ -1704,33 +1700,29 @@ void launch_tune_skin(hipStream_t st, DevSys *sys, int nsys, const int *dmax, in
                        tune);
 }
 
-void launch_zero_int(hipStream_t st, int *p, int n) {
-    if (n > 0) hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, st, p, n, 0);
-}
 
 void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
-    launch_zero_int(st, a.cell_count, a.ncell_total);
     hipLaunchKernelGGL(k_cell_count, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.mem, a.s, a.nitems_total,
                        a.mem_cell, a.cell_count);
-    const int ntile = (a.ncell_total + kBlock - 1) / kBlock;
+    const int ntile = (a.ncell_total + 1 + kBlock - 1) / kBlock;   // the starts have one more entry
     hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(kBlock), 0, st, a.cell_count, a.cell_start, a.cell_fill,
                        a.tile_sum, a.ncell_total);
     hipLaunchKernelGGL(k_scan_add, dim3(ntile), dim3(kBlock), 0, st, a.cell_start, a.tile_sum, a.ncell_total);
     hipLaunchKernelGGL(k_place, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.mem, a.mem_cell, a.cell_start,
                        a.cell_fill, a.tmp);
-    hipLaunchKernelGGL(k_sort_cells, dim3((a.ncell_total + kBlock / kWave - 1) / (kBlock / kWave)), dim3(kBlock), 0, st,
+    hipLaunchKernelGGL(k_sort_cells, dim3((a.ncell_total + kBlock / kSortLanes - 1) / (kBlock / kSortLanes)), dim3(kBlock), 0, st,
                        a.cell_sys, a.sys, a.g, a.cell_start,
-                       a.cell_count, a.tmp, a.mem, a.perm, a.slot_cell, a.flags, a.ncell_total);
+                       a.tmp, a.mem, a.perm, a.slot_cell, a.flags, a.ncell_total);
     hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
                        a.itype, a.label, a.r64, a.pos4, a.inv_perm, a.pos4_ref, a.dmax, a.nsys, 0);
     const int nwg_list = a.nwg * (kBlock / kListAtoms);
     if (a.cubic)
         hipLaunchKernelGGL(k_build_list<true>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
-                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap,
+                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.excl_off, a.excl, a.cap,
                            a.list, a.nneb, a.flags, nwg_list);
     else
         hipLaunchKernelGGL(k_build_list<false>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
-                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.cell_count, a.excl_off, a.excl, a.cap,
+                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.excl_off, a.excl, a.cap,
                            a.list, a.nneb, a.flags, nwg_list);
 }
 

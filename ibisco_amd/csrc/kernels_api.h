@@ -93,7 +93,6 @@ struct FinLaunch {
     int mode, nsys;
 };
 
-void launch_zero_int(hipStream_t st, int *p, int n);
 void launch_mts(hipStream_t st, bool save, float *fnb, const int *label, int n, double *part, int npart,
                 float *mts_f, double *mts_part, int m, double I);
 void launch_tune_skin(hipStream_t st, DevSys *sys, int nsys, const int *dmax, int nmove, int tune);

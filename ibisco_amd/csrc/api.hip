@@ -114,9 +114,11 @@ struct ibx_ctx {
     DBuf<double> mts_part;           // [3 slots][9 fields][workgroups] bead-group energy/virial partials
     double rdf_dr = 0;
     // graph
-    hipGraphExec_t gexec = nullptr;
-    int graph_steps = 0;
-    int graph_par = 0, graph_bd_par = 0;   // nb_par, bd_par when the graph was captured (and after each replay)
+    struct GraphSlot {
+        hipGraphExec_t exec = nullptr;
+        int steps = 0;
+        int par = 0, bd_par = 0;   // nb_par, bd_par when the graph was captured (and after each replay)
+    } graph[2];                    // one block period; several periods (fewer launches per step)
     // timing
     bool timing = false;
     Timing tm;
@@ -124,7 +126,8 @@ struct ibx_ctx {
     std::string err;
     ~ibx_ctx() {
         std::lock_guard<std::mutex> lk(g_capture_mu);
-        if (gexec) (void)hipGraphExecDestroy(gexec);
+        for (auto &g : graph)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
         for (auto e : evpool) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -167,10 +170,12 @@ int block_period(const ibx_ctx *c) {
 }
 
 void drop_graph(ibx_ctx *c) {
-    if (!c->gexec) return;
     std::lock_guard<std::mutex> lk(g_capture_mu);
-    (void)hipGraphExecDestroy(c->gexec);
-    c->gexec = nullptr;
+    for (auto &g : c->graph) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        g.exec = nullptr;
+        g.steps = 0;
+    }
 }
 
 // Copies and fills go through the context's own (non-blocking) stream, never
@@ -1009,43 +1014,56 @@ int ibx_step(ibx_ctx *c, int nsteps, int step0) {
     return rc;
 }
 
+// nblk replays of graph slot `slot`, which holds `steps` steps from a step that starts a block period
+// (captured on first use, or again when the partial buffers it expects have changed).
+int run_graph_blocks(ibx_ctx *c, int slot, int steps, int nblk, int cur) {
+    ibx_ctx::GraphSlot &g = c->graph[slot];
+    // a block starts by reading the partial buffers the capture found latest
+    if (!g.exec || g.steps != steps || g.par != c->nb_par || g.bd_par != c->bd_par) {
+        if (g.exec) {
+            std::lock_guard<std::mutex> lk(g_capture_mu);
+            (void)hipGraphExecDestroy(g.exec);
+            g.exec = nullptr;
+        }
+        std::lock_guard<std::mutex> lk(g_capture_mu);
+        hipGraph_t graph;
+        // relaxed: other host threads (other contexts, the serving loop) keep allocating and copying
+        // while this one captures; nothing captured here is a call that capture forbids
+        HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
+        for (int k = 0; k < steps; ++k) enqueue_step(c, steps + k, nullptr);   // block starts at a rebuild
+        flush_fin(c);
+        HIPCHK(c, hipStreamEndCapture(c->st, &graph));
+        hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) return fail(c, IBX_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+        g.steps = steps;
+        g.par = c->nb_par;
+        g.bd_par = c->bd_par;
+    }
+    HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
+    HIPCHK(c, hipMemcpyAsync(c->step.p, &cur, sizeof(int), hipMemcpyHostToDevice, c->st));
+    for (int b = 0; b < nblk; ++b) HIPCHK(c, hipGraphLaunch(g.exec, c->st));
+    return check_flags(c);
+}
+
 int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
     if (!c || nsteps < 0) return fail(c, IBX_E_INVALID, "invalid arguments");
     // eager head until a step that starts a list interval (and a multiple-time-step cycle), graph
-    // blocks of that period, eager tail
+    // blocks of several periods, then of one period, eager tail
+    constexpr int kGraphSteps = 40;   // steps per long block (each block ends with its own field sums)
     const int period = block_period(c);
+    const int longer = period * std::max(1, kGraphSteps / period);
     int head = 0;
     while (head < nsteps && (step0 + 1 + head) % period != 0) ++head;
     int rc = ibx_step(c, head, step0);
     if (rc) return rc;
     int done = head;
-    int nblk = (nsteps - done) / period;
-    if (nblk > 0) {
-        // a block starts by reading the partial buffer the capture found latest
-        if (!c->gexec || c->graph_steps != period || c->graph_par != c->nb_par || c->graph_bd_par != c->bd_par) {
-            drop_graph(c);
-            std::lock_guard<std::mutex> lk(g_capture_mu);
-            hipGraph_t graph;
-            // relaxed: other host threads (other contexts, the serving loop) keep allocating and copying
-            // while this one captures; nothing captured here is a call that capture forbids
-            HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
-            for (int k = 0; k < period; ++k) enqueue_step(c, period + k, nullptr);   // block starts at a rebuild
-            flush_fin(c);
-            HIPCHK(c, hipStreamEndCapture(c->st, &graph));
-            hipError_t e = hipGraphInstantiate(&c->gexec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            if (e != hipSuccess) return fail(c, IBX_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
-            c->graph_steps = period;
-            c->graph_par = c->nb_par;
-            c->graph_bd_par = c->bd_par;
-        }
-        HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
-        int cur = step0 + done;
-        HIPCHK(c, hipMemcpyAsync(c->step.p, &cur, sizeof(int), hipMemcpyHostToDevice, c->st));
-        for (int b = 0; b < nblk; ++b) HIPCHK(c, hipGraphLaunch(c->gexec, c->st));
-        rc = check_flags(c);
-        if (rc) return rc;
-        done += nblk * period;
+    for (int slot = 1; slot >= 0; --slot) {
+        const int steps = slot ? longer : period;
+        const int nblk = (nsteps - done) / steps;
+        if (nblk == 0) continue;
+        if ((rc = run_graph_blocks(c, slot, steps, nblk, step0 + done))) return rc;
+        done += nblk * steps;
     }
     return ibx_step(c, nsteps - done, step0 + done);
 }

@@ -105,11 +105,23 @@ def test_batch_equals_alone():
         assert np.array_equal(oa["vnb"], ob["vnb"]) and np.array_equal(oa["pt"], ob["pt"]), k
 
 
-def test_graph_equals_eager():
+@pytest.mark.parametrize("calls", [(35,), (64, 64), (139,)])
+def test_graph_equals_eager(calls):
+    """Graph replays (blocks of one list interval and of four, eager head and tail) against eager steps:
+    35 steps use the one-interval block only; 64 + 64 and 139 use both, the second call reusing them."""
+    import copy
+    from ibisco_amd import units, workloads
     s = _sys_cg(dt=0.002)
-    a, b = Engine(s), Engine(s)
-    a.step(35)
-    b.step(35, graph=True)
+    pre = Engine(s)                     # equilibrated start: the raw melt heats up within ~60 steps
+    workloads.equilibrate(pre, s, stage_steps=100, final_steps=200)
+    t = copy.deepcopy(s)
+    t.s, v = pre.state()
+    t.v = units.vel_from_reduced(v)
+    pre.close()
+    a, b = Engine(t), Engine(t)
+    a.step(sum(calls))
+    for n in calls:
+        b.step(n, graph=True)
     sa, va = a.state()
     sb, vb = b.state()
     assert np.array_equal(sa, sb)

@@ -660,6 +660,7 @@ RebuildArgs rebuild_args(ibx_ctx *c, int g) {
     a.cell_fill = G.cell_fill.p; a.tile_sum = G.tile_sum.p; a.tmp = G.tmp.p; a.perm = G.perm.p; a.slot_cell = G.slot_cell.p;
     a.r64 = G.r64.p; a.pos4 = G.pos4.p; a.inv_perm = G.inv_perm.p; a.pos4_ref = G.pos4_ref.p;
     a.dmax = c->dmax.p + (size_t)g * c->nsys; a.nsys = c->nsys;
+    a.tune = -1; a.nmove = c->nupdate; a.tune_sys = c->sys.p; a.dmax_all = c->dmax.p;
     a.excl_off = c->excl_off.p; a.excl = c->excl.p; a.list = G.list.p; a.nneb = G.nneb.p; a.flags = c->flags.p;
     return a;
 }
@@ -667,9 +668,13 @@ RebuildArgs rebuild_args(ibx_ctx *c, int g) {
 // tune: a rebuild of the step schedule, after a full interval of moves (the
 // list-skin autotuner reads the displacement that interval reached)
 void enqueue_rebuild(ibx_ctx *c, bool tune = false) {
-    if (tune) launch_tune_skin(c->st, c->sys.p, c->nsys, c->dmax.p, c->nupdate, c->skin_tune ? 1 : 0);
-    for (int g = 0; g < 2; ++g)
-        if (c->gr[g].present) launch_rebuild(c->st, rebuild_args(c, g));
+    for (int g = 0; g < 2; ++g) {
+        if (!c->gr[g].present) continue;
+        RebuildArgs a = rebuild_args(c, g);
+        if (tune) a.tune = c->skin_tune ? 1 : 0;   // with the first group's cell count, for both groups
+        tune = false;
+        launch_rebuild(c->st, a);
+    }
 }
 
 void enqueue_rdf(ibx_ctx *c) {

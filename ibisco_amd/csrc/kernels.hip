@@ -182,11 +182,21 @@ __device__ __forceinline__ void block_max_to(float x, int *dst, float *lds /* kB
     __syncthreads();
 }
 
+__device__ void tune_skin(DevSys *sys, int nsys, const int *dmax, int nmove, int tune, int t);
+
 // Cell of every member from its folded FP64 position (LINKS.f90:16-18, clamped).
-__global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
+// Workgroups past nwg run the list-skin tuner instead (one thread per system
+// and group): it writes each group's inner radius and displacement bound, which
+// the cell count does not read, so the two share a launch.
+__global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *sys, const int *__restrict__ wg_sys, int nwg,
                                                        int g, const int *__restrict__ mem, const double *__restrict__ s,
                                                        int nitems_total, int *__restrict__ mem_cell,
-                                                       int *__restrict__ cell_count) {
+                                                       int *__restrict__ cell_count, DevSys *tune_sys, int nsys,
+                                                       const int *__restrict__ dmax_all, int nmove, int tune) {
+    if ((int)blockIdx.x >= nwg) {
+        tune_skin(tune_sys, nsys, dmax_all, nmove, tune, ((int)blockIdx.x - nwg) * kBlock + threadIdx.x);
+        return;
+    }
     const DevSys &S = sys[wg_sys[blockIdx.x]];
     const DevGroup &G = S.g[g];
     int m = blockIdx.x * kBlock + threadIdx.x;
@@ -1607,8 +1617,7 @@ __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, 
 // system and group: the choice depends only on the system's own trajectory, so
 // a system tunes the same alone and in a batch.  tune == 0 restores the fixed
 // delta = kInnerFrac (rlist - rc).  Runs only where a list is about to be built.
-__global__ void k_tune_skin(DevSys *__restrict__ sys, int nsys, const int *__restrict__ dmax, int nmove, int tune) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ void tune_skin(DevSys *sys, int nsys, const int *dmax, int nmove, int tune, int t) {
     if (t >= 2 * nsys) return;
     const int g = t / nsys, s_id = t - g * nsys;
     DevSys &S = sys[s_id];
@@ -1695,15 +1704,11 @@ void launch_mts(hipStream_t st, bool save, float *fnb, const int *label, int n, 
         hipLaunchKernelGGL(k_mts_approx, grid, dim3(kBlock), 0, st, fnb, label, n, part, npart, mts_f, mts_part, I);
 }
 
-void launch_tune_skin(hipStream_t st, DevSys *sys, int nsys, const int *dmax, int nmove, int tune) {
-    hipLaunchKernelGGL(k_tune_skin, dim3((2 * nsys + kWave - 1) / kWave), dim3(kWave), 0, st, sys, nsys, dmax, nmove,
-                       tune);
-}
-
 
 void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
-    hipLaunchKernelGGL(k_cell_count, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.mem, a.s, a.nitems_total,
-                       a.mem_cell, a.cell_count);
+    const int ntune = a.tune >= 0 ? (2 * a.nsys + kBlock - 1) / kBlock : 0;
+    hipLaunchKernelGGL(k_cell_count, dim3(a.nwg + ntune), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.nwg, a.g, a.mem, a.s,
+                       a.nitems_total, a.mem_cell, a.cell_count, a.tune_sys, a.nsys, a.dmax_all, a.nmove, a.tune);
     const int ntile = (a.ncell_total + 1 + kBlock - 1) / kBlock;   // the starts have one more entry
     hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(kBlock), 0, st, a.cell_count, a.cell_start, a.cell_fill,
                        a.tile_sum, a.ncell_total);

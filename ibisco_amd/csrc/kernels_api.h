@@ -22,6 +22,11 @@ struct RebuildArgs {
     int *inv_perm;
     int *dmax;          // this group's per-system largest displacement^2 since the rebuild (float bits)
     int nsys;
+    // list-skin tuning in the same launch as the cell count (-1: none; 0: restore the fixed inner
+    // radius; 1: tune), from every group's displacement over nmove moves
+    int tune, nmove;
+    DevSys *tune_sys;
+    const int *dmax_all;
     const int *excl_off, *excl;
     int *list;
     int2 *nneb;
@@ -95,7 +100,6 @@ struct FinLaunch {
 
 void launch_mts(hipStream_t st, bool save, float *fnb, const int *label, int n, double *part, int npart,
                 float *mts_f, double *mts_part, int m, double I);
-void launch_tune_skin(hipStream_t st, DevSys *sys, int nsys, const int *dmax, int nmove, int tune);
 void launch_rebuild(hipStream_t st, const RebuildArgs &a);
 void launch_pack_only(hipStream_t st, const RebuildArgs &a);
 void launch_nonbonded(hipStream_t st, const NbArgs &a);

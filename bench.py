@@ -66,7 +66,7 @@ def cpu_baseline(sysm, nsteps, threads):
 
 def replicas_compare(nrep, steps, warmup, seed=3):
     """CG melt replicas (log-uniform 2k-50k beads, each with its own corrected table): atom-steps/s of all
-    replicas in one context (every kernel launch covers all of them, HIP graph per list interval) against
+    replicas in one context (every kernel launch covers all of them, HIP graphs of four list intervals) against
     the plain path, each replica alone in its own context, one after the other, from the same equilibrated
     states.  RDFs accumulate on the device at every list rebuild in the batched run."""
     import copy
@@ -395,7 +395,7 @@ def main():
                    "rc_nm": 1.0, "rlist_nm": 1.3, "nupdate": 10, "dt_fs": 1.0, "ensemble": "NVE",
                    "time_stepping": "single dt (the reference has no multiple time step for non-hybrid systems)",
                    "path": "one system per launch, " + ("eager launches" if args.eager else
-                                                         "HIP graph per list interval"),
+                                                         "HIP graphs of four list intervals"),
                    "precision": "FP32 forces, FP64 integration and reductions"},
         "step_breakdown_ms": {k: round(tm[k] / tm["nsteps_timed"], 5) for k in
                               ("nonbonded", "bonded", "integrate", "rebuild", "step")},

@@ -97,8 +97,10 @@ const char *ibx_last_error(const ibx_ctx *ctx);   /* ctx may be NULL: last creat
  * the reference when mts_quirk is set (else I = 3+nmts). */
 int ibx_step(ibx_ctx *ctx, int nsteps, int step0);
 
-/* The same steps as ibx_step, launched as a captured HIP graph replayed per
- * list interval (batched replicas and serving use this path). */
+/* The same steps as ibx_step, bit for bit, launched as captured HIP graphs:
+ * eager steps up to the start of a list interval, graphs of four intervals,
+ * of one, then eager steps for the rest (batched replicas and serving use
+ * this path). */
 int ibx_step_graph(ibx_ctx *ctx, int nsteps, int step0);
 
 /* NEW_FORCE.F90:7-83 at the current positions without moving: fold, virtual

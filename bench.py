@@ -1,27 +1,33 @@
 """Throughput of the per-step force + integration path on one MI355X.
 
-Default workload: the united-atom melt (400 chains x 250 CH2 = 100,000 sites,
-LJ 0.401 nm / 0.39 kJ/mol to 1.0 nm, harmonic bonds and angles, Ryckaert-
-Bellemans torsion table, 450 K, 1 fs, list 1.3 nm rebuilt every 10 steps),
-synthetic and equilibrated on the device before timing.  One "step" is one
-full NEW_LOOP step (list rebuild when due, nonbonded, bonded, leap-frog,
-observables) advanced through HIP graphs.
+Headline workload: 256 independent CG melt replicas (log-uniform 2,000-50,000
+beads, chain length 20-100, each with its own Gaussian-corrected LJ table,
+8 beads/nm^3, rc 1.5 nm, list 1.8 nm rebuilt every 10 steps, 5 fs, NVE, RDF
+accumulated on the device at every rebuild), synthetic and equilibrated on the
+device before timing, all advanced together by HIP graph replays of one
+batched context.  One "step" is one full NEW_LOOP step of every replica.  The
+plain path (each replica alone in its own context, one after the other, same
+protocol) is timed beside it, and the ratio reported.
+
+The warm-up captures and instantiates every graph the timed region replays:
+after the --warmup steps, the --steps steps run once untimed, followed by the
+eager steps that return to the same phase of the graph block period, so the
+timed --steps steps replay exactly those graphs (the counters prove it).
 
 Prints one JSON line: whole-job atom-steps/s, the nonbonded kernel's
-algorithmic-byte roofline fraction (HIP events on the library's stream), and
-the reference CPU path (oracle/_ref, the reference's own routines built from
-source) timed on this host's cores on a bounded sample of the same workload.
+algorithmic-byte roofline fraction (HIP events on the library's stream), the
+force/energy/pressure error of the device against the FP64 oracle on sampled
+replicas, and the FP32 OpenMP restatement of the reference loop (oracle/
+cpu_port.c) timed on this host's cores on a bounded sample of the replicas.
+Secondary records: the served Poisson stream, the 100k-site united-atom melt
+(with its multiple time step) and the 500k-atom hybrid system.
 """
 from __future__ import annotations
 
 import argparse
 import json
 import os
-import resource
-import shutil
-import subprocess
 import sys
-import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -31,84 +37,182 @@ METRIC = "atom-steps/s on 1 GPU, batched/served vs plain, at ≤1e-5 rel RMS for
 HBM_PEAK_GBS = 8000.0
 
 
-def _stack_unlimited():
+def steady_run(e, steps, warmup, graph=True):
+    """Warm-up, then the timed ``steps``.  With graphs the warm-up ends with one untimed pass of the same
+    ``steps`` and the eager steps that return to its phase in the block period, so the timed pass replays
+    graphs captured before it.  Returns (seconds, counters of the timed pass)."""
+    e.step(warmup, graph=graph)
+    if graph:
+        period = e.counters()["period"]
+        e.step(steps, graph=True)
+        back = (-steps) % period
+        if back:
+            e.step(back)
+    e.obs()   # synchronises
+    c0 = e.counters()
+    t0 = time.perf_counter()
+    e.step(steps, graph=graph)
+    e.obs()
+    secs = time.perf_counter() - t0
+    c1 = e.counters()
+    tc = {k: c1[k] - c0[k] for k in ("eager_steps", "graph_steps", "graph_launches", "captures")}
+    tc["period"], tc["long_block"] = c1["period"], c1["long_block"]
+    return secs, tc
+
+
+def cpu_model():
     try:
-        resource.setrlimit(resource.RLIMIT_STACK, (resource.RLIM_INFINITY, resource.RLIM_INFINITY))
-    except (ValueError, OSError):
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
         pass
+    return "unknown"
 
 
-def cpu_baseline(sysm, nsteps, threads):
-    """Times the reference's own routines (oracle/_ref/ref_harness) on this host."""
-    exe = os.path.join(ROOT, "oracle", "_ref", "ref_harness")
-    if not os.path.exists(exe):
-        return None
-    from ibisco_amd import system
-    d = tempfile.mkdtemp(prefix="ibx_ref_")
-    try:
-        system.write_dir(sysm, d)
-        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores",
-                   OMP_STACKSIZE="256M")
-        t0 = time.time()
-        r = subprocess.run([exe, "time", str(nsteps)], cwd=d, env=env, capture_output=True, text=True,
-                           preexec_fn=_stack_unlimited, timeout=600)
-        wall = time.time() - t0
-        line = [x for x in r.stdout.splitlines() if x.startswith("LOOP_SECONDS")]
-        if r.returncode != 0 or not line:
-            return {"error": f"reference harness rc={r.returncode}: {r.stderr[-300:]}"}
-        secs = float(line[0].split()[1])
-        return {"value": sysm.natoms * nsteps / secs, "unit": "atom-steps/s", "cores": threads, "kind": "reference",
-                "sample": f"{nsteps} steps of the same {sysm.natoms}-site system (list rebuilt at step 10), "
-                          f"loop {secs:.2f} s, wall incl. set-up {wall:.1f} s, OpenMP FP32 reference routines"}
-    finally:
-        shutil.rmtree(d, ignore_errors=True)
+def host_threads():
+    """Threads for the OpenMP baseline: the CPUs this process may run on, capped by OMP_NUM_THREADS."""
+    n = len(os.sched_getaffinity(0))
+    env = os.environ.get("OMP_NUM_THREADS")
+    if env and env.isdigit():
+        n = min(n, int(env))
+    return max(1, n)
 
 
-def replicas_compare(nrep, steps, warmup, seed=3):
-    """CG melt replicas (log-uniform 2k-50k beads, each with its own corrected table): atom-steps/s of all
-    replicas in one context (every kernel launch covers all of them, HIP graphs of four list intervals) against
-    the plain path, each replica alone in its own context, one after the other, from the same equilibrated
-    states.  RDFs accumulate on the device at every list rebuild in the batched run."""
+def cpu_baseline(systems, states, target_s=4.0, runs=3):
+    """The FP32 OpenMP restatement of the reference loop (oracle/cpu_port.c) on the sampled replicas, run one
+    after the other from their equilibrated states; rate = sum of beads x steps / sum of loop seconds, median
+    of ``runs`` runs, steps chosen so one run takes about ``target_s`` seconds."""
+    import copy
+    from ibisco_amd import units
+    threads = host_threads()
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
+    from oracle.oracle import CpuPort
+
+    def one_run(nsteps):
+        secs, work = 0.0, 0
+        for sy, (s, v) in zip(systems, states):
+            one = copy.copy(sy)
+            one.s, one.v = s[:sy.natoms].copy(), units.vel_from_reduced(v)
+            p = CpuPort(one)
+            secs += p.run(nsteps, 0)
+            work += sy.natoms * nsteps
+            del p
+        return secs, work
+
+    secs, work = one_run(10)
+    nupdate = systems[0].control.nupdate
+    nsteps = max(2 * nupdate, int(round(target_s / max(secs, 1e-6) * 10 / nupdate)) * nupdate)
+    nsteps = min(nsteps, 2000)
+    rates = []
+    for _ in range(runs):
+        secs, work = one_run(nsteps)
+        rates.append(work / secs)
+    rates.sort()
+    return {"value": rates[len(rates) // 2], "unit": "atom-steps/s", "cores": CpuPort.threads(), "kind": "port",
+            "sample": f"{len(systems)} of the replicas ({sum(sy.natoms for sy in systems)} beads) one after the "
+                      f"other, {nsteps} steps each from their equilibrated states (list rebuilt every {nupdate}), "
+                      f"median of {runs} runs (rates {', '.join(f'{r:.3e}' for r in rates)}); FP32 OpenMP "
+                      f"restatement of the reference loop, gcc -O2 -fopenmp",
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "omp_threads": threads}
+
+
+def accuracy(e, systems, idx):
+    """Device forces, energies and virial of the sampled systems of context ``e`` at their current positions
+    against the FP64 oracle on the same positions (each with a fresh list)."""
+    import copy
+    import numpy as np
+    from ibisco_amd import units
+    from oracle.oracle import Oracle
+    e.compute_forces(rebuild=True)
+    worst = {"force_rel_rms": 0.0, "energy_rel": 0.0, "virial_rel": 0.0, "pressure_rel": 0.0}
+    for k in idx:
+        sy = systems[k]
+        s, v = e.state(k)
+        one = copy.copy(sy)
+        one.s, one.v = s[:sy.natoms].copy(), units.vel_from_reduced(v)
+        o = Oracle(one)
+        o.update_lists()
+        o.new_force()
+        fo, fg = o.forces(), e.forces(k)
+        oo, og = o.obs(), e.obs(k)
+        eo = sum(float(oo[q].sum()) for q in ("vnb", "vbond", "vangle", "vtors", "voop"))
+        eg = sum(float(og[q].sum()) for q in ("vnb", "vbond", "vangle", "vtors", "voop"))
+        po, pg = oo["pt"], og["pt"]
+        tr = abs(po[:3].sum())
+        worst["force_rel_rms"] = max(worst["force_rel_rms"], float(np.linalg.norm(fg - fo) / np.linalg.norm(fo)))
+        worst["energy_rel"] = max(worst["energy_rel"], abs(eg - eo) / abs(eo))
+        worst["virial_rel"] = max(worst["virial_rel"], float(np.abs(pg - po).max()) / tr)
+        worst["pressure_rel"] = max(worst["pressure_rel"], abs(pg[:3].sum() - po[:3].sum()) / tr)
+    worst.update(systems_checked=len(idx), reference="FP64 oracle (oracle/ibx_oracle.c) on the same positions",
+                 gates={"force_rel_rms": 1e-5, "energy_rel": 1e-7, "virial_rel": 1e-7})
+    return worst
+
+
+def sample_indices(systems, n):
+    """n systems spread over the size range (by bead count)."""
+    order = sorted(range(len(systems)), key=lambda k: systems[k].natoms)
+    if n >= len(order):
+        return order
+    return [order[round(q * (len(order) - 1) / (n - 1))] for q in range(n)]
+
+
+def batched_leg(args, seed=3, with_plain=True):
+    """Config 3: the replicas in one context (every kernel launch covers all of them), then each alone."""
     import copy
     from ibisco_amd import units, workloads
     from ibisco_amd.engine import Engine
 
-    systems = workloads.cg_replicas(nrep=nrep, seed=seed)
+    systems = workloads.cg_replicas(nrep=args.replicas, seed=seed)
     beads = [s.natoms for s in systems]
     total = sum(beads)
-    print(f"[bench] {nrep} replicas, {total} beads (min {min(beads)}, max {max(beads)})", file=sys.stderr, flush=True)
+    print(f"[bench] {len(systems)} replicas, {total} beads (min {min(beads)}, max {max(beads)})", file=sys.stderr,
+          flush=True)
     eb = Engine(systems)
     workloads.equilibrate(eb, systems, stage_steps=100, final_steps=400)
     print("[bench] replicas equilibrated", file=sys.stderr, flush=True)
     eb.rdf_enable(150, 0.01, every=1)
-    eb.step(warmup, graph=True)
-    eb.obs()
-    t0 = time.perf_counter()
-    eb.step(steps, graph=True)
-    eb.obs()
-    tb = time.perf_counter() - t0
-    states = [eb.state(k) for k in range(nrep)]
+    secs, tc = steady_run(eb, args.steps, args.warmup, graph=not args.eager)
+    value = total * args.steps / secs
+    print(f"[bench] batched: {args.steps} steps in {secs:.4f} s ({value:.3e} atom-steps/s), timed counters {tc}",
+          file=sys.stderr, flush=True)
+    states = [eb.state(k) for k in range(len(systems))]
+    # nonbonded kernel: HIP events on the library stream over eager steps (kernel classes launched apart)
+    nt = 20
+    eb.set_timing(True)
+    eb.step(nt)
+    tm = eb.timing()
+    eb.set_timing(False)
+    nb_ms = tm["nonbonded"] / tm["nsteps_timed"]
+    prl = sum(eb.pair_count(k) for k in range(len(systems)))   # half-list pairs of the last rebuild
+    idx = sample_indices(systems, args.check)
+    acc = accuracy(eb, systems, idx)
+    print(f"[bench] batched: accuracy {acc}", file=sys.stderr, flush=True)
     eb.close()
+    # the plain path: each replica alone, from the same state, same protocol
     tp = 0.0
-    for k, sy in enumerate(systems):
+    plain_tc = {"captures": 0, "eager_steps": 0, "graph_steps": 0}
+    for k, sy in enumerate(systems if with_plain else []):
         one = copy.copy(sy)
         one.s, one.v = states[k][0][:sy.natoms].copy(), units.vel_from_reduced(states[k][1])
         e = Engine(one)
-        e.step(warmup, graph=True)
-        e.obs()
-        t0 = time.perf_counter()
-        e.step(steps, graph=True)
-        e.obs()
-        tp += time.perf_counter() - t0
+        e.rdf_enable(150, 0.01, every=1)
+        t, c = steady_run(e, args.steps, args.warmup, graph=not args.eager)
+        tp += t
+        for q in plain_tc:
+            plain_tc[q] += c[q]
         e.close()
-        if (k + 1) % 32 == 0:
-            print(f"[bench] plain path: {k + 1}/{nrep} replicas", file=sys.stderr, flush=True)
-    vb, vp = total * steps / tb, total * steps / tp
-    print(f"[bench] replicas: batched {vb:.3e}, plain {vp:.3e} atom-steps/s", file=sys.stderr, flush=True)
-    return {"workload": "cg_replicas_corrected_tables", "replicas": nrep, "beads": total, "steps": steps,
-            "batched": {"value": vb, "ms_per_step": 1e3 * tb / steps},
-            "plain": {"value": vp, "ms_per_step": 1e3 * tp / steps, "path": "one context per replica, in turn"},
-            "ratio": vb / vp}
+        if (k + 1) % 64 == 0:
+            print(f"[bench] plain path: {k + 1}/{len(systems)} replicas", file=sys.stderr, flush=True)
+    vp = total * args.steps / tp if tp > 0 else None
+    if vp:
+        print(f"[bench] replicas: batched {value:.3e}, plain {vp:.3e} atom-steps/s", file=sys.stderr, flush=True)
+    return dict(systems=systems, states=states, total=total, secs=secs, value=value, timed_counters=tc, tm=tm,
+                nb_ms=nb_ms, prl=prl, accuracy=acc, plain_value=vp, plain_secs=tp, plain_counters=plain_tc,
+                check=idx)
 
 
 def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=3, nsample=12, ncheck=4, step_band=1.5):
@@ -251,20 +355,59 @@ def hybrid_run(steps, warmup, seed=7):
             "temperature_K": float(ob["temp"] * 300.0)}
 
 
-TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles", "k_nonbonded_ua100k_traffic.md")
+def ua_leg(args):
+    """Config 1 (secondary record): the 100k-site united-atom melt alone, single 1 fs steps, then bonded 1 fs /
+    nonbonded 4 fs (RESPA), with its nonbonded kernel's roofline."""
+    from ibisco_amd import workloads
+    from ibisco_amd.engine import Engine
+    sysm = workloads.ua_melt_100k()
+    n = sysm.natoms
+    print(f"[bench] UA melt: {n} sites, box {sysm.box[0]:.3f} nm", file=sys.stderr, flush=True)
+    e = Engine(sysm)
+    workloads.equilibrate(e, sysm, stage_steps=200, final_steps=1000)
+    secs, tc = steady_run(e, args.ua_steps, args.warmup, graph=not args.eager)
+    e.set_respa(4)
+    nr = max(80, args.ua_steps // 40 * 40)
+    tr, _ = steady_run(e, nr, 40, graph=not args.eager)
+    e.set_respa(1)
+    e.set_timing(True)
+    e.step(50)
+    tm = e.timing()
+    e.set_timing(False)
+    nb_ms = tm["nonbonded"] / tm["nsteps_timed"]
+    b_nb = 4.0 * e.pair_count() / n + 28.0
+    acc = accuracy(e, [sysm], [0])
+    e.close()
+    ach = b_nb * n / (nb_ms * 1e-3) / 1e9
+    print(f"[bench] UA melt: {n * args.ua_steps / secs:.3e} atom-steps/s, NB {nb_ms * 1e3:.1f} us", file=sys.stderr,
+          flush=True)
+    return {"workload": "ua_melt_100k", "sites": n, "steps": args.ua_steps, "value": n * args.ua_steps / secs,
+            "ms_per_step": 1e3 * secs / args.ua_steps, "timed_counters": tc,
+            "step_breakdown_ms": {k: round(tm[k] / tm["nsteps_timed"], 5) for k in
+                                  ("nonbonded", "bonded", "integrate", "rebuild", "step")},
+            "nonbonded_roofline": {"achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
+                                   "bytes_per_site_step": round(b_nb, 1), "kernel_ms": round(nb_ms, 5),
+                                   "traffic": measured_traffic("ua_melt_100k")},
+            "multiple_time_step": {"inner_dt_fs": 1.0, "nonbonded_every": 4, "inner_steps": nr,
+                                   "ms_per_outer_step": 1e3 * tr / (nr / 4), "ns_per_day": nr * 1e-6 / tr * 86400,
+                                   "plain_1fs_ns_per_day": args.ua_steps * 1e-6 / secs * 86400},
+            "accuracy": acc}
 
 
-def measured_traffic(sites):
-    """HBM bytes per k_nonbonded launch from the committed rocprofv3 counter summary
-    (FETCH_SIZE and WRITE_SIZE scaled by a device copy), when it covers this workload."""
+TRAFFIC_FILES = {"cg_replicas_256": "k_nonbonded_cg_replicas_traffic.md", "ua_melt_100k": "k_nonbonded_ua100k_traffic.md"}
+
+
+def measured_traffic(workload):
+    """HBM bytes per nonbonded launch from the committed rocprofv3 counter summary of this workload
+    (FETCH_SIZE and WRITE_SIZE scaled by a device copy), when there is one."""
     try:
-        with open(TRAFFIC_FILE) as fh:
+        with open(os.path.join(ROOT, "profiles", TRAFFIC_FILES[workload])) as fh:
             for line in fh:
                 if line.startswith("traffic_json:"):
                     t = json.loads(line.split(":", 1)[1])
-                    if t.get("workload") == "ua_melt_100k_plain" and t.get("sites") == sites:
+                    if t.get("workload") == workload:
                         return t["bytes"]
-    except OSError:
+    except (OSError, KeyError, ValueError):
         pass
     return None
 
@@ -272,27 +415,21 @@ def measured_traffic(sites):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20000)
-    ap.add_argument("--warmup", type=int, default=500)
-    ap.add_argument("--cpu-steps", type=int, default=20)
-    ap.add_argument("--cpu-threads", type=int, default=int(os.environ.get("OMP_NUM_THREADS", "16")))
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--nchains", type=int, default=400)
-    ap.add_argument("--nbeads", type=int, default=250)
+    ap.add_argument("--steps", type=int, default=200, help="timed steps of the batched replicas")
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--replicas", type=int, default=256)
+    ap.add_argument("--check", type=int, default=8, help="replicas checked against the FP64 oracle")
+    ap.add_argument("--legs", default="plain,cpu,served,ua,hybrid",
+                    help="secondary legs besides the batched headline: plain, cpu, served, ua, hybrid")
     ap.add_argument("--eager", action="store_true",
                     help="launch kernel by kernel instead of through HIP graphs (profiling runs)")
-    ap.add_argument("--replicas", type=int, default=256, metavar="NREP",
-                    help="also compare NREP CG melt replicas batched in one context against one context each "
-                         "(0 skips it)")
-    ap.add_argument("--replica-steps", type=int, default=200)
+    ap.add_argument("--ua-steps", type=int, default=2000)
+    ap.add_argument("--hybrid-steps", type=int, default=500)
+    ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-cohorts", type=int, default=3, help="batches the serving loop runs at once")
     ap.add_argument("--serve-band", type=float, default=1.5,
                     help="serving: batch only jobs whose step counts are within this factor (<= 1: off)")
-    ap.add_argument("--hybrid-steps", type=int, default=500,
-                    help="also time the 500k-atom hybrid system for this many steps (0 skips it)")
-    ap.add_argument("--serve-seconds", type=float, default=20.0,
-                    help="also run the serving loop on Poisson arrivals (20 jobs/s) for this many seconds "
-                         "(0 skips it)")
+    ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline: seconds per run (3 runs)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB five times (HBM counter calibration)")
     args = ap.parse_args()
@@ -310,106 +447,62 @@ def main():
         print(f"bench.py measures one GPU only (asked for {args.gpus}); replicas and jobs shard across GPUs "
               "without communication (one serving loop or batch per GPU)", file=sys.stderr)
         sys.exit(2)
+    legs = set(x for x in args.legs.split(",") if x)
+    if "plain" not in legs:
+        args_plain = False
+    else:
+        args_plain = True
 
-    from ibisco_amd import workloads
-    from ibisco_amd.engine import Engine
-
-    sysm = workloads.ua_melt_100k(nchains=args.nchains, nbeads=args.nbeads)
-    n = sysm.natoms
-    print(f"[bench] {n} sites, box {sysm.box[0]:.3f} nm", file=sys.stderr, flush=True)
-    e = Engine(sysm)
-    workloads.equilibrate(e, sysm, stage_steps=200, final_steps=1000)
-    print("[bench] equilibrated", file=sys.stderr, flush=True)
-    e.step(args.warmup, graph=not args.eager)
-    e.obs()   # synchronise
-    t0 = time.perf_counter()
-    e.step(args.steps, graph=not args.eager)
-    ob = e.obs()
-    t1 = time.perf_counter()
-    secs = t1 - t0
-    value = n * args.steps / secs
-    print(f"[bench] {args.steps} steps in {secs:.3f} s, T = {ob['temp'] * 300:.1f} K", file=sys.stderr, flush=True)
-
-    # multiple time step: bonded terms every 1 fs step, nonbonded every 4 (r-RESPA impulse form)
-    e.set_respa(4)
-    e.step(40, graph=not args.eager)
-    e.obs()
-    nr = max(400, (args.steps // 10) // 40 * 40)
-    t0 = time.perf_counter()
-    e.step(nr, graph=not args.eager)
-    e.obs()
-    tr = time.perf_counter() - t0
-    e.set_respa(1)
-    respa = {"inner_dt_fs": 1.0, "nonbonded_every": 4, "inner_steps": nr,
-             "atom_inner_steps_per_s": n * nr / tr, "atom_outer_steps_per_s": n * nr / 4 / tr,
-             "ms_per_outer_step": 1e3 * tr / (nr / 4), "ns_per_day": nr * 1e-6 / tr * 86400,
-             "plain_1fs_ns_per_day": args.steps * 1e-6 / secs * 86400}
-    # eager timing with the list-skin autotuner off (fixed delta = skin/3 from the next rebuild)
-    nt = min(200, max(20, args.steps // 10))
-    e.set_skin_tuning(False)
-    e.step(10)
-    e.set_timing(True)
-    e.step(nt)
-    tm_fixed = e.timing()
-    e.set_timing(False)
-    e.set_skin_tuning(True)
-    e.step(20)
-    # nonbonded kernel: HIP events on the library stream over an eager run (the last kernels of this context,
-    # so a kernel trace of this command ends with these launches)
-    e.set_timing(True)
-    e.step(nt)
-    tm = e.timing()
-    e.set_timing(False)
-    nb_ms = tm["nonbonded"] / tm["nsteps_timed"]
-    delta = float(e.skin(0)[0])
-    skin_tuning = {"delta_nm": round(delta, 4),
-                   "fixed_delta_nm": round((sysm.control.rlist - sysm.control.cutoff) / 3.0, 4),
-                   "nb_ms_tuned": round(nb_ms, 5),
-                   "nb_ms_fixed": round(tm_fixed["nonbonded"] / tm_fixed["nsteps_timed"], 5)}
-    pairs = e.pairs(0, 0)
-    prl_n = len(pairs) / n                        # half-list pairs per site
-    b_nb = 4.0 * prl_n + 28.0                     # algorithmic bytes per site-step
-    achieved = b_nb * n / (nb_ms * 1e-3) / 1e9    # GB/s
+    B = batched_leg(args, with_plain=args_plain)
+    total, secs = B["total"], B["secs"]
+    b_nb = 4.0 * B["prl"] / total + 28.0          # algorithmic bytes per bead-step (half-list int32 entries)
+    achieved = b_nb * total / (B["nb_ms"] * 1e-3) / 1e9
+    tm = B["tm"]
     roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(n),
-            "kernel": "k_nonbonded", "kernel_ms": round(nb_ms, 5), "timed_launches": int(tm["nsteps_timed"]),
-            "bytes_per_site_step": round(b_nb, 1), "half_list_pairs_per_site": round(prl_n, 2)}
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic("cg_replicas_256"),
+            "kernel": "k_nonbonded (the nonbonded role of k_step_forces in graph steps)",
+            "kernel_ms": round(B["nb_ms"], 5), "timed_launches": int(tm["nsteps_timed"]),
+            "bytes_per_bead_step": round(b_nb, 1), "half_list_pairs_per_bead": round(B["prl"] / total, 2),
+            "units_per_launch": total}
     cpu = None
-    if not args.no_cpu_baseline:
-        from ibisco_amd import units
-        pos, vel = e.state()
-        sysm.s = pos[:n].copy()
-        sysm.v = units.vel_from_reduced(vel)
-        print("[bench] timing the reference CPU path", file=sys.stderr, flush=True)
-        cpu = cpu_baseline(sysm, args.cpu_steps, args.cpu_threads)
-    batched = replicas_compare(args.replicas, args.replica_steps, 50) if args.replicas > 0 else None
-    served = serve_stream(20.0, args.serve_seconds, max_cohorts=args.serve_cohorts,
-                          step_band=args.serve_band) if args.serve_seconds > 0 else None
-    hybrid = hybrid_run(args.hybrid_steps, 50) if args.hybrid_steps > 0 else None
+    if "cpu" in legs:
+        print("[bench] timing the FP32 OpenMP restatement on the host", file=sys.stderr, flush=True)
+        idx = B["check"]
+        cpu = cpu_baseline([B["systems"][k] for k in idx], [B["states"][k] for k in idx], target_s=args.cpu_seconds)
+        cpu["accuracy_of_gpu_path"] = B["accuracy"]
     out = {
-        "metric": METRIC, "value": value, "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
+        "metric": METRIC, "value": B["value"], "unit": "atom-steps/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": 1e3 * secs / args.steps, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded random-walk melt, equilibrated on device)",
-        "config": {"workload": "ua_melt_100k_plain", "sites": n, "chains": args.nchains, "chain_length": args.nbeads,
-                   "rc_nm": 1.0, "rlist_nm": 1.3, "nupdate": 10, "dt_fs": 1.0, "ensemble": "NVE",
-                   "time_stepping": "single dt (the reference has no multiple time step for non-hybrid systems)",
-                   "path": "one system per launch, " + ("eager launches" if args.eager else
-                                                         "HIP graphs of four list intervals"),
+        "data": "synthetic (seeded random-walk CG melts with seeded Gaussian-corrected tables, equilibrated on device)",
+        "config": {"workload": "cg_replicas_256_batched", "replicas": len(B["systems"]), "beads": total,
+                   "beads_per_replica": "log-uniform 2,000-50,000, chain length 20-100", "density_per_nm3": 8.0,
+                   "rc_nm": 1.5, "rlist_nm": 1.8, "nupdate": 10, "dt_fs": 5.0, "ensemble": "NVE",
+                   "rdf": "150 bins of 0.01 nm, every rebuild",
+                   "path": "one batched context, " + ("eager launches" if args.eager else
+                                                      "HIP graph replays captured in the warm-up"),
                    "precision": "FP32 forces, FP64 integration and reductions"},
+        "timed_counters": B["timed_counters"],
         "step_breakdown_ms": {k: round(tm[k] / tm["nsteps_timed"], 5) for k in
                               ("nonbonded", "bonded", "integrate", "rebuild", "step")},
-        "list_skin_autotuner": skin_tuning,
-        "multiple_time_step": respa,
+        "accuracy": B["accuracy"],
         "roofline": roof,
         "cpu_baseline": cpu,
     }
-    if batched:
-        out["batched_vs_plain"] = batched
-    if served:
-        out["served_vs_plain"] = served
-    if hybrid:
-        out["hybrid"] = hybrid
+    if args_plain:
+        out["batched_vs_plain"] = {"batched": B["value"], "plain": B["plain_value"],
+                                   "ratio": B["value"] / B["plain_value"],
+                                   "plain_ms_per_step": 1e3 * B["plain_secs"] / args.steps,
+                                   "plain_path": "each replica alone in its own context, one after the other, "
+                                                 "same warm-up and graph protocol",
+                                   "plain_timed_counters": B["plain_counters"]}
+    if "served" in legs and args.serve_seconds > 0:
+        out["served_vs_plain"] = serve_stream(20.0, args.serve_seconds, max_cohorts=args.serve_cohorts,
+                                              step_band=args.serve_band)
+    if "ua" in legs:
+        out["ua_melt_100k"] = ua_leg(args)
+    if "hybrid" in legs and args.hybrid_steps > 0:
+        out["hybrid"] = hybrid_run(args.hybrid_steps, 50)
     print(json.dumps(out))
 
 

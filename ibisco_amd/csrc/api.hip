@@ -76,6 +76,7 @@ struct Timing {
 
 struct ibx_ctx {
     int nsys = 0;
+    int device = 0;                          // HIP device the context was created on
     std::vector<HostSystem> hs;
     std::vector<DevSys> dsys;
     std::vector<int> item_off, slot_off[2];
@@ -113,12 +114,18 @@ struct ibx_ctx {
     DBuf<float> mts_f;               // [3 slots][xyz][items] bead-group forces of the explicit steps
     DBuf<double> mts_part;           // [3 slots][9 fields][workgroups] bead-group energy/virial partials
     double rdf_dr = 0;
-    // graph
+    // graph blocks, keyed by length, phase in the block period and the partial buffers they start from
     struct GraphSlot {
         hipGraphExec_t exec = nullptr;
-        int steps = 0;
+        int steps = 0, phase = 0;
         int par = 0, bd_par = 0;   // nb_par, bd_par when the graph was captured (and after each replay)
-    } graph[2];                    // one block period; several periods (fewer launches per step)
+        uint64_t used = 0;         // last use, for eviction
+    };
+    static constexpr int kGraphSlots = 8;
+    GraphSlot graph[kGraphSlots];
+    uint64_t graph_clock = 0;
+    // counters: [0] eager steps, [1] steps replayed from graphs, [2] graph launches, [3] captures
+    int64_t counters[4] = {0, 0, 0, 0};
     // timing
     bool timing = false;
     Timing tm;
@@ -173,8 +180,7 @@ void drop_graph(ibx_ctx *c) {
     std::lock_guard<std::mutex> lk(g_capture_mu);
     for (auto &g : c->graph) {
         if (g.exec) (void)hipGraphExecDestroy(g.exec);
-        g.exec = nullptr;
-        g.steps = 0;
+        g = ibx_ctx::GraphSlot();
     }
 }
 
@@ -569,6 +575,7 @@ int build_device(ibx_ctx *c) {
         HIPCHK(c, G.pos4.alloc(slot));
         HIPCHK(c, G.pos4_ref.alloc(slot));
         HIPCHK(c, G.nneb.alloc(slot));
+        HIPCHK(c, memset_st(c, G.nneb.p, 0, G.nneb.bytes()));   // padding slots keep empty rows
         HIPCHK(c, G.part.alloc(2 * (size_t)G.nwg * (kBlock / kNbAtomsPerBlock) * kNbPart));
         HIPCHK(c, memset_st(c, G.part.p, 0, G.part.bytes()));
         HIPCHK(c, G.list.alloc((size_t)slot * G.cap));
@@ -900,6 +907,7 @@ int ibx::create_context(std::vector<HostSystem> &&hs, ibx_ctx **out) {
         return fail(nullptr, IBX_E_DEVICE, "device %s is not gfx950 (MI355X)", prop.gcnArchName);
     std::unique_ptr<ibx_ctx> c(new ibx_ctx());
     c->nsys = nsys;
+    c->device = dev;
     c->hs = std::move(hs);
     for (int s = 0; s < nsys; ++s) {
         if (c->hs[s].nupdate != c->hs[0].nupdate)
@@ -1001,6 +1009,7 @@ int ibx_step(ibx_ctx *c, int nsteps, int step0) {
         enqueue_step(c, step0 + 1 + k, c->timing ? &e : nullptr);
     }
     flush_fin(c);
+    c->counters[0] += nsteps;
     int rc = check_flags(c);
     if (c->timing) {
         for (int k = 0; k < nsteps; ++k) {
@@ -1019,58 +1028,87 @@ int ibx_step(ibx_ctx *c, int nsteps, int step0) {
     return rc;
 }
 
-// nblk replays of graph slot `slot`, which holds `steps` steps from a step that starts a block period
-// (captured on first use, or again when the partial buffers it expects have changed).
-int run_graph_blocks(ibx_ctx *c, int slot, int steps, int nblk, int cur) {
-    ibx_ctx::GraphSlot &g = c->graph[slot];
-    // a block starts by reading the partial buffers the capture found latest
-    if (!g.exec || g.steps != steps || g.par != c->nb_par || g.bd_par != c->bd_par) {
-        if (g.exec) {
-            std::lock_guard<std::mutex> lk(g_capture_mu);
-            (void)hipGraphExecDestroy(g.exec);
-            g.exec = nullptr;
-        }
+// Steps per long graph block (each block ends with its own field sums).
+constexpr int kGraphSteps = 40;
+
+int long_block(const ibx_ctx *c) {
+    const int period = block_period(c);
+    return period * std::max(1, kGraphSteps / period);
+}
+
+// nblk replays of a graph of `steps` steps (a whole number of block periods)
+// starting after step `cur`.  The step schedule (rebuilds, multiple-time-step
+// cycles) repeats with the block period, so one graph serves every start step
+// of the same phase cur % period; it is captured on first use, or again when
+// the partial buffers it expects have changed.
+int run_graph_blocks(ibx_ctx *c, int steps, int nblk, int cur) {
+    const int phase = cur % block_period(c);
+    ibx_ctx::GraphSlot *g = nullptr;
+    for (auto &q : c->graph)
+        if (q.exec && q.steps == steps && q.phase == phase && q.par == c->nb_par && q.bd_par == c->bd_par) g = &q;
+    if (!g) {
+        // an empty slot, else the least recently used one
+        g = &c->graph[0];
+        for (auto &q : c->graph)
+            if (!q.exec || (g->exec && q.used < g->used)) g = &q;
         std::lock_guard<std::mutex> lk(g_capture_mu);
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        *g = ibx_ctx::GraphSlot();
         hipGraph_t graph;
+        const int par = c->nb_par, bd_par = c->bd_par;
         // relaxed: other host threads (other contexts, the serving loop) keep allocating and copying
         // while this one captures; nothing captured here is a call that capture forbids
         HIPCHK(c, hipStreamBeginCapture(c->st, hipStreamCaptureModeRelaxed));
-        for (int k = 0; k < steps; ++k) enqueue_step(c, steps + k, nullptr);   // block starts at a rebuild
+        for (int k = 0; k < steps; ++k) enqueue_step(c, phase + 1 + k, nullptr);
         flush_fin(c);
         HIPCHK(c, hipStreamEndCapture(c->st, &graph));
-        hipError_t e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
         (void)hipGraphDestroy(graph);
-        if (e != hipSuccess) return fail(c, IBX_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
-        g.steps = steps;
-        g.par = c->nb_par;
-        g.bd_par = c->bd_par;
+        if (e != hipSuccess) {
+            g->exec = nullptr;
+            return fail(c, IBX_E_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+        }
+        // a block holds an even number of evaluations, so it ends on the buffers it started from
+        if (c->nb_par != par || c->bd_par != bd_par)
+            return fail(c, IBX_E_DEVICE, "graph block of %d steps does not return to its partial buffers", steps);
+        g->steps = steps;
+        g->phase = phase;
+        g->par = par;
+        g->bd_par = bd_par;
+        ++c->counters[3];
     }
+    g->used = ++c->graph_clock;
     HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
     HIPCHK(c, hipMemcpyAsync(c->step.p, &cur, sizeof(int), hipMemcpyHostToDevice, c->st));
-    for (int b = 0; b < nblk; ++b) HIPCHK(c, hipGraphLaunch(g.exec, c->st));
+    for (int b = 0; b < nblk; ++b) HIPCHK(c, hipGraphLaunch(g->exec, c->st));
+    c->counters[1] += (int64_t)nblk * steps;
+    c->counters[2] += nblk;
     return check_flags(c);
 }
 
 int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
-    if (!c || nsteps < 0) return fail(c, IBX_E_INVALID, "invalid arguments");
-    // eager head until a step that starts a list interval (and a multiple-time-step cycle), graph
-    // blocks of several periods, then of one period, eager tail
-    constexpr int kGraphSteps = 40;   // steps per long block (each block ends with its own field sums)
+    if (!c || nsteps < 0 || step0 < 0) return fail(c, IBX_E_INVALID, "invalid arguments");
+    // graph blocks of several block periods, then of one period, from step0 whatever its phase; eager tail
     const int period = block_period(c);
-    const int longer = period * std::max(1, kGraphSteps / period);
-    int head = 0;
-    while (head < nsteps && (step0 + 1 + head) % period != 0) ++head;
-    int rc = ibx_step(c, head, step0);
-    if (rc) return rc;
-    int done = head;
-    for (int slot = 1; slot >= 0; --slot) {
-        const int steps = slot ? longer : period;
+    const int longer = long_block(c);
+    int done = 0, rc;
+    for (int steps : {longer, period}) {
         const int nblk = (nsteps - done) / steps;
         if (nblk == 0) continue;
-        if ((rc = run_graph_blocks(c, slot, steps, nblk, step0 + done))) return rc;
+        if ((rc = run_graph_blocks(c, steps, nblk, step0 + done))) return rc;
         done += nblk * steps;
     }
     return ibx_step(c, nsteps - done, step0 + done);
+}
+
+int ibx_get_counters(ibx_ctx *c, int64_t out[8]) {
+    if (!c || !out) return fail(c, IBX_E_INVALID, "invalid arguments");
+    for (int k = 0; k < 4; ++k) out[k] = c->counters[k];
+    out[4] = block_period(c);
+    out[5] = long_block(c);
+    out[6] = 0;
+    out[7] = 0;
+    return 0;
 }
 
 int ibx_set_respa(ibx_ctx *c, int k) {
@@ -1196,12 +1234,7 @@ int ibx_set_state(ibx_ctx *c, int sys, const double *s, const double *v) {
         }
     }
     HIPCHK(c, memset_st(c, c->sysstate.p + 8 * (size_t)sys + 2, 0, 3 * sizeof(double)));
-    // refresh the sorted copies with the current order (lists unchanged)
-    for (int g = 0; g < 2; ++g) {
-        if (!c->gr[g].present) continue;
-        RebuildArgs a = rebuild_args(c, g);
-        launch_pack_only(c->st, a);
-    }
+    if (s) return rebuild_checked(c);   // lists of the new positions (every system of the context)
     HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
@@ -1272,10 +1305,21 @@ int ibx_get_pairs(ibx_ctx *c, int sys, int group, int *out, int64_t cap, int64_t
     const DevGroup &dg = c->dsys[sys].g[group];
     if (dg.nmem == 0) return 0;
     HIPCHK(c, hipStreamSynchronize(c->st));
-    std::vector<int> perm(dg.nslot_pad), list((size_t)dg.nslot_pad * G.cap);
     std::vector<int2> nneb(dg.nslot_pad);
-    HIPCHK(c, memcpy_st(c, perm.data(), G.perm.p + dg.slot_off, perm.size() * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<int> perm(dg.nslot_pad);
     HIPCHK(c, memcpy_st(c, nneb.data(), G.nneb.p + dg.slot_off, nneb.size() * sizeof(int2), hipMemcpyDeviceToHost));
+    HIPCHK(c, memcpy_st(c, perm.data(), G.perm.p + dg.slot_off, perm.size() * sizeof(int), hipMemcpyDeviceToHost));
+    if (!out) {   // rows are full (each pair in both members' rows): the count is half the row lengths
+        int64_t n = 0;
+        for (int i = 0; i < dg.nslot_pad; ++i) {
+            if (perm[i] < 0) continue;
+            const int n_in = std::min(nneb[i].x, G.cap);
+            n += n_in + std::min(nneb[i].y, G.cap - n_in);
+        }
+        *count = n / 2;
+        return 0;
+    }
+    std::vector<int> list((size_t)dg.nslot_pad * G.cap);
     HIPCHK(c, memcpy_st(c, list.data(), G.list.p + (size_t)dg.slot_off * G.cap, list.size() * sizeof(int),
                         hipMemcpyDeviceToHost));
     const int o = c->item_off[sys];

@@ -291,11 +291,13 @@ __global__ void __launch_bounds__(kBlock) k_sort_cells(const int *__restrict__ c
         base = G.slot_off + cell_start[gc] - cell_start[G.cell_off];
         lc = gc - G.cell_off;
     }
-    if (n > kCap) {   // too crowded to sort in LDS: keep arrival order, flag it
-        if (lane == 0) atomicMax(&flags[2], n);
+    if (n > kCap) {   // too crowded for LDS: rank against the arrival copy in global memory (same order)
         for (int i = lane; i < n; i += kSortLanes) {
-            perm[base + i] = mem[tmp[base + i]];
-            slot_cell[base + i] = lc;
+            const int v = tmp[base + i];
+            int rank = 0;
+            for (int k = 0; k < n; ++k) rank += tmp[base + k] < v;
+            perm[base + rank] = mem[v];
+            slot_cell[base + rank] = lc;
         }
         n = 0;
     }
@@ -1729,13 +1731,6 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
         hipLaunchKernelGGL(k_build_list<false>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
                            a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.excl_off, a.excl, a.cap,
                            a.list, a.nneb, a.flags, nwg_list);
-}
-
-// New positions without a rebuild: the displacement bound no longer holds, so
-// the nonbonded kernel scans whole lists until the next rebuild.
-void launch_pack_only(hipStream_t st, const RebuildArgs &a) {
-    hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
-                       a.itype, a.label, a.r64, a.pos4, a.inv_perm, (int4 *)nullptr, a.dmax, a.nsys, 0x7f800000);
 }
 
 template <bool M, bool L, int G, bool K>

@@ -101,7 +101,6 @@ struct FinLaunch {
 void launch_mts(hipStream_t st, bool save, float *fnb, const int *label, int n, double *part, int npart,
                 float *mts_f, double *mts_part, int m, double I);
 void launch_rebuild(hipStream_t st, const RebuildArgs &a);
-void launch_pack_only(hipStream_t st, const RebuildArgs &a);
 void launch_nonbonded(hipStream_t st, const NbArgs &a);
 void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &b, const double *s, int nitems_total,
                   int nterms, float4 *contrib, int ncontrib, double *part, int *flags);

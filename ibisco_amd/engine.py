@@ -56,7 +56,7 @@ EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_s
            "ibx_get_pairs", "ibx_rdf_enable", "ibx_rdf_get", "ibx_get_timing", "ibx_set_timing",
            "ibx_get_info", "ibx_retire", "ibx_server_create", "ibx_server_submit", "ibx_server_wait",
            "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes",
-           "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa")
+           "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa", "ibx_get_counters")
 
 _lib = None
 
@@ -93,6 +93,7 @@ def load_library(path=LIB_PATH):
     L.ibx_get_timing.argtypes = [vp, _dp]
     L.ibx_set_timing.argtypes = [vp, C.c_int]
     L.ibx_get_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
+    L.ibx_get_counters.argtypes = [vp, C.POINTER(C.c_int64)]
     L.ibx_retire.argtypes = [vp, C.c_int]
     L.ibx_set_skin_tuning.argtypes = [vp, C.c_int]
     L.ibx_set_respa.argtypes = [vp, C.c_int]
@@ -259,6 +260,12 @@ class Engine:
         self._chk(self.L.ibx_get_pairs(self.h, sys, group, _p(out, C.c_int), n.value, C.byref(n)))
         return out[:n.value]
 
+    def pair_count(self, sys=0, group=0):
+        """Number of unordered pairs in the current list (no copy of the list itself)."""
+        n = C.c_int64()
+        self._chk(self.L.ibx_get_pairs(self.h, sys, group, None, 0, C.byref(n)))
+        return n.value
+
     def rdf_enable(self, nbin, dr, every=1):
         self._chk(self.L.ibx_rdf_enable(self.h, nbin, dr, every))
         self._rdf_nbin = nbin
@@ -281,6 +288,14 @@ class Engine:
         i = (C.c_int64 * 8)()
         self._chk(self.L.ibx_get_info(self.h, sys, i))
         return list(i)
+
+    def counters(self):
+        """Step counters (ibx_get_counters): eager steps, graph-replayed steps, graph launches, captures,
+        the block period and the long block's steps."""
+        i = (C.c_int64 * 8)()
+        self._chk(self.L.ibx_get_counters(self.h, i))
+        return dict(eager_steps=i[0], graph_steps=i[1], graph_launches=i[2], captures=i[3], period=i[4],
+                    long_block=i[5])
 
     def retire(self, sys):
         self._chk(self.L.ibx_retire(self.h, sys))

@@ -98,10 +98,18 @@ const char *ibx_last_error(const ibx_ctx *ctx);   /* ctx may be NULL: last creat
 int ibx_step(ibx_ctx *ctx, int nsteps, int step0);
 
 /* The same steps as ibx_step, bit for bit, launched as captured HIP graphs:
- * eager steps up to the start of a list interval, graphs of four intervals,
- * of one, then eager steps for the rest (batched replicas and serving use
- * this path). */
+ * blocks of four list intervals (and multiple-time-step cycles), then of one,
+ * then eager steps for the rest (batched replicas and serving use this path).
+ * A block may start at any step: graphs are kept per block length and phase
+ * of step0 in the block period (the period is ibx_get_counters()[4]), so a
+ * caller stepping in chunks of whole periods replays after the first call. */
 int ibx_step_graph(ibx_ctx *ctx, int nsteps, int step0);
+
+/* Step counters of the context since creation: [0] steps run eagerly, [1]
+ * steps replayed from graphs, [2] graph launches, [3] graph captures; [4] the
+ * block period (steps), [5] steps of a long block, [6..7] 0.  No counterpart
+ * in the reference (instrumentation). */
+int ibx_get_counters(ibx_ctx *ctx, int64_t out[8]);
 
 /* NEW_FORCE.F90:7-83 at the current positions without moving: fold, virtual
  * sites, (re)build the lists when rebuild != 0, forces, energies, virial. */
@@ -125,6 +133,9 @@ int ibx_get_forces(ibx_ctx *ctx, int sys, double *f);
 /* State of system sys, xyz interleaved: s (natoms+nvs)*3 unwrapped positions
  * (SXYZ), v natoms*3 reduced velocities (VXYZ).  Either pointer may be NULL. */
 int ibx_get_state(ibx_ctx *ctx, int sys, double *s, double *v);
+/* Replaces the state of system sys (same layout; virtual sites are placed from
+ * their atoms).  New positions rebuild the lists of every system of the
+ * context at their current positions (main.F90:234 after RDCOOR). */
 int ibx_set_state(ibx_ctx *ctx, int sys, const double *s, const double *v);
 /* Replaces nonbonded table t of system sys (raw rows as in ibx_tables). */
 int ibx_set_nb_table(ibx_ctx *ctx, int sys, int t, int nrow, const double *x, const double *v);

@@ -15,6 +15,7 @@ from ibisco_amd.arrays import flatten
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "libibx_oracle.so")
+PORT_LIB = os.path.join(HERE, "libibx_cpuport.so")
 
 _ip = C.POINTER(C.c_int)
 _dp = C.POINTER(C.c_double)
@@ -40,10 +41,36 @@ class _Desc(C.Structure):
 
 
 _lib = None
+_port = None
 
 
 def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def _declare(L):
+    L.ox_create.restype = C.c_void_p
+    L.ox_create.argtypes = [C.POINTER(_Desc), C.c_char_p, C.c_int]
+    for nm in ("ox_destroy", "ox_shift", "ox_virtual_def", "ox_new_force", "ox_move", "ox_momentum"):
+        getattr(L, nm).argtypes = [C.c_void_p]
+        getattr(L, nm).restype = None
+    L.ox_update_lists.argtypes = [C.c_void_p, C.c_int]
+    L.ox_pairs.restype = C.c_int64
+    L.ox_pairs.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int64]
+    L.ox_exclusions.argtypes = [C.c_void_p, _ip, _ip, C.c_int]
+    L.ox_terms.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int]
+    L.ox_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.ox_get_obs.argtypes = [C.c_void_p, _dp]
+    L.ox_get_obs.restype = None
+    L.ox_nitems.argtypes = [C.c_void_p]
+    L.ox_get_forces.argtypes = [C.c_void_p, _dp]
+    L.ox_get_forces.restype = None
+    L.ox_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
+    L.ox_get_state.restype = None
+    L.ox_set_state.argtypes = [C.c_void_p, _dp, _dp]
+    L.ox_set_state.restype = None
+    L.ox_table.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int]
+    return L
 
 
 def lib():
@@ -51,30 +78,31 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        L.ox_create.restype = C.c_void_p
-        L.ox_create.argtypes = [C.POINTER(_Desc), C.c_char_p, C.c_int]
-        for nm in ("ox_destroy", "ox_shift", "ox_virtual_def", "ox_new_force", "ox_move", "ox_momentum"):
-            getattr(L, nm).argtypes = [C.c_void_p]
-            getattr(L, nm).restype = None
-        L.ox_update_lists.argtypes = [C.c_void_p, C.c_int]
-        L.ox_pairs.restype = C.c_int64
-        L.ox_pairs.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int64]
-        L.ox_exclusions.argtypes = [C.c_void_p, _ip, _ip, C.c_int]
-        L.ox_terms.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int]
-        L.ox_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
-        L.ox_get_obs.argtypes = [C.c_void_p, _dp]
-        L.ox_get_obs.restype = None
-        L.ox_nitems.argtypes = [C.c_void_p]
-        L.ox_get_forces.argtypes = [C.c_void_p, _dp]
-        L.ox_get_forces.restype = None
-        L.ox_get_state.argtypes = [C.c_void_p, _dp, _dp, _dp]
-        L.ox_get_state.restype = None
-        L.ox_set_state.argtypes = [C.c_void_p, _dp, _dp]
-        L.ox_set_state.restype = None
-        L.ox_table.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp, _dp, _dp, C.c_int]
-        _lib = L
+        _lib = _declare(C.CDLL(LIB))
     return _lib
+
+
+def port_lib():
+    """The FP32 OpenMP port (cpu_port.c) with its own copy of the oracle's set-up."""
+    global _port
+    if _port is None:
+        if not os.path.exists(PORT_LIB):
+            build()
+        L = _declare(C.CDLL(PORT_LIB))
+        L.cp_create.restype = C.c_void_p
+        L.cp_create.argtypes = [C.c_void_p, C.c_char_p, C.c_int]
+        L.cp_destroy.argtypes = [C.c_void_p]
+        L.cp_destroy.restype = None
+        L.cp_init_lists.argtypes = [C.c_void_p]
+        L.cp_run.argtypes = [C.c_void_p, C.c_int, C.c_int, _dp]
+        L.cp_threads.argtypes = []
+        for nm in ("cp_get_forces", "cp_get_obs"):
+            getattr(L, nm).argtypes = [C.c_void_p, _dp]
+            getattr(L, nm).restype = None
+        L.cp_get_state.argtypes = [C.c_void_p, _dp, _dp]
+        L.cp_get_state.restype = None
+        _port = L
+    return _port
 
 
 def _p(a, t):
@@ -87,7 +115,7 @@ OBS_KEYS = ("vnb", "vbond", "vangle", "vtors", "voop")
 class Oracle:
     """FP64 restatement of the reference path for one system."""
 
-    def __init__(self, sysm, mts_quirk=False):
+    def __init__(self, sysm, mts_quirk=False, library=None):
         self.arrays = a = flatten(sysm)
         self._keep = []
         d = _Desc()
@@ -118,7 +146,7 @@ class Oracle:
         d.nonbexc_atom, d.nonbexc_bead = a["nonbexc_atom"], a["nonbexc_bead"]
         d.nupdate, d.halt_drift, d.nmts, d.mts_quirk = a["nupdate"], a["halt_drift"], a["nmts"], int(mts_quirk)
         err = C.create_string_buffer(256)
-        self.L = lib()
+        self.L = library if library is not None else lib()
         self.h = self.L.ox_create(C.byref(d), err, 256)
         if not self.h:
             raise RuntimeError("oracle: " + err.value.decode())
@@ -191,3 +219,50 @@ class Oracle:
         x, V, F = np.zeros(n), np.zeros(n), np.zeros(n)
         self.L.ox_table(self.h, kind, t, _p(x, C.c_double), _p(V, C.c_double), _p(F, C.c_double), n)
         return x, V, F
+
+
+class CpuPort:
+    """The FP32 OpenMP restatement of the reference's step loop (cpu_port.c) for
+    one non-hybrid system with bonds and angles: the CPU baseline."""
+
+    def __init__(self, sysm):
+        self.L = port_lib()
+        self._setup = Oracle(sysm, library=self.L)
+        err = C.create_string_buffer(256)
+        self.h = self.L.cp_create(self._setup.h, err, 256)
+        if not self.h:
+            raise RuntimeError("cpu port: " + err.value.decode())
+        self.natoms = self._setup.natoms
+        if self.L.cp_init_lists(self.h):
+            raise RuntimeError("cpu port: LIST TOO SMALL")
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.L.cp_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def threads():
+        return int(port_lib().cp_threads())
+
+    def run(self, nsteps, step0=0):
+        """Runs nsteps; returns the wall time of the step loop in seconds."""
+        t = C.c_double()
+        if self.L.cp_run(self.h, nsteps, step0, C.byref(t)):
+            raise RuntimeError("cpu port: LIST TOO SMALL")
+        return t.value
+
+    def forces(self):
+        f = np.zeros((self.natoms, 3))
+        self.L.cp_get_forces(self.h, _p(f, C.c_double))
+        return f
+
+    def obs(self):
+        o = np.zeros(32)
+        self.L.cp_get_obs(self.h, _p(o, C.c_double))
+        return o
+
+    def state(self):
+        s, v = np.zeros((self.natoms, 3)), np.zeros((self.natoms, 3))
+        self.L.cp_get_state(self.h, _p(s, C.c_double), _p(v, C.c_double))
+        return s, v

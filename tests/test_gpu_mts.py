@@ -57,10 +57,17 @@ def test_mts_cycle_matches_oracle_and_reference(oracle_mod):
 
 
 def test_mts_graph_equals_eager():
+    """Graph blocks of whole MTS cycles, from a step inside a cycle (7) and again (25 steps: two blocks of the
+    10-step period from phase 7, then eager steps), against eager launches: the same bits, with graph
+    replays actually taken."""
     _, s = _load()
     a, b = Engine(s, mts_quirk=True), Engine(s, mts_quirk=True)
-    a.step(12)
-    b.step(12, graph=True)
+    a.step(32)
+    b.step(7, graph=True)
+    b.step(25, graph=True)
+    c = b.counters()
+    print("counters", c, flush=True)
+    assert c["period"] == 10 and c["graph_launches"] >= 2 and c["graph_steps"] >= 20
     sa, va = a.state()
     sb, vb = b.state()
     assert np.array_equal(sa, sb) and np.array_equal(va, vb)

@@ -122,6 +122,10 @@ def test_graph_equals_eager(calls):
     a.step(sum(calls))
     for n in calls:
         b.step(n, graph=True)
+    c = b.counters()
+    print("counters", c, flush=True)
+    assert c["graph_steps"] >= sum(calls) - len(calls) * (c["period"] - 1)
+    assert c["graph_launches"] >= len(calls) and c["captures"] >= 1
     sa, va = a.state()
     sb, vb = b.state()
     assert np.array_equal(sa, sb)

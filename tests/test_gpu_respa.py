@@ -55,6 +55,28 @@ def test_ratio_one_is_the_plain_loop():
     assert np.array_equal(sa, sb) and np.array_equal(va, vb)
 
 
+def test_respa_graph_equals_eager():
+    """RESPA k = 4 (block period 40: lcm of the list interval 10 and k is 20, which holds an odd number of
+    nonbonded evaluations, so it doubles) stepped through graphs from steps inside a cycle, in calls of 3, 47
+    and 40 steps (blocks at phases 3 and 10): the same bits as eager steps."""
+    s = _ua()                           # equilibrated: a raw random-walk melt blows up within 90 steps
+    a, b = Engine(s), Engine(s)
+    for e in (a, b):
+        e.set_respa(4)
+    a.step(90)
+    for n in (3, 47, 40):
+        b.step(n, graph=True)
+    c = b.counters()
+    print("counters", c, flush=True)
+    assert c["period"] == 40 and c["long_block"] == 40
+    assert c["graph_steps"] >= 80 and c["graph_launches"] >= 2
+    sa, va = a.state()
+    sb, vb = b.state()
+    assert np.all(np.isfinite(sa)) and np.all(np.isfinite(va))
+    assert np.array_equal(sa, sb) and np.array_equal(va, vb)
+    assert np.array_equal(a.obs()["vnb"], b.obs()["vnb"])
+
+
 def test_nonbonded_only_on_cycle_steps():
     s = system.cg_melt(nchains=30, nbeads=50, seed=4, avoid=0.8, dt=0.001)
     e = Engine(s)

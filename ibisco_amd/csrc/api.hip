@@ -8,6 +8,7 @@
 #include <cstddef>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -1026,6 +1027,101 @@ int ibx_step(ibx_ctx *c, int nsteps, int step0) {
         c->tm.ms[6] += nsteps;
     }
     return rc;
+}
+
+}  // extern "C"
+
+// ---- scratch of ibx_step_noalloc -----------------------------------------------------------------
+// The per-workgroup reduction partials (nonbonded of both groups, bonded, integrator), the per-system
+// field sums and the per-system counters of finished field-sum workgroups live, for ibx_step_noalloc,
+// in this array instead of the context's buffers.  Each device holds its own copy until the process
+// ends; host code never passes the array's name to a kernel or a copy, only its device address, taken
+// once per device with hipGetSymbolAddress.
+static constexpr size_t kScratchBytes = (size_t)16 << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;                     // one ibx_step_noalloc at a time in the process
+void *g_scratch_ptr[kMaxDevices] = {};       // device address of `scratch` on each device
+
+int scratch_address(ibx_ctx *c, unsigned char **ptr) {
+    int dev = 0;
+    HIPCHK(c, hipGetDevice(&dev));
+    if (dev != c->device)
+        return fail(c, IBX_E_INVALID, "the current device (%d) is not the context's (%d)", dev, c->device);
+    if (dev < 0 || dev >= kMaxDevices) return fail(c, IBX_E_DEVICE, "device index %d out of range", dev);
+    if (!g_scratch_ptr[dev]) {
+        void *p = nullptr;
+        HIPCHK(c, hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[dev] = p;
+    }
+    *ptr = (unsigned char *)g_scratch_ptr[dev];
+    return 0;
+}
+
+// The context buffers ibx_step_noalloc redirects into the scratch array, in order, with their
+// offsets (each aligned to 256 bytes).  While a redirection is alive the buffers point into the
+// array; its destructor points them back.
+struct ScratchRedirect {
+    struct Entry {
+        std::function<void(void *)> set;   // stores an address into the buffer's pointer
+        void *own;                         // the context's own allocation
+        size_t bytes, off;
+    };
+    std::vector<Entry> e;
+    size_t total = 0;
+    template <typename T>
+    void add(DBuf<T> &b) {
+        if (!b.p) return;
+        DBuf<T> *q = &b;
+        e.push_back(Entry{[q](void *a) { q->p = (T *)a; }, (void *)b.p, b.bytes(), total});
+        total += (b.bytes() + 255) / 256 * 256;
+    }
+    void point_to(unsigned char *base) {
+        for (auto &x : e) x.set(base + x.off);
+    }
+    ~ScratchRedirect() {
+        for (auto &x : e) x.set(x.own);
+    }
+};
+
+}  // namespace
+
+extern "C" {
+
+int ibx_step_noalloc(ibx_ctx *c, int nsteps, int step0) {
+    if (!c || nsteps < 0) return fail(c, IBX_E_INVALID, "invalid arguments");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    unsigned char *base = nullptr;
+    int rc = scratch_address(c, &base);
+    if (rc) return rc;
+    flush_fin(c);   // nothing may be left reading the context's own partials
+    ScratchRedirect r;
+    for (int g = 0; g < 2; ++g)
+        if (c->gr[g].present) r.add(c->gr[g].part);
+    r.add(c->part_bd);
+    r.add(c->part_mv);
+    r.add(c->fsum);
+    r.add(c->fin_done);
+    if (r.total > kScratchBytes)
+        return fail(c, IBX_E_INVALID,
+                    "ibx_step_noalloc: this context needs %zu bytes of scratch, more than the %zu of the static array",
+                    r.total, kScratchBytes);
+    // the array keeps the previous call's contents (maybe of another context): start from this context's
+    // partials (the last evaluation's, which a step without a nonbonded evaluation still reads) and counters
+    for (auto &x : r.e)
+        HIPCHK(c, hipMemcpyAsync(base + x.off, x.own, x.bytes, hipMemcpyDeviceToDevice, c->st));
+    r.point_to(base);
+    launch_fill_int(c->st, c->flags.p, (int)c->flags.n, 0);
+    HIPCHK(c, hipMemcpyAsync(c->step.p, &step0, sizeof(int), hipMemcpyHostToDevice, c->st));
+    for (int k = 0; k < nsteps; ++k) enqueue_step(c, step0 + 1 + k, nullptr);
+    flush_fin(c);
+    for (auto &x : r.e)   // and hand the partials and counters back to the context
+        HIPCHK(c, hipMemcpyAsync(x.own, base + x.off, x.bytes, hipMemcpyDeviceToDevice, c->st));
+    c->counters[0] += nsteps;
+    return check_flags(c);   // waits for the stream, so no two calls overlap on the array
 }
 
 // Steps per long graph block (each block ends with its own field sums).

@@ -1707,6 +1707,16 @@ void launch_mts(hipStream_t st, bool save, float *fnb, const int *label, int n, 
 }
 
 
+__global__ void __launch_bounds__(kBlock) k_fill_int(int *__restrict__ p, int n, int v) {
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+void launch_fill_int(hipStream_t st, int *p, int n, int v) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, st, p, n, v);
+}
+
 void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
     const int ntune = a.tune >= 0 ? (2 * a.nsys + kBlock - 1) / kBlock : 0;
     hipLaunchKernelGGL(k_cell_count, dim3(a.nwg + ntune), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.nwg, a.g, a.mem, a.s,

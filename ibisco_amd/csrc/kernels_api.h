@@ -115,6 +115,8 @@ void launch_finalize(hipStream_t st, const FinLaunch &f);
 void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const int *wg_terms, const TermLaunch &b,
                         const double *s, int nitems_total, int nterms, float4 *contrib, int ncontrib, double *part_bd,
                         int *flags, const FinLaunch *fin);
+// Sets n ints to v (a kernel, not a memset node, so that code a caller may capture stays replayable).
+void launch_fill_int(hipStream_t st, int *p, int n, int v);
 void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
                 const int *list, const int2 *nneb, int cap, int nbin, float inv_dr, unsigned long long *hist,
                 int *samples);

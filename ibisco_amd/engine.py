@@ -56,7 +56,7 @@ EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_s
            "ibx_get_pairs", "ibx_rdf_enable", "ibx_rdf_get", "ibx_get_timing", "ibx_set_timing",
            "ibx_get_info", "ibx_retire", "ibx_server_create", "ibx_server_submit", "ibx_server_wait",
            "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes",
-           "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa", "ibx_get_counters")
+           "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa", "ibx_get_counters", "ibx_step_noalloc")
 
 _lib = None
 
@@ -76,7 +76,7 @@ def load_library(path=LIB_PATH):
     L.ibx_destroy.restype = None
     L.ibx_last_error.argtypes = [vp]
     L.ibx_last_error.restype = C.c_char_p
-    for nm in ("ibx_step", "ibx_step_graph"):
+    for nm in ("ibx_step", "ibx_step_graph", "ibx_step_noalloc"):
         getattr(L, nm).argtypes = [vp, C.c_int, C.c_int]
     L.ibx_compute_forces.argtypes = [vp, C.c_int]
     L.ibx_rebuild_lists.argtypes = [vp]
@@ -210,6 +210,12 @@ class Engine:
     def step(self, nsteps, graph=False):
         fn = self.L.ibx_step_graph if graph else self.L.ibx_step
         self._chk(fn(self.h, nsteps, self.step_count))
+        self.step_count += nsteps
+
+    def step_noalloc(self, nsteps):
+        """ibx_step_noalloc: the steps of ibx_step with the library's static scratch array for the reduction
+        partials (no device allocation in the call)."""
+        self._chk(self.L.ibx_step_noalloc(self.h, nsteps, self.step_count))
         self.step_count += nsteps
 
     def compute_forces(self, rebuild=True):

@@ -105,6 +105,20 @@ int ibx_step(ibx_ctx *ctx, int nsteps, int step0);
  * caller stepping in chunks of whole periods replays after the first call. */
 int ibx_step_graph(ibx_ctx *ctx, int nsteps, int step0);
 
+/* The steps of ibx_step, bit for bit, for callers that cannot allocate device
+ * memory (tight serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  The reduction partials and counters the
+ * kernels write (per-workgroup partial sums of every kernel class, per-system
+ * field sums and completion counters) live in a static 16 MiB device array of
+ * the library instead of the context's buffers; the call copies the context's
+ * into the array first and back at the end, and allocates nothing.  A context
+ * needing more than 16 MiB of them (about 2 million particles) is rejected
+ * with IBX_E_INVALID.  The calling thread's current device must be the
+ * context's.  Calls hold a process-wide lock and wait for their stream, so two
+ * calls never overlap on the array (a call that overlapped another on the same
+ * GPU would race on it). */
+int ibx_step_noalloc(ibx_ctx *ctx, int nsteps, int step0);
+
 /* Step counters of the context since creation: [0] steps run eagerly, [1]
  * steps replayed from graphs, [2] graph launches, [3] graph captures; [4] the
  * block period (steps), [5] steps of a long block, [6..7] 0.  No counterpart

@@ -97,6 +97,7 @@ struct ibx_ctx {
     DBuf<float4> contrib;           // bonded contribution slots {x, y, z, 0}
     int nterms = 0, term_nwg = 0, ncontrib = 0;
     DBuf<double> s, v, mass, invmass, vs_coeff, vs_invmass, part_bd, part_mv, fsum, obs, avg, sysstate;
+    DBuf<double> vs_com;             // per virtual site (SoA like s): the mass-weighted centre of its atoms
     double *part_bd_buf(int par) { return part_bd.p + (size_t)par * (part_bd.n / 2); }   // two buffers
     DBuf<float> fnb, ftot;
     DBuf<float4> nb_rows, bd_rows;
@@ -243,6 +244,17 @@ NbTab nb_header(const Table &t, double unit, int row_off, std::vector<float> &xo
     return nt;
 }
 
+// The mass-weighted centre of virtual site v's atoms (unwrapped): where DISTRIBUTE_VSFORCE.f90:27-37 puts a
+// force on the site (each atom takes m_j/M of it), so where the bonded virial of NEW_FORCE.F90:73-80 counts it.
+void vs_centre_of_atoms(const HostSystem &h, int v, double com[3]) {
+    com[0] = com[1] = com[2] = 0.0;
+    for (int q = h.vs_off[v]; q < h.vs_off[v + 1]; ++q) {
+        const int a = h.vs_atoms[q];
+        for (int k = 0; k < 3; ++k) com[k] += h.mass[h.itype[a]] * h.s[3 * (size_t)a + k];
+    }
+    for (int k = 0; k < 3; ++k) com[k] *= h.vs_invmass[v];
+}
+
 int build_device(ibx_ctx *c) {
     const int S = c->nsys;
     c->item_off.resize(S + 1);
@@ -265,6 +277,7 @@ int build_device(ibx_ctx *c) {
     std::vector<int> hit(N, -1), hlab(N, 0), hvs(N, -1), hvcen(N, -1), hwg(c->item_nwg, 0);
     std::vector<int> hex_off(N + 1, 0), hex;
     std::vector<int> hvm_off(N + 1, 0), hvm;
+    std::vector<double> hcom;         // virtual sites' centres of their atoms, when there are sites
     std::vector<int4> htm;            // term members
     std::vector<int> hmeta, hslot, hwgt;
     std::vector<BdTab> hbt;
@@ -295,10 +308,14 @@ int build_device(ibx_ctx *c) {
         for (int i = 0; i < h.nitems; ++i)
             for (int q = h.excl_off[i]; q < h.excl_off[i + 1]; ++q) hex.push_back(o + h.excl[q]);
         // virtual sites
+        if (h.nvs > 0 && hcom.empty()) hcom.assign(3 * (size_t)N, 0.0);
         for (int v = 0; v < h.nvs; ++v) {
             const int vi = o + h.natoms + v;
             hvcen[vi] = h.vs_center[v] >= 0 ? o + h.vs_center[v] : -1;
             hvim[vi] = h.vs_invmass[v];
+            double com[3];
+            vs_centre_of_atoms(h, v, com);
+            for (int k = 0; k < 3; ++k) hcom[(size_t)k * N + vi] = com[k];
             for (int q = h.vs_off[v]; q < h.vs_off[v + 1]; ++q) {
                 hvs[o + h.vs_atoms[q]] = vi;
                 hvc[o + h.vs_atoms[q]] = h.vs_coeff[q];
@@ -329,7 +346,10 @@ int build_device(ibx_ctx *c) {
         auto add_term = [&](int kind, int tab, const std::array<int, 4> &m, int nm) {
             int4 g4 = make_int4(o + m[0], o + m[1], nm > 2 ? o + m[2] : -1, nm > 3 ? o + m[3] : -1);
             htm.push_back(g4);
-            hmeta.push_back(kind | (tab << 2));
+            int vsmask = 0;   // members that are virtual sites (terms of the virtangles file)
+            for (int r = 0; r < nm; ++r)
+                if (m[r] >= h.natoms) vsmask |= 1 << r;
+            hmeta.push_back(kind | (tab << 2) | (vsmask << kTermVsShift));
             hslot.push_back(nslot);
             for (int r = 0; r < nm; ++r) inc[o + m[r]].push_back(nslot + r);
             nslot += nm;
@@ -453,7 +473,7 @@ int build_device(ibx_ctx *c) {
         (rc = upload(c, c->term_slot, hslot4)) || (rc = upload(c, c->wg_terms, hwgt)) ||
         (rc = upload(c, c->bd_tabs, hbt)) || (rc = upload(c, c->bd_rows, hbrows)) || (rc = upload(c, c->bd_x, hbx)) ||
         (rc = upload(c, c->nb_rows, hnrows)) || (rc = upload(c, c->nb_tabs, hntab)) || (rc = upload(c, c->nb_map, hnmap)) ||
-        (rc = upload(c, c->nb_xoff, hnxoff)))
+        (rc = upload(c, c->nb_xoff, hnxoff)) || (rc = upload(c, c->vs_com, hcom)))
         return rc;
     c->h_nb_tabs = hntab;
     HIPCHK(c, c->fnb.alloc(3 * (size_t)N));
@@ -597,7 +617,7 @@ size_t device_bytes(const ibx_ctx *c) {
                c->vs_invmass.bytes() + c->part_bd.bytes() + c->part_mv.bytes() + c->fsum.bytes() + c->obs.bytes() +
                c->avg.bytes() + c->sysstate.bytes() + c->fnb.bytes() + c->ftot.bytes() + c->nb_rows.bytes() +
                c->bd_rows.bytes() + c->bd_x.bytes() + c->nb_tabs.bytes() + c->nb_xoff.bytes() + c->bd_tabs.bytes() +
-               c->rdf.bytes() + c->rdf_samples.bytes() + c->mts_f.bytes() + c->mts_part.bytes();
+               c->rdf.bytes() + c->rdf_samples.bytes() + c->mts_f.bytes() + c->mts_part.bytes() + c->vs_com.bytes();
     for (const GroupDev &G : c->gr)
         b += G.nneb.bytes() + G.mem.bytes() + G.mem_cell.bytes() + G.cell_count.bytes() + G.cell_start.bytes() +
              G.cell_fill.bytes() + G.tile_sum.bytes() + G.tmp.bytes() + G.perm.bytes() + G.slot_cell.bytes() + G.inv_perm.bytes() +
@@ -633,7 +653,8 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
                + at_least1(bd_rows) * (16 + 8) + at_least1(h.bt.size() + h.at.size() + h.tt.size() + h.ot.size()) *
                                                      sizeof(BdTab)
                + at_least1(nb_rows) * (16 + 4) + at_least1(h.nt.size()) * sizeof(NbTab) +
-               at_least1((size_t)h.ntype * h.ntype) * 4;
+               at_least1((size_t)h.ntype * h.ntype) * 4
+               + (h.nvs > 0 ? 24 * N : 8);                          // vs_com
     if (rdf_nbin > 0) b += (size_t)rdf_nbin * 8 + 4;
     if (h.hybrid && h.nmts > 0 && h.ngroup > 1 && !h.members[1].empty())   // MTS slots
         b += 9 * N * 4 + 27 * (size_t)(pad_to((int)h.members[1].size(), kBlock) / kBlock) * (kBlock / kNbAtomsPerBlock) * 8;
@@ -740,7 +761,8 @@ void enqueue_forces(ibx_ctx *c, int mode, int step = 0) {
 }
 
 TermLaunch term_launch(const ibx_ctx *c) {
-    return TermLaunch{c->sys.p, c->term_mem.p, c->term_meta.p, c->term_slot.p, c->bd_tabs.p, c->bd_rows.p, c->bd_x.p};
+    return TermLaunch{c->sys.p, c->term_mem.p, c->term_meta.p, c->term_slot.p, c->bd_tabs.p, c->bd_rows.p, c->bd_x.p,
+                      c->vs_com.p};
 }
 
 void enqueue_terms(ibx_ctx *c) {
@@ -791,7 +813,8 @@ void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev, float nb_we
     if (ev) (void)hipEventRecord((*ev)[3], c->st);
     if (mode == 0 && c->gr[1].present && c->hs[0].hybrid)
         launch_vs_place(c->st, c->item_nwg, c->sys.p, c->wg_items.p, c->s.p, c->n_items, c->label.p, c->itype.p,
-                        c->vs_center.p, c->vs_moff.p, c->vs_members.p, c->mass.p, c->vs_invmass.p, c->gr[1].pos4.p,
+                        c->vs_center.p, c->vs_moff.p, c->vs_members.p, c->mass.p, c->vs_invmass.p, c->vs_com.p,
+                        c->gr[1].pos4.p,
                         c->gr[1].pos4_ref.p, c->gr[1].inv_perm.p, c->dmax.p + c->nsys);
     c->fin = fin_launch(c, mode);
     c->fin_pending = true;
@@ -1321,6 +1344,15 @@ int ibx_set_state(ibx_ctx *c, int sys, const double *s, const double *v) {
         for (int k = 0; k < 3; ++k) {
             for (int i = 0; i < h.nitems; ++i) tmp[i] = h.s[3 * (size_t)i + k];
             HIPCHK(c, memcpy_st(c, c->s.p + (size_t)k * N + o, tmp.data(), h.nitems * sizeof(double), hipMemcpyHostToDevice));
+        }
+        if (h.nvs > 0) {
+            std::vector<double> com(3 * (size_t)h.nvs);
+            for (int vv = 0; vv < h.nvs; ++vv) vs_centre_of_atoms(h, vv, &com[3 * (size_t)vv]);
+            for (int k = 0; k < 3; ++k) {
+                for (int vv = 0; vv < h.nvs; ++vv) tmp[vv] = com[3 * (size_t)vv + k];
+                HIPCHK(c, memcpy_st(c, c->vs_com.p + (size_t)k * N + o + h.natoms, tmp.data(), h.nvs * sizeof(double),
+                                    hipMemcpyHostToDevice));
+            }
         }
     }
     if (v) {

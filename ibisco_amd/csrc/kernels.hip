@@ -876,11 +876,12 @@ __global__ void __launch_bounds__(kBlock) k_nonbonded(IBX_NB_PARAMS) {
 struct TermArgs {
     const DevSys *sys;    // per-system constants (retired systems are skipped)
     const int4 *mem;      // members (unused -1)
-    const int *meta;      // kind | table << 2, -1 for padding
+    const int *meta;      // kind | table << 2 | virtual-site members << kTermVsShift, -1 for padding
     const int4 *slot;     // contribution slot of each member
     const BdTab *tabs;    // all bonded table headers
     const float4 *rows;   // all bonded rows {F_k, V_k, F_k+1, V_k+1}
     const double *xs;     // abscissa of every bonded row
+    const double *vs_com; // per virtual site: the mass-weighted centre of its atoms (SoA like s)
 };
 
 // Table row index from a non-negative abscissa ratio, clamped to [0, ndat]
@@ -1124,7 +1125,7 @@ __device__ __forceinline__ void terms_block(const int b, const int nwg, const in
         const int mr = (meta >= 0 && mem[r] >= 0) ? mem[r] : 0;
         for (int k = 0; k < 3; ++k) P[r][k] = s[k * n + mr];
     }
-    const BdTab tb = A.tabs[max(meta, 0) >> 2];
+    const BdTab tb = A.tabs[(max(meta, 0) >> 2) & kTermTabMask];
     issue_here(s, A.tabs, A.mem);
     if (!live) return;
     double en[19];   // 0..11 unused here: a term's energy is vt in class ek (see the sums below)
@@ -1158,6 +1159,13 @@ __device__ __forceinline__ void terms_block(const int b, const int nwg, const in
             if (r >= nm) break;
             contrib[sl[r]] = make_float4((float)fm[r][0], (float)fm[r][1], (float)fm[r][2], 0.f);
         }
+        // NEW_FORCE.F90:73-80 sums the bonded virial over atoms after DISTRIBUTE_VSFORCE.f90:27-37 has moved a
+        // virtual site's force to its atoms (m_j/M each): a site's force counts at the centre of its atoms
+        const int vsmask = meta >> kTermVsShift;
+        if (vsmask)
+            for (int r = 0; r < nm; ++r)
+                if (vsmask & (1 << r))
+                    for (int k = 0; k < 3; ++k) P[r][k] = A.vs_com[k * n + mem[r]];
 #pragma unroll
         for (int r = 1; r < 4; ++r) {
             if (r >= nm) break;
@@ -1370,6 +1378,7 @@ __global__ void __launch_bounds__(kBlock) k_vs_place(const DevSys *__restrict__ 
                                                      const int *__restrict__ itype, const int *__restrict__ vs_center,
                                                      const int *__restrict__ vs_moff, const int *__restrict__ vs_members,
                                                      const double *__restrict__ mass, const double *__restrict__ vs_invmass,
+                                                     double *__restrict__ vs_com,
                                                      int4 *__restrict__ pos4, const int4 *__restrict__ pos4_ref,
                                                      const int *__restrict__ inv_perm, int *__restrict__ dmax) {
     __shared__ float mx[kBlock / kWave];
@@ -1380,19 +1389,20 @@ __global__ void __launch_bounds__(kBlock) k_vs_place(const DevSys *__restrict__ 
     const int n = nitems_total;
     float disp = 0.f;
     if (itype[it] >= 0 && label[it] == 3) {
-        double p[3];
-        const int c = vs_center[it];
-        if (c >= 0) {
-            for (int k = 0; k < 3; ++k) p[k] = s[k * n + c];
-        } else {
-            double sum[3] = {0, 0, 0};
-            for (int q = vs_moff[it]; q < vs_moff[it + 1]; ++q) {
-                const int a = vs_members[q];
-                const double m = mass[a];
-                for (int k = 0; k < 3; ++k) sum[k] += m * s[k * n + a];
-            }
-            for (int k = 0; k < 3; ++k) p[k] = sum[k] * vs_invmass[it];
+        // the mass-weighted centre of the site's atoms: the site itself when it is defined as one, and where a
+        // bonded force on it counts in the virial
+        double p[3], sum[3] = {0, 0, 0};
+        for (int q = vs_moff[it]; q < vs_moff[it + 1]; ++q) {
+            const int a = vs_members[q];
+            const double m = mass[a];
+            for (int k = 0; k < 3; ++k) sum[k] += m * s[k * n + a];
         }
+        for (int k = 0; k < 3; ++k) {
+            sum[k] *= vs_invmass[it];
+            vs_com[k * n + it] = sum[k];
+        }
+        const int c = vs_center[it];
+        for (int k = 0; k < 3; ++k) p[k] = c >= 0 ? s[k * n + c] : sum[k];
         for (int k = 0; k < 3; ++k) s[k * n + it] = p[k];
         const int slot = inv_perm[it];
         if (slot >= 0) {
@@ -1778,7 +1788,7 @@ void launch_nonbonded(hipStream_t st, const NbArgs &a) {
 
 void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &b, const double *s, int nitems_total,
                   int nterms, float4 *contrib, int ncontrib, double *part, int *flags) {
-    TermArgs A{b.sys, b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs};
+    TermArgs A{b.sys, b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs, b.vs_com};
     hipLaunchKernelGGL(k_terms, dim3(nwg), dim3(kBlock), 0, st, wg_sys, A, s, nitems_total, nterms, contrib, ncontrib,
                        part, flags);
 }
@@ -1797,10 +1807,10 @@ void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, 
 
 void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, double *s, int nitems_total,
                      const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,
-                     const double *mass, const double *vs_invmass, int4 *pos4, const int4 *pos4_ref,
+                     const double *mass, const double *vs_invmass, double *vs_com, int4 *pos4, const int4 *pos4_ref,
                      const int *inv_perm, int *dmax) {
     hipLaunchKernelGGL(k_vs_place, dim3(nwg), dim3(kBlock), 0, st, sys, wg_sys, s, nitems_total, label, itype, vs_center,
-                       vs_moff, vs_members, mass, vs_invmass, pos4, pos4_ref, inv_perm, dmax);
+                       vs_moff, vs_members, mass, vs_invmass, vs_com, pos4, pos4_ref, inv_perm, dmax);
 }
 
 void launch_finalize(hipStream_t st, const FinLaunch &f) {
@@ -1831,7 +1841,7 @@ void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const in
         multi = nb->multi; lds = nb->max_rows_lds > 0; cubic = nb->cubic;
         bytes = std::max(lds ? (size_t)nb->max_rows_lds * sizeof(float4) : 0, (size_t)kNbFields * kBlock * sizeof(float));
     }
-    TermLaunchK t{wg_terms, TermArgs{b.sys, b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs}, s, nitems_total, nterms,
+    TermLaunchK t{wg_terms, TermArgs{b.sys, b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs, b.vs_com}, s, nitems_total, nterms,
                   contrib, ncontrib, part_bd, flags, term_nwg};
     FinLaunch f{};
     const int fin_blocks = fin ? fin->nsys * kFields : 0;

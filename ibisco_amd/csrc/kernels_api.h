@@ -61,6 +61,7 @@ struct TermLaunch {
     const BdTab *tabs;
     const float4 *rows;
     const double *xs;
+    const double *vs_com;   // per virtual site: the mass-weighted centre of its atoms
 };
 
 struct MoveLaunch {
@@ -107,7 +108,7 @@ void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &
 void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const MoveLaunch &m);
 void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, double *s, int nitems_total,
                      const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,
-                     const double *mass, const double *vs_invmass, int4 *pos4, const int4 *pos4_ref,
+                     const double *mass, const double *vs_invmass, double *vs_com, int4 *pos4, const int4 *pos4_ref,
                      const int *inv_perm, int *dmax);
 void launch_finalize(hipStream_t st, const FinLaunch &f);
 // The force phase of a step in one launch: the group-0 nonbonded force (nb null: none this step), the bonded

@@ -316,6 +316,7 @@ void ox_destroy(ox_ctx *c);
 
 ox_ctx *ox_create(const ox_desc *d, char *err, int errlen) {
     ox_ctx *c = (ox_ctx *)calloc(1, sizeof(ox_ctx));
+    c->respa = 1;
     c->natoms = d->natoms; c->nvs = d->nvs; c->nitems = d->natoms + d->nvs; c->ntype = d->ntype;
     c->hybrid = d->nvs > 0;
     int n = c->nitems;
@@ -476,7 +477,7 @@ void ox_destroy(ox_ctx *c) {
     for (int g = 0; g < 2; ++g) { free(c->g[g].nneb); free(c->g[g].off); free(c->g[g].nb); }
     free(c->atom_list); free(c->bead_list);
     free(c->vs_type); free(c->vs_center); free(c->vs_off); free(c->vs_atoms);
-    free(c->vs_invmass); free(c->vs_coeff); free(c->mts_f);
+    free(c->vs_invmass); free(c->vs_coeff); free(c->mts_f); free(c->respa_f);
     free(c);
 }
 
@@ -861,6 +862,11 @@ static int bonded(ox_ctx *c) {
             else if (cost < -1.0) cost = -1.0;
             double phi = acos(cost) * R2D;
             if (signt < 0) phi = 360.0 - phi;
+            if (!(phi >= 0.0 && phi <= 360.0)) {   /* NaN: collinear atoms the 0.99 perturbation left collinear */
+                snprintf(c->err, sizeof c->err, "torsion %d-%d-%d-%d undefined", I + 1, J + 1, K + 1, L + 1);
+                free(fl);
+                return -1;
+            }
             int ni = (int)(phi / t->bin);
             if (ni > t->ndat) { snprintf(c->err, sizeof c->err, "torsion table %d overflow", tt + 1); free(fl); return -1; }
             double fijkl, vijkl;
@@ -1062,6 +1068,37 @@ void ox_momentum(ox_ctx *c) {
         for (int k = 0; k < 3; ++k) c->v[3 * i + k] -= sum[k] * c->invmass[c->itype[i]];
 }
 
+/* Multiple time step for non-hybrid systems (r-RESPA in leap-frog form; no counterpart in the reference,
+ * so no reference file:line): the nonbonded force is evaluated on steps 1, k+1, 2k+1, ... and applied with
+ * weight k (the two half kicks of the slow force around an outer step boundary merged into one), the bonded
+ * terms on every step.  The other steps report the last evaluation's nonbonded energy and virial.  With
+ * k = 1 this is NEW_FORCE.  Pinned by k = 1 against the plain loop and by time reversibility. */
+static int respa_force(ox_ctx *c, int step) {
+    const int nb_step = (step - 1) % c->respa == 0;
+    const size_t m = 3 * (size_t)c->nitems;
+    zero_force_obs(c);
+    if (nb_step) {
+        nonbonded(c, &c->g[0]);
+        memcpy(c->respa_f, c->f, sizeof(double) * m);
+        for (int k = 0; k < 3; ++k) c->respa_vnb[k] = c->vnb[k];
+        for (int k = 0; k < 6; ++k) c->respa_pt[k] = c->pt[k];
+    } else {
+        for (int k = 0; k < 3; ++k) c->vnb[k] = c->respa_vnb[k];
+        for (int k = 0; k < 6; ++k) c->pt[k] = c->respa_pt[k];
+    }
+    if (finish_force(c)) return -1;
+    if (nb_step)
+        for (size_t q = 0; q < m; ++q) c->f[q] += (c->respa - 1) * c->respa_f[q];
+    return 0;
+}
+
+int ox_set_respa(ox_ctx *c, int k) {
+    if (k < 1 || (k > 1 && (c->hybrid || c->nmts > 0))) return -1;
+    c->respa = k;
+    if (!c->respa_f) c->respa_f = (double *)calloc(3 * (size_t)c->nitems, sizeof(double));
+    return 0;
+}
+
 /* NEW_LOOP.F90:19-156 and MTS_LOOP.f90:13-60 (NVE/NVT; no barostat) */
 int ox_run(ox_ctx *c, int nsteps, int step0) {
     for (int step = step0 + 1; step <= step0 + nsteps; ++step) {
@@ -1071,6 +1108,8 @@ int ox_run(ox_ctx *c, int nsteps, int step0) {
             if (ox_update_lists(c, 0)) return -1;
         if (c->nmts > 0 && c->hybrid) {
             if (mts_force(c, step)) return -1;
+        } else if (c->respa > 1) {
+            if (respa_force(c, step)) return -1;
         } else {
             zero_force_obs(c);
             nonbonded(c, &c->g[0]);

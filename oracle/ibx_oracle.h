@@ -84,6 +84,9 @@ void ox_virtual_def(ox_ctx *c);
 void ox_new_force(ox_ctx *c);
 void ox_move(ox_ctx *c);
 void ox_momentum(ox_ctx *c);
+/* Multiple time step of non-hybrid systems: nonbonded forces on steps 1, k+1, ... with weight k, bonded
+ * terms every step (r-RESPA; no counterpart in the reference).  Returns -1 for k < 1 or hybrid systems. */
+int ox_set_respa(ox_ctx *c, int k);
 /* nsteps of NEW_LOOP (or MTS_LOOP when nmts>0) starting after step0. */
 int ox_run(ox_ctx *c, int nsteps, int step0);
 

@@ -60,6 +60,7 @@ def _declare(L):
     L.ox_exclusions.argtypes = [C.c_void_p, _ip, _ip, C.c_int]
     L.ox_terms.argtypes = [C.c_void_p, C.c_int, _ip, C.c_int]
     L.ox_run.argtypes = [C.c_void_p, C.c_int, C.c_int]
+    L.ox_set_respa.argtypes = [C.c_void_p, C.c_int]
     L.ox_get_obs.argtypes = [C.c_void_p, _dp]
     L.ox_get_obs.restype = None
     L.ox_nitems.argtypes = [C.c_void_p]
@@ -188,6 +189,11 @@ class Oracle:
     def run(self, nsteps, step0=0):
         if self.L.ox_run(self.h, nsteps, step0):
             raise RuntimeError("oracle: run failed")
+
+    def set_respa(self, k):
+        """Nonbonded forces every k steps with weight k, bonded every step (ox_set_respa)."""
+        if self.L.ox_set_respa(self.h, int(k)):
+            raise RuntimeError("oracle: invalid multiple-time-step ratio")
 
     def move(self):
         self.L.ox_move(self.h)

@@ -68,6 +68,11 @@ struct ox_ctx {
     double pt[6];   /* 11 22 33 12 13 23 */
     double ek[2], temp, temp_atom, temp_bead;
     long overext;
+    /* multiple time step (r-RESPA, no counterpart in the reference): nonbonded forces of the last
+     * evaluation, its energy and virial */
+    int respa;
+    double *respa_f;         /* [nitems][3] */
+    double respa_vnb[3], respa_pt[6];
     /* MTS history */
     double *mts_f;           /* [3][ncoarse][3] */
     double mts_vnb[3][3], mts_pt[3][6];

@@ -18,6 +18,7 @@ Inputs are stored in single precision, which is what the reference's REAL*4
 readers see, so the oracle and the reference start from identical numbers.
 
     python tests/golden/make_golden.py [case ...]
+    python tests/golden/make_golden.py cg10k_nve_drift STATE.npz   (a state dump: s nm, v reduced)
 """
 from __future__ import annotations
 
@@ -175,6 +176,105 @@ def mts_case():
     return s
 
 
+def virt_case():
+    """hybridPA with a virtbonds and a virtangles file (RDVIRTBONDS.f90:42-64, RDVIRTANGLES.f90:42-50): each
+    bead's nearest virtual site within 0.6 nm (unwrapped, as bonded terms see them) excluded from it, and
+    angles bead-site-bead over the two nearest beads of sites that have two within 0.6 nm, with harmonic
+    angle tables (theta0 120 degrees, 40 kJ/mol/rad^2) added for each type triple they use.  NVE."""
+    s = testsuite_case("hybridPA")
+    s.control.ensemble = "NVE"
+    lab = np.array(s.type_label)[s.itype]
+    vs_types = set(int(t) for t in s.vs_type)
+    beads = np.array([i for i in range(s.natoms) if lab[i] == "B" and int(s.itype[i]) not in vs_types])
+    vpos = s.s[s.vs_center]
+    d = np.linalg.norm(vpos[:, None, :] - s.s[beads][None, :, :], axis=2)   # unwrapped, no minimum image
+    vb, va = [], []
+    for k in range(s.nvs):
+        near = np.argsort(d[k])[:2]
+        near = [int(beads[q]) for q in near if d[k, q] < 0.6]
+        if near:
+            vb.append((s.natoms + k, near[0]))
+        if len(near) == 2 and len(va) < 200:
+            va.append((near[0], s.natoms + k, near[1]))
+    s.virtbonds = np.array(vb, np.int32)
+    s.virtangles = np.array(va, np.int32)
+    triples = sorted({(int(s.itype[i]), int(s.vs_type[j - s.natoms]), int(s.itype[k])) for i, j, k in va})
+    have = {tuple(t.types) for t in s.angle_tables}
+    x, v = system.harmonic_angle_table(120.0, 40.0)
+    for t in triples:
+        if t not in have and t[::-1] not in have:
+            s.angle_tables.append(system.Table(t, x, v, f"vang{len(s.angle_tables)}"))
+            have.add(t)
+    return s
+
+
+def collinear_case():
+    """A UA torsion melt whose chains end in a head type (1) and a tail type (2) without angle tables
+    (their end angles only exclude, SETLIS.f90) but with torsion tables, and in which chain 0's first three
+    atoms and chain 1's last three atoms lie exactly on a line (coordinates exact in single precision):
+    the torsions there take the degenerate branches of TORSIONS.inc:47-59 (RM = 0) and :75-87 (RN = 0)."""
+    s = system.ua_melt(nchains=20, nbeads=100, seed=5, avoid=0.6)
+    n = s.natoms
+    starts = np.cumsum([0] + list(s.mol_sizes[:-1]))
+    s.itype = s.itype.copy()
+    s.itype[starts] = 1
+    s.itype[starts + np.array(s.mol_sizes) - 1] = 2
+    s.type_names = list(s.type_names) + ["CH3h", "CH3t"]
+    s.type_mass = np.concatenate([s.type_mass, [s.type_mass[0], s.type_mass[0]]])
+    s.type_label = list(s.type_label) + [s.type_label[0], s.type_label[0]]
+    nb0, b0, t0 = s.nb_tables[0], s.bond_tables[0], s.torsion_tables[0]
+    s.nb_tables = [system.Table((a, b), nb0.x, nb0.v, f"nb{a}{b}") for a in range(3) for b in range(a, 3)]
+    s.bond_tables = [system.Table((a, 0), b0.x, b0.v, f"bond{a}") for a in range(3)]
+    s.torsion_tables = [system.Table((0, 0, 0, 0), t0.x, t0.v, "tors0"),
+                        system.Table((1, 0, 0, 0), t0.x, t0.v, "torsh"),
+                        system.Table((0, 0, 0, 2), t0.x, t0.v, "tors_t")]
+    # The reference breaks the line by scaling one x coordinate by 0.99, so neither line may run along x:
+    # chain 0's head along (1, 1, 0), chain 1's tail along y.
+    q = 1.0 / 64.0   # exact in single precision
+    a0 = int(starts[0])
+    base = np.round(s.s[a0 + 1] / q) * q
+    for m in range(3):
+        s.s[a0 + m] = base + (m - 1) * np.array([6 * q, 6 * q, 0.0])   # 0.133 nm apart
+    e1 = int(starts[1] + s.mol_sizes[1] - 1)   # chain 1's last atom
+    base = np.round(s.s[e1 - 1] / q) * q
+    for m in range(3):
+        s.s[e1 - 2 + m] = base + (m - 1) * np.array([0.0, 10 * q, 0.0])   # 0.156 nm apart
+    return s
+
+
+def drift_fit(etot, dt_ps):
+    """Slope (per ps) of a straight-line fit to a total-energy series sampled every step."""
+    t = np.arange(1, len(etot) + 1) * dt_ps
+    return float(np.polyfit(t, etot, 1)[0])
+
+
+def drift_case(state_path, nsteps=1000):
+    """The 10,000-bead CG melt (200 x 50, 8 beads/nm^3, LJ 0.47 nm / 3.5 kJ/mol to 1.5 nm, list 1.8 nm every 10
+    steps, 5 fs, NVE) from an equilibrated state (positions nm, velocities reduced, as a state dump of the
+    device path holds them), run for ``nsteps`` by the reference's own routines: its per-step energies and the
+    drift of its total energy (V_NB + V_BOND + V_ANGLE + EK, reduced units) per ps."""
+    from ibisco_amd import workloads
+    sysm = workloads.cg_melt_10k()
+    st = np.load(state_path)
+    sysm.s = st["s"][:sysm.natoms].astype(np.float64)
+    sysm.v = units.vel_from_reduced(st["v"].astype(np.float64))
+    sysm = single(sysm)
+    d = tempfile.mkdtemp(prefix="ibx_drift_")
+    try:
+        system.write_dir(sysm, d)
+        run_harness(d, "run", str(nsteps))
+        obs = np.fromfile(os.path.join(d, "ref_obs.bin"), "<f8").reshape(nsteps, -1)
+        etot = obs[:, 0:15].sum(1) + obs[:, 21] + obs[:, 22]
+        slope = drift_fit(etot, sysm.control.dt)
+        data = {"ref_obs": obs, "ref_etot": etot, "ref_drift_per_ps": np.array(slope), "nsteps": np.array(nsteps)}
+        data.update({"sys_" + k: v for k, v in system.to_arrays(sysm, f32=True).items()})
+        np.savez_compressed(os.path.join(HERE, "cg10k_nve_drift.npz"), **data)
+        print(f"cg10k_nve_drift: {sysm.natoms} beads, {nsteps} steps, reference drift {slope:.4e} per ps "
+              f"(E from {etot[0]:.6e} to {etot[-1]:.6e})")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def make_case(name, sysm, runs=()):
     sysm = single(sysm)
     d = tempfile.mkdtemp(prefix="ibx_gold_")
@@ -208,6 +308,13 @@ def main(only=()):
                 make_case("cg_ortho", ortho_case(), runs=(10,))
             if name == "hybridPA_mts":
                 make_case("hybridPA_mts", mts_case(), runs=(12,))
+            if name == "cg10k_nve_drift":   # needs the state file after the name
+                drift_case(only[only.index(name) + 1])
+                return
+            if name == "hybridPA_virt":
+                make_case("hybridPA_virt", virt_case(), runs=(10,))
+            if name == "ua_collinear":
+                make_case("ua_collinear", collinear_case(), runs=(10,))
             if name == "comb_oop_only":
                 make_case("comb_oop_only", system.comb_melt(nchains=36, seed=1, oop_only=True), runs=(10,))
         return
@@ -230,6 +337,8 @@ def main(only=()):
     make_case("comb_oop_only", system.comb_melt(nchains=36, seed=1, oop_only=True), runs=(10,))
     make_case("hybridPA_mts", mts_case(), runs=(12,))
     make_case("cg_ortho", ortho_case(), runs=(10,))
+    make_case("hybridPA_virt", virt_case(), runs=(10,))
+    make_case("ua_collinear", collinear_case(), runs=(10,))
 
 
 if __name__ == "__main__":

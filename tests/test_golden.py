@@ -31,7 +31,7 @@ from ibisco_amd.units import vel_from_reduced
 
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 CASES = ("cg2000", "ua2000_excl5", "cg1500_strained", "twoatombox", "hybridPA", "atomisticPA", "comb_oop",
-         "comb_oop_only", "cg_ortho")
+         "comb_oop_only", "cg_ortho", "hybridPA_virt", "ua_collinear")
 
 FP32_R2_TOL = 1e-5        # relative band around rlist^2 / rc^2 that FP32 folding and distances can cross
 FORCE_RTOL = 5e-5         # FP32 reference vs FP64: measured 3e-6 .. 1.8e-5 relative RMS
@@ -260,7 +260,7 @@ def test_energies_and_virial(case, oracle_mod):
 
 
 RUNS = [("cg2000", 10), ("twoatombox", 40), ("atomisticPA", 10), ("atomisticPA", 40), ("comb_oop", 10),
-        ("comb_oop_only", 10), ("cg_ortho", 10)]
+        ("comb_oop_only", 10), ("cg_ortho", 10), ("hybridPA_virt", 10), ("ua_collinear", 10)]
 
 
 @pytest.mark.parametrize("case,n", RUNS)

@@ -3,7 +3,10 @@
 The fixtures hold the reference's own testsuite inputs (hybridPA with 960
 virtual sites and two neighbour groups, atomisticPA with torsions and NVT,
 twoatombox), a torsion melt with ``non_bonded 5``, comb polymers with
-out-of-plane terms and a melt in an orthorhombic box.  The oracle is pinned to
+out-of-plane terms, a melt in an orthorhombic box, a strained melt whose
+over-extended bonds take the clamp of BONDS.inc:21-31, hybridPA with virtbonds
+and virtangles files, and a torsion melt with exactly collinear atoms (the
+0.99 perturbations of TORSIONS.inc:47-59 and :75-87).  The oracle is pinned to
 the reference's outputs on these inputs in test_golden.py; here the device path
 must match the oracle to the same gates as test_gpu_parity.py.  Forces do not
 depend on the ensemble, so the NPT input (hybridPA, which the device path does
@@ -33,7 +36,7 @@ def _load(name):
 
 
 @pytest.mark.parametrize("case", ["twoatombox", "ua2000_excl5", "hybridPA", "atomisticPA", "comb_oop", "comb_oop_only",
-                                  "cg_ortho"])
+                                  "cg_ortho", "cg1500_strained", "hybridPA_virt", "ua_collinear"])
 def test_golden_inputs_forces(case, oracle_mod):
     sysm = _load(case)
     print("case", case, flush=True)
@@ -52,6 +55,8 @@ def test_golden_inputs_forces(case, oracle_mod):
     print(case, "force rel rms", rr)
     assert rr <= FORCE_RTOL
     oo, og = o.obs(), e.obs()
+    print(case, "bond over-extensions", og["overext"], oo["overext"])
+    assert og["overext"] == oo["overext"]   # BONDS.inc:21-31 clamp count
     epot = sum(float(np.abs(oo[k]).sum()) for k in ("vnb", "vbond", "vangle", "vtors", "voop"))
     for k in ("vnb", "vbond", "vangle", "vtors", "voop"):
         scale = max(float(np.abs(oo[k]).sum()), 1e-3 * epot)

@@ -115,3 +115,59 @@ def test_energy_conservation_against_plain_steps():
     assert res["respa_4x1fs"][0] <= 5.0 * res["plain_1fs"][0] + 1e-4 * scale
     assert res["respa_4x1fs"][0] < res["plain_4fs"][0]
     assert abs(res["respa_4x1fs"][2] - res["plain_1fs"][2]) <= 0.1 * res["plain_1fs"][2]
+
+
+def test_respa_matches_oracle(oracle_mod):
+    """k = 4 on the equilibrated UA melt for three cycles, step by step against the FP64 oracle's RESPA (itself
+    pinned by time reversibility and the impulse, test_respa_oracle.py): energies and virial within 1e-7 of
+    their scale, final positions and velocities as in the plain short runs."""
+    s = _ua()
+    e = Engine(s)
+    e.set_respa(4)
+    o = oracle_mod.Oracle(s)
+    o.set_respa(4)
+    for k in range(12):
+        e.step(1)
+        o.run(1, k)
+        og, oo = e.obs(), o.obs()
+        eg = np.concatenate([og[q] for q in ("vnb", "vbond", "vangle", "vtors")])
+        eo = np.concatenate([oo[q] for q in ("vnb", "vbond", "vangle", "vtors")])
+        scale = np.abs(eo).sum()
+        assert np.all(np.abs(eg - eo) <= 1e-7 * scale), (k + 1, eg, eo)
+        assert np.all(np.abs(og["pt"] - oo["pt"]) <= 1e-7 * np.abs(oo["pt"][:3]).sum()), (k + 1, og["pt"], oo["pt"])
+    sg, vg = e.state()
+    so, vo, _ = o.state()
+    ds = np.abs(sg - so).max()
+    dv = float(np.linalg.norm(vg - vo) / np.linalg.norm(vo))
+    print("RESPA k=4, 12 steps: max |ds|", ds, "velocity rel", dv, flush=True)
+    assert ds <= 1e-6 and dv <= 1e-5
+
+
+def test_respa_time_reversible():
+    """On the device: 40 steps of k = 4 forward, then from (x_40, -v_{40+1/2}) 40 steps again return to x_0
+    (FP32 forces, FP64 integration: to well under the 1e-3 nm a misplaced kick would leave)."""
+    import copy
+    s = _ua()
+    e = Engine(s)
+    e.set_respa(4)
+    s0, _ = e.state()
+    e.step(1)
+    _, v_half = e.state()
+    e.close()
+    e = Engine(s)
+    e.set_respa(4)
+    e.step(40)
+    sn, _ = e.state()
+    e.step(1)
+    _, vn1 = e.state()
+    e.close()
+    rev = copy.copy(s)
+    rev.s, rev.v = sn[:s.natoms].copy(), units.vel_from_reduced(-vn1)
+    r = Engine(rev)
+    r.set_respa(4)
+    r.step(40)
+    sr, vr = r.state()
+    ds = np.abs(sr - s0).max()
+    dv = np.abs(vr + v_half).max() / np.abs(v_half).max()
+    print("device RESPA k=4: 40 steps forward and back, max |x - x0|", ds, "velocity", dv, flush=True)
+    assert ds <= 1e-6 and dv <= 1e-5

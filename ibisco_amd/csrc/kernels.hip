@@ -1145,10 +1145,10 @@ __device__ __forceinline__ void terms_block(const int b, const int nwg, const in
             angle_term<double>(tb, A.rows, A.xs, P, fm, V);
             nm = 3;
         } else if (kind == KTORS) {
-            torsion_term<double>(tb, A.rows, A.xs, P, fm, V, flags);
+            torsion_term<double>(tb, A.rows, A.xs, P, fm, V, flags + 1);   // word 1: table bits
             nm = 4;
         } else {
-            oop_term<double>(tb, A.rows, A.xs, P, fm, V, flags);
+            oop_term<double>(tb, A.rows, A.xs, P, fm, V, flags + 1);
             nm = 4;
         }
         ek = 3 * kind + tb.label - 1;

@@ -126,9 +126,10 @@ def accuracy(e, systems, idx):
     import copy
     import numpy as np
     from ibisco_amd import units
-    from oracle.oracle import Oracle
+    from oracle.oracle import Oracle, nb_virial_magnitude
     e.compute_forces(rebuild=True)
-    worst = {"force_rel_rms": 0.0, "energy_rel": 0.0, "virial_rel": 0.0, "pressure_rel": 0.0}
+    worst = {"force_rel_rms": 0.0, "energy_rel": 0.0, "virial_rel": 0.0, "virial_rel_of_magnitude": 0.0,
+             "pressure_rel": 0.0}
     for k in idx:
         sy = systems[k]
         s, v = e.state(k)
@@ -145,10 +146,17 @@ def accuracy(e, systems, idx):
         tr = abs(po[:3].sum())
         worst["force_rel_rms"] = max(worst["force_rel_rms"], float(np.linalg.norm(fg - fo) / np.linalg.norm(fo)))
         worst["energy_rel"] = max(worst["energy_rel"], abs(eg - eo) / abs(eo))
+        worst["pressure_rel"] = float(worst["pressure_rel"])
         worst["virial_rel"] = max(worst["virial_rel"], float(np.abs(pg - po).max()) / tr)
+        mag = nb_virial_magnitude(o, one)
+        if mag:
+            worst["virial_rel_of_magnitude"] = max(worst["virial_rel_of_magnitude"],
+                                                   float(np.abs(pg - po).max()) / mag)
         worst["pressure_rel"] = max(worst["pressure_rel"], abs(pg[:3].sum() - po[:3].sum()) / tr)
     worst.update(systems_checked=len(idx), reference="FP64 oracle (oracle/ibx_oracle.c) on the same positions",
-                 gates={"force_rel_rms": 1e-5, "energy_rel": 1e-7, "virial_rel": 1e-7})
+                 gates={"force_rel_rms": 1e-5, "energy_rel": 1e-7, "virial_rel": 1e-7},
+                 virial_scales="virial_rel: max |dPT| / |PT11+PT22+PT33| (the net virial); virial_rel_of_magnitude: "
+                               "max |dPT| / sum over in-cutoff pairs of |F| d_k^2 (the terms it sums)")
     return worst
 
 

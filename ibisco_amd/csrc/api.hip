@@ -91,7 +91,6 @@ struct ibx_ctx {
     DBuf<int> wg_items, itype, label, excl_off, excl, vs_of, vs_center, vs_moff, vs_members;
     DBuf<int> term_meta, inc_off, wg_terms, nb_map, flags, step;
     DBuf<int> dmax;                  // [group][system] largest displacement^2 since the rebuild (float bits)
-    DBuf<int> fin_done;              // per system: field-sum workgroups finished in this launch
     DBuf<int4> term_slot;           // contribution slot of each member of a term
     DBuf<int4> term_mem;
     DBuf<float4> contrib;           // bonded contribution slots {x, y, z, 0}
@@ -487,8 +486,6 @@ int build_device(ibx_ctx *c) {
     HIPCHK(c, memset_st(c, c->part_bd.p, 0, c->part_bd.bytes()));
     HIPCHK(c, memset_st(c, c->part_mv.p, 0, c->part_mv.bytes()));
     HIPCHK(c, c->fsum.alloc((size_t)S * kFields));
-    HIPCHK(c, c->fin_done.alloc((size_t)S));
-    HIPCHK(c, memset_st(c, c->fin_done.p, 0, c->fin_done.bytes()));
     HIPCHK(c, c->obs.alloc((size_t)S * kObs));
     HIPCHK(c, c->avg.alloc((size_t)S * kObs));
     HIPCHK(c, c->sysstate.alloc((size_t)S * 8));
@@ -612,7 +609,7 @@ size_t device_bytes(const ibx_ctx *c) {
     size_t b = c->sys.bytes() + c->wg_items.bytes() + c->itype.bytes() + c->label.bytes() + c->excl_off.bytes() +
                c->excl.bytes() + c->vs_of.bytes() + c->vs_center.bytes() + c->vs_moff.bytes() + c->vs_members.bytes() +
                c->term_meta.bytes() + c->inc_off.bytes() + c->wg_terms.bytes() + c->nb_map.bytes() + c->flags.bytes() +
-               c->step.bytes() + c->dmax.bytes() + c->fin_done.bytes() + c->term_slot.bytes() + c->term_mem.bytes() + c->contrib.bytes() +
+               c->step.bytes() + c->dmax.bytes() + c->term_slot.bytes() + c->term_mem.bytes() + c->contrib.bytes() +
                c->s.bytes() + c->v.bytes() + c->mass.bytes() + c->invmass.bytes() + c->vs_coeff.bytes() +
                c->vs_invmass.bytes() + c->part_bd.bytes() + c->part_mv.bytes() + c->fsum.bytes() + c->obs.bytes() +
                c->avg.bytes() + c->sysstate.bytes() + c->fnb.bytes() + c->ftot.bytes() + c->nb_rows.bytes() +
@@ -648,8 +645,8 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
                + nterm_pad * (4 + 16 + 16) + (nterm_pad / kBlock) * 4   // meta, members, slots, wg_terms
                + at_least1(members) * 16                           // contributions
                + 2 * (nterm_pad / kBlock) * kBdPart * 8 + (N / kBlock) * kMvPart * 8
-               + (size_t)(kFields + 2 * kObs + 8) * 8 + 2 * 4 + 8 * 4 + 4 + 4   // fsum obs avg sysstate dmax flags
-                                                                           // step fin_done
+               + (size_t)(kFields + 2 * kObs + 8) * 8 + 2 * 4 + 8 * 4 + 4   // fsum obs avg sysstate dmax flags
+                                                                       // step
                + at_least1(bd_rows) * (16 + 8) + at_least1(h.bt.size() + h.at.size() + h.tt.size() + h.ot.size()) *
                                                      sizeof(BdTab)
                + at_least1(nb_rows) * (16 + 4) + at_least1(h.nt.size()) * sizeof(NbTab) +
@@ -779,7 +776,7 @@ FinLaunch fin_launch(ibx_ctx *c, int mode) {
     f.sys = c->sys.p; f.part_nb0 = c->gr[0].part_buf(c->nb_par); f.part_nb1 = G1.part_buf(c->nb_par);
     f.part_bd = c->part_bd_buf(c->bd_par); f.part_mv = c->part_mv.p;
     f.snb0 = per * c->gr[0].nwg; f.snb1 = per * G1.nwg; f.sbd = c->term_nwg; f.smv = c->item_nwg;
-    f.fsum = c->fsum.p; f.done = c->fin_done.p; f.obs = c->obs.p; f.avg = c->avg.p; f.sysstate = c->sysstate.p;
+    f.fsum = c->fsum.p; f.obs = c->obs.p; f.avg = c->avg.p; f.sysstate = c->sysstate.p;
     f.step = c->step.p; f.mode = mode; f.nsys = c->nsys;
     return f;
 }
@@ -1127,7 +1124,6 @@ int ibx_step_noalloc(ibx_ctx *c, int nsteps, int step0) {
     r.add(c->part_bd);
     r.add(c->part_mv);
     r.add(c->fsum);
-    r.add(c->fin_done);
     if (r.total > kScratchBytes)
         return fail(c, IBX_E_INVALID,
                     "ibx_step_noalloc: this context needs %zu bytes of scratch, more than the %zu of the static array",
@@ -1141,7 +1137,7 @@ int ibx_step_noalloc(ibx_ctx *c, int nsteps, int step0) {
     HIPCHK(c, hipMemcpyAsync(c->step.p, &step0, sizeof(int), hipMemcpyHostToDevice, c->st));
     for (int k = 0; k < nsteps; ++k) enqueue_step(c, step0 + 1 + k, nullptr);
     flush_fin(c);
-    for (auto &x : r.e)   // and hand the partials and counters back to the context
+    for (auto &x : r.e)   // and hand the partials and field sums back to the context
         HIPCHK(c, hipMemcpyAsync(x.own, base + x.off, x.bytes, hipMemcpyDeviceToDevice, c->st));
     c->counters[0] += nsteps;
     return check_flags(c);   // waits for the stream, so no two calls overlap on the array

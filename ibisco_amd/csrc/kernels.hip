@@ -1432,8 +1432,7 @@ __device__ __forceinline__ double strided_sum(const double *__restrict__ p, int 
 
 // Observables of one system from its field sums (AVERAGE.f90:10-47 window
 // sums included), the temperatures the next MOVE uses, and the MOMENTUM
-// correction when the step is a HALT_DRIFT step.  red was written by other
-// workgroups of the launch: the caller's device-scope fence comes first.
+// correction when the step is a HALT_DRIFT step.
 __device__ void finalize_system(const DevSys &S, int s_id, const double *red, double *__restrict__ obs,
                                 double *__restrict__ avg, double *__restrict__ sysstate, const int *__restrict__ step,
                                 int mode) {
@@ -1476,49 +1475,48 @@ __device__ void finalize_system(const DevSys &S, int s_id, const double *red, do
     a[28] += 1.0;
 }
 
-// Per-system sums of the workgroup partials: one workgroup per (system,
-// field), fields 0..8 nonbonded (both groups), 9..27 bonded, 28..42
-// integrate.  Each sum is taken in a fixed order, so results are deterministic.
-// The last workgroup of a system to finish (a per-system counter, reset by that
-// workgroup for the next launch) forms the system's observables.
+// Per-system sums of the workgroup partials: one workgroup per system, whose
+// wave w sums fields w, w + 4, ... (0..8 nonbonded of both groups, 9..27
+// bonded, 28..42 integrate), each in a fixed order (the lanes stride over the
+// partials, then a fixed-order wave sum), so results are deterministic and a
+// system's sums do not depend on the other systems of a batch.  The workgroup
+// then forms the system's observables.  Workgroups past nsys (the grid is a
+// whole number of XCD rounds) do nothing.
 #define IBX_FIN_PARAMS                                                                                          \
     const DevSys *__restrict__ sys, const double *__restrict__ part_nb0, const double *__restrict__ part_nb1,      \
         const double *__restrict__ part_bd, const double *__restrict__ part_mv, int snb0, int snb1, int sbd,      \
-        int smv, double *__restrict__ fsum, int *__restrict__ done, double *__restrict__ obs,                     \
-        double *__restrict__ avg, double *__restrict__ sysstate, const int *__restrict__ step, int mode
-#define IBX_FIN_ARGS sys, part_nb0, part_nb1, part_bd, part_mv, snb0, snb1, sbd, smv, fsum, done, obs, avg, sysstate, \
-                     step, mode
+        int smv, double *__restrict__ fsum, double *__restrict__ obs, double *__restrict__ avg,                   \
+        double *__restrict__ sysstate, const int *__restrict__ step, int mode, int nsys
+#define IBX_FIN_ARGS sys, part_nb0, part_nb1, part_bd, part_mv, snb0, snb1, sbd, smv, fsum, obs, avg, sysstate, step, \
+                     mode, nsys
 
 // Workgroup b of the field sums (also one role of k_step_forces).
 __device__ __forceinline__ void field_sums_block(const int b, IBX_FIN_PARAMS) {
-    __shared__ double red[kBlock / kWave];
-    const int s_id = b / kFields, f = b % kFields;
+    __shared__ double red[kFields];
+    const int s_id = b;
+    if (s_id >= nsys) return;
     const DevSys &S = sys[s_id];
-    const int t = threadIdx.x;
+    const int lane = threadIdx.x & (kWave - 1), wv = threadIdx.x / kWave;
     constexpr int per = kBlock / kNbAtomsPerBlock;   // nonbonded workgroups per 256-slot block
-    double x = 0.0;
-    if (f < 9) {
-        for (int g = 0; g < S.ngroup; ++g) {
-            const double *pp = (g == 0 ? part_nb0 + (size_t)f * snb0 : part_nb1 + (size_t)f * snb1);
-            x += strided_sum(pp, per * S.g[g].wg_off, per * (S.g[g].wg_off + S.g[g].nwg), t, kBlock);
+    for (int f = wv; f < kFields; f += kBlock / kWave) {
+        double x = 0.0;
+        if (f < 9) {
+            for (int g = 0; g < S.ngroup; ++g) {
+                const double *pp = (g == 0 ? part_nb0 + (size_t)f * snb0 : part_nb1 + (size_t)f * snb1);
+                x += strided_sum(pp, per * S.g[g].wg_off, per * (S.g[g].wg_off + S.g[g].nwg), lane, kWave);
+            }
+        } else if (f < 28) {
+            x = strided_sum(part_bd + (size_t)(f - 9) * sbd, S.term_wg_off, S.term_wg_off + S.term_nwg, lane, kWave);
+        } else {
+            x = strided_sum(part_mv + (size_t)(f - 28) * smv, S.item_wg_off, S.item_wg_off + S.item_nwg, lane, kWave);
         }
-    } else if (f < 28) {
-        x = strided_sum(part_bd + (size_t)(f - 9) * sbd, S.term_wg_off, S.term_wg_off + S.term_nwg, t, kBlock);
-    } else {
-        x = strided_sum(part_mv + (size_t)(f - 28) * smv, S.item_wg_off, S.item_wg_off + S.item_nwg, t, kBlock);
+        x = wave_sum(x);
+        if (lane == 0) red[f] = x;
     }
-    x = wave_sum(x);
-    if ((t & (kWave - 1)) == 0) red[t / kWave] = x;
     __syncthreads();
-    if (t == 0) {
-        fsum[b] = ((red[0] + red[1]) + red[2]) + red[3];
-        __threadfence();
-        // the last of the system's kFields workgroups turns the sums into observables
-        if (atomicAdd(&done[s_id], 1) == kFields - 1) {
-            done[s_id] = 0;
-            __threadfence();
-            if (S.live) finalize_system(S, s_id, fsum + (size_t)s_id * kFields, obs, avg, sysstate, step, mode);
-        }
+    if (threadIdx.x == 0) {
+        for (int k = 0; k < kFields; ++k) fsum[(size_t)s_id * kFields + k] = red[k];
+        if (S.live) finalize_system(S, s_id, red, obs, avg, sysstate, step, mode);
     }
 }
 
@@ -1564,7 +1562,15 @@ struct TermLaunchK {   // the arguments of k_terms
 template <bool kMulti, bool kLds, bool kCubic>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_step_forces(NbLaunchK nb, int nb_grid, TermLaunchK tk, FinLaunch fk,
                                                         int fin_blocks) {
+    // the previous step's field sums first (a whole number of XCD rounds, so the nonbonded workgroups keep
+    // their XCD-aware numbering), so they run beside the nonbonded role rather than after it
     int b = (int)blockIdx.x;
+    if (b < fin_blocks) {
+        field_sums_block(b, fk.sys, fk.part_nb0, fk.part_nb1, fk.part_bd, fk.part_mv, fk.snb0, fk.snb1, fk.sbd, fk.smv,
+                         fk.fsum, fk.obs, fk.avg, fk.sysstate, fk.step, fk.mode, fk.nsys);
+        return;
+    }
+    b -= fin_blocks;
     if (b < nb_grid) {
         nb_block<kMulti, kLds, 0, kCubic>(b, nb.sys, nb.wg_sys, nb.perm, nb.pos4, nb.list, nb.nneb, nb.dmax, nb.nsys,
                                           nb.cap, nb.rows, nb.xoff, nb.tabs, nb.map, nb.nitems_total, nb.s, nb.fnb,
@@ -1572,18 +1578,42 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
         return;
     }
     b -= nb_grid;
-    if (b < tk.nwg) {
+    if (b < tk.nwg)
         terms_block(b, tk.nwg, tk.wg_sys, tk.A, tk.s, tk.nitems_total, tk.nterms, tk.contrib, tk.ncontrib, tk.part,
                     tk.flags);
-        return;
-    }
-    b -= tk.nwg;
-    if (b < fin_blocks)
-        field_sums_block(b, fk.sys, fk.part_nb0, fk.part_nb1, fk.part_bd, fk.part_mv, fk.snb0, fk.snb1, fk.sbd, fk.smv,
-                         fk.fsum, fk.done, fk.obs, fk.avg, fk.sysstate, fk.step, fk.mode);
 }
 
-// Radial distribution function histogram over the current list (group 0).
+// Bin of one listed neighbour (r from the fixed-point displacement, in this system's working units).
+__device__ __forceinline__ void rdf_add(const int4 pi, const int4 pj, const float (&rho)[3], float inv_dr_u, int nbin,
+                                        unsigned int *h) {
+    const float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x) * rho[0];
+    const float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y) * rho[1];
+    const float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z) * rho[2];
+    const int b = (int)(sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz))) * inv_dr_u);
+    if (b < nbin) atomicAdd(&h[b], 1u);
+}
+
+// m entries of one run of a slot's list, from l stepping kStep words, four in flight.
+template <int kStep>
+__device__ __forceinline__ void rdf_run(const int *__restrict__ l, int m, const int4 *__restrict__ pos4, const int4 pi,
+                                        const float (&rho)[3], float inv_dr_u, int nbin, unsigned int *h) {
+    int k = 0;
+    for (; k + 4 <= m; k += 4, l += 4 * kStep) {
+        int j[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) j[u] = l[u * kStep];
+        int4 p[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) p[u] = ld_pos(pos4, j[u]);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) rdf_add(pi, p[u], rho, inv_dr_u, nbin, h);
+    }
+    for (; k < m; ++k, l += kStep) rdf_add(pi, ld_pos(pos4, l[0]), rho, inv_dr_u, nbin, h);
+}
+
+// Radial distribution histogram over the group-0 lists at a rebuild: kLanes lanes per slot read its row in
+// the interleaved layout the nonbonded kernel reads (both runs), each with four entries in flight; counts
+// go to a per-workgroup LDS histogram, then one global add per bin.  Integer counts: the order is free.
 __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
                                                 const int *__restrict__ perm, const int4 *__restrict__ pos4,
                                                 const int *__restrict__ list, const int2 *__restrict__ nneb, int cap,
@@ -1594,22 +1624,21 @@ __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, 
     const DevSys &S = sys[s_id];
     if (!S.live) return;
     const float inv_dr_u = (float)S.unit * inv_dr;   // bins in this system's working units
+    const float rho[3] = {S.rho[0], S.rho[1], S.rho[2]};
     if (threadIdx.x == 0 && (int)blockIdx.x == S.g[0].wg_off) atomicAdd(&samples[s_id], 1);
     for (int b = threadIdx.x; b < nbin; b += kBlock) h[b] = 0;
     __syncthreads();
-    const int i = blockIdx.x * kBlock + threadIdx.x;
-    if (perm[i] >= 0) {
+    const int q = threadIdx.x % kLanes;
+    for (int pass = 0; pass < kLanes; ++pass) {   // kNbAtomsPerBlock slots a pass, kBlock slots a workgroup
+        const int i = blockIdx.x * kBlock + pass * kNbAtomsPerBlock + threadIdx.x / kLanes;
+        if (perm[i] < 0) continue;
         const int4 pi = pos4[i];
         const int2 nn = nneb[i];
         const int n_in = min(nn.x, cap), n_out = min(nn.y, cap - n_in);
-        for (int k = 0; k < n_in + n_out; ++k) {
-            const int4 pj = pos4[list[list_pos(i, k < n_in ? k : cap - 1 - (k - n_in), cap)]];
-            const float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x) * S.rho[0];
-            const float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y) * S.rho[1];
-            const float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z) * S.rho[2];
-            const int b = (int)(sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz))) * inv_dr_u);
-            if (b < nbin) atomicAdd(&h[b], 1u);
-        }
+        const int m_in = n_in > q ? (n_in - q + kLanes - 1) / kLanes : 0;
+        const int m_out = n_out > q ? (n_out - q + kLanes - 1) / kLanes : 0;
+        rdf_run<kWave>(list + list_pos(i, q, cap), m_in, pos4, pi, rho, inv_dr_u, nbin, h);
+        rdf_run<-kWave>(list + list_pos(i, cap - 1 - q, cap), m_out, pos4, pi, rho, inv_dr_u, nbin, h);
     }
     __syncthreads();
     for (int b = threadIdx.x; b < nbin; b += kBlock)
@@ -1814,9 +1843,9 @@ void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_s
 }
 
 void launch_finalize(hipStream_t st, const FinLaunch &f) {
-    hipLaunchKernelGGL(k_field_sums, dim3(f.nsys * kFields), dim3(kBlock), 0, st, f.sys, f.part_nb0, f.part_nb1,
-                       f.part_bd, f.part_mv, f.snb0, f.snb1, f.sbd, f.smv, f.fsum, f.done, f.obs, f.avg, f.sysstate,
-                       f.step, f.mode);
+    hipLaunchKernelGGL(k_field_sums, dim3(f.nsys), dim3(kBlock), 0, st, f.sys, f.part_nb0, f.part_nb1,
+                       f.part_bd, f.part_mv, f.snb0, f.snb1, f.sbd, f.smv, f.fsum, f.obs, f.avg, f.sysstate,
+                       f.step, f.mode, f.nsys);
 }
 
 template <bool M, bool L, bool K>
@@ -1844,7 +1873,7 @@ void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const in
     TermLaunchK t{wg_terms, TermArgs{b.sys, b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs, b.vs_com}, s, nitems_total, nterms,
                   contrib, ncontrib, part_bd, flags, term_nwg};
     FinLaunch f{};
-    const int fin_blocks = fin ? fin->nsys * kFields : 0;
+    const int fin_blocks = fin ? xcd_grid(fin->nsys) : 0;
     if (fin) f = *fin;
     const dim3 grid(nb_grid + term_nwg + fin_blocks);
     if (grid.x == 0) return;

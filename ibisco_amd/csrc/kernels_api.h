@@ -93,7 +93,6 @@ struct FinLaunch {
     const double *part_nb0, *part_nb1, *part_bd, *part_mv;
     int snb0, snb1, sbd, smv;   // workgroups (field strides) of each partial buffer
     double *fsum;
-    int *done;
     double *obs, *avg, *sysstate;
     const int *step;
     int mode, nsys;

@@ -107,9 +107,9 @@ int ibx_step_graph(ibx_ctx *ctx, int nsteps, int step0);
 
 /* The steps of ibx_step, bit for bit, for callers that cannot allocate device
  * memory (tight serving loops, harnesses that forbid hipMalloc).  No
- * counterpart in the reference.  The reduction partials and counters the
- * kernels write (per-workgroup partial sums of every kernel class, per-system
- * field sums and completion counters) live in a static 16 MiB device array of
+ * counterpart in the reference.  The reduction partials and field sums the
+ * kernels write (per-workgroup partial sums of every kernel class and the
+ * per-system field sums) live in a static 16 MiB device array of
  * the library instead of the context's buffers; the call copies the context's
  * into the array first and back at the end, and allocates nothing.  A context
  * needing more than 16 MiB of them (about 2 million particles) is rejected

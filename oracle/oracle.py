@@ -272,3 +272,28 @@ class CpuPort:
         s, v = np.zeros((self.natoms, 3)), np.zeros((self.natoms, 3))
         self.L.cp_get_state(self.h, _p(s, C.c_double), _p(v, C.c_double))
         return s, v
+
+
+def nb_virial_magnitude(o, sysm, group=0):
+    """The scale of the terms a nonbonded virial sums: over the in-cutoff pairs of the oracle's current list,
+    sum_k |F(r)| d_k^2 (k = x, y, z), FP64 with the oracle's table (NONBONDED_FORCE.F90:66-74, 92-94).  The
+    net virial of a dense melt is a small difference of such terms, so an error relative to it measures the
+    arithmetic, not the cancellation.  One nonbonded table only (else None)."""
+    if len(sysm.nb_tables) != 1:
+        return None
+    p = o.pairs(group)
+    _, _, r = o.state()
+    L = np.asarray(sysm.box, dtype=np.float64)
+    d = r[p[:, 0]] - r[p[:, 1]]
+    d -= np.round(d / L) * L
+    r2 = (d * d).sum(1)
+    rc = sysm.control.cutoff if group == 0 else sysm.control.bead_cutoff
+    keep = r2 < rc * rc
+    d, rr = d[keep], np.sqrt(r2[keep])
+    x, _V, F = o.table(3, 0)
+    ndat = len(x) - 2
+    bin_ = x[1] - x[0]
+    ni = np.minimum((rr / bin_).astype(np.int64), ndat)
+    alpha = (rr - x[ni]) / bin_
+    f = F[ni] * (1.0 - alpha) + F[ni + 1] * alpha
+    return float((np.abs(f)[:, None] * d * d).sum())

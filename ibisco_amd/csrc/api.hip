@@ -75,8 +75,18 @@ struct Timing {
 
 }  // namespace
 
+// Where one place of a batch lives in the device arrays, and what it holds
+// (a system assigned to the place later must fit cap).
+struct SysLayout {
+    SlotSizes cap;
+    int item_off = 0, term_off = 0, contrib_off = 0, excl_off = 0, vsm_off = 0;
+    int nb_row_off = 0, nb_tab_off = 0, nb_map_off = 0, bd_tab_off = 0, bd_row_off = 0;
+    int slot_off[2] = {0, 0}, cell_off[2] = {0, 0};
+};
+
 struct ibx_ctx {
     int nsys = 0;
+    std::vector<SysLayout> lay;
     int device = 0;                          // HIP device the context was created on
     std::vector<HostSystem> hs;
     std::vector<DevSys> dsys;
@@ -254,6 +264,61 @@ void vs_centre_of_atoms(const HostSystem &h, int v, double com[3]) {
     for (int k = 0; k < 3; ++k) com[k] *= h.vs_invmass[v];
 }
 
+// Constants of system h in its record (the caller sets the offsets into the batch's arrays).
+void set_sys_constants(const HostSystem &h, DevSys &D) {
+    for (int k = 0; k < 3; ++k) {
+        D.box[k] = h.box[k];
+        D.boxinv[k] = 1.0 / h.box[k];
+        D.box2[k] = h.box[k] / 2.0;
+        D.inv_box_frac[k] = 1.0 / h.box[k];
+        D.rho[k] = (float)(h.box[k] / h.box[0]);
+    }
+    D.unit = h.box[0] / 4294967296.0;   // fixed-point unit of the system
+    D.natoms = h.natoms;
+    D.nitems = h.nitems;
+    D.nvs = h.nvs;
+    D.hybrid = h.hybrid;
+    D.ngroup = h.ngroup;
+    D.dt = h.dt; D.temp_in = h.temp_in; D.fac = h.fac; D.ensemble = h.ensemble;
+    D.mktemp = h.mktemp; D.mktemp_atom = h.mktemp_atom; D.mktemp_bead = h.mktemp_bead;
+    D.volume = h.box[0] * h.box[1] * h.box[2];
+    D.halt_drift = h.halt_drift < 0 ? -h.halt_drift : h.halt_drift;
+    D.nmts = h.nmts; D.mts_quirk = h.mts_quirk; D.nupdate = h.nupdate;
+    D.live = 1;
+    D.step0 = 0;
+}
+
+// Cell grid, radii and FP32 bands of group g of h in a system of fixed-point unit `unit` (the caller sets the
+// offsets and member count).
+void set_group_constants(const HostSystem &h, int g, double unit, DevGroup &dg) {
+    const double rc = h.rcut[g], rl = h.rlist[g];
+    int nc[3];
+    for (int k = 0; k < 3; ++k) {
+        nc[k] = (int)(kCellDiv * h.box[k] / rl);   // cells of edge >= rlist/kCellDiv (the reference: rlist)
+        if (nc[k] < 1) nc[k] = 1;
+        dg.nc[k] = nc[k];
+    }
+    dg.all_pairs = (nc[0] < kMinCells || nc[1] < kMinCells || nc[2] < kMinCells) ? 1 : 0;
+    dg.ncell = nc[0] * nc[1] * nc[2];
+    dg.rl2 = rl * rl;
+    const double rl2u = (rl / unit) * (rl / unit);
+    const double lband = 4e-6 + 16.0 * unit / rl;
+    dg.rl2u_lo = (float)(rl2u * (1.0 - lband));
+    dg.rl2u_hi = (float)(rl2u * (1.0 + lband));
+    dg.rc2u = (float)((rc / unit) * (rc / unit));
+    // FP32 r^2 from the fixed-point displacement is within ~2e-7 of the FP64 value, plus the
+    // fixed-point rounding of the two positions (a few units over a distance rc)
+    const double band = 1e-6 + 16.0 * unit / rc;
+    dg.rc2 = rc * rc;
+    dg.rc2u_lo = (float)((rc / unit) * (rc / unit) * (1.0 - band));
+    dg.rc2u_hi = (float)((rc / unit) * (rc / unit) * (1.0 + band));
+    // front run: pairs closer than rc + delta at the rebuild; the back run is scanned only once a
+    // member has moved 0.49 delta (two members closing delta apart), so no pair is ever missed
+    const double delta = kInnerFrac * std::max(0.0, rl - rc);
+    dg.rin2u = (float)(((rc + delta) / unit) * ((rc + delta) / unit));
+    dg.dmax2u = (float)((0.49 * delta / unit) * (0.49 * delta / unit));
+}
+
 int build_device(ibx_ctx *c) {
     const int S = c->nsys;
     c->item_off.resize(S + 1);
@@ -261,9 +326,11 @@ int build_device(ibx_ctx *c) {
     c->slot_off[1].assign(S + 1, 0);
     c->dsys.assign(S, DevSys{});
     int off = 0;
+    c->lay.assign(S, SysLayout());
     for (int s = 0; s < S; ++s) {
+        const HostSystem &h = c->hs[s];
         c->item_off[s] = off;
-        off += pad_to(c->hs[s].nitems, kBlock);
+        off += pad_to(std::max(h.nitems, h.reserve.items) + (h.reserve.items > 0 ? 1 : 0), kBlock);
     }
     c->item_off[S] = off;
     c->n_items = off;
@@ -338,8 +405,17 @@ int build_device(ibx_ctx *c) {
             }
             return base;
         };
+        SysLayout &L = c->lay[s];
+        L.bd_tab_off = (int)hbt.size();
+        L.bd_row_off = (int)hbrows.size();
         int bbase = add_tabs(h.bt, h.bond_lab), abase = add_tabs(h.at, h.angle_lab);
         int tbase = add_tabs(h.tt, h.tors_lab), obase = add_tabs(h.ot, h.oop_lab);
+        // reserved room for a later system's tables (zero headers and rows, never read)
+        L.cap.bd_tabs = std::max((int)hbt.size() - L.bd_tab_off, h.reserve.bd_tabs);
+        L.cap.bd_rows = std::max((int)hbrows.size() - L.bd_row_off, h.reserve.bd_rows);
+        hbt.resize(L.bd_tab_off + L.cap.bd_tabs, BdTab{});
+        hbrows.resize(L.bd_row_off + L.cap.bd_rows, make_float4(0.f, 0.f, 0.f, 0.f));
+        hbx.resize(L.bd_row_off + L.cap.bd_rows, 0.0);
         // terms of this system, padded to whole workgroups (one system per workgroup)
         const int term0 = (int)hmeta.size();
         auto add_term = [&](int kind, int tab, const std::array<int, 4> &m, int nm) {
@@ -357,7 +433,7 @@ int build_device(ibx_ctx *c) {
         for (size_t q = 0; q < h.angles.size(); ++q) add_term(KANGLE, abase + h.angle_tab[q], h.angles[q], 3);
         for (size_t q = 0; q < h.tors.size(); ++q) add_term(KTORS, tbase + h.tors_tab[q], h.tors[q], 4);
         for (size_t q = 0; q < h.oops.size(); ++q) add_term(KOOP, obase + h.oop_tab[q], h.oops[q], 4);
-        int nt_pad = pad_to(std::max(1, (int)hmeta.size() - term0), kBlock);
+        int nt_pad = pad_to(std::max(1, std::max((int)hmeta.size() - term0, h.reserve.terms)), kBlock);
         while ((int)hmeta.size() < term0 + nt_pad) {
             htm.push_back(make_int4(-1, -1, -1, -1));
             hmeta.push_back(-1);
@@ -365,6 +441,8 @@ int build_device(ibx_ctx *c) {
         }
         D.term_wg_off = term0 / kBlock;
         D.term_nwg = nt_pad / kBlock;
+        L.term_off = term0;
+        L.cap.terms = nt_pad;
         for (int w = 0; w < D.term_nwg; ++w) hwgt.push_back(s);
         // nonbonded tables: fixed-point unit of this system
         const double unit = h.box[0] / 4294967296.0;
@@ -382,30 +460,21 @@ int build_device(ibx_ctx *c) {
         if (h.nt.size() > 1 && h.ntype > kMaxNbTypes)
             return fail(c, IBX_E_INVALID, "more than %d types with several nonbonded tables", kMaxNbTypes);
         for (int q = 0; q < h.ntype * h.ntype; ++q) hnmap.push_back(h.inbondt[q] < 0 ? -1 : D.nb_tab_off + h.inbondt[q]);
-        // system constants
-        for (int k = 0; k < 3; ++k) {
-            D.box[k] = h.box[k];
-            D.boxinv[k] = 1.0 / h.box[k];
-            D.box2[k] = h.box[k] / 2.0;
-            D.inv_box_frac[k] = 1.0 / h.box[k];
-            D.rho[k] = (float)(h.box[k] / h.box[0]);
-        }
-        D.unit = unit;
+        L.nb_row_off = D.nb_row_off;
+        L.nb_tab_off = D.nb_tab_off;
+        L.nb_map_off = D.nbmap_off;
+        L.cap.nb_rows = std::max(D.nb_nrow, h.reserve.nb_rows);
+        L.cap.nb_tabs = std::max(D.nb_ntab, h.reserve.nb_tabs);
+        L.cap.nb_map = std::max(h.ntype * h.ntype, h.reserve.nb_map);
+        hnrows.resize(L.nb_row_off + L.cap.nb_rows, make_float4(0.f, 0.f, 0.f, 0.f));
+        hnxoff.resize(L.nb_row_off + L.cap.nb_rows, 0.f);
+        hntab.resize(L.nb_tab_off + L.cap.nb_tabs, NbTab{});
+        hnmap.resize(L.nb_map_off + L.cap.nb_map, -1);
+        set_sys_constants(h, D);
         D.item_off = o;
         D.nitems_pad = c->item_off[s + 1] - o;
-        D.natoms = h.natoms;
-        D.nitems = h.nitems;
-        D.nvs = h.nvs;
         D.item_wg_off = o / kBlock;
         D.item_nwg = D.nitems_pad / kBlock;
-        D.hybrid = h.hybrid;
-        D.ngroup = h.ngroup;
-        D.dt = h.dt; D.temp_in = h.temp_in; D.fac = h.fac; D.ensemble = h.ensemble;
-        D.mktemp = h.mktemp; D.mktemp_atom = h.mktemp_atom; D.mktemp_bead = h.mktemp_bead;
-        D.volume = h.box[0] * h.box[1] * h.box[2];
-        D.halt_drift = h.halt_drift < 0 ? -h.halt_drift : h.halt_drift;
-        D.nmts = h.nmts; D.mts_quirk = h.mts_quirk; D.nupdate = h.nupdate;
-        D.live = 1;
     }
     for (int i = 0; i < N; ++i) {
         hex_off[i + 1] = (int)hex.size();   // placeholder, rebuilt below
@@ -416,12 +485,17 @@ int build_device(ibx_ctx *c) {
         for (int s = 0; s < S; ++s) {
             HostSystem &h = c->hs[s];
             const int o = c->item_off[s];
+            SysLayout &L = c->lay[s];
+            L.excl_off = (int)hex.size();
             for (int i = o; i < c->item_off[s + 1]; ++i) {
                 hex_off[i] = (int)hex.size();
                 int li = i - o;
                 if (li < h.nitems)
                     for (int q = h.excl_off[li]; q < h.excl_off[li + 1]; ++q) hex.push_back(o + h.excl[q]);
             }
+            // reserved room: entries of the last (padding) item, never read
+            while ((int)hex.size() < L.excl_off + h.reserve.excl) hex.push_back(o);
+            L.cap.excl = (int)hex.size() - L.excl_off;
         }
         hex_off[N] = (int)hex.size();
     }
@@ -433,6 +507,7 @@ int build_device(ibx_ctx *c) {
         for (int s = 0; s < S; ++s) {
             HostSystem &h = c->hs[s];
             const int o = c->item_off[s];
+            c->lay[s].vsm_off = (int)hvm.size();
             for (int i = o; i < c->item_off[s + 1]; ++i) {
                 hvm_off[i] = (int)hvm.size();
                 int li = i - o;
@@ -446,9 +521,16 @@ int build_device(ibx_ctx *c) {
     }
     // contribution slots renumbered so each item's slots are one contiguous range, in term order
     std::vector<int> hinc_off(N + 1, 0), renum((size_t)std::max(1, nslot), 0);
-    for (int i = 0; i < N; ++i) {
-        hinc_off[i + 1] = hinc_off[i] + (int)inc[i].size();
-        for (size_t q = 0; q < inc[i].size(); ++q) renum[inc[i][q]] = hinc_off[i] + (int)q;
+    for (int s = 0; s < S; ++s) {
+        const int o = c->item_off[s], e = c->item_off[s + 1];
+        c->lay[s].contrib_off = hinc_off[o];
+        for (int i = o; i < e; ++i) {
+            hinc_off[i + 1] = hinc_off[i] + (int)inc[i].size();
+            for (size_t q = 0; q < inc[i].size(); ++q) renum[inc[i][q]] = hinc_off[i] + (int)q;
+        }
+        // reserved room: slots of the last (padding) item, never gathered
+        hinc_off[e] = std::max(hinc_off[e], c->lay[s].contrib_off + c->hs[s].reserve.contrib);
+        c->lay[s].cap.contrib = hinc_off[e] - c->lay[s].contrib_off;
     }
     std::vector<int4> hslot4(hslot.size());
     for (size_t t = 0; t < hslot.size(); ++t) {
@@ -459,7 +541,11 @@ int build_device(ibx_ctx *c) {
     }
     c->nterms = (int)hmeta.size();
     c->term_nwg = c->nterms / kBlock;
-    c->ncontrib = std::max(1, nslot);
+    c->ncontrib = std::max(1, hinc_off[N]);
+    for (int s = 0; s < S; ++s) {
+        c->lay[s].item_off = c->item_off[s];
+        c->lay[s].cap.items = c->item_off[s + 1] - c->item_off[s];
+    }
 
     int rc;
     if ((rc = upload(c, c->s, hs_s)) || (rc = upload(c, c->v, hv)) || (rc = upload(c, c->mass, hm)) ||
@@ -521,51 +607,31 @@ int build_device(ibx_ctx *c) {
             const int nm = (g < h.ngroup) ? (int)h.members[g].size() : 0;
             dg.slot_off = slot;
             dg.nmem = nm;
-            dg.nslot_pad = pad_to(nm, kBlock);
+            dg.nslot_pad = pad_to(std::max(nm, h.reserve.members[g]), kBlock);
             dg.mem_off = slot;
             dg.wg_off = slot / kBlock;
             dg.nwg = dg.nslot_pad / kBlock;
+            c->lay[s].slot_off[g] = slot;
+            c->lay[s].cap.members[g] = dg.nslot_pad;
             for (int k = 0; k < dg.nslot_pad; ++k) mem.push_back(k < nm ? c->item_off[s] + h.members[g][k] : -1);
             for (int w = 0; w < dg.nwg; ++w) wg.push_back(s);
             slot += dg.nslot_pad;
-            const double rc = h.rcut[g], rl = h.rlist[g];
-            int nc[3];
-            for (int k = 0; k < 3; ++k) {
-                nc[k] = (int)(kCellDiv * h.box[k] / rl);   // cells of edge >= rlist/kCellDiv (the reference: rlist)
-                if (nc[k] < 1) nc[k] = 1;
-                dg.nc[k] = nc[k];
-            }
-            dg.all_pairs = (nc[0] < kMinCells || nc[1] < kMinCells || nc[2] < kMinCells) ? 1 : 0;
-            dg.ncell = nc[0] * nc[1] * nc[2];
+            set_group_constants(h, g, D.unit, dg);
             dg.cell_off = cell;
-            for (int q = 0; q < dg.ncell; ++q) cellsys.push_back(s);
-            cell += dg.ncell;
-            const double unit = D.unit;
-            dg.rl2 = rl * rl;
-            const double rl2u = (rl / unit) * (rl / unit);
-            const double lband = 4e-6 + 16.0 * unit / rl;
-            dg.rl2u_lo = (float)(rl2u * (1.0 - lband));
-            dg.rl2u_hi = (float)(rl2u * (1.0 + lband));
-            dg.rc2u = (float)((rc / unit) * (rc / unit));
-            // FP32 r^2 from the fixed-point displacement is within ~2e-7 of the FP64 value, plus the
-            // fixed-point rounding of the two positions (a few units over a distance rc)
-            const double band = 1e-6 + 16.0 * unit / rc;
-            dg.rc2 = rc * rc;
-            dg.rc2u_lo = (float)((rc / unit) * (rc / unit) * (1.0 - band));
-            dg.rc2u_hi = (float)((rc / unit) * (rc / unit) * (1.0 + band));
-            // front run: pairs closer than rc + delta at the rebuild; the back run is scanned only once a
-            // member has moved 0.49 delta (two members closing delta apart), so no pair is ever missed
-            const double delta = kInnerFrac * std::max(0.0, rl - rc);
-            dg.rin2u = (float)(((rc + delta) / unit) * ((rc + delta) / unit));
-            dg.dmax2u = (float)((0.49 * delta / unit) * (0.49 * delta / unit));
+            const int ncell_res = std::max(dg.ncell, h.reserve.cells[g]);   // reserved cells stay empty
+            c->lay[s].cell_off[g] = cell;
+            c->lay[s].cap.cells[g] = ncell_res;
+            for (int q = 0; q < ncell_res; ++q) cellsys.push_back(s);
+            cell += ncell_res;
             if (nm > 0) {
                 any = true;
+                const double rl = h.rlist[g];
                 double vol = h.box[0] * h.box[1] * h.box[2];
                 double nbr = 4.0 / 3.0 * M_PI * rl * rl * rl * nm / vol;
                 max_dens_vol = std::max(max_dens_vol, nbr);
                 margin = std::max(margin, h.list_margin);
-                maxrows = std::max(maxrows, D.nb_nrow);
-                multi = multi || D.nb_ntab > 1;
+                maxrows = std::max(maxrows, c->lay[s].cap.nb_rows);   // rows a later system may bring too
+                multi = multi || c->lay[s].cap.nb_tabs > 1;
             }
         }
         G.present = any;
@@ -625,49 +691,76 @@ size_t device_bytes(const ibx_ctx *c) {
 
 }  // namespace
 
+// What h needs of a place in a batch.
+SlotSizes ibx::slot_needs(const HostSystem &h) {
+    SlotSizes n;
+    n.items = h.nitems;
+    n.terms = (int)(h.bonds.size() + h.angles.size() + h.tors.size() + h.oops.size());
+    n.contrib = (int)(2 * h.bonds.size() + 3 * h.angles.size() + 4 * (h.tors.size() + h.oops.size()));
+    n.excl = (int)h.excl.size();
+    for (int g = 0; g < 2; ++g) {
+        n.members[g] = g < h.ngroup ? (int)h.members[g].size() : 0;
+        if (n.members[g] == 0) continue;
+        int cells = 1;
+        for (int k = 0; k < 3; ++k) cells *= std::max(1, (int)(kCellDiv * h.box[k] / h.rlist[g]));
+        n.cells[g] = cells;
+    }
+    for (const Table &t : h.nt) n.nb_rows += t.ndat + 1;
+    n.nb_tabs = (int)h.nt.size();
+    n.nb_map = h.ntype * h.ntype;
+    for (const auto *v : {&h.bt, &h.at, &h.tt, &h.ot})
+        for (const Table &t : *v) n.bd_rows += t.ndat + 1;
+    n.bd_tabs = (int)(h.bt.size() + h.at.size() + h.tt.size() + h.ot.size());
+    return n;
+}
+
+bool ibx::slot_fits(const SlotSizes &have, const SlotSizes &need) {
+    bool ok = need.items <= have.items && need.terms <= have.terms && need.contrib <= have.contrib &&
+              need.excl <= have.excl && need.nb_rows <= have.nb_rows && need.nb_tabs <= have.nb_tabs &&
+              need.nb_map <= have.nb_map && need.bd_tabs <= have.bd_tabs && need.bd_rows <= have.bd_rows;
+    for (int g = 0; g < 2; ++g) ok = ok && need.members[g] <= have.members[g] && need.cells[g] <= have.cells[g];
+    return ok;
+}
+
 // Device bytes a one-system context of h would allocate at creation (the sizes
-// build_device computes), plus an RDF of rdf_nbin bins.
+// build_device computes, its reservation included), plus an RDF of rdf_nbin bins.
 size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
     auto at_least1 = [](size_t n) { return n ? n : (size_t)1; };
-    const size_t N = (size_t)pad_to(h.nitems, kBlock);
-    const size_t nterm = h.bonds.size() + h.angles.size() + h.tors.size() + h.oops.size();
-    const size_t members = 2 * h.bonds.size() + 3 * h.angles.size() + 4 * (h.tors.size() + h.oops.size());
-    const size_t nterm_pad = (size_t)pad_to(std::max<int>(1, (int)nterm), kBlock);
-    size_t bd_rows = 0, nb_rows = 0;
-    for (const auto *v : {&h.bt, &h.at, &h.tt, &h.ot})
-        for (const Table &tb : *v) bd_rows += (size_t)tb.ndat + 1;
-    for (const Table &tb : h.nt) nb_rows += (size_t)tb.ndat + 1;
+    const SlotSizes need = slot_needs(h), &res = h.reserve;
+    auto cap = [](int a, int b) { return (size_t)std::max(a, b); };
+    const size_t N = (size_t)pad_to(std::max(h.nitems, res.items) + (res.items > 0 ? 1 : 0), kBlock);
+    const size_t nterm_pad = (size_t)pad_to(std::max(1, std::max(need.terms, res.terms)), kBlock);
+    const size_t members = cap(need.contrib, res.contrib);
+    const size_t bd_rows = cap(need.bd_rows, res.bd_rows), nb_rows = cap(need.nb_rows, res.nb_rows);
     size_t b = sizeof(DevSys) + (N / kBlock) * 4                 // sys, wg_items
                + N * (4 + 4 + 4 + 4 + 8 + 8 + 8 + 8 + 24 + 24 + 12 + 12)   // itype label vs_of vs_center mass
                                                                   // invmass vs_coeff vs_invmass s v fnb ftot
                + 3 * (N + 1) * 4                                  // excl_off vs_moff inc_off
-               + at_least1(h.excl.size()) * 4 + at_least1(h.vs_atoms.size()) * 4
+               + at_least1(cap(need.excl, res.excl)) * 4 + at_least1(h.vs_atoms.size()) * 4
                + nterm_pad * (4 + 16 + 16) + (nterm_pad / kBlock) * 4   // meta, members, slots, wg_terms
                + at_least1(members) * 16                           // contributions
                + 2 * (nterm_pad / kBlock) * kBdPart * 8 + (N / kBlock) * kMvPart * 8
                + (size_t)(kFields + 2 * kObs + 8) * 8 + 2 * 4 + 8 * 4 + 4   // fsum obs avg sysstate dmax flags
                                                                        // step
-               + at_least1(bd_rows) * (16 + 8) + at_least1(h.bt.size() + h.at.size() + h.tt.size() + h.ot.size()) *
-                                                     sizeof(BdTab)
-               + at_least1(nb_rows) * (16 + 4) + at_least1(h.nt.size()) * sizeof(NbTab) +
-               at_least1((size_t)h.ntype * h.ntype) * 4
+               + at_least1(bd_rows) * (16 + 8) + at_least1(cap(need.bd_tabs, res.bd_tabs)) * sizeof(BdTab)
+               + at_least1(nb_rows) * (16 + 4) + at_least1(cap(need.nb_tabs, res.nb_tabs)) * sizeof(NbTab) +
+               at_least1(cap(need.nb_map, res.nb_map)) * 4
                + (h.nvs > 0 ? 24 * N : 8);                          // vs_com
     if (rdf_nbin > 0) b += (size_t)rdf_nbin * 8 + 4;
     if (h.hybrid && h.nmts > 0 && h.ngroup > 1 && !h.members[1].empty())   // MTS slots
         b += 9 * N * 4 + 27 * (size_t)(pad_to((int)h.members[1].size(), kBlock) / kBlock) * (kBlock / kNbAtomsPerBlock) * 8;
     for (int g = 0; g < 2; ++g) {
-        const int nm = g < h.ngroup ? (int)h.members[g].size() : 0;
-        const size_t slots = (size_t)pad_to(nm, kBlock), nwg = slots / kBlock;
-        size_t ncell = 1;
-        for (int k = 0; k < 3; ++k) ncell *= (size_t)std::max(1, (int)(kCellDiv * h.box[k] / h.rlist[g]));
+        const int nm = need.members[g];
         if (nm == 0) continue;   // an absent group allocates nothing
+        const size_t slots = (size_t)pad_to(std::max(nm, res.members[g]), kBlock), nwg = slots / kBlock;
+        const size_t ncell = cap(need.cells[g], res.cells[g]);
         const double nbr = 4.0 / 3.0 * M_PI * h.rlist[g] * h.rlist[g] * h.rlist[g] * nm /
                            (h.box[0] * h.box[1] * h.box[2]);
-        const size_t cap = (size_t)pad_to((int)(std::max(1.5, h.list_margin) * nbr) + 64, 8);
+        const size_t lcap = (size_t)pad_to((int)(std::max(1.5, h.list_margin) * nbr) + 64, 8);
         b += slots * (4 * 5 + 32 + 16 + 16 + 8)                  // mem mem_cell tmp perm slot_cell r64 pos4 pos4_ref nneb
              + N * 4 + nwg * 4 + ncell * 16 + 4                   // inv_perm, wg_sys, cell arrays (one more start)
              + (ncell + 1 + kBlock - 1) / kBlock * 4              // scan tile sums
-             + 2 * nwg * (kBlock / kNbAtomsPerBlock) * kNbPart * 8 + slots * cap * 4;
+             + 2 * nwg * (kBlock / kNbAtomsPerBlock) * kNbPart * 8 + slots * lcap * 4;
     }
     return b;
 }
@@ -688,6 +781,7 @@ RebuildArgs rebuild_args(ibx_ctx *c, int g) {
     a.dmax = c->dmax.p + (size_t)g * c->nsys; a.nsys = c->nsys;
     a.tune = -1; a.nmove = c->nupdate; a.tune_sys = c->sys.p; a.dmax_all = c->dmax.p;
     a.excl_off = c->excl_off.p; a.excl = c->excl.p; a.list = G.list.p; a.nneb = G.nneb.p; a.flags = c->flags.p;
+    a.only = -1;
     return a;
 }
 
@@ -1258,6 +1352,280 @@ int ibx_retire(ibx_ctx *c, int sys) {
     c->dsys[sys].live = 0;
     const int zero = 0;
     HIPCHK(c, memcpy_st(c, (char *)(c->sys.p + sys) + offsetof(DevSys, live), &zero, sizeof zero, hipMemcpyHostToDevice));
+    return 0;
+}
+
+}  // extern "C"
+
+namespace {
+
+// Host copy of n elements into b at element offset off (the context's stream, then a wait).
+template <typename T>
+hipError_t put(ibx_ctx *c, DBuf<T> &b, size_t off, const std::vector<T> &h) {
+    if (h.empty()) return hipSuccess;
+    if (off + h.size() > b.n) return hipErrorInvalidValue;
+    return hipMemcpyAsync(b.p + off, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->st);
+}
+
+// n ints of value v from p on (the fill kernel; also zeroes floats and doubles).
+void fill_words(ibx_ctx *c, void *p, size_t nwords, int v) { launch_fill_int(c->st, (int *)p, (int)nwords, v); }
+
+}  // namespace
+
+// Place k of a batch taken over by system h (see ibx_assign).
+int ibx::assign_system(ibx_ctx *c, int k, HostSystem &&h, int step0) {
+    if (!c || k < 0 || k >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
+    if (c->dsys[k].live) return fail(c, IBX_E_INVALID, "place %d still holds a live system (retire it first)", k);
+    if (h.nvs > 0 || h.hybrid || h.ngroup != 1)
+        return fail(c, IBX_E_INVALID, "only systems without virtual sites and with one nonbonded group can take over a place");
+    if (c->mts || c->gr[1].present || !c->gr[0].present)
+        return fail(c, IBX_E_INVALID, "places of this batch cannot be taken over");
+    if (h.nupdate != c->nupdate) return fail(c, IBX_E_INVALID, "update_neighbour_list differs from the batch's");
+    if (step0 < 0 || step0 % c->nupdate || step0 % c->respa)
+        return fail(c, IBX_E_INVALID, "step0 must be a multiple of the list interval and of the RESPA factor");
+    const SysLayout &L = c->lay[k];
+    const SlotSizes need = slot_needs(h);
+    if (!slot_fits(L.cap, need) || h.nitems >= L.cap.items)
+        return fail(c, IBX_E_NOMEM, "the system does not fit place %d", k);
+    GroupDev &G = c->gr[0];
+    {
+        const double rl = h.rlist[0], vol = h.box[0] * h.box[1] * h.box[2];
+        const double nbr = 4.0 / 3.0 * M_PI * rl * rl * rl * need.members[0] / vol;
+        if ((int)(std::max(1.5, h.list_margin) * nbr) + 64 > G.cap)
+            return fail(c, IBX_E_NOMEM, "the system's lists need more than the batch's list capacity %d", G.cap);
+    }
+    if (G.max_rows > 0 && need.nb_rows > G.max_rows)
+        return fail(c, IBX_E_NOMEM, "nonbonded tables of %d rows exceed the batch's %d", need.nb_rows, G.max_rows);
+    if (need.nb_tabs > 1 && !G.multi) return fail(c, IBX_E_INVALID, "several nonbonded tables in a batch of one each");
+    if (need.nb_tabs > kMaxNbTabs) return fail(c, IBX_E_INVALID, "more than %d nonbonded tables", kMaxNbTabs);
+    if (need.nb_tabs > 1 && h.ntype > kMaxNbTypes)
+        return fail(c, IBX_E_INVALID, "more than %d types with several nonbonded tables", kMaxNbTypes);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+
+    const int o = L.item_off, np = L.cap.items, N = c->n_items;
+    // items
+    std::vector<double> hs(3 * (size_t)np, 0.0), hv(3 * (size_t)np, 0.0), hm(np, 1.0), him(np, 1.0), hzero(np, 0.0);
+    std::vector<int> hit(np, -1), hlab(np, 0), hneg(np, -1), hvm(np, L.vsm_off), hexo(np), hinco(np);
+    for (int i = 0; i < h.nitems; ++i) {
+        for (int q = 0; q < 3; ++q) hs[(size_t)q * np + i] = h.s[3 * (size_t)i + q];
+        if (i < h.natoms)
+            for (int q = 0; q < 3; ++q) hv[(size_t)q * np + i] = h.v[3 * (size_t)i + q];
+        hit[i] = h.itype[i];
+        hlab[i] = h.type_label[i];
+        hm[i] = h.mass[h.itype[i]];
+        him[i] = h.invmass[h.itype[i]];
+    }
+    // exclusions in global indices; the room left belongs to the last (padding) item
+    std::vector<int> hex;
+    for (int i = 0; i < np; ++i) {
+        hexo[i] = L.excl_off + (int)hex.size();
+        if (i < h.nitems)
+            for (int q = h.excl_off[i]; q < h.excl_off[i + 1]; ++q) hex.push_back(o + h.excl[q]);
+    }
+    hex.resize(L.cap.excl, o);
+    // bonded tables and terms, members' contribution slots contiguous per item in term order
+    std::vector<BdTab> hbt;
+    std::vector<float4> hbrows;
+    std::vector<double> hbx;
+    auto add_tabs = [&](const std::vector<Table> &tabs, const std::vector<int> &labs) {
+        const int base = L.bd_tab_off + (int)hbt.size();
+        for (size_t t = 0; t < tabs.size(); ++t) {
+            BdTab b;
+            b.row_off = L.bd_row_off + (int)hbrows.size();
+            b.ndat = tabs[t].ndat;
+            b.x0 = tabs[t].x0;
+            b.bin = tabs[t].bin;
+            b.inv_bin = 1.0 / tabs[t].bin;
+            b.pad = 0;
+            b.label = labs.empty() ? 1 : labs[t];
+            hbt.push_back(b);
+            append_rows(hbrows, tabs[t]);
+            for (int q = 0; q <= tabs[t].ndat; ++q) hbx.push_back(tabs[t].x[q]);
+        }
+        return base;
+    };
+    const int bbase = add_tabs(h.bt, h.bond_lab), abase = add_tabs(h.at, h.angle_lab);
+    const int tbase = add_tabs(h.tt, h.tors_lab), obase = add_tabs(h.ot, h.oop_lab);
+    if (L.bd_tab_off + L.cap.bd_tabs - 1 > kTermTabMask) return fail(c, IBX_E_INVALID, "too many bonded tables");
+    hbt.resize(L.cap.bd_tabs, BdTab{});
+    hbrows.resize(L.cap.bd_rows, make_float4(0.f, 0.f, 0.f, 0.f));
+    hbx.resize(L.cap.bd_rows, 0.0);
+    std::vector<int4> htm;
+    std::vector<int> hmeta, hslot;
+    std::vector<std::vector<int>> inc(np);
+    int nslot = 0;
+    auto add_term = [&](int kind, int tab, const std::array<int, 4> &m, int nm) {
+        htm.push_back(make_int4(o + m[0], o + m[1], nm > 2 ? o + m[2] : -1, nm > 3 ? o + m[3] : -1));
+        hmeta.push_back(kind | (tab << 2));
+        hslot.push_back(nslot);
+        for (int r = 0; r < nm; ++r) inc[m[r]].push_back(nslot + r);
+        nslot += nm;
+    };
+    for (size_t q = 0; q < h.bonds.size(); ++q) add_term(KBOND, bbase + h.bond_tab[q], h.bonds[q], 2);
+    for (size_t q = 0; q < h.angles.size(); ++q) add_term(KANGLE, abase + h.angle_tab[q], h.angles[q], 3);
+    for (size_t q = 0; q < h.tors.size(); ++q) add_term(KTORS, tbase + h.tors_tab[q], h.tors[q], 4);
+    for (size_t q = 0; q < h.oops.size(); ++q) add_term(KOOP, obase + h.oop_tab[q], h.oops[q], 4);
+    std::vector<int> renum(std::max(1, nslot), 0);
+    for (int i = 0, at = 0; i < np; ++i) {
+        hinco[i] = L.contrib_off + at;
+        for (size_t q = 0; q < inc[i].size(); ++q) renum[inc[i][q]] = at + (int)q;
+        at += (int)inc[i].size();
+    }
+    std::vector<int4> hslot4(L.cap.terms, make_int4(0, 0, 0, 0));
+    for (size_t t = 0; t < hslot.size(); ++t) {
+        const int4 m = htm[t];
+        const int b = hslot[t], co = L.contrib_off;
+        hslot4[t] = make_int4(co + renum[b], co + renum[b + 1], m.z >= 0 ? co + renum[b + 2] : 0,
+                              m.w >= 0 ? co + renum[b + 3] : 0);
+    }
+    htm.resize(L.cap.terms, make_int4(-1, -1, -1, -1));
+    hmeta.resize(L.cap.terms, -1);
+    // nonbonded tables
+    DevSys D{};
+    set_sys_constants(h, D);
+    std::vector<float4> hnrows;
+    std::vector<float> hnxoff;
+    std::vector<NbTab> hntab;
+    for (const Table &t : h.nt) {
+        hntab.push_back(nb_header(t, D.unit, L.nb_row_off + (int)hnrows.size(), hnxoff));
+        append_nb_rows(hnrows, t);
+    }
+    D.nb_row_off = L.nb_row_off;
+    D.nb_nrow = (int)hnrows.size();
+    D.nb_tab_off = L.nb_tab_off;
+    D.nb_ntab = (int)h.nt.size();
+    D.ntype = h.ntype;
+    D.nbmap_off = L.nb_map_off;
+    hnrows.resize(L.cap.nb_rows, make_float4(0.f, 0.f, 0.f, 0.f));
+    hnxoff.resize(L.cap.nb_rows, 0.f);
+    hntab.resize(L.cap.nb_tabs, NbTab{});
+    std::vector<int> hnmap(L.cap.nb_map, -1);
+    for (int q = 0; q < h.ntype * h.ntype; ++q) hnmap[q] = h.inbondt[q] < 0 ? -1 : L.nb_tab_off + h.inbondt[q];
+    // the record: offsets of the place, the system's constants
+    const DevSys &P = c->dsys[k];
+    D.item_off = o;
+    D.nitems_pad = np;
+    D.item_wg_off = P.item_wg_off;
+    D.item_nwg = P.item_nwg;
+    D.term_wg_off = P.term_wg_off;
+    D.term_nwg = P.term_nwg;
+    for (int g = 0; g < 2; ++g) {
+        DevGroup &dg = D.g[g];
+        dg = P.g[g];   // offsets of the place
+        if (g < h.ngroup) set_group_constants(h, g, D.unit, dg);
+        dg.nmem = need.members[g];
+        dg.cell_off = L.cell_off[g];
+    }
+    D.step0 = step0;
+    std::vector<int> hmem(L.cap.members[0], -1);
+    for (int q = 0; q < need.members[0]; ++q) hmem[q] = o + h.members[0][q];
+
+    // device writes, all on the context's stream
+    hipError_t e = hipSuccess;
+    for (int q = 0; q < 3 && e == hipSuccess; ++q) {
+        std::vector<double> a(hs.begin() + (size_t)q * np, hs.begin() + (size_t)(q + 1) * np);
+        std::vector<double> b(hv.begin() + (size_t)q * np, hv.begin() + (size_t)(q + 1) * np);
+        e = put(c, c->s, (size_t)q * N + o, a);
+        if (e == hipSuccess) e = put(c, c->v, (size_t)q * N + o, b);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->st);   // a and b leave scope
+    }
+    auto chain = [&](hipError_t x) { if (e == hipSuccess) e = x; };
+    chain(put(c, c->mass, o, hm));
+    chain(put(c, c->invmass, o, him));
+    chain(put(c, c->itype, o, hit));
+    chain(put(c, c->label, o, hlab));
+    chain(put(c, c->vs_of, o, hneg));
+    chain(put(c, c->vs_center, o, hneg));
+    chain(put(c, c->vs_coeff, o, hzero));
+    chain(put(c, c->vs_invmass, o, hzero));
+    chain(put(c, c->vs_moff, o, hvm));
+    chain(put(c, c->excl_off, o, hexo));
+    chain(put(c, c->excl, L.excl_off, hex));
+    chain(put(c, c->inc_off, o, hinco));
+    chain(put(c, c->term_mem, L.term_off, htm));
+    chain(put(c, c->term_meta, L.term_off, hmeta));
+    chain(put(c, c->term_slot, L.term_off, hslot4));
+    chain(put(c, c->bd_tabs, L.bd_tab_off, hbt));
+    chain(put(c, c->bd_rows, L.bd_row_off, hbrows));
+    chain(put(c, c->bd_x, L.bd_row_off, hbx));
+    chain(put(c, c->nb_rows, L.nb_row_off, hnrows));
+    chain(put(c, c->nb_xoff, L.nb_row_off, hnxoff));
+    chain(put(c, c->nb_tabs, L.nb_tab_off, hntab));
+    chain(put(c, c->nb_map, L.nb_map_off, hnmap));
+    chain(put(c, G.mem, L.slot_off[0], hmem));
+    if (e != hipSuccess) {
+        (void)hipStreamSynchronize(c->st);   // no copy left reading the host arrays
+        return fail(c, IBX_E_DEVICE, "assign: %s", hipGetErrorString(e));
+    }
+    // state of the place left by the system before: cleared
+    fill_words(c, c->contrib.p + L.contrib_off, 4 * (size_t)L.cap.contrib, 0);
+    for (int q = 0; q < 3; ++q) {
+        fill_words(c, c->fnb.p + (size_t)q * N + o, np, 0);
+        fill_words(c, c->ftot.p + (size_t)q * N + o, np, 0);
+    }
+    fill_words(c, G.perm.p + L.slot_off[0], L.cap.members[0], -1);
+    fill_words(c, G.nneb.p + L.slot_off[0], 2 * (size_t)L.cap.members[0], 0);
+    fill_words(c, G.inv_perm.p + o, np, -1);
+    fill_words(c, c->obs.p + (size_t)k * kObs, 2 * (size_t)kObs, 0);
+    fill_words(c, c->avg.p + (size_t)k * kObs, 2 * (size_t)kObs, 0);
+    if (c->rdf_nbin) {
+        fill_words(c, c->rdf.p + (size_t)k * c->rdf_nbin, 2 * (size_t)c->rdf_nbin, 0);
+        fill_words(c, c->rdf_samples.p + k, 1, 0);
+    }
+    std::vector<double> st(8, 0.0);
+    st[0] = h.temp_atom0;
+    st[1] = h.temp_bead0;
+    chain(put(c, c->sysstate, 8 * (size_t)k, st));
+    std::vector<DevSys> one(1, D);
+    chain(put(c, c->sys, (size_t)k, one));
+    chain(hipStreamSynchronize(c->st));
+    if (e != hipSuccess) return fail(c, IBX_E_DEVICE, "assign: %s", hipGetErrorString(e));
+    for (int t = 0; t < (int)h.nt.size(); ++t) c->h_nb_tabs[L.nb_tab_off + t] = hntab[t];
+    // a non-cubic box in a batch of cubic ones needs the general kernels: graphs are captured again
+    bool cubic_before = true, cubic_after = true;
+    for (int s = 0; s < c->nsys; ++s) {
+        const DevSys &X = s == k ? D : c->dsys[s];
+        cubic_before = cubic_before && c->dsys[s].rho[1] == 1.0f && c->dsys[s].rho[2] == 1.0f;
+        cubic_after = cubic_after && X.rho[1] == 1.0f && X.rho[2] == 1.0f;
+    }
+    if (cubic_before != cubic_after) drop_graph(c);
+    c->dsys[k] = D;
+    c->hs[k] = std::move(h);
+    // the place's first lists; the other systems' lists are untouched
+    HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
+    RebuildArgs a = rebuild_args(c, 0);
+    a.only = k;
+    launch_rebuild(c->st, a);
+    int f0 = 0;
+    HIPCHK(c, memcpy_st(c, &f0, c->flags.p, sizeof f0, hipMemcpyDeviceToHost));
+    if (f0 > 0) {
+        (void)ibx_retire(c, k);
+        return fail(c, IBX_E_LIST_OVERFLOW, "LIST TOO SMALL: %d neighbours exceed the list capacity", f0);
+    }
+    return 0;
+}
+
+SlotSizes ibx::slot_capacity(const ibx_ctx *c, int k) { return c->lay[k].cap; }
+
+int ibx::batch_period(const ibx_ctx *c) { return block_period(c); }
+
+extern "C" {
+
+int ibx_assign(ibx_ctx *c, int sys, const ibx_system_desc *desc, int step0) {
+    if (!c || !desc) return fail(c, IBX_E_INVALID, "invalid arguments");
+    HostSystem h;
+    std::string err;
+    int rc = build_host_system(desc, h, err);
+    if (rc) return fail(c, rc, "%s", err.c_str());
+    return assign_system(c, sys, std::move(h), step0);
+}
+
+int ibx_get_capacity(ibx_ctx *c, int sys, int64_t cap[16]) {
+    if (!c || !cap || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
+    const SlotSizes &z = c->lay[sys].cap;
+    const int64_t v[16] = {z.items, z.terms, z.contrib, z.excl, z.members[0], z.members[1], z.cells[0], z.cells[1],
+                           z.nb_rows, z.nb_tabs, z.nb_map, z.bd_tabs, z.bd_rows, c->gr[0].cap, block_period(c), 0};
+    for (int q = 0; q < 16; ++q) cap[q] = v[q];
     return 0;
 }
 

@@ -22,6 +22,18 @@ struct Table {
 
 
 
+// Sizes of one system's share of a batch's device arrays: what a place in a
+// batch holds, and what a system needs to take a place over.
+struct SlotSizes {
+    int items = 0;                       // items (atoms, beads, virtual sites), padding included
+    int terms = 0, contrib = 0;          // bonded terms; their members' contribution slots
+    int excl = 0;                        // exclusion entries
+    int members[2] = {0, 0};             // nonbonded group members (list slots)
+    int cells[2] = {0, 0};               // list cells of each group
+    int nb_rows = 0, nb_tabs = 0, nb_map = 0;   // nonbonded table rows, headers, type-pair map entries
+    int bd_tabs = 0, bd_rows = 0;        // bonded table headers and rows
+};
+
 struct HostSystem {
     int natoms = 0, nvs = 0, nitems = 0, ntype = 0;
     bool hybrid = false;
@@ -62,7 +74,16 @@ struct HostSystem {
     // list capacity per member as a multiple of the expected neighbours within rlist
     // (a batch takes the largest of its systems')
     double list_margin = 1.5;
+    // Room kept for this system's place in a batch, so that a later system of up to these sizes can take
+    // the place over (ibx_assign); zero fields: the system's own sizes.  A place with a reservation
+    // (reserve.items > 0) also keeps at least one padding item.
+    SlotSizes reserve;
 };
+
+// What a system needs of a place in a batch (its own sizes).
+SlotSizes slot_needs(const HostSystem &h);
+// Every field of need within have.
+bool slot_fits(const SlotSizes &have, const SlotSizes &need);
 
 // Set-up of main.F90:107-234 from a descriptor (no device work).
 int build_host_system(const ibx_system_desc *d, HostSystem &hs, std::string &err);
@@ -84,5 +105,17 @@ int create_context(std::vector<HostSystem> &&hs, ibx_ctx **out);
 size_t estimate_device_bytes(const HostSystem &h, int rdf_nbin);
 // Free device memory now (hipMemGetInfo).
 int device_free_bytes(size_t *free_bytes);
+
+// Place k of context c, whose system has been retired, taken over by h (moved in) from context step step0 on
+// (a multiple of the list interval and of the RESPA factor): the place's share of every device array is
+// rewritten, its record, observables, window sums and RDF start afresh, and its first lists are built; the
+// other systems' data are untouched, so they continue bit for bit.  Systems without virtual sites and with
+// one nonbonded group only, into a batch of such systems.
+int assign_system(ibx_ctx *c, int k, HostSystem &&h, int step0);
+// What place k of c holds.
+SlotSizes slot_capacity(const ibx_ctx *c, int k);
+// Steps of one graph block of c (list interval, RESPA factor and MTS cycle); a place taken over at a multiple
+// of it starts its own steps in phase with the batch's.
+int batch_period(const ibx_ctx *c);
 
 }  // namespace ibx

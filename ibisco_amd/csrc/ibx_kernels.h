@@ -91,6 +91,8 @@ struct DevSys {
     int nbead;                     // members of group 1 (beads+VS) for MTS
     int live;                      // 0 once retired: the system's workgroups return at once and its
                                    // state, observables, window sums and RDF keep their values
+    int step0;                     // context step at which this system's own step 0 was (a system assigned
+                                   // to a slot of a running batch); MOMENTUM counts the system's own steps
 };
 
 struct NbTab {                     // one nonbonded table as seen by a system

@@ -192,11 +192,13 @@ __global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *sys, const 
                                                        int g, const int *__restrict__ mem, const double *__restrict__ s,
                                                        int nitems_total, int *__restrict__ mem_cell,
                                                        int *__restrict__ cell_count, DevSys *tune_sys, int nsys,
-                                                       const int *__restrict__ dmax_all, int nmove, int tune) {
+                                                       const int *__restrict__ dmax_all, int nmove, int tune,
+                                                       int only) {
     if ((int)blockIdx.x >= nwg) {
         tune_skin(tune_sys, nsys, dmax_all, nmove, tune, ((int)blockIdx.x - nwg) * kBlock + threadIdx.x);
         return;
     }
+    if (only >= 0 && wg_sys[blockIdx.x] != only) return;
     const DevSys &S = sys[wg_sys[blockIdx.x]];
     const DevGroup &G = S.g[g];
     int m = blockIdx.x * kBlock + threadIdx.x;
@@ -258,7 +260,8 @@ __global__ void __launch_bounds__(kBlock) k_scan_add(int *__restrict__ start, co
 __global__ void __launch_bounds__(kBlock) k_place(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, int g,
                                                   const int *__restrict__ mem, const int *__restrict__ mem_cell,
                                                   const int *__restrict__ cell_start, int *__restrict__ cell_fill,
-                                                  int *__restrict__ tmp) {
+                                                  int *__restrict__ tmp, int only) {
+    if (only >= 0 && wg_sys[blockIdx.x] != only) return;
     const DevSys &S = sys[wg_sys[blockIdx.x]];
     const DevGroup &G = S.g[g];
     int m = blockIdx.x * kBlock + threadIdx.x;
@@ -277,14 +280,14 @@ __global__ void __launch_bounds__(kBlock) k_sort_cells(const int *__restrict__ c
                                                        const int *__restrict__ cell_start,
                                                        const int *__restrict__ tmp, const int *__restrict__ mem,
                                                        int *__restrict__ perm, int *__restrict__ slot_cell,
-                                                       int *__restrict__ flags, int ncell) {
+                                                       int *__restrict__ flags, int ncell, int only) {
     // kSortLanes lanes per cell (cells hold a few members each), kBlock / kSortLanes cells per workgroup
     constexpr int kCap = 256;
     __shared__ int vals[kBlock / kSortLanes][kCap];
     const int w = threadIdx.x / kSortLanes, lane = threadIdx.x & (kSortLanes - 1);
     const int gc = blockIdx.x * (kBlock / kSortLanes) + w;
     int n = 0, base = 0, lc = 0;
-    if (gc < ncell) {
+    if (gc < ncell && (only < 0 || cell_sys[gc] == only)) {
         const DevSys &S = sys[cell_sys[gc]];
         const DevGroup &G = S.g[g];
         n = cell_start[gc + 1] - cell_start[gc];
@@ -320,10 +323,12 @@ __global__ void __launch_bounds__(kBlock) k_pack(const DevSys *__restrict__ sys,
                                                  const int *__restrict__ label, double4 *__restrict__ r64,
                                                  int4 *__restrict__ pos4, int *__restrict__ inv_perm,
                                                  int4 *__restrict__ pos4_ref, int *__restrict__ dmax, int nsys,
-                                                 int dmax_value) {
+                                                 int dmax_value, int only) {
     const DevSys &S = sys[wg_sys[blockIdx.x]];
     int slot = blockIdx.x * kBlock + threadIdx.x;
-    if (slot < nsys) dmax[slot] = dmax_value;   // the group's displacement bound restarts (every system has a block)
+    // the group's displacement bound restarts (every system has a block, so slots 0..nsys-1 exist)
+    if (slot < nsys && (only < 0 || slot == only)) dmax[slot] = dmax_value;
+    if (only >= 0 && wg_sys[blockIdx.x] != only) return;
     int it = perm[slot];
     if (it < 0) {
         pos4[slot] = make_int4(0, 0, 0, -1);
@@ -373,12 +378,13 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
                                                        const int *__restrict__ cell_start,
                                                        const int *__restrict__ excl_off, const int *__restrict__ excl,
                                                        int cap, int *__restrict__ list, int2 *__restrict__ nneb,
-                                                       int *__restrict__ flags, int nblk) {
+                                                       int *__restrict__ flags, int nblk, int only) {
     // range r of a member: {candidates up to its end, slot of candidate p in it minus p} (one LDS read)
     __shared__ int2 rng[kListAtoms][kListRanges];
     const int blk = xcd_block(nblk);
     if (blk >= nblk) return;
     const int slot0 = blk * kListAtoms;
+    if (only >= 0 && wg_sys[slot0 / kBlock] != only) return;
     const DevSys &S = sys[wg_sys[slot0 / kBlock]];
     if (!S.live) return;
     const DevGroup &G = S.g[g];
@@ -1463,7 +1469,7 @@ __device__ void finalize_system(const DevSys &S, int s_id, const double *red, do
     o[26] = S.volume;
     st[0] = o[24];
     st[1] = o[25];
-    const int cur = *step;
+    const int cur = *step - S.step0;   // this system's own step
     if (cur % S.halt_drift == 0) {
         for (int k = 0; k < 3; ++k) st[2 + k] = mom[k] / (double)S.natoms;
     } else {
@@ -1759,27 +1765,28 @@ void launch_fill_int(hipStream_t st, int *p, int n, int v) {
 void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
     const int ntune = a.tune >= 0 ? (2 * a.nsys + kBlock - 1) / kBlock : 0;
     hipLaunchKernelGGL(k_cell_count, dim3(a.nwg + ntune), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.nwg, a.g, a.mem, a.s,
-                       a.nitems_total, a.mem_cell, a.cell_count, a.tune_sys, a.nsys, a.dmax_all, a.nmove, a.tune);
+                       a.nitems_total, a.mem_cell, a.cell_count, a.tune_sys, a.nsys, a.dmax_all, a.nmove, a.tune,
+                       a.only);
     const int ntile = (a.ncell_total + 1 + kBlock - 1) / kBlock;   // the starts have one more entry
     hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(kBlock), 0, st, a.cell_count, a.cell_start, a.cell_fill,
                        a.tile_sum, a.ncell_total);
     hipLaunchKernelGGL(k_scan_add, dim3(ntile), dim3(kBlock), 0, st, a.cell_start, a.tile_sum, a.ncell_total);
     hipLaunchKernelGGL(k_place, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.mem, a.mem_cell, a.cell_start,
-                       a.cell_fill, a.tmp);
+                       a.cell_fill, a.tmp, a.only);
     hipLaunchKernelGGL(k_sort_cells, dim3((a.ncell_total + kBlock / kSortLanes - 1) / (kBlock / kSortLanes)), dim3(kBlock), 0, st,
                        a.cell_sys, a.sys, a.g, a.cell_start,
-                       a.tmp, a.mem, a.perm, a.slot_cell, a.flags, a.ncell_total);
+                       a.tmp, a.mem, a.perm, a.slot_cell, a.flags, a.ncell_total, a.only);
     hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
-                       a.itype, a.label, a.r64, a.pos4, a.inv_perm, a.pos4_ref, a.dmax, a.nsys, 0);
+                       a.itype, a.label, a.r64, a.pos4, a.inv_perm, a.pos4_ref, a.dmax, a.nsys, 0, a.only);
     const int nwg_list = a.nwg * (kBlock / kListAtoms);
     if (a.cubic)
         hipLaunchKernelGGL(k_build_list<true>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
                            a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.excl_off, a.excl, a.cap,
-                           a.list, a.nneb, a.flags, nwg_list);
+                           a.list, a.nneb, a.flags, nwg_list, a.only);
     else
         hipLaunchKernelGGL(k_build_list<false>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
                            a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.excl_off, a.excl, a.cap,
-                           a.list, a.nneb, a.flags, nwg_list);
+                           a.list, a.nneb, a.flags, nwg_list, a.only);
 }
 
 template <bool M, bool L, int G, bool K>

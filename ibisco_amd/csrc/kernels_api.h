@@ -31,6 +31,7 @@ struct RebuildArgs {
     int *list;
     int2 *nneb;
     int *flags;
+    int only;           // >= 0: rebuild only this system (the others keep their lists), else every system
 };
 
 struct NbArgs {

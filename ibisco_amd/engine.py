@@ -56,7 +56,8 @@ EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_s
            "ibx_get_pairs", "ibx_rdf_enable", "ibx_rdf_get", "ibx_get_timing", "ibx_set_timing",
            "ibx_get_info", "ibx_retire", "ibx_server_create", "ibx_server_submit", "ibx_server_wait",
            "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes",
-           "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa", "ibx_get_counters", "ibx_step_noalloc")
+           "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa", "ibx_get_counters", "ibx_step_noalloc",
+           "ibx_assign", "ibx_get_capacity")
 
 _lib = None
 
@@ -95,6 +96,8 @@ def load_library(path=LIB_PATH):
     L.ibx_get_info.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.ibx_get_counters.argtypes = [vp, C.POINTER(C.c_int64)]
     L.ibx_retire.argtypes = [vp, C.c_int]
+    L.ibx_assign.argtypes = [vp, C.c_int, C.POINTER(IbxSystemDesc), C.c_int]
+    L.ibx_get_capacity.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.ibx_set_skin_tuning.argtypes = [vp, C.c_int]
     L.ibx_set_respa.argtypes = [vp, C.c_int]
     L.ibx_get_skin.argtypes = [vp, C.c_int, _dp]
@@ -169,6 +172,8 @@ class Engine:
     def __init__(self, systems, mts_quirk=False, load_dir=None):
         self.L = load_library()
         self.h = C.c_void_p()
+        self._keep = []
+        self.arrays = []
         if load_dir is not None:
             rc = self.L.ibx_load_dir(load_dir.encode(), C.byref(self.h))
             self.nsys = 1
@@ -176,7 +181,6 @@ class Engine:
         else:
             if not isinstance(systems, (list, tuple)):
                 systems = [systems]
-            self._keep = []
             descs = [make_desc(s, self._keep, mts_quirk) for s in systems]
             self.arrays = [a for _, a in descs]
             ptrs = (C.POINTER(IbxSystemDesc) * len(descs))(*[C.pointer(d) for d, _ in descs])
@@ -305,6 +309,24 @@ class Engine:
 
     def retire(self, sys):
         self._chk(self.L.ibx_retire(self.h, sys))
+
+    def assign(self, sys, sysm, step0=None):
+        """Place ``sys`` (retired) taken over by system ``sysm`` from step ``step0`` (default: the context's
+        current step) on (ibx_assign); its own steps count from there."""
+        step0 = self.step_count if step0 is None else int(step0)
+        d, a = make_desc(sysm, self._keep)
+        self._chk(self.L.ibx_assign(self.h, int(sys), C.byref(d), step0))
+        self.arrays[sys] = a
+        self.natoms[sys] = a["natoms"]
+        self.nitems[sys] = a["natoms"] + a["nvs"]
+
+    def capacity(self, sys=0):
+        """What place ``sys`` holds (ibx_get_capacity)."""
+        i = (C.c_int64 * 16)()
+        self._chk(self.L.ibx_get_capacity(self.h, sys, i))
+        keys = ("items", "terms", "term_slots", "excl", "members0", "members1", "cells0", "cells1", "nb_rows",
+                "nb_tabs", "nb_map", "bd_tabs", "bd_rows", "list_cap", "period")
+        return {k: int(i[q]) for q, k in enumerate(keys)}
 
     def set_respa(self, k):
         """Nonbonded forces every k steps with weight k, bonded every step (ibx_set_respa)."""

@@ -205,12 +205,35 @@ int ibx_set_skin_tuning(ibx_ctx *ctx, int on);
 /* Current inner skin delta (nm) of each group of system sys. */
 int ibx_get_skin(ibx_ctx *ctx, int sys, double delta[2]);
 
-/* Stops advancing system sys of a batch for the rest of the context's life:
- * its workgroups return at once, and its state, observables, window sums and
- * RDF keep the values of its last step.  The other systems are unaffected
- * (bit for bit).  No counterpart in the reference (one system per run); the
- * serving loop uses it when a job of a batch reaches its step count. */
+/* Stops advancing system sys of a batch: its workgroups return at once, and
+ * its state, observables, window sums and RDF keep the values of its last step
+ * until ibx_assign gives the place to another system.  The other systems are
+ * unaffected (bit for bit).  No counterpart in the reference (one system per
+ * run); the serving loop uses it when a job of a batch reaches its step count. */
 int ibx_retire(ibx_ctx *ctx, int sys);
+
+/* Gives place sys of a batch, whose system has been retired, to the system of
+ * desc from context step step0 on (the step0 the next ibx_step* call passes; a
+ * multiple of update_neighbour_list and of the RESPA factor).  The system must
+ * fit what the place holds (ibx_get_capacity; a place holds its first system's
+ * sizes, or the reservation its creator asked for), have no virtual sites and
+ * one nonbonded group, and share the batch's update_neighbour_list.  Its own
+ * steps count from step0 (MOMENTUM, averages), its observables, window sums
+ * and RDF start at zero, and its first lists are built at once; the other
+ * systems of the batch continue bit for bit, and captured graphs stay valid.
+ * IBX_E_NOMEM when it does not fit, IBX_E_INVALID for the other conditions.
+ * No counterpart in the reference (one system per run): the serving loop uses
+ * it to admit a queued job into a running batch. */
+int ibx_assign(ibx_ctx *ctx, int sys, const ibx_system_desc *desc, int step0);
+
+/* What place sys of a batch holds: [0] items (padding included), [1] bonded
+ * terms, [2] term member slots, [3] exclusion entries, [4..5] members of
+ * nonbonded groups 0 and 1, [6..7] list cells of each group, [8] nonbonded
+ * table rows, [9] nonbonded tables, [10] type-pair map entries, [11] bonded
+ * tables, [12] bonded table rows; [13] the batch's list capacity per member,
+ * [14] steps of a graph block (ibx_assign's step0 alignment in the serving
+ * loop). */
+int ibx_get_capacity(ibx_ctx *ctx, int sys, int64_t cap[16]);
 
 /* ---- Serving loop (no counterpart in the reference) ----------------------
  * Jobs (one system and a step count each) are admitted in arrival order into

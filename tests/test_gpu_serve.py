@@ -14,11 +14,12 @@ import numpy as np
 import pytest
 
 from ibisco_amd import system, units, workloads
-from ibisco_amd.engine import Engine, Server, estimate_bytes
+from ibisco_amd.engine import Engine, IbxError, Server, estimate_bytes
 
 pytestmark = pytest.mark.gpu
 
 NBIN, DR = 150, 0.01
+IBX_E_NOMEM, IBX_E_INVALID = -5, -1
 HERE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
@@ -139,3 +140,56 @@ def test_step_band_groups_similar_jobs():
     for sy, sp, r in zip(systems, specs, res):
         ref = _alone(sy, sp[3])
         assert np.array_equal(r["s"], ref["s"]) and np.array_equal(r["rdf"], ref["rdf"])
+
+
+def test_assigned_place_equals_alone():
+    """A place of a running batch, retired, is given to another system at a list-interval boundary
+    (ibx_assign): the newcomer's state, observables, window sums and RDF equal the system stepped alone from
+    step 0, bit for bit, and so do those of the system that ran on beside it; a second newcomer reuses the
+    place again; the graphs captured before the hand-overs are replayed after them, with no new capture."""
+    a, b, c, d = _equilibrated([_job(12, 40, 51), _job(20, 30, 52), _job(10, 40, 53), _job(9, 30, 54)])
+    e = Engine([a, b])
+    e.rdf_enable(NBIN, DR)
+    cap = e.capacity(1)
+    print("place 1 holds", cap, flush=True)
+    assert cap["period"] == 10
+    e.step(40, graph=True)                         # captures the 40-step block
+    e.step(10, graph=True)                         # and the 10-step block
+    e.step(3, graph=True)
+    e.retire(1)                                    # b's job ends at its step 53
+    e.step(7, graph=True)                          # to step 60, a multiple of the block period
+    with pytest.raises(IbxError) as ei:            # a live place is not given away
+        e.assign(0, c)
+    assert ei.value.code == IBX_E_INVALID
+    with pytest.raises(IbxError) as ei:            # a newcomer starts on a list-interval boundary
+        e.assign(1, c, step0=e.step_count + 1)
+    assert ei.value.code == IBX_E_INVALID
+    with pytest.raises(IbxError) as ei:            # a system larger than the place
+        e.assign(1, _job(30, 40, 55))
+    assert ei.value.code == IBX_E_NOMEM
+    captures = e.counters()["captures"]
+    e.assign(1, c)
+    e.step(47, graph=True)                         # c: its steps 1..47 (a 40-step block, 7 eager steps)
+    sc, vc = e.state(1)
+    rc = dict(rdf=e.rdf(1), sums=e.averages(1), obs=e.obs(1))
+    e.retire(1)
+    e.step(3, graph=True)                          # to step 110
+    e.assign(1, d)
+    e.step(35, graph=True)                         # d: its steps 1..35 (three 10-step blocks); a: 145 in all
+    sd, vd = e.state(1)
+    rd = dict(rdf=e.rdf(1), sums=e.averages(1), obs=e.obs(1))
+    sa, va = e.state(0)
+    ra = dict(rdf=e.rdf(0), sums=e.averages(0))
+    cnt = e.counters()
+    e.close()
+    print("counters", cnt, "captures before the first hand-over", captures, flush=True)
+    assert captures == 2 and cnt["captures"] == captures and cnt["graph_steps"] == 40 + 10 + 40 + 30
+    for got_s, got_v, got, sy, n in ((sc, vc, rc, c, 47), (sd, vd, rd, d, 35)):
+        ref = _alone(sy, n)
+        assert np.array_equal(got_s, ref["s"]) and np.array_equal(got_v, ref["v"])
+        assert np.array_equal(got["rdf"], ref["rdf"])
+        assert np.array_equal(got["sums"][:29], ref["sums"][:29]) and got["sums"][28] == n
+        assert all(np.array_equal(got["obs"][k], ref["obs"][k]) for k in ("vnb", "pt", "ek"))
+    ref = _alone(a, 145)
+    assert np.array_equal(sa, ref["s"]) and np.array_equal(va, ref["v"])
+    assert np.array_equal(ra["rdf"], ref["rdf"]) and np.array_equal(ra["sums"][:29], ref["sums"][:29])

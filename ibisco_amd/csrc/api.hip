@@ -714,6 +714,13 @@ SlotSizes ibx::slot_needs(const HostSystem &h) {
     return n;
 }
 
+int ibx::list_capacity_needed(const HostSystem &h) {
+    const int nm = h.members[0].empty() ? 0 : (int)h.members[0].size();
+    const double rl = h.rlist[0], vol = h.box[0] * h.box[1] * h.box[2];
+    const double nbr = 4.0 / 3.0 * M_PI * rl * rl * rl * nm / vol;
+    return (int)(std::max(1.5, h.list_margin) * nbr) + 64;
+}
+
 bool ibx::slot_fits(const SlotSizes &have, const SlotSizes &need) {
     bool ok = need.items <= have.items && need.terms <= have.terms && need.contrib <= have.contrib &&
               need.excl <= have.excl && need.nb_rows <= have.nb_rows && need.nb_tabs <= have.nb_tabs &&
@@ -723,10 +730,13 @@ bool ibx::slot_fits(const SlotSizes &have, const SlotSizes &need) {
 }
 
 // Device bytes a one-system context of h would allocate at creation (the sizes
-// build_device computes, its reservation included), plus an RDF of rdf_nbin bins.
-size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) {
+// build_device computes, its reservation included), plus an RDF of rdf_nbin bins;
+// the second form with reservation res in place of h's own.
+size_t ibx::estimate_device_bytes(const HostSystem &h, int rdf_nbin) { return estimate_device_bytes(h, h.reserve, rdf_nbin); }
+
+size_t ibx::estimate_device_bytes(const HostSystem &h, const SlotSizes &res, int rdf_nbin) {
     auto at_least1 = [](size_t n) { return n ? n : (size_t)1; };
-    const SlotSizes need = slot_needs(h), &res = h.reserve;
+    const SlotSizes need = slot_needs(h);
     auto cap = [](int a, int b) { return (size_t)std::max(a, b); };
     const size_t N = (size_t)pad_to(std::max(h.nitems, res.items) + (res.items > 0 ? 1 : 0), kBlock);
     const size_t nterm_pad = (size_t)pad_to(std::max(1, std::max(need.terms, res.terms)), kBlock);
@@ -1388,12 +1398,8 @@ int ibx::assign_system(ibx_ctx *c, int k, HostSystem &&h, int step0) {
     if (!slot_fits(L.cap, need) || h.nitems >= L.cap.items)
         return fail(c, IBX_E_NOMEM, "the system does not fit place %d", k);
     GroupDev &G = c->gr[0];
-    {
-        const double rl = h.rlist[0], vol = h.box[0] * h.box[1] * h.box[2];
-        const double nbr = 4.0 / 3.0 * M_PI * rl * rl * rl * need.members[0] / vol;
-        if ((int)(std::max(1.5, h.list_margin) * nbr) + 64 > G.cap)
-            return fail(c, IBX_E_NOMEM, "the system's lists need more than the batch's list capacity %d", G.cap);
-    }
+    if (list_capacity_needed(h) > G.cap)
+        return fail(c, IBX_E_NOMEM, "the system's lists need more than the batch's list capacity %d", G.cap);
     if (G.max_rows > 0 && need.nb_rows > G.max_rows)
         return fail(c, IBX_E_NOMEM, "nonbonded tables of %d rows exceed the batch's %d", need.nb_rows, G.max_rows);
     if (need.nb_tabs > 1 && !G.multi) return fail(c, IBX_E_INVALID, "several nonbonded tables in a batch of one each");

@@ -84,6 +84,8 @@ struct HostSystem {
 SlotSizes slot_needs(const HostSystem &h);
 // Every field of need within have.
 bool slot_fits(const SlotSizes &have, const SlotSizes &need);
+// List capacity per member (group 0) the system's density asks for (a batch takes the largest of its systems').
+int list_capacity_needed(const HostSystem &h);
 
 // Set-up of main.F90:107-234 from a descriptor (no device work).
 int build_host_system(const ibx_system_desc *d, HostSystem &hs, std::string &err);
@@ -103,6 +105,7 @@ int create_context(std::vector<HostSystem> &&hs, ibx_ctx **out);
 // Device bytes a one-system context of h allocates at creation, plus an RDF of
 // rdf_nbin bins (admission by footprint).
 size_t estimate_device_bytes(const HostSystem &h, int rdf_nbin);
+size_t estimate_device_bytes(const HostSystem &h, const SlotSizes &res, int rdf_nbin);
 // Free device memory now (hipMemGetInfo).
 int device_free_bytes(size_t *free_bytes);
 

@@ -40,7 +40,8 @@ class IbxSystemDesc(C.Structure):
 
 class IbxServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_int64), ("max_cohorts", C.c_int), ("max_cohort_jobs", C.c_int),
-                ("chunk_steps", C.c_int), ("rdf_nbin", C.c_int), ("rdf_dr", C.c_double), ("step_band", C.c_double)]
+                ("chunk_steps", C.c_int), ("rdf_nbin", C.c_int), ("rdf_dr", C.c_double), ("step_band", C.c_double),
+                ("spare_places", C.c_int), ("class_ratio", C.c_double)]
 
 
 class IbxJobResult(C.Structure):
@@ -361,11 +362,11 @@ class Server:
     footprint into batched contexts that run concurrently; see include/ibx.h."""
 
     def __init__(self, hbm_budget=200e9, max_cohorts=3, max_cohort_jobs=256, chunk_steps=200, rdf_nbin=150,
-                 rdf_dr=0.01, step_band=1.5):
+                 rdf_dr=0.01, step_band=1.5, spare_places=0, class_ratio=2.0):
         self.L = load_library()
         self.h = C.c_void_p()
         cfg = IbxServerConfig(int(hbm_budget), max_cohorts, max_cohort_jobs, chunk_steps, rdf_nbin, rdf_dr,
-                              step_band)
+                              step_band, int(spare_places), float(class_ratio))
         rc = self.L.ibx_server_create(C.byref(cfg), C.byref(self.h))
         if rc != 0:
             raise IbxError(rc, self.L.ibx_server_last_error().decode())
@@ -408,7 +409,8 @@ class Server:
         if rc != 0:
             raise IbxError(rc, self.L.ibx_server_last_error().decode())
         keys = ("submitted", "completed", "failed", "queued", "running", "cohorts", "peak_running", "bytes_in_use",
-                "peak_bytes", "atom_steps", "clock_s", "largest_cohort", "mean_cohort_jobs", "budget")
+                "peak_bytes", "atom_steps", "clock_s", "largest_cohort", "mean_cohort_jobs", "budget",
+                "admitted_running", "atom_steps_run")
         return {k: float(st[i]) for i, k in enumerate(keys)}
 
     def close(self):

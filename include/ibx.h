@@ -236,14 +236,22 @@ int ibx_assign(ibx_ctx *ctx, int sys, const ibx_system_desc *desc, int step0);
 int ibx_get_capacity(ibx_ctx *ctx, int sys, int64_t cap[16]);
 
 /* ---- Serving loop (no counterpart in the reference) ----------------------
- * Jobs (one system and a step count each) are admitted in arrival order into
- * batched contexts while the estimated device bytes of all resident jobs stay
- * within hbm_budget and fewer than max_cohorts batches run; batches admitted
- * at different times run concurrently on their own streams and host threads.
- * A job that reaches its step count is retired from its batch, its results are
- * copied out, and the batch's bytes return to the budget when its last job
- * ends.  Each job's results equal those of the same system stepped alone in
- * its own context (ibx_step_graph from step 0, RDF every rebuild). */
+ * Jobs (one system and a step count each) are queued in arrival order.  A
+ * queued job goes to a free place of a running batch that holds it (the
+ * smallest such place), where it starts at the batch's next graph-block
+ * boundary (ibx_assign); otherwise, while the estimated device bytes of all
+ * batches stay within hbm_budget and fewer than max_cohorts batches run, a new
+ * batch is created from the queue.  Each place of a batch is reserved for a
+ * size class (1024 items times a power of class_ratio, the other sizes scaled
+ * from its job), and a new batch also keeps spare_places free places per size
+ * class seen so far, for jobs that arrive while it runs.  Batches run
+ * concurrently on their own streams and host threads.  A job that reaches its
+ * step count is retired, its results are copied out and its place is free for
+ * the next job; a batch with no job left ends and its bytes return to the
+ * budget.  When the oldest queued job fits no free place and no new batch can
+ * start, the batch with the least work left stops taking jobs, so that it ends
+ * and makes room.  Each job's results equal those of the same system stepped
+ * alone in its own context (ibx_step_graph from step 0, RDF every rebuild). */
 typedef struct ibx_server ibx_server;
 
 typedef struct {
@@ -253,8 +261,10 @@ typedef struct {
     int chunk_steps;      /* most steps between completion checks of a batch (<= 0: 200) */
     int rdf_nbin;         /* RDF bins per job (0: none) */
     double rdf_dr;        /* RDF bin width, nm */
-    double step_band;     /* > 1: a batch takes, besides the oldest queued job, only jobs whose step counts
+    double step_band;     /* > 1: a new batch takes, besides the oldest queued job, only jobs whose step counts
                              are within this factor of its own, so its members end close together (<= 1: off) */
+    int spare_places;     /* free places per size class seen so far in a new batch (0: 2; < 0: none) */
+    double class_ratio;   /* growth between size classes of places (<= 1: 2) */
 } ibx_server_config;
 
 typedef struct {
@@ -286,8 +296,11 @@ int ibx_server_wait(ibx_server *srv, int64_t job, double timeout_s, ibx_job_resu
                     double *v);
 /* [0] submitted, [1] completed, [2] failed, [3] queued, [4] running batches,
  * [5] batches created, [6] peak running batches, [7] bytes in use (estimated),
- * [8] peak bytes, [9] atom-steps completed, [10] seconds since creation,
- * [11] largest batch, [12] mean jobs per batch, [13] effective budget. */
+ * [8] peak bytes, [9] atom-steps of completed jobs, [10] seconds since
+ * creation, [11] most places of a batch, [12] mean jobs per batch (those it
+ * started with and those given its free places), [13] effective budget,
+ * [14] jobs given a free place of a running batch, [15] atom-steps run so far
+ * (every live place of every batch, jobs still running included). */
 int ibx_server_stats(ibx_server *srv, double st[16]);
 /* Runs every submitted job to its end, then stops the threads and frees. */
 void ibx_server_destroy(ibx_server *srv);

@@ -123,11 +123,13 @@ def test_served_jobs_equal_alone():
 
 
 def test_step_band_groups_similar_jobs():
-    """With a step band, a batch holds the oldest queued job and only jobs whose step counts are within the
-    band of its own; every result still equals the job alone."""
+    """With a step band, a new batch holds the oldest queued job and only jobs whose step counts are within
+    the band of its own (here with hand-overs of free places off, so that batches take jobs only when they
+    are created); every result still equals the job alone."""
     specs = [(8, 40, 41, 200), (8, 40, 42, 30), (8, 40, 43, 190), (8, 40, 44, 35), (8, 40, 45, 210)]
     *systems, slow = _equilibrated([_job(nc, nb, sd) for nc, nb, sd, _ in specs] + [_job(4, 40, 49)])
-    srv = Server(max_cohorts=1, max_cohort_jobs=8, chunk_steps=64, rdf_nbin=NBIN, rdf_dr=DR, step_band=1.5)
+    srv = Server(max_cohorts=1, max_cohort_jobs=8, chunk_steps=64, rdf_nbin=NBIN, rdf_dr=DR, step_band=1.5,
+                 spare_places=-1)
     blocker = srv.submit(slow, 400)     # keeps the only batch slot busy while the rest queue up
     ids = [srv.submit(sy, sp[3], keep_state=True) for sy, sp in zip(systems, specs)]
     res = [srv.wait(j, timeout=120) for j in ids]
@@ -139,6 +141,61 @@ def test_step_band_groups_similar_jobs():
     assert cohort[30] == cohort[35] != cohort[200]
     for sy, sp, r in zip(systems, specs, res):
         ref = _alone(sy, sp[3])
+        assert np.array_equal(r["s"], ref["s"]) and np.array_equal(r["rdf"], ref["rdf"])
+
+
+def _running(srv, timeout=60.0):
+    t = time.time()
+    while srv.stats()["running"] < 1:
+        assert time.time() - t < timeout
+        time.sleep(0.01)
+
+
+def test_jobs_join_a_running_batch():
+    """Continuous batching: jobs that arrive while the only batch runs are given its spare places, and then
+    the place a finished job left, each at a block boundary; every result equals the job alone, bit for bit."""
+    long, *short = _equilibrated([_job(50, 40, 61), _job(8, 30, 62), _job(10, 25, 63), _job(9, 30, 64)])
+    nlong = 40000
+    srv = Server(max_cohorts=1, chunk_steps=50, rdf_nbin=NBIN, rdf_dr=DR, step_band=0.0, spare_places=2)
+    jl = srv.submit(long, nlong, keep_state=True)
+    _running(srv)
+    time.sleep(0.2)
+    steps = (37, 52, 29)
+    ids = [srv.submit(sy, n, keep_state=True) for sy, n in zip(short, steps)]
+    res = [srv.wait(j, timeout=120) for j in ids]
+    rl = srv.wait(jl, timeout=120)
+    st = srv.stats()
+    srv.close()
+    print("stats", st, flush=True)
+    print("long job", rl["t_admit"], rl["t_done"], [(r["t_admit"], r["t_done"]) for r in res], flush=True)
+    assert st["cohorts"] == 1 and st["admitted_running"] == 3 and st["failed"] == 0
+    assert all(r["cohort"] == rl["cohort"] and r["t_admit"] < rl["t_done"] for r in res)
+    # two spare places for three newcomers: the third started after one of the first two had finished
+    assert res[2]["t_admit"] >= min(res[0]["t_done"], res[1]["t_done"])
+    for sy, n, r in zip(short + [long], steps + (nlong,), res + [rl]):
+        ref = _alone(sy, n)
+        assert np.array_equal(r["s"], ref["s"]) and np.array_equal(r["v"], ref["v"])
+        assert np.array_equal(r["rdf"], ref["rdf"]) and np.array_equal(r["sums"][:29], ref["sums"][:29])
+
+
+def test_job_fitting_no_place_gets_a_new_batch():
+    """A job larger than every free place, with no room for a second batch, waits until the running batch,
+    which then takes no more jobs, has ended; it then runs in a batch of its own and equals the job alone."""
+    small, big, late = _equilibrated([_job(8, 40, 71), _job(60, 50, 72), _job(8, 30, 73)])
+    srv = Server(max_cohorts=1, chunk_steps=50, rdf_nbin=NBIN, rdf_dr=DR, step_band=0.0, spare_places=1)
+    js = srv.submit(small, 100000)
+    _running(srv)
+    jb = srv.submit(big, 60, keep_state=True)       # 3,000 beads: no place of the running batch holds it
+    time.sleep(0.5)
+    jl = srv.submit(late, 40, keep_state=True)      # fits the spare place, but the batch is draining
+    rs, rb, rl = srv.wait(js, timeout=120), srv.wait(jb, timeout=120), srv.wait(jl, timeout=120)
+    st = srv.stats()
+    srv.close()
+    print("stats", st, [(r["cohort"], r["t_admit"], r["t_done"]) for r in (rs, rb, rl)], flush=True)
+    assert st["failed"] == 0 and st["cohorts"] == 2 and st["admitted_running"] == 0
+    assert rb["cohort"] != rs["cohort"] and rb["t_admit"] >= rs["t_done"]
+    for sy, n, r in ((big, 60, rb), (late, 40, rl)):
+        ref = _alone(sy, n)
         assert np.array_equal(r["s"], ref["s"]) and np.array_equal(r["rdf"], ref["rdf"])
 
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdarg>
 #include <cstddef>
@@ -286,6 +287,7 @@ void set_sys_constants(const HostSystem &h, DevSys &D) {
     D.nmts = h.nmts; D.mts_quirk = h.mts_quirk; D.nupdate = h.nupdate;
     D.live = 1;
     D.step0 = 0;
+    D.last_step = INT_MAX;
 }
 
 // Cell grid, radii and FP32 bands of group g of h in a system of fixed-point unit `unit` (the caller sets the
@@ -581,6 +583,7 @@ int build_device(ibx_ctx *c) {
     for (int s = 0; s < S; ++s) {
         st[8 * s] = c->hs[s].temp_atom0;
         st[8 * s + 1] = c->hs[s].temp_bead0;
+        st[8 * s + 5] = 1.0;   // the next move advances the system
     }
     if ((rc = upload(c, c->sysstate, st))) return rc;
     HIPCHK(c, c->dmax.alloc(2 * (size_t)S));
@@ -592,6 +595,7 @@ int build_device(ibx_ctx *c) {
     HIPCHK(c, memset_st(c, c->flags.p, 0, c->flags.bytes()));
     HIPCHK(c, c->step.alloc(1));
     HIPCHK(c, memset_st(c, c->step.p, 0, c->step.bytes()));
+    for (DevSys &D : c->dsys) D.step_ctr = c->step.p;
 
     // groups
     for (int g = 0; g < 2; ++g) {
@@ -1362,6 +1366,11 @@ int ibx_retire(ibx_ctx *c, int sys) {
     c->dsys[sys].live = 0;
     const int zero = 0;
     HIPCHK(c, memcpy_st(c, (char *)(c->sys.p + sys) + offsetof(DevSys, live), &zero, sizeof zero, hipMemcpyHostToDevice));
+    // the integrator's and the bonded terms' workgroups of the system return before loading anything
+    const DevSys &D = c->dsys[sys];
+    launch_fill_int(c->st, c->wg_items.p + D.item_wg_off, D.item_nwg, sys | kWgRetired);
+    launch_fill_int(c->st, c->wg_terms.p + D.term_wg_off, D.term_nwg, sys | kWgRetired);
+    HIPCHK(c, hipStreamSynchronize(c->st));
     return 0;
 }
 
@@ -1523,6 +1532,7 @@ int ibx::assign_system(ibx_ctx *c, int k, HostSystem &&h, int step0) {
         dg.cell_off = L.cell_off[g];
     }
     D.step0 = step0;
+    D.step_ctr = c->step.p;
     std::vector<int> hmem(L.cap.members[0], -1);
     for (int q = 0; q < need.members[0]; ++q) hmem[q] = o + h.members[0][q];
 
@@ -1581,9 +1591,12 @@ int ibx::assign_system(ibx_ctx *c, int k, HostSystem &&h, int step0) {
     std::vector<double> st(8, 0.0);
     st[0] = h.temp_atom0;
     st[1] = h.temp_bead0;
+    st[5] = 1.0;
     chain(put(c, c->sysstate, 8 * (size_t)k, st));
     std::vector<DevSys> one(1, D);
     chain(put(c, c->sys, (size_t)k, one));
+    launch_fill_int(c->st, c->wg_items.p + D.item_wg_off, D.item_nwg, k);   // no longer retired
+    launch_fill_int(c->st, c->wg_terms.p + D.term_wg_off, D.term_nwg, k);
     chain(hipStreamSynchronize(c->st));
     if (e != hipSuccess) return fail(c, IBX_E_DEVICE, "assign: %s", hipGetErrorString(e));
     for (int t = 0; t < (int)h.nt.size(); ++t) c->h_nb_tabs[L.nb_tab_off + t] = hntab[t];
@@ -1632,6 +1645,20 @@ int ibx_get_capacity(ibx_ctx *c, int sys, int64_t cap[16]) {
     const int64_t v[16] = {z.items, z.terms, z.contrib, z.excl, z.members[0], z.members[1], z.cells[0], z.cells[1],
                            z.nb_rows, z.nb_tabs, z.nb_map, z.bd_tabs, z.bd_rows, c->gr[0].cap, block_period(c), 0};
     for (int q = 0; q < 16; ++q) cap[q] = v[q];
+    return 0;
+}
+
+int ibx_set_last_step(ibx_ctx *c, int sys, int nsteps) {
+    if (!c || sys < 0 || sys >= c->nsys) return fail(c, IBX_E_INVALID, "invalid system index");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    DevSys &D = c->dsys[sys];
+    D.last_step = nsteps < 0 ? INT_MAX : nsteps;
+    int cur = 0;
+    HIPCHK(c, memcpy_st(c, &cur, c->step.p, sizeof cur, hipMemcpyDeviceToHost));
+    const double go = alive_at(D, cur + 1) ? 1.0 : 0.0;   // whether the next move advances it
+    HIPCHK(c, memcpy_st(c, (char *)(c->sys.p + sys) + offsetof(DevSys, last_step), &D.last_step, sizeof(int),
+                        hipMemcpyHostToDevice));
+    HIPCHK(c, memcpy_st(c, c->sysstate.p + 8 * (size_t)sys + 5, &go, sizeof go, hipMemcpyHostToDevice));
     return 0;
 }
 

@@ -93,7 +93,17 @@ struct DevSys {
                                    // state, observables, window sums and RDF keep their values
     int step0;                     // context step at which this system's own step 0 was (a system assigned
                                    // to a slot of a running batch); MOMENTUM counts the system's own steps
+    int last_step;                 // the system's own last step: later steps leave it as after that step
+    const int *step_ctr;           // the context's step counter (steps completed; k_move advances it)
 };
+
+// Flag on the workgroup -> system words of the integrator and bonded launches:
+// the system is retired, so the workgroup returns before loading anything.
+constexpr int kWgRetired = 1 << 30;
+
+// Whether the system takes part in step t of the context (t = the counter + 1
+// for work before the move of step t).
+__host__ __device__ inline bool alive_at(const DevSys &S, int t) { return S.live && t - S.step0 <= S.last_step; }
 
 struct NbTab {                     // one nonbonded table as seen by a system
     int row_off, ndat;

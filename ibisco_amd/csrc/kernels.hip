@@ -200,6 +200,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *sys, const 
     }
     if (only >= 0 && wg_sys[blockIdx.x] != only) return;
     const DevSys &S = sys[wg_sys[blockIdx.x]];
+    if (!alive_at(S, *S.step_ctr + 1)) return;   // a stopped system's cells stay empty
     const DevGroup &G = S.g[g];
     int m = blockIdx.x * kBlock + threadIdx.x;
     int it = mem[m];
@@ -263,6 +264,7 @@ __global__ void __launch_bounds__(kBlock) k_place(const DevSys *__restrict__ sys
                                                   int *__restrict__ tmp, int only) {
     if (only >= 0 && wg_sys[blockIdx.x] != only) return;
     const DevSys &S = sys[wg_sys[blockIdx.x]];
+    if (!alive_at(S, *S.step_ctr + 1)) return;
     const DevGroup &G = S.g[g];
     int m = blockIdx.x * kBlock + threadIdx.x;
     if (mem[m] < 0) return;
@@ -329,6 +331,7 @@ __global__ void __launch_bounds__(kBlock) k_pack(const DevSys *__restrict__ sys,
     // the group's displacement bound restarts (every system has a block, so slots 0..nsys-1 exist)
     if (slot < nsys && (only < 0 || slot == only)) dmax[slot] = dmax_value;
     if (only >= 0 && wg_sys[blockIdx.x] != only) return;
+    if (!alive_at(S, *S.step_ctr + 1)) return;   // a stopped system keeps its sorted copies
     int it = perm[slot];
     if (it < 0) {
         pos4[slot] = make_int4(0, 0, 0, -1);
@@ -386,7 +389,7 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
     const int slot0 = blk * kListAtoms;
     if (only >= 0 && wg_sys[slot0 / kBlock] != only) return;
     const DevSys &S = sys[wg_sys[slot0 / kBlock]];
-    if (!S.live) return;
+    if (!alive_at(S, *S.step_ctr + 1)) return;
     const DevGroup &G = S.g[g];
     const int sub = threadIdx.x & (kListLanes - 1), grp = threadIdx.x / kListLanes;
     const int i = slot0 + grp;
@@ -765,7 +768,7 @@ __device__ __forceinline__ void nb_block(const int bidx, const DevSys *__restric
     issue_here(perm, pos4, nneb);
     const int sid = wg_sys[blk / (kBlock / kNbAtomsPerBlock)];
     const DevSys &S = sys[sid];
-    if (!S.live) return;
+    if (!alive_at(S, *S.step_ctr + 1)) return;
     const DevGroup &G = S.g[kGroup];
     if (kLds) {
         // four rows per thread in flight at once (the last row is read again past the end)
@@ -1116,7 +1119,10 @@ __device__ __forceinline__ void terms_block(const int b, const int nwg, const in
                                             float4 *__restrict__ contrib, int ncontrib, double *__restrict__ part,
                                             int *__restrict__ flags) {
     __shared__ double red[19 * 4];
-    const int live = A.sys[wg_sys[b]].live;   // first, through the scalar cache
+    const int wsys = wg_sys[b];
+    if (wsys & kWgRetired) return;   // a retired system: nothing loaded
+    const DevSys &Sx = A.sys[wsys];
+    const int live = alive_at(Sx, *Sx.step_ctr + 1);   // first, through the scalar cache
     const int t = b * kBlock + threadIdx.x;
     const int n = nitems_total;
     // the term and its members' positions are loaded before the system record
@@ -1282,14 +1288,20 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     // The system's constants and state (temperatures of the two populations,
     // pending momentum) are read first, before any store (so through the scalar
     // cache, and not again after each store that might alias them).
-    const int s_id = wg_sys[blockIdx.x];
+    if (A.mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.step, 1);   // the step is done
+    const int w = wg_sys[blockIdx.x];
+    if (w & kWgRetired) return;   // a retired system: nothing loaded
+    const int s_id = w;
     const DevSys &S = sys[s_id];
-    const int live = S.live, ensemble = S.ensemble, hybrid = S.hybrid;
+    const int ensemble = S.ensemble, hybrid = S.hybrid;
     const double fac = S.fac, temp_in = S.temp_in, dt = S.dt;
     const double ibf[3] = {S.inv_box_frac[0], S.inv_box_frac[1], S.inv_box_frac[2]};
     const float rho1 = S.rho[1], rho2 = S.rho[2];
     const double *stp = A.sysstate + (size_t)s_id * 8;
     const double st[5] = {stp[0], stp[1], stp[2], stp[3], stp[4]};
+    // whether the system advances this step: written by the field sums of the step before (race-free, unlike
+    // the step counter, which this launch advances)
+    const int live = S.live && stp[5] != 0.0;
     // Every per-item load is issued before the system record is checked (the
     // arrays are padded to whole blocks): the item's own data, then what that
     // data points at (its contribution slots, its virtual site's, the reference
@@ -1317,7 +1329,6 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
     int4 ref[2];
     for (int g = 0; g < 2; ++g) ref[g] = A.pos4_ref[g][max(slot_g[g], 0)];   // used only where slot_g[g] >= 0
     issue_here();
-    if (A.mode == 0 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.step, 1);
     if (!live) return;
     double acc[kNf];
     for (int k = 0; k < kNf; ++k) acc[k] = 0.0;
@@ -1388,9 +1399,11 @@ __global__ void __launch_bounds__(kBlock) k_vs_place(const DevSys *__restrict__ 
                                                      int4 *__restrict__ pos4, const int4 *__restrict__ pos4_ref,
                                                      const int *__restrict__ inv_perm, int *__restrict__ dmax) {
     __shared__ float mx[kBlock / kWave];
-    const int s_id = wg_sys[blockIdx.x];
+    const int w = wg_sys[blockIdx.x];
+    if (w & kWgRetired) return;
+    const int s_id = w;
     const DevSys &S = sys[s_id];
-    if (!S.live) return;
+    if (!alive_at(S, *S.step_ctr)) return;   // after the move: the counter holds this step
     const int it = blockIdx.x * kBlock + threadIdx.x;
     const int n = nitems_total;
     float disp = 0.f;
@@ -1522,7 +1535,10 @@ __device__ __forceinline__ void field_sums_block(const int b, IBX_FIN_PARAMS) {
     __syncthreads();
     if (threadIdx.x == 0) {
         for (int k = 0; k < kFields; ++k) fsum[(size_t)s_id * kFields + k] = red[k];
-        if (S.live) finalize_system(S, s_id, red, obs, avg, sysstate, step, mode);
+        // the step summed is the counter's (the moves of its step are done, the next step's are not)
+        const int t = *step;
+        if (mode == 1 ? S.live : alive_at(S, t)) finalize_system(S, s_id, red, obs, avg, sysstate, step, mode);
+        if (mode == 0 && S.live) sysstate[(size_t)s_id * 8 + 5] = alive_at(S, t + 1) ? 1.0 : 0.0;   // next move
     }
 }
 
@@ -1628,7 +1644,7 @@ __global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, 
     extern __shared__ unsigned int h[];
     const int s_id = wg_sys[blockIdx.x];
     const DevSys &S = sys[s_id];
-    if (!S.live) return;
+    if (!alive_at(S, *S.step_ctr + 1)) return;
     const float inv_dr_u = (float)S.unit * inv_dr;   // bins in this system's working units
     const float rho[3] = {S.rho[0], S.rho[1], S.rho[2]};
     if (threadIdx.x == 0 && (int)blockIdx.x == S.g[0].wg_off) atomicAdd(&samples[s_id], 1);

@@ -13,11 +13,12 @@
 // also holds spare places, per size class seen so far, for the jobs that
 // arrive while it runs.  Each cohort has its own host thread and HIP stream,
 // so cohorts overlap on the device, and every kernel launch of a cohort covers
-// all of its places.  A cohort steps in chunks that end where one of its jobs
-// reaches its step count or where a hand-over is due; a finished job's results
-// (observables, window sums, mean pressure, RDF, optionally the state) are
-// copied out, its place is retired and free again, and the other jobs
-// continue bit for bit.  A cohort with no job left ends and its bytes return
+// all of its places.  Every job stops by itself after its last step
+// (ibx_set_last_step), so a cohort steps in whole graph blocks, in chunks that
+// end at the block boundary after a job's last step or where a hand-over is
+// due.  A finished job's results (observables, window sums, mean pressure, RDF,
+// optionally the state) are then copied out, its place is retired and free
+// again, and the other jobs continue bit for bit.  A cohort with no job left ends and its bytes return
 // to the budget.  When the oldest queued job has waited for a place that no
 // cohort has free and no new cohort can start, the cohort with the least work
 // left stops taking jobs, so that it ends and makes room.
@@ -154,6 +155,10 @@ struct ibx_server {
     int running = 0, peak_running = 0, ncohorts = 0, max_places = 0;
     int64_t nsubmitted = 0, ncompleted = 0, nfailed = 0, cohort_jobs_total = 0, nadmit_running = 0;
     double atom_steps = 0, atom_steps_run = 0;
+    // host seconds of the cohort threads: stepping, hand-overs, result copies, context creation; chunks,
+    // hand-overs, steps (summed over cohorts)
+    double t_step = 0, t_assign = 0, t_collect = 0, t_create = 0;
+    int64_t nchunks = 0, nassign = 0, nsteps_run = 0;
     bool stop = false;
     std::thread sched;
     std::map<int, std::thread> workers;
@@ -227,6 +232,7 @@ void run_cohort(ibx_server *S, CohortPtr C, std::vector<JobPtr> jobs, std::vecto
     const int nplace = (int)hs.size();
     ibx_ctx *ctx = nullptr;
     std::string err;
+    const double tc0 = S->now();
     int rc = create_context(std::move(hs), &ctx);
     if (rc) err = ibx_last_error(nullptr);
     if (!rc && S->cfg.rdf_nbin > 0) {
@@ -235,10 +241,14 @@ void run_cohort(ibx_server *S, CohortPtr C, std::vector<JobPtr> jobs, std::vecto
     }
     for (int k = n; !rc && k < nplace; ++k)
         if ((rc = ibx_retire(ctx, k))) err = ibx_last_error(ctx);
+    // each job stops by itself after its last step, so the cohort steps in whole graph blocks
+    for (int k = 0; !rc && k < n; ++k)
+        if ((rc = ibx_set_last_step(ctx, k, jobs[k]->nsteps))) err = ibx_last_error(ctx);
     int64_t cap[16] = {};
     if (!rc && (rc = ibx_get_capacity(ctx, 0, cap))) err = ibx_last_error(ctx);
     {
         std::lock_guard<std::mutex> lk(S->mu);
+        S->t_create += S->now() - tc0;
         if (!rc) {
             for (int k = 0; k < nplace; ++k) C->places[k].cap = slot_capacity(ctx, k);
             C->period = (int)cap[14];
@@ -250,19 +260,24 @@ void run_cohort(ibx_server *S, CohortPtr C, std::vector<JobPtr> jobs, std::vecto
         S->cv_sched.notify_all();
     }
     const int period = std::max(1, C->period);
-    int done = 0;
+    const int chunk = std::max(period, S->cfg.chunk_steps / period * period);
+    int done = 0;   // always a multiple of the block period
     while (!rc) {
         // hand-overs due at this block boundary
         std::vector<std::pair<int, JobPtr>> due;
-        if (done % period == 0) {
+        {
             std::lock_guard<std::mutex> lk(S->mu);
             for (int k = 0; k < nplace; ++k)
                 if (C->places[k].pending) due.emplace_back(k, C->places[k].pending);
         }
         for (auto &d : due) {
             Job &j = *d.second;
-            const int r = assign_system(ctx, d.first, std::move(j.hs), done);   // j.hs kept when it fails
+            const double ta0 = S->now();
+            int r = assign_system(ctx, d.first, std::move(j.hs), done);   // j.hs kept when it fails
+            if (!r) r = ibx_set_last_step(ctx, d.first, j.nsteps);
             std::lock_guard<std::mutex> lk(S->mu);
+            S->t_assign += S->now() - ta0;
+            ++S->nassign;
             Place &p = C->places[d.first];
             p.pending.reset();
             if (!r) {
@@ -280,7 +295,8 @@ void run_cohort(ibx_server *S, CohortPtr C, std::vector<JobPtr> jobs, std::vecto
                 S->cv_sched.notify_all();
             }
         }
-        // the next chunk ends at the first job's last step, or at the next hand-over
+        // the next chunk ends at the block boundary at or after the first job's last step, or at the next
+        // block boundary when a hand-over is due
         int target = INT_MAX;
         bool any = false, pending = false;
         double natoms = 0;
@@ -299,13 +315,13 @@ void run_cohort(ibx_server *S, CohortPtr C, std::vector<JobPtr> jobs, std::vecto
                 break;
             }
         }
-        if (pending) target = std::min(target, done - done % period + period);
-        if (!any) {                         // nothing to step until the hand-over
-            done = target;
-            continue;
-        }
-        const int nstep = std::min(target - done, S->cfg.chunk_steps);
+        if (!any) continue;                 // a hand-over is due now
+        target = (target + period - 1) / period * period;
+        if (pending) target = std::min(target, done + period);
+        const int nstep = std::min(target - done, chunk);
+        const double ts0 = S->now();
         rc = ibx_step_graph(ctx, nstep, done);
+        const double ts1 = S->now();
         if (rc) {
             err = ibx_last_error(ctx);
             break;
@@ -316,12 +332,20 @@ void run_cohort(ibx_server *S, CohortPtr C, std::vector<JobPtr> jobs, std::vecto
             std::lock_guard<std::mutex> lk(S->mu);
             C->done = done;
             S->atom_steps_run += natoms * nstep;
-            for (int k = 0; k < nplace; ++k)
-                if (C->places[k].job && C->places[k].job->step0 + C->places[k].job->nsteps == done)
+            S->t_step += ts1 - ts0;
+            ++S->nchunks;
+            S->nsteps_run += nstep;
+            for (int k = 0; k < nplace; ++k)   // stopped by themselves at their last steps
+                if (C->places[k].job && C->places[k].job->step0 + C->places[k].job->nsteps <= done)
                     ended.emplace_back(k, C->places[k].job);
         }
         for (auto &e : ended) {
+            const double tq0 = S->now();
             int r = collect(S, ctx, e.first, *e.second);
+            {
+                std::lock_guard<std::mutex> lk(S->mu);
+                S->t_collect += S->now() - tq0;
+            }
             finish_job(S, *e.second, r, r ? ibx_last_error(ctx) : "");
             std::lock_guard<std::mutex> lk(S->mu);
             C->places[e.first].job.reset();
@@ -561,8 +585,8 @@ int ibx_server_create(const ibx_server_config *cfg, ibx_server **out) {
     if (c.max_cohorts <= 0) c.max_cohorts = 3;
     if (c.max_cohort_jobs <= 0) c.max_cohort_jobs = 256;
     if (c.chunk_steps <= 0) c.chunk_steps = 200;
-    if (c.spare_places == 0) c.spare_places = 2;
-    if (!(c.class_ratio > 1.0)) c.class_ratio = 2.0;
+    if (c.spare_places == 0) c.spare_places = 1;
+    if (!(c.class_ratio > 1.0)) c.class_ratio = 1.25;
     if (c.rdf_nbin < 0 || c.rdf_nbin > 8192 || (c.rdf_nbin > 0 && !(c.rdf_dr > 0)))
         return serr(IBX_E_INVALID, "invalid RDF settings");
     size_t free_b = 0;
@@ -669,6 +693,15 @@ int ibx_server_stats(ibx_server *S, double st[16]) {
     st[13] = (double)S->budget;
     st[14] = (double)S->nadmit_running;
     st[15] = S->atom_steps_run;
+    return IBX_OK;
+}
+
+int ibx_server_timing(ibx_server *S, double t[8]) {
+    if (!S || !t) return serr(IBX_E_INVALID, "invalid arguments");
+    std::lock_guard<std::mutex> lk(S->mu);
+    const double v[8] = {S->t_step, S->t_assign, S->t_collect, S->t_create,
+                         (double)S->nchunks, (double)S->nassign, (double)S->nsteps_run, 0.0};
+    for (int k = 0; k < 8; ++k) t[k] = v[k];
     return IBX_OK;
 }
 

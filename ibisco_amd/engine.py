@@ -58,7 +58,7 @@ EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_s
            "ibx_get_info", "ibx_retire", "ibx_server_create", "ibx_server_submit", "ibx_server_wait",
            "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes",
            "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa", "ibx_get_counters", "ibx_step_noalloc",
-           "ibx_assign", "ibx_get_capacity")
+           "ibx_assign", "ibx_get_capacity", "ibx_server_timing", "ibx_set_last_step")
 
 _lib = None
 
@@ -98,6 +98,7 @@ def load_library(path=LIB_PATH):
     L.ibx_get_counters.argtypes = [vp, C.POINTER(C.c_int64)]
     L.ibx_retire.argtypes = [vp, C.c_int]
     L.ibx_assign.argtypes = [vp, C.c_int, C.POINTER(IbxSystemDesc), C.c_int]
+    L.ibx_set_last_step.argtypes = [vp, C.c_int, C.c_int]
     L.ibx_get_capacity.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.ibx_set_skin_tuning.argtypes = [vp, C.c_int]
     L.ibx_set_respa.argtypes = [vp, C.c_int]
@@ -106,6 +107,7 @@ def load_library(path=LIB_PATH):
     L.ibx_server_submit.argtypes = [vp, C.POINTER(IbxSystemDesc), C.c_int, C.c_int, C.POINTER(C.c_int64)]
     L.ibx_server_wait.argtypes = [vp, C.c_int64, C.c_double, C.POINTER(IbxJobResult), _dp, _dp, _dp]
     L.ibx_server_stats.argtypes = [vp, _dp]
+    L.ibx_server_timing.argtypes = [vp, _dp]
     L.ibx_server_destroy.argtypes = [vp]
     L.ibx_server_destroy.restype = None
     L.ibx_estimate_bytes.argtypes = [C.POINTER(IbxSystemDesc), C.c_int, C.POINTER(C.c_int64)]
@@ -321,6 +323,10 @@ class Engine:
         self.natoms[sys] = a["natoms"]
         self.nitems[sys] = a["natoms"] + a["nvs"]
 
+    def set_last_step(self, sys, nsteps):
+        """System ``sys`` stops by itself after its own step ``nsteps`` (< 0: no limit; ibx_set_last_step)."""
+        self._chk(self.L.ibx_set_last_step(self.h, int(sys), int(nsteps)))
+
     def capacity(self, sys=0):
         """What place ``sys`` holds (ibx_get_capacity)."""
         i = (C.c_int64 * 16)()
@@ -362,7 +368,7 @@ class Server:
     footprint into batched contexts that run concurrently; see include/ibx.h."""
 
     def __init__(self, hbm_budget=200e9, max_cohorts=3, max_cohort_jobs=256, chunk_steps=200, rdf_nbin=150,
-                 rdf_dr=0.01, step_band=1.5, spare_places=0, class_ratio=2.0):
+                 rdf_dr=0.01, step_band=1.5, spare_places=0, class_ratio=0.0):
         self.L = load_library()
         self.h = C.c_void_p()
         cfg = IbxServerConfig(int(hbm_budget), max_cohorts, max_cohort_jobs, chunk_steps, rdf_nbin, rdf_dr,
@@ -412,6 +418,15 @@ class Server:
                 "peak_bytes", "atom_steps", "clock_s", "largest_cohort", "mean_cohort_jobs", "budget",
                 "admitted_running", "atom_steps_run")
         return {k: float(st[i]) for i, k in enumerate(keys)}
+
+    def timing(self):
+        """Host seconds of the batches' threads (ibx_server_timing)."""
+        t = np.zeros(8)
+        rc = self.L.ibx_server_timing(self.h, _p(t, C.c_double))
+        if rc != 0:
+            raise IbxError(rc, self.L.ibx_server_last_error().decode())
+        return dict(step_s=t[0], assign_s=t[1], collect_s=t[2], create_s=t[3], step_calls=int(t[4]),
+                    assigns=int(t[5]), steps=int(t[6]))
 
     def close(self):
         """Runs every submitted job to its end, then frees the server."""

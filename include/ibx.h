@@ -212,6 +212,16 @@ int ibx_get_skin(ibx_ctx *ctx, int sys, double delta[2]);
  * run); the serving loop uses it when a job of a batch reaches its step count. */
 int ibx_retire(ibx_ctx *ctx, int sys);
 
+/* System sys of a batch advances through its own step nsteps (counted from the
+ * step0 it started at: 0, or that of ibx_assign) and then stops by itself: its
+ * workgroups return at once and its state, observables, window sums and RDF
+ * keep the values of that step while the context steps on.  nsteps < 0: no
+ * limit (the default).  The serving loop sets it so that a batch always steps
+ * in whole graph blocks and collects a job's results at the next block
+ * boundary.  The reference runs one system for NSTEP steps (RDCONTROL); this is
+ * that limit per member of a batch. */
+int ibx_set_last_step(ibx_ctx *ctx, int sys, int nsteps);
+
 /* Gives place sys of a batch, whose system has been retired, to the system of
  * desc from context step step0 on (the step0 the next ibx_step* call passes; a
  * multiple of update_neighbour_list and of the RESPA factor).  The system must
@@ -263,8 +273,9 @@ typedef struct {
     double rdf_dr;        /* RDF bin width, nm */
     double step_band;     /* > 1: a new batch takes, besides the oldest queued job, only jobs whose step counts
                              are within this factor of its own, so its members end close together (<= 1: off) */
-    int spare_places;     /* free places per size class seen so far in a new batch (0: 2; < 0: none) */
-    double class_ratio;   /* growth between size classes of places (<= 1: 2) */
+    int spare_places;     /* free places per size class seen so far in a new batch (0: 1); < 0: places
+                             are not handed over (a batch takes jobs only when it is created) */
+    double class_ratio;   /* growth between size classes of places (<= 1: 1.25) */
 } ibx_server_config;
 
 typedef struct {
@@ -302,6 +313,11 @@ int ibx_server_wait(ibx_server *srv, int64_t job, double timeout_s, ibx_job_resu
  * [14] jobs given a free place of a running batch, [15] atom-steps run so far
  * (every live place of every batch, jobs still running included). */
 int ibx_server_stats(ibx_server *srv, double st[16]);
+/* Host seconds the batches' threads spent, summed over batches: [0] in
+ * ibx_step_graph, [1] handing places over (ibx_assign), [2] copying results
+ * out, [3] creating contexts; [4] step calls, [5] hand-overs, [6] steps run
+ * (each batch's steps counted once). */
+int ibx_server_timing(ibx_server *srv, double t[8]);
 /* Runs every submitted job to its end, then stops the threads and frees. */
 void ibx_server_destroy(ibx_server *srv);
 /* Message of the last failed ibx_server_* call on the calling thread. */

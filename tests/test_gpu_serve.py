@@ -144,6 +144,30 @@ def test_step_band_groups_similar_jobs():
         assert np.array_equal(r["s"], ref["s"]) and np.array_equal(r["rdf"], ref["rdf"])
 
 
+def test_last_step_stops_a_member_on_the_device():
+    """A member given a last step (ibx_set_last_step) stops by itself there while the batch steps on in whole
+    graph blocks: its state, observables, window sums and RDF equal the system alone after that many steps,
+    and the other member equals itself alone after all of them."""
+    a, b = _equilibrated([_job(12, 40, 81), _job(10, 30, 82)])
+    e = Engine([a, b])
+    e.rdf_enable(NBIN, DR)
+    e.set_last_step(1, 27)
+    e.step(40, graph=True)
+    e.step(10, graph=True)
+    sb, vb = e.state(1)
+    rb = dict(rdf=e.rdf(1), sums=e.averages(1), obs=e.obs(1))
+    sa, va = e.state(0)
+    cnt = e.counters()
+    e.close()
+    assert cnt["eager_steps"] == 0
+    ref = _alone(b, 27)
+    assert np.array_equal(sb, ref["s"]) and np.array_equal(vb, ref["v"])
+    assert np.array_equal(rb["rdf"], ref["rdf"]) and np.array_equal(rb["sums"][:29], ref["sums"][:29])
+    assert rb["sums"][28] == 27 and all(np.array_equal(rb["obs"][k], ref["obs"][k]) for k in ("vnb", "pt", "ek"))
+    ref = _alone(a, 50)
+    assert np.array_equal(sa, ref["s"]) and np.array_equal(va, ref["v"])
+
+
 def _running(srv, timeout=60.0):
     t = time.time()
     while srv.stats()["running"] < 1:

@@ -796,26 +796,32 @@ RebuildArgs rebuild_args(ibx_ctx *c, int g) {
     a.tune = -1; a.nmove = c->nupdate; a.tune_sys = c->sys.p; a.dmax_all = c->dmax.p;
     a.excl_off = c->excl_off.p; a.excl = c->excl.p; a.list = G.list.p; a.nneb = G.nneb.p; a.flags = c->flags.p;
     a.only = -1;
+    a.rdf_hist = nullptr;
+    a.rdf_samples = nullptr;
+    a.rdf_nbin = 0;
+    a.rdf_inv_dr = 0.f;
     return a;
 }
 
 // tune: a rebuild of the step schedule, after a full interval of moves (the
-// list-skin autotuner reads the displacement that interval reached)
+// list-skin autotuner reads the displacement that interval reached); with the
+// RDF on, the group-0 list build also adds its listed pairs to each system's
+// histogram (one sample a build).
 void enqueue_rebuild(ibx_ctx *c, bool tune = false) {
+    const bool rdf = tune && c->rdf_nbin > 0 && c->rdf_every > 0;
     for (int g = 0; g < 2; ++g) {
         if (!c->gr[g].present) continue;
         RebuildArgs a = rebuild_args(c, g);
         if (tune) a.tune = c->skin_tune ? 1 : 0;   // with the first group's cell count, for both groups
         tune = false;
+        if (rdf && g == 0) {
+            a.rdf_hist = c->rdf.p;
+            a.rdf_samples = c->rdf_samples.p;
+            a.rdf_nbin = c->rdf_nbin;
+            a.rdf_inv_dr = (float)(1.0 / c->rdf_dr);
+        }
         launch_rebuild(c->st, a);
     }
-}
-
-void enqueue_rdf(ibx_ctx *c) {
-    if (!c->rdf_nbin || !c->gr[0].present) return;
-    GroupDev &G = c->gr[0];
-    launch_rdf(c->st, G.nwg, c->sys.p, G.wg_sys.p, G.perm.p, G.pos4.p, G.list.p, G.nneb.p, G.cap, c->rdf_nbin,
-               (float)(1.0 / c->rdf_dr), c->rdf.p, c->rdf_samples.p);
 }
 
 NbArgs nb_args(ibx_ctx *c, int g) {
@@ -970,10 +976,7 @@ void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
     if (ev) {
         flush_fin(c);
         (void)hipEventRecord((*ev)[0], c->st);
-        if (step % c->nupdate == 0) {
-            enqueue_rebuild(c, true);
-            if (c->rdf_nbin && c->rdf_every > 0) enqueue_rdf(c);
-        }
+        if (step % c->nupdate == 0) enqueue_rebuild(c, true);
         (void)hipEventRecord((*ev)[4], c->st);
         if (nb_step) enqueue_forces(c, 0, step);
         (void)hipEventRecord((*ev)[1], c->st);
@@ -983,10 +986,7 @@ void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
         (void)hipEventRecord((*ev)[5], c->st);
         return;
     }
-    if (step % c->nupdate == 0) {
-        enqueue_rebuild(c, true);
-        if (c->rdf_nbin && c->rdf_every > 0) enqueue_rdf(c);
-    }
+    if (step % c->nupdate == 0) enqueue_rebuild(c, true);
     if (nb_step) c->nb_par ^= 1;
     c->bd_par ^= 1;
     NbArgs a0;

@@ -349,6 +349,9 @@ __global__ void __launch_bounds__(kBlock) k_pack(const DevSys *__restrict__ sys,
     inv_perm[it] = slot;
 }
 
+// RDF bin of a pair at squared distance r2u (working units^2), inv_dr_u bins per working unit.
+__device__ __forceinline__ int rdf_bin(float r2u, float inv_dr_u) { return (int)(__builtin_amdgcn_sqrtf(r2u) * inv_dr_u); }
+
 // Gap along one axis between a point at fraction f of its cell and the cell d
 // cells away (cell edge a); 0 for its own cell.
 __device__ __forceinline__ float cell_gap(int d, float f, float a) {
@@ -370,7 +373,7 @@ __device__ __forceinline__ float cell_gap(int d, float f, float a) {
 // Pairs closer than rc + delta when the list is built fill the front run, the
 // others the back run.  With fewer than five cells along an axis every member scans the
 // whole group once.
-template <bool kCubic>
+template <bool kCubic, bool kRdf>
 // Held to 64 VGPRs (eight waves per SIMD, measured faster than seven: the candidate loop waits on one position
 // gather per round, and more waves hide more of it).
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(8)))
@@ -381,16 +384,26 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
                                                        const int *__restrict__ cell_start,
                                                        const int *__restrict__ excl_off, const int *__restrict__ excl,
                                                        int cap, int *__restrict__ list, int2 *__restrict__ nneb,
-                                                       int *__restrict__ flags, int nblk, int only) {
+                                                       int *__restrict__ flags, int nblk, int only,
+                                                       unsigned long long *__restrict__ rdf_hist,
+                                                       int *__restrict__ rdf_samples, int nbin, float inv_dr) {
     // range r of a member: {candidates up to its end, slot of candidate p in it minus p} (one LDS read)
     __shared__ int2 rng[kListAtoms][kListRanges];
+    extern __shared__ unsigned int rdf_h[];   // nbin counts of this workgroup's listed pairs (RDF on)
     const int blk = xcd_block(nblk);
     if (blk >= nblk) return;
     const int slot0 = blk * kListAtoms;
-    if (only >= 0 && wg_sys[slot0 / kBlock] != only) return;
-    const DevSys &S = sys[wg_sys[slot0 / kBlock]];
+    const int s_id = wg_sys[slot0 / kBlock];
+    if (only >= 0 && s_id != only) return;
+    const DevSys &S = sys[s_id];
     if (!alive_at(S, *S.step_ctr + 1)) return;
     const DevGroup &G = S.g[g];
+    constexpr bool rdf = kRdf;   // the RDF of the listed pairs (kRdf: rdf_hist holds the histograms)
+    if (rdf) {
+        for (int b = threadIdx.x; b < nbin; b += kBlock) rdf_h[b] = 0;
+        if (threadIdx.x == 0 && slot0 == G.slot_off) atomicAdd(&rdf_samples[s_id], 1);   // one sample a build
+    }
+    const float inv_dr_u = (float)S.unit * inv_dr;   // bins in this system's working units
     const int sub = threadIdx.x & (kListLanes - 1), grp = threadIdx.x / kListLanes;
     const int i = slot0 + grp;
     const int I = perm[i];
@@ -534,29 +547,50 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
 
     int tmax = T;   // the wave walks as far as its longest member
     for (int off = kListLanes; off < kWave; off <<= 1) tmax = max(tmax, __shfl_xor(tmax, off, kWave));
+    // exclusion registers worth testing: the wave's largest count (entries past a member's own are -1)
+    int nex = valid ? min(ex1 - ex0, 8) : 0;
+    for (int off = 1; off < kWave; off <<= 1) nex = max(nex, __shfl_xor(nex, off, kWave));
+    nex = __builtin_amdgcn_readfirstlane(nex);
     const int gshift = (threadIdx.x & (kWave - 1)) & ~(kListLanes - 1);   // this group's bits in a wave ballot
     const unsigned below = (1u << sub) - 1u;
+    int *const lrow = list + (valid ? list_pos(i, 0, cap) : 0);   // entry k of the row: lrow[kofs(k)]
     int n_in = 0, n_out = 0, r = 0;
+    // slot of candidate p (p ascending: the range walk only moves forward)
+    auto cand_slot = [&](int p) {
+        int2 e = rng[grp][r];
+        while (p >= e.x) e = rng[grp][++r];
+        return e.y + p;
+    };
+    // each step's candidate and its position are loaded during the step before
+    int jn = -1;
+    int4 pn = make_int4(0, 0, 0, 0);
+    if (sub < T) {
+        jn = cand_slot(sub);
+        pn = pos4[jn];
+    }
     for (int t0 = 0; t0 < tmax; t0 += kListLanes) {
         const int p = t0 + sub;
+        const int j = jn;
+        const int4 pj = pn;
+        if (p + kListLanes < T) {
+            jn = cand_slot(p + kListLanes);
+            pn = pos4[jn];
+        }
         bool hit = false, inner = false;
-        int j = -1;
+        float r2u = 0.f;
         if (p < T) {
-            int2 e = rng[grp][r];
-            while (p >= e.x) e = rng[grp][++r];
-            j = e.y + p;
-            const int4 pj = pos4[j];
             float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
             float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
             float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
             if (!kCubic) { dx *= rho0; dy *= rho1; dz *= rho2; }
-            const float r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
+            r2u = fmaf(dx, dx, fmaf(dy, dy, dz * dz));
             bool cand = r2u < lo;
             // the member's own FP64 position is read again here (rare) rather than held in registers
             if (r2u >= lo && r2u <= hi) cand = exact_r2(r64[i], r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2;
             bool skip = (j == i) || (hybrid && li == 3 && pos_label(pj.w) == 3);
 #pragma unroll
-            for (int e = 0; e < 8; ++e) skip |= (ex[e] == j);
+            for (int e = 0; e < 8; ++e)
+                if (e < nex) skip |= (ex[e] == j);
             if (cand && !skip && ex1 - ex0 > 8)
                 for (int e = ex0 + 8; e < ex1; ++e) skip |= (inv_perm[excl[e]] == j);
             hit = cand && !skip;
@@ -564,10 +598,13 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
         }
         const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
         const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
-        if (hit) {
-            const unsigned k = inner ? (unsigned)n_in + __popc(gi & below) : (unsigned)(cap - 1 - n_out - __popc(go & below));
-            const bool fits = inner ? (int)k < cap : n_out + __popc(go & below) < cap;
-            if (fits) list[list_pos(i, (int)k, cap)] = j;
+        const int k_in = n_in + __popc(gi & below);
+        const int k_out = cap - 1 - n_out - __popc(go & below);
+        const int k = inner ? k_in : k_out;
+        if (hit && (inner ? k_in < cap : k_out >= 0)) lrow[((k >> 2) << 6) + (k & 3)] = j;   // list_pos(i, k, cap)
+        if (rdf && hit) {
+            const int b = rdf_bin(r2u, inv_dr_u);
+            if (b < nbin) atomicAdd(&rdf_h[b], 1u);
         }
         n_in += __popc(gi);
         n_out += __popc(go);
@@ -575,6 +612,11 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
     if (valid && sub == 0) {
         nneb[i] = make_int2(n_in, n_out);
         if (n_in + n_out > cap) atomicMax(&flags[0], n_in + n_out);
+    }
+    if (rdf) {
+        __syncthreads();
+        for (int b = threadIdx.x; b < nbin; b += kBlock)
+            if (rdf_h[b]) atomicAdd(&rdf_hist[(size_t)s_id * nbin + b], (unsigned long long)rdf_h[b]);
     }
 }
 
@@ -1605,67 +1647,6 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
                     tk.flags);
 }
 
-// Bin of one listed neighbour (r from the fixed-point displacement, in this system's working units).
-__device__ __forceinline__ void rdf_add(const int4 pi, const int4 pj, const float (&rho)[3], float inv_dr_u, int nbin,
-                                        unsigned int *h) {
-    const float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x) * rho[0];
-    const float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y) * rho[1];
-    const float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z) * rho[2];
-    const int b = (int)(sqrtf(fmaf(dx, dx, fmaf(dy, dy, dz * dz))) * inv_dr_u);
-    if (b < nbin) atomicAdd(&h[b], 1u);
-}
-
-// m entries of one run of a slot's list, from l stepping kStep words, four in flight.
-template <int kStep>
-__device__ __forceinline__ void rdf_run(const int *__restrict__ l, int m, const int4 *__restrict__ pos4, const int4 pi,
-                                        const float (&rho)[3], float inv_dr_u, int nbin, unsigned int *h) {
-    int k = 0;
-    for (; k + 4 <= m; k += 4, l += 4 * kStep) {
-        int j[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) j[u] = l[u * kStep];
-        int4 p[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) p[u] = ld_pos(pos4, j[u]);
-#pragma unroll
-        for (int u = 0; u < 4; ++u) rdf_add(pi, p[u], rho, inv_dr_u, nbin, h);
-    }
-    for (; k < m; ++k, l += kStep) rdf_add(pi, ld_pos(pos4, l[0]), rho, inv_dr_u, nbin, h);
-}
-
-// Radial distribution histogram over the group-0 lists at a rebuild: kLanes lanes per slot read its row in
-// the interleaved layout the nonbonded kernel reads (both runs), each with four entries in flight; counts
-// go to a per-workgroup LDS histogram, then one global add per bin.  Integer counts: the order is free.
-__global__ void __launch_bounds__(kBlock) k_rdf(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
-                                                const int *__restrict__ perm, const int4 *__restrict__ pos4,
-                                                const int *__restrict__ list, const int2 *__restrict__ nneb, int cap,
-                                                int nbin, float inv_dr, unsigned long long *__restrict__ hist,
-                                                int *__restrict__ samples) {
-    extern __shared__ unsigned int h[];
-    const int s_id = wg_sys[blockIdx.x];
-    const DevSys &S = sys[s_id];
-    if (!alive_at(S, *S.step_ctr + 1)) return;
-    const float inv_dr_u = (float)S.unit * inv_dr;   // bins in this system's working units
-    const float rho[3] = {S.rho[0], S.rho[1], S.rho[2]};
-    if (threadIdx.x == 0 && (int)blockIdx.x == S.g[0].wg_off) atomicAdd(&samples[s_id], 1);
-    for (int b = threadIdx.x; b < nbin; b += kBlock) h[b] = 0;
-    __syncthreads();
-    const int q = threadIdx.x % kLanes;
-    for (int pass = 0; pass < kLanes; ++pass) {   // kNbAtomsPerBlock slots a pass, kBlock slots a workgroup
-        const int i = blockIdx.x * kBlock + pass * kNbAtomsPerBlock + threadIdx.x / kLanes;
-        if (perm[i] < 0) continue;
-        const int4 pi = pos4[i];
-        const int2 nn = nneb[i];
-        const int n_in = min(nn.x, cap), n_out = min(nn.y, cap - n_in);
-        const int m_in = n_in > q ? (n_in - q + kLanes - 1) / kLanes : 0;
-        const int m_out = n_out > q ? (n_out - q + kLanes - 1) / kLanes : 0;
-        rdf_run<kWave>(list + list_pos(i, q, cap), m_in, pos4, pi, rho, inv_dr_u, nbin, h);
-        rdf_run<-kWave>(list + list_pos(i, cap - 1 - q, cap), m_out, pos4, pi, rho, inv_dr_u, nbin, h);
-    }
-    __syncthreads();
-    for (int b = threadIdx.x; b < nbin; b += kBlock)
-        if (h[b]) atomicAdd(&hist[(size_t)s_id * nbin + b], (unsigned long long)h[b]);
-}
 
 // List-skin autotuner (no counterpart in the reference).  Before a scheduled
 // rebuild, each system's inner skin delta (the front run holds the pairs closer
@@ -1795,14 +1776,14 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
     hipLaunchKernelGGL(k_pack, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.perm, a.s, a.nitems_total,
                        a.itype, a.label, a.r64, a.pos4, a.inv_perm, a.pos4_ref, a.dmax, a.nsys, 0, a.only);
     const int nwg_list = a.nwg * (kBlock / kListAtoms);
-    if (a.cubic)
-        hipLaunchKernelGGL(k_build_list<true>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
-                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.excl_off, a.excl, a.cap,
-                           a.list, a.nneb, a.flags, nwg_list, a.only);
-    else
-        hipLaunchKernelGGL(k_build_list<false>, dim3(xcd_grid(nwg_list)), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g,
-                           a.perm, a.inv_perm, a.pos4, a.r64, a.cell_start, a.excl_off, a.excl, a.cap,
-                           a.list, a.nneb, a.flags, nwg_list, a.only);
+    const size_t rdf_lds = a.rdf_hist ? (size_t)a.rdf_nbin * sizeof(unsigned int) : 0;
+    auto go = [&](auto kernel) {
+        hipLaunchKernelGGL(kernel, dim3(xcd_grid(nwg_list)), dim3(kBlock), rdf_lds, st, a.sys, a.wg_sys, a.g, a.perm,
+                           a.inv_perm, a.pos4, a.r64, a.cell_start, a.excl_off, a.excl, a.cap, a.list, a.nneb, a.flags,
+                           nwg_list, a.only, a.rdf_hist, a.rdf_samples, a.rdf_nbin, a.rdf_inv_dr);
+    };
+    if (a.rdf_hist) a.cubic ? go(k_build_list<true, true>) : go(k_build_list<false, true>);
+    else a.cubic ? go(k_build_list<true, false>) : go(k_build_list<false, false>);
 }
 
 template <bool M, bool L, int G, bool K>
@@ -1913,11 +1894,5 @@ void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const in
     }
 }
 
-void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
-                const int *list, const int2 *nneb, int cap, int nbin, float inv_dr, unsigned long long *hist,
-                int *samples) {
-    hipLaunchKernelGGL(k_rdf, dim3(nwg), dim3(kBlock), (size_t)nbin * sizeof(unsigned int), st, sys, wg_sys, perm, pos4,
-                       list, nneb, cap, nbin, inv_dr, hist, samples);
-}
 
 }  // namespace ibx

@@ -32,6 +32,11 @@ struct RebuildArgs {
     int2 *nneb;
     int *flags;
     int only;           // >= 0: rebuild only this system (the others keep their lists), else every system
+    // RDF of the listed group-0 pairs, accumulated by the list build (null: none)
+    unsigned long long *rdf_hist;
+    int *rdf_samples;
+    int rdf_nbin;
+    float rdf_inv_dr;   // 1 / bin width (nm^-1)
 };
 
 struct NbArgs {
@@ -118,8 +123,6 @@ void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const in
                         int *flags, const FinLaunch *fin);
 // Sets n ints to v (a kernel, not a memset node, so that code a caller may capture stays replayable).
 void launch_fill_int(hipStream_t st, int *p, int n, int v);
-void launch_rdf(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const int *perm, const int4 *pos4,
-                const int *list, const int2 *nneb, int cap, int nbin, float inv_dr, unsigned long long *hist,
-                int *samples);
+
 
 }  // namespace ibx

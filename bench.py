@@ -494,6 +494,8 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline: seconds per run (3 runs)")
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB five times (HBM counter calibration)")
+    ap.add_argument("--only-ua", action="store_true",
+                    help="only the 100k-site united-atom leg, printed alone (profiling runs)")
     args = ap.parse_args()
     if args.calib_copy:
         import torch
@@ -509,6 +511,9 @@ def main():
         print(f"bench.py measures one GPU only (asked for {args.gpus}); replicas and jobs shard across GPUs "
               "without communication (one serving loop or batch per GPU)", file=sys.stderr)
         sys.exit(2)
+    if args.only_ua:
+        print(json.dumps({"ua_melt_100k": ua_leg(args)}))
+        return
     legs = set(x for x in args.legs.split(",") if x)
     if "plain" not in legs:
         args_plain = False

@@ -561,24 +561,15 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
         while (p >= e.x) e = rng[grp][++r];
         return e.y + p;
     };
-    // each step's candidate and its position are loaded during the step before
-    int jn = -1;
-    int4 pn = make_int4(0, 0, 0, 0);
-    if (sub < T) {
-        jn = cand_slot(sub);
-        pn = pos4[jn];
-    }
     for (int t0 = 0; t0 < tmax; t0 += kListLanes) {
         const int p = t0 + sub;
-        const int j = jn;
-        const int4 pj = pn;
-        if (p + kListLanes < T) {
-            jn = cand_slot(p + kListLanes);
-            pn = pos4[jn];
-        }
+        int j = -1;
+        int4 pj = make_int4(0, 0, 0, 0);
         bool hit = false, inner = false;
         float r2u = 0.f;
         if (p < T) {
+            j = cand_slot(p);
+            pj = pos4[j];
             float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
             float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
             float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);

@@ -1769,8 +1769,6 @@ int ibx_set_nb_table(ibx_ctx *c, int sys, int t, int nrow, const double *x, cons
     HostSystem &h = c->hs[sys];
     if (t < 0 || t >= (int)h.nt.size()) return fail(c, IBX_E_INVALID, "invalid table index");
     // same load + FTABLE as set-up
-    ibx_tables one{1, nullptr, nullptr, nullptr, x, v};
-    (void)one;
     Table nt;
     {
         nt.x.assign(nrow + 4, 0.0);

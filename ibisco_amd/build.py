@@ -14,7 +14,7 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libibx.so")
 OBJDIR = os.path.join(HERE, "_build")
-SOURCES = ["kernels.hip", "api.hip", "topology.cpp", "io.cpp", "serve.cpp"]
+SOURCES = ["kernels.hip", "api.hip", "topology.cpp", "io.cpp", "serve.cpp", "output.cpp"]
 ARCH = os.environ.get("IBX_ARCH", "gfx950")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")

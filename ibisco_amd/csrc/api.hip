@@ -1626,6 +1626,8 @@ int ibx::assign_system(ibx_ctx *c, int k, HostSystem &&h, int step0) {
 
 SlotSizes ibx::slot_capacity(const ibx_ctx *c, int k) { return c->lay[k].cap; }
 
+const HostSystem *ibx::host_system(const ibx_ctx *c, int k) { return (c && k >= 0 && k < c->nsys) ? &c->hs[k] : nullptr; }
+
 int ibx::batch_period(const ibx_ctx *c) { return block_period(c); }
 
 extern "C" {

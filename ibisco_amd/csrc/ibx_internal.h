@@ -117,6 +117,8 @@ int device_free_bytes(size_t *free_bytes);
 int assign_system(ibx_ctx *c, int k, HostSystem &&h, int step0);
 // What place k of c holds.
 SlotSizes slot_capacity(const ibx_ctx *c, int k);
+// System k of c as set up on the host (null for an invalid index).
+const HostSystem *host_system(const ibx_ctx *c, int k);
 // Steps of one graph block of c (list interval, RESPA factor and MTS cycle); a place taken over at a multiple
 // of it starts its own steps in phase with the batch's.
 int batch_period(const ibx_ctx *c);

@@ -58,7 +58,8 @@ EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_s
            "ibx_get_info", "ibx_retire", "ibx_server_create", "ibx_server_submit", "ibx_server_wait",
            "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes",
            "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa", "ibx_get_counters", "ibx_step_noalloc",
-           "ibx_assign", "ibx_get_capacity", "ibx_server_timing", "ibx_set_last_step")
+           "ibx_assign", "ibx_get_capacity", "ibx_server_timing", "ibx_set_last_step", "ibx_trz_write_frame",
+           "ibx_restart_write", "ibx_write_trz", "ibx_write_restart", "ibx_output_last_error")
 
 _lib = None
 
@@ -99,6 +100,15 @@ def load_library(path=LIB_PATH):
     L.ibx_retire.argtypes = [vp, C.c_int]
     L.ibx_assign.argtypes = [vp, C.c_int, C.POINTER(IbxSystemDesc), C.c_int]
     L.ibx_set_last_step.argtypes = [vp, C.c_int, C.c_int]
+    L.ibx_trz_write_frame.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, _dp, _dp,
+                                      C.c_double, C.c_double, C.c_double, _dp, _dp]
+    L.ibx_restart_write.argtypes = [C.c_char_p, C.c_char_p, C.c_double, _dp, C.c_int, _ip, _ip, _ip, _dp, _dp,
+                                    C.c_int, _ip, C.POINTER(C.c_char_p)]
+    L.ibx_write_trz.argtypes = [vp, C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double]
+    L.ibx_write_restart.argtypes = [vp, C.c_int, C.c_char_p, C.c_char_p, C.c_int, C.c_double, C.c_int, _ip,
+                                    C.POINTER(C.c_char_p)]
+    L.ibx_output_last_error.argtypes = []
+    L.ibx_output_last_error.restype = C.c_char_p
     L.ibx_get_capacity.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.ibx_set_skin_tuning.argtypes = [vp, C.c_int]
     L.ibx_set_respa.argtypes = [vp, C.c_int]
@@ -323,6 +333,22 @@ class Engine:
         self.natoms[sys] = a["natoms"]
         self.nitems[sys] = a["natoms"] + a["nvs"]
 
+    def write_trz(self, path, frame, step, sys=0, title="ibx", ntrj=1, initime_ps=0.0):
+        """Appends a TRZ frame of system ``sys`` (ibx_write_trz; frame 1 creates the file)."""
+        rc = self.L.ibx_write_trz(self.h, int(sys), path.encode(), title.encode(), int(frame), int(ntrj), int(step),
+                                  float(initime_ps))
+        if rc != 0:
+            raise IbxError(rc, self.L.ibx_output_last_error().decode())
+
+    def write_restart(self, path, step, mol_sizes, mol_names, sys=0, title="ibx", initime_ps=0.0):
+        """The restart file of system ``sys`` (ibx_write_restart)."""
+        natm = np.ascontiguousarray(mol_sizes, np.int32)
+        names = (C.c_char_p * len(mol_names))(*[str(n).encode() for n in mol_names])
+        rc = self.L.ibx_write_restart(self.h, int(sys), path.encode(), title.encode(), int(step), float(initime_ps),
+                                      len(natm), _p(natm, C.c_int), names)
+        if rc != 0:
+            raise IbxError(rc, self.L.ibx_output_last_error().decode())
+
     def set_last_step(self, sys, nsteps):
         """System ``sys`` stops by itself after its own step ``nsteps`` (< 0: no limit; ibx_set_last_step)."""
         self._chk(self.L.ibx_set_last_step(self.h, int(sys), int(nsteps)))
@@ -439,3 +465,35 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def trz_write_frame(path, title, frame, ntrj, time_ps, box, pt, epot, ek, mktemp, s, v):
+    """One TRZ frame from host arrays (ibx_trz_write_frame; reduced units, s nm and v reduced, natoms x 3)."""
+    L = load_library()
+    s = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+    box = np.ascontiguousarray(box, np.float64)
+    pt = np.ascontiguousarray(pt, np.float64)
+    rc = L.ibx_trz_write_frame(path.encode(), title.encode(), int(frame), int(ntrj), len(s), float(time_ps),
+                               _p(box, C.c_double), _p(pt, C.c_double), float(epot), float(ek), float(mktemp),
+                               _p(s, C.c_double), _p(v, C.c_double))
+    if rc != 0:
+        raise IbxError(rc, L.ibx_output_last_error().decode())
+
+
+def restart_write(path, title, time_ps, box, itype, nbonds, jbond, s, v, mol_sizes, mol_names):
+    """The restart file from host arrays (ibx_restart_write; itype, jbond 0-based, jbond natoms x 4)."""
+    L = load_library()
+    s = np.ascontiguousarray(s, np.float64).reshape(-1, 3)
+    v = np.ascontiguousarray(v, np.float64).reshape(-1, 3)
+    box = np.ascontiguousarray(box, np.float64)
+    itype = np.ascontiguousarray(itype, np.int32)
+    nbonds = np.ascontiguousarray(nbonds, np.int32)
+    jbond = np.ascontiguousarray(jbond, np.int32).reshape(-1, 4)
+    natm = np.ascontiguousarray(mol_sizes, np.int32)
+    names = (C.c_char_p * len(mol_names))(*[str(n).encode() for n in mol_names])
+    rc = L.ibx_restart_write(path.encode(), title.encode(), float(time_ps), _p(box, C.c_double), len(s),
+                             _p(itype, C.c_int), _p(nbonds, C.c_int), _p(jbond, C.c_int), _p(s, C.c_double),
+                             _p(v, C.c_double), len(natm), _p(natm, C.c_int), names)
+    if rc != 0:
+        raise IbxError(rc, L.ibx_output_last_error().decode())

@@ -1,9 +1,11 @@
 !> A step driver in the shape of NEW_LOOP.F90:19-156 over libibx.so: reads the
 !> input directory (the reference's control/interaction/coordinate files), runs
 !> NSTEP steps in blocks of NTRJ on the GPU and prints, per block, the
-!> observables OUTPUT.F90 reports (energies in kJ/mol, temperature in K).
+!> observables OUTPUT.F90 reports (energies in kJ/mol, temperature in K); with
+!> a fourth argument, also appends a TRZ frame per block to that file, as
+!> WRITETRJ.F90 does every NTRJ steps (title: the coordinate file's first word).
 !>
-!>   ibx_driver <input dir> <nsteps> <block>
+!>   ibx_driver <input dir> <nsteps> <block> [trajectory file]
 PROGRAM IBX_DRIVER
   USE, INTRINSIC :: ISO_C_BINDING
   USE IBX_MOD
@@ -12,8 +14,9 @@ PROGRAM IBX_DRIVER
   REAL(8), PARAMETER :: KB = 1.380658D-23, NA = 6.0221367D23
   REAL(8), PARAMETER :: CONV = NA * KB * 300.0D0 / 1000.0D0, TEMPSCALE = 300.0D0
   TYPE(C_PTR) :: ctx
-  CHARACTER(LEN=1024) :: dir, arg
-  INTEGER :: nstep, ntrj, step, n, rc
+  CHARACTER(LEN=1024) :: dir, arg, trz
+  CHARACTER(LEN=80) :: title
+  INTEGER :: nstep, ntrj, step, n, rc, frame
   REAL(C_DOUBLE) :: obs(32)
   REAL(8) :: epot, ekin
 
@@ -22,6 +25,15 @@ PROGRAM IBX_DRIVER
   READ(arg, *) nstep
   CALL GET_COMMAND_ARGUMENT(3, arg)
   READ(arg, *) ntrj
+  trz = ''
+  IF (COMMAND_ARGUMENT_COUNT() >= 4) CALL GET_COMMAND_ARGUMENT(4, trz)
+  title = 'ibx'
+  IF (LEN_TRIM(trz) > 0) THEN   ! RDCOOR.f90:55-58: the title is the first word of the coordinate file
+     OPEN(31, FILE=TRIM(dir) // '/coordinate', STATUS='OLD', ACTION='READ')
+     READ(31, *) title
+     CLOSE(31)
+  END IF
+  frame = 0
 
   rc = ibx_load_dir(TRIM(dir) // C_NULL_CHAR, ctx)
   IF (rc /= IBX_OK) THEN
@@ -39,6 +51,16 @@ PROGRAM IBX_DRIVER
         STOP 1
      END IF
      step = step + n
+     IF (LEN_TRIM(trz) > 0) THEN
+        frame = frame + 1
+        rc = ibx_write_trz(ctx, 0, TRIM(trz) // C_NULL_CHAR, TRIM(title) // C_NULL_CHAR, frame, ntrj, step, &
+             0.0_C_DOUBLE)
+        IF (rc /= IBX_OK) THEN
+           WRITE(*, '(A,I0)') 'ibx_write_trz failed: ', rc
+           CALL ibx_destroy(ctx)
+           STOP 1
+        END IF
+     END IF
      rc = ibx_get_observables(ctx, 0, obs)
      epot = SUM(obs(1:15))
      ekin = obs(22) + obs(23)

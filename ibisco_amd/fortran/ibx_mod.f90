@@ -62,6 +62,17 @@ MODULE IBX_MOD
        REAL(C_DOUBLE), INTENT(OUT) :: sums(32)
      END FUNCTION ibx_get_averages
 
+     ! one TRZ frame of system sys (WRITETRJ.F90 format), frame 1 creating the file; time = step*DT + initime
+     INTEGER(C_INT) FUNCTION ibx_write_trz(ctx, sys, path, title, frame, ntrj, step, initime) &
+          BIND(C, NAME='ibx_write_trz')
+       IMPORT :: C_INT, C_PTR, C_CHAR, C_DOUBLE
+       TYPE(C_PTR), VALUE :: ctx
+       INTEGER(C_INT), VALUE :: sys
+       CHARACTER(KIND=C_CHAR), DIMENSION(*), INTENT(IN) :: path, title
+       INTEGER(C_INT), VALUE :: frame, ntrj, step
+       REAL(C_DOUBLE), VALUE :: initime
+     END FUNCTION ibx_write_trz
+
      ! s(3, NITEMS) unwrapped positions (SXYZ), v(3, NATOMS) reduced velocities (VXYZ)
      INTEGER(C_INT) FUNCTION ibx_get_state(ctx, sys, s, v) BIND(C, NAME='ibx_get_state')
        IMPORT :: C_INT, C_PTR, C_DOUBLE

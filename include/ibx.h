@@ -245,6 +245,45 @@ int ibx_assign(ibx_ctx *ctx, int sys, const ibx_system_desc *desc, int step0);
  * loop). */
 int ibx_get_capacity(ibx_ctx *ctx, int sys, int64_t cap[16]);
 
+/* ---- Trajectory and restart output (WRITETRJ.F90) ------------------------
+ * Host-side writers of the reference's output formats, fed from the device
+ * state at output steps only (the window sums stay on the device between
+ * them).  Errors are reported through ibx_output_last_error. */
+
+/* Appends one TRZ frame to path as WRITETRJ.F90:31-104 writes it (frame 1
+ * creates the file and writes the header records: the title in 80
+ * characters, then 10).  Fortran unformatted sequential records, each between
+ * 4-byte length markers: frame, frame*ntrj, natoms, time (ps); the box as a
+ * 3x3 matrix (nm); pressure and tensor in kPa (p, 11, 12, 22, 13, 23, 33); 6,
+ * then total, potential and kinetic energy (kJ/mol), the temperature as
+ * SUM(EK)*MKTEMP (reduced, as the reference writes it), 0, 0; x, y, z (nm) and
+ * vx, vy, vz (nm/ps) in single precision.  Inputs in reduced units: pt the
+ * pressure tensor (11 22 33 12 13 23), epot and ek the potential and kinetic
+ * energy, mktemp as in ibx_system; s natoms*3 (nm) and v natoms*3 (reduced),
+ * xyz interleaved. */
+int ibx_trz_write_frame(const char *path, const char *title, int frame, int ntrj, int natoms, double time_ps,
+                        const double box[3], const double pt[6], double epot, double ek, double mktemp,
+                        const double *s, const double *v);
+/* The restart file of WRITETRJ.F90:115-178, which RDCOOR.f90 reads back as a
+ * coordinate file: title, time (ps), box, the record description, then per
+ * molecule (nmol of natm[m] atoms named names[m]) each atom's index, type,
+ * bond count and position (nm), and its velocity (nm/ps) with its bonded
+ * atoms.  itype (0-based), nbonds and jbond (natoms*4, 0-based) as in
+ * ibx_system_desc; v reduced. */
+int ibx_restart_write(const char *path, const char *title, double time_ps, const double box[3], int natoms,
+                      const int *itype, const int *nbonds, const int *jbond, const double *s, const double *v,
+                      int nmol, const int *natm, const char *const *names);
+/* A TRZ frame of system sys from its current state and observables
+ * (ibx_get_state, ibx_get_observables after a step); time = step * dt +
+ * initime_ps (WRITETRJ.F90:57). */
+int ibx_write_trz(ibx_ctx *ctx, int sys, const char *path, const char *title, int frame, int ntrj, int step,
+                  double initime_ps);
+/* The restart file of system sys from its current state (molecules as in
+ * ibx_restart_write). */
+int ibx_write_restart(ibx_ctx *ctx, int sys, const char *path, const char *title, int step, double initime_ps,
+                      int nmol, const int *natm, const char *const *names);
+const char *ibx_output_last_error(void);
+
 /* ---- Serving loop (no counterpart in the reference) ----------------------
  * Jobs (one system and a step count each) are queued in arrival order.  A
  * queued job goes to a free place of a running batch that holds it (the

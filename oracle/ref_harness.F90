@@ -7,6 +7,8 @@
 !   run N        N steps in NEW_LOOP order (MTS_FORCE when mts>0), dump per-step
 !                observables (ref_obs.bin) and the final state (ref_state.bin)
 !   time N       the same N steps timed with OMP_GET_WTIME (prints LOOP_SECONDS)
+!   trz N        N steps, then the reference's WRITETRJ twice at that state (s-md.trj with two frames,
+!                config.xyz, restart), and the values it wrote them from (ref_trz_vals.bin)
 ! For systems above 20000 items the exclusion lists are built by an O(N)
 ! routine of this file (same lists as BUILD_CONNECTIVITY.f90, whose NITEMS^2
 ! LOGICAL matrix would not fit in memory).  Without virtual sites the force
@@ -146,6 +148,23 @@ PROGRAM REF_HARNESS
      WRITE(22) NITEMS, NATOMS
      WRITE(22) SXYZ, VXYZ, FXYZ
      CLOSE(22)
+  ELSE IF (MODE == 'trz') THEN
+     DO K = 1, NRUN
+        STEP = K
+        CALL ONE_STEP(STEP)
+     END DO
+     OPEN(113, FILE='s-md.trj', FORM='UNFORMATTED', ACCESS='SEQUENTIAL', STATUS='REPLACE')
+     NFRAME = 0
+     CALL WRITETRJ(NRUN)
+     CALL WRITETRJ(NRUN)
+     CLOSE(113)
+     OPEN(23, FILE='ref_trz_vals.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+     WRITE(23) NATOMS, NTRJ
+     WRITE(23) DBLE(PT11), DBLE(PT22), DBLE(PT33), DBLE(PT12), DBLE(PT13), DBLE(PT23)
+     WRITE(23) SUM(V_BOND) + SUM(V_ANGLE) + SUM(V_TORSION) + SUM(V_OOP) + SUM(V_NB), SUM(EK), DBLE(MKTEMP)
+     WRITE(23) DBLE(BOX(1)), DBLE(BOX(2)), DBLE(BOX(3)), DBLE(DT), DBLE(INITIME)
+     WRITE(23) SXYZ(:, 1:NATOMS), VXYZ(:, 1:NATOMS)
+     CLOSE(23)
   ELSE IF (MODE == 'time') THEN
      T0 = OMP_GET_WTIME()
      DO K = 1, NRUN

@@ -12,7 +12,8 @@ testsuite compares against (testsuite/tests.py:47-68).  comb_oop is a
 generated comb polymer whose branch atoms carry out-of-plane terms, and
 comb_oop_only the same with every other table zero (out-of-plane forces alone);
 hybridPA_mts runs hybridPA through MTS_LOOP's force (MTS_FORCE, NMTS 2) for 12 steps;
-cg_ortho is a melt in an orthorhombic box.
+cg_ortho is a melt in an orthorhombic box; trz_cg2000 keeps the trajectory
+frame and restart file the reference's WRITETRJ writes after 10 steps of cg2000.
 
 Inputs are stored in single precision, which is what the reference's REAL*4
 readers see, so the oracle and the reference start from identical numbers.
@@ -275,6 +276,36 @@ def drift_case(state_path, nsteps=1000):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def trz_case(nsteps=10):
+    """The reference's own WRITETRJ (TRZ frame twice, restart, config.xyz) after nsteps of cg2000: its files as
+    bytes and the values it wrote them from (trz_cg2000.npz)."""
+    sysm = single(system.cg_melt(nchains=40, nbeads=50, seed=3, avoid=0.8))
+    d = tempfile.mkdtemp(prefix="ibx_trz_")
+    try:
+        system.write_dir(sysm, d)
+        run_harness(d, "trz", str(nsteps))
+        b = open(os.path.join(d, "ref_trz_vals.bin"), "rb").read()
+        natoms, ntrj = np.frombuffer(b, "<i4", 2)
+        off = 8
+        pt = np.frombuffer(b, "<f8", 6, off); off += 48
+        epot, ek, mktemp = np.frombuffer(b, "<f8", 3, off); off += 24
+        box_dt_init = np.frombuffer(b, "<f8", 5, off); off += 40
+        s = np.frombuffer(b, "<f4", 3 * natoms, off).reshape(-1, 3); off += 12 * natoms
+        v = np.frombuffer(b, "<f4", 3 * natoms, off).reshape(-1, 3)
+        data = dict(trj=np.frombuffer(open(os.path.join(d, "s-md.trj"), "rb").read(), np.uint8),
+                    restart=np.frombuffer(open(os.path.join(d, "restart"), "rb").read(), np.uint8),
+                    natoms=natoms, ntrj=ntrj, nsteps=nsteps, pt=pt, epot=epot, ek=ek, mktemp=mktemp,
+                    box=box_dt_init[:3], dt=box_dt_init[3], initime=box_dt_init[4], s=s.copy(), v=v.copy(),
+                    title=np.array(sysm.title), mol_sizes=np.array(sysm.mol_sizes),
+                    mol_names=np.array(sysm.mol_names), itype=np.array(sysm.itype, np.int32),
+                    nbonds=np.array([len(j) for j in sysm.jbond], np.int32),
+                    jbond=np.array([(list(j) + [-1] * 4)[:4] for j in sysm.jbond], np.int32))
+        np.savez_compressed(os.path.join(HERE, "trz_cg2000.npz"), **data)
+        print(f"trz_cg2000: {natoms} atoms, {len(data['trj'])} TRZ bytes, {len(data['restart'])} restart bytes")
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+
+
 def make_case(name, sysm, runs=()):
     sysm = single(sysm)
     d = tempfile.mkdtemp(prefix="ibx_gold_")
@@ -317,6 +348,8 @@ def main(only=()):
                 make_case("ua_collinear", collinear_case(), runs=(10,))
             if name == "comb_oop_only":
                 make_case("comb_oop_only", system.comb_melt(nchains=36, seed=1, oop_only=True), runs=(10,))
+            if name == "trz_cg2000":
+                trz_case()
         return
     make_case("cg2000", system.cg_melt(nchains=40, nbeads=50, seed=3, avoid=0.8), runs=(10,))
     ua = system.ua_melt(nchains=20, nbeads=100, seed=5, avoid=0.6)   # 3 cells per edge at rlist 1.3
@@ -339,6 +372,7 @@ def main(only=()):
     make_case("cg_ortho", ortho_case(), runs=(10,))
     make_case("hybridPA_virt", virt_case(), runs=(10,))
     make_case("ua_collinear", collinear_case(), runs=(10,))
+    trz_case()
 
 
 if __name__ == "__main__":

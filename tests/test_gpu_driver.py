@@ -12,7 +12,7 @@ import tempfile
 import numpy as np
 import pytest
 
-from ibisco_amd import system
+from ibisco_amd import system, units
 from ibisco_amd.build import DRIVER
 from ibisco_amd.engine import Engine
 from ibisco_amd.units import CONV
@@ -20,9 +20,9 @@ from ibisco_amd.units import CONV
 pytestmark = pytest.mark.gpu
 
 
-def _run_driver(d, nsteps, block):
+def _run_driver(d, nsteps, block, *extra):
     assert os.path.exists(DRIVER), "ibx_driver is not built: run `python -m ibisco_amd.build`"
-    r = subprocess.run([DRIVER, d, str(nsteps), str(block)], capture_output=True, text=True, timeout=120)
+    r = subprocess.run([DRIVER, d, str(nsteps), str(block), *extra], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     rows = [list(map(float, ln.split())) for ln in r.stdout.splitlines() if ln.strip() and not ln.startswith("#")]
     return np.array(rows)
@@ -71,3 +71,39 @@ def test_fortran_driver_runs_the_mts_cycle():
         print("step", step, "V_NB driver", rows[k, 5], "reference", vnb_ref, flush=True)
         assert abs(rows[k, 5] - vnb_ref) <= 1e-4 * abs(epot_ref)
         assert abs(rows[k, 2] - epot_ref) <= 1e-4 * abs(epot_ref)
+
+
+def _records(b):
+    out, off = [], 0
+    while off < len(b):
+        n = int(np.frombuffer(b, "<u4", 1, off)[0])
+        out.append(b[off + 4:off + 4 + n])
+        off += 8 + n
+    return out
+
+
+def test_fortran_driver_writes_the_trajectory():
+    """With a trajectory file the driver appends a TRZ frame per block (WRITETRJ.F90 format): two frames after
+    20 steps in blocks of 10, whose positions and velocities are the state the Python binding reaches, in single
+    precision, and whose energies are its observables."""
+    sysm = system.cg_melt(nchains=20, nbeads=50, seed=4, avoid=0.8, dt=0.002)
+    with tempfile.TemporaryDirectory() as d:
+        system.write_dir(sysm, d)
+        trj = os.path.join(d, "s-md.trj")
+        print("driver on", d, flush=True)
+        _run_driver(d, 20, 10, trj)
+        rec = _records(open(trj, "rb").read())
+        e = Engine(None, load_dir=d)
+        e.step(20, graph=True)
+        s, v = e.state()
+        ob = e.obs()
+        e.close()
+    n = sysm.natoms
+    assert len(rec) == 2 + 2 * 10 and rec[0].decode().strip() == sysm.title
+    assert np.frombuffer(rec[12], "<i4", 3).tolist() == [2, 20, n]
+    for k in range(3):
+        assert np.array_equal(np.frombuffer(rec[12 + 4 + k], "<f4"), s[:n, k].astype(np.float32))
+        vk = (v[:, k] * (1.0 / 1.0e12 / units.TIMESCALE)).astype(np.float32)
+        assert np.array_equal(np.frombuffer(rec[12 + 7 + k], "<f4"), vk)
+    epot = sum(float(ob[k].sum()) for k in ("vnb", "vbond", "vangle", "vtors", "voop"))
+    assert abs(np.frombuffer(rec[12 + 3], "<f8", 1, 12)[0] - epot * CONV) <= 1e-12 * abs(epot * CONV)

@@ -56,6 +56,10 @@ def flatten(sysm: System):
                 for k in TABLE_KINDS},
         ensemble={"NVE": 0, "NVT": 1, "NPT": 2}[c.ensemble.upper()],
         dt=dt, temp_in=units.temp_to_reduced(c.temperature), fac=dt / taut,
+        # SCALEBP.f90: PRESSURE = PRESSURE0/PSCALE, PFAC = BETA*DT*LIMAVP/TAUP (RDCONTROL.f90:107-108,317-337,
+        # ALLOCATEVAR.f90:17, LIMAVP = 10)
+        pressure=c.pressure / units.PSCALE,
+        pfac=(c.beta * units.PSCALE) * dt * 10 / units.dt_to_reduced(c.taup),
         rcut_atom=c.cutoff, rlist_atom=c.rlist, rcut_bead=c.bead_cutoff, rlist_bead=c.bead_rlist,
         nonbexc_atom=c.non_bonded, nonbexc_bead=c.non_bonded_bead,
         nupdate=c.nupdate, halt_drift=c.halt_drift, nmts=c.mts,

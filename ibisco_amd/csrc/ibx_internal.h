@@ -67,6 +67,7 @@ struct HostSystem {
     int ngroup = 1;
 
     int ensemble = 0, nupdate = 1, halt_drift = 1, nmts = 0, mts_quirk = 0;
+    double pressure = 0, pfac = 0;      // NPT barostat (SCALEBP.f90), reduced
     double dt = 0, temp_in = 0, fac = 0;
     int numatoms = 0, numbeads = 0;
     double mktemp = 0, mktemp_atom = 0, mktemp_bead = 0;

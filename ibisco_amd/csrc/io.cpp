@@ -24,6 +24,7 @@ struct Units {
     double timescale = rscale * std::sqrt(massscale / escale);
     double vscale = rscale / timescale;
     double tempscale = escale / kKB;
+    double pscale = escale / (rscale * rscale * rscale) / 1000.0;   // reduced pressure -> kPa
 };
 
 std::vector<std::string> tokens(const std::string &line) {
@@ -95,7 +96,8 @@ int read_input_dir(const char *dir, DescStorage &st, std::string &err) {
     for (auto &ch : ens) ch = (char)toupper(ch);
     if (ens == "NVE") D.ensemble = 0;
     else if (ens == "NVT") D.ensemble = 1;
-    else { err = "You did not input a valid Ensemble in control (NVE/NVT on the device path)"; return IBX_E_INVALID; }
+    else if (ens == "NPT") D.ensemble = 2;
+    else { err = "You did not input a valid Ensemble in control (NVE/NVT/NPT on the device path)"; return IBX_E_INVALID; }
     const double temp_k = num("temperature", 300, true);
     const int natoms = (int)num("atoms", 0, true);
     st.nstep = (int)num("num_of_time_steps", 0, true);
@@ -111,11 +113,17 @@ int read_input_dir(const char *dir, DescStorage &st, std::string &err) {
     D.halt_drift = (int)num("halt_drift", 0, true);
     st.ntrj = (int)num("trajectory", 0, true);
     const double taut = num("temperature_coupling_time", 0.1, true);
+    // RDCONTROL.f90:99-108, 307-345: the barostat's keywords are required for NPT
+    const double pressure_kpa = num("pressure", 101.3, D.ensemble == 2);
+    const double taup = num("pressure_coupling_time", 5.0, D.ensemble == 2);
+    const double beta = num("isothermal_compressibility", 1.0e-6, D.ensemble == 2);
     D.nmts = (int)num("mts", 0, true);
     if (!err.empty()) return IBX_E_INVALID;
     D.dt = dt_ps * 1.0e-12 / U.timescale;
     D.temp_in = temp_k / U.tempscale;
     D.fac = D.dt / (taut * 1.0e-12 / U.timescale);
+    D.pressure = pressure_kpa / U.pscale;
+    D.pfac = beta * U.pscale * D.dt * 10.0 / (taup * 1.0e-12 / U.timescale);   // ALLOCATEVAR.f90:17, LIMAVP = 10
     // the reference's MTS_LOOP takes I = MOD(STEP, 3+NMTS) = 0 on the last step of each cycle
     // (MTS_LOOP.f90:100, :176): an input directory runs the same cycle
     D.mts_quirk = 1;

@@ -345,7 +345,11 @@ int build_host_system(const ibx_system_desc *d, HostSystem &hs, std::string &err
     hs.dt = d->dt; hs.temp_in = d->temp_in; hs.fac = d->fac;
     hs.nupdate = d->nupdate; hs.halt_drift = d->halt_drift; hs.nmts = d->nmts; hs.mts_quirk = d->mts_quirk;
     if (hs.nupdate <= 0 || hs.halt_drift == 0) { err = "update_neighbour_list and halt_drift must be positive"; return IBX_E_INVALID; }
-    if (hs.ensemble != 0 && hs.ensemble != 1) { err = "only NVE and NVT are supported on the device path"; return IBX_E_INVALID; }
+    if (hs.ensemble < 0 || hs.ensemble > 2) { err = "ensemble must be NVE, NVT or NPT"; return IBX_E_INVALID; }
+    hs.pressure = d->pressure;
+    hs.pfac = d->pfac;
+    if (hs.ensemble == 2 && !(hs.pfac >= 0.0)) { err = "NPT needs a non-negative BETA*DT*LIMAVP/TAUP"; return IBX_E_INVALID; }
+    if (hs.ensemble == 2) { err = "NPT is not supported on the device path yet"; return IBX_E_INVALID; }
 
     // initial temperatures (main.F90:173-195) drive the first Berendsen factor
     hs.mktemp = 2.0 / (3.0 * (na - 1));

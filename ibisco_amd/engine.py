@@ -35,6 +35,7 @@ class IbxSystemDesc(C.Structure):
         ("rcut_atom", C.c_double), ("rlist_atom", C.c_double), ("rcut_bead", C.c_double),
         ("rlist_bead", C.c_double), ("nonbexc_atom", C.c_int), ("nonbexc_bead", C.c_int),
         ("nupdate", C.c_int), ("halt_drift", C.c_int), ("nmts", C.c_int), ("mts_quirk", C.c_int),
+        ("pressure", C.c_double), ("pfac", C.c_double),
     ]
 
 
@@ -166,6 +167,7 @@ def make_desc(sysm, keep, mts_quirk=False):
     d.rcut_bead, d.rlist_bead = a["rcut_bead"], a["rlist_bead"]
     d.nonbexc_atom, d.nonbexc_bead = a["nonbexc_atom"], a["nonbexc_bead"]
     d.nupdate, d.halt_drift, d.nmts, d.mts_quirk = a["nupdate"], a["halt_drift"], a["nmts"], int(mts_quirk)
+    d.pressure, d.pfac = a["pressure"], a["pfac"]
     keep.append(d)
     return d, a
 

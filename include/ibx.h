@@ -65,12 +65,16 @@ typedef struct {
     const int *virtbonds;      /* nvirtbonds*2 */
     int nvirtangles;
     const int *virtangles;     /* nvirtangles*3 */
-    int ensemble;              /* 0 NVE, 1 NVT (Berendsen) */
+    int ensemble;              /* 0 NVE, 1 NVT (Berendsen), 2 NPT (Berendsen thermostat and barostat) */
     double dt, temp_in, fac;   /* reduced time step, target temperature, dt/taut */
     double rcut_atom, rlist_atom, rcut_bead, rlist_bead;
     int nonbexc_atom, nonbexc_bead; /* 4: exclude 1-2,1-3; 5: also 1-4 */
     int nupdate, halt_drift, nmts;
     int mts_quirk;             /* 1 reproduces MTS_LOOP.f90:100 at MOD==0 */
+    /* ensemble 2 (NPT, Berendsen barostat SCALEBP.f90): target pressure and
+     * PFAC = BETA*DT*LIMAVP/TAUP, reduced (RDCONTROL.f90:107-108,317-337,
+     * ALLOCATEVAR.f90:17) */
+    double pressure, pfac;
 } ibx_system_desc;
 
 /* Replaces the set-up half of main.F90:107-234 (RDCONTROL .. UPDATE_NEIGHBOURLIST):

@@ -353,6 +353,7 @@ ox_ctx *ox_create(const ox_desc *d, char *err, int errlen) {
         c->box2[k] = d->box[k] / 2.0;
     }
     c->ensemble = d->ensemble; c->dt = d->dt; c->temp_in = d->temp_in; c->fac = d->fac;
+    c->pressure = d->pressure; c->pfac = d->pfac; c->iavp = 0;
     c->nupdate = d->nupdate; c->halt_drift = d->halt_drift; c->nmts = d->nmts;
     c->mts_quirk = d->mts_quirk;
 
@@ -1099,7 +1100,28 @@ int ox_set_respa(ox_ctx *c, int k) {
     return 0;
 }
 
-/* NEW_LOOP.F90:19-156 and MTS_LOOP.f90:13-60 (NVE/NVT; no barostat) */
+/* SCALEBP.f90:6-44: the instantaneous pressure (PT11 + PT22 + PT33)/3 after MOVE goes into a ring of LIMAVP
+ * = 10 entries; every LIMAVP-th step its mean P scales the box and the positions of the atoms and beads by
+ * MIU = (1 + (P - PRESSURE) PFAC)^(1/3) (virtual sites follow at the next VIRTUAL_DEF).  The cell grid follows
+ * the box at the next list update (UPDATE_NEIGHBOURLIST.F90:14-48; build_group takes it from the box). */
+void ox_scalebp(ox_ctx *c, int step) {
+    c->iavp = c->iavp % 10 + 1;
+    c->sp[c->iavp - 1] = (c->pt[0] + c->pt[1] + c->pt[2]) / 3.0;
+    if (step % 10 != 0) return;
+    double avp = 0.0;
+    for (int k = 0; k < 10; ++k) avp += c->sp[k];
+    const double p = avp / 10.0;
+    const double miu = cbrt(1.0 + (p - c->pressure) * c->pfac);
+    for (int k = 0; k < 3; ++k) {
+        c->box[k] *= miu;
+        c->boxinv[k] = 1.0 / c->box[k];
+        c->box2[k] = c->box[k] / 2.0;
+    }
+    for (int i = 0; i < c->natoms; ++i)
+        for (int k = 0; k < 3; ++k) c->s[3 * i + k] *= miu;
+}
+
+/* NEW_LOOP.F90:19-156 and MTS_LOOP.f90:13-60 */
 int ox_run(ox_ctx *c, int nsteps, int step0) {
     for (int step = step0 + 1; step <= step0 + nsteps; ++step) {
         ox_shift(c);
@@ -1118,6 +1140,7 @@ int ox_run(ox_ctx *c, int nsteps, int step0) {
         }
         ox_move(c);
         if (step % c->halt_drift == 0) ox_momentum(c);
+        if (c->ensemble == 2) ox_scalebp(c, step);   /* NEW_LOOP.F90:120-122, MTS_LOOP.f90:42-44 */
     }
     return 0;
 }
@@ -1132,6 +1155,7 @@ void ox_get_obs(ox_ctx *c, double *o) {
     o[23] = c->temp; o[24] = c->temp_atom; o[25] = c->temp_bead;
     o[26] = c->box[0] * c->box[1] * c->box[2];
     o[27] = (double)c->overext;
+    for (int k = 0; k < 3; ++k) o[28 + k] = c->box[k];
 }
 
 int ox_nitems(ox_ctx *c) { return c->nitems; }

@@ -54,12 +54,13 @@ typedef struct {
     int nvirtangles;
     const int *virtangles;     /* nvirtangles*3 */
     /* control */
-    int ensemble;              /* 0 NVE, 1 NVT */
+    int ensemble;              /* 0 NVE, 1 NVT, 2 NPT */
     double dt, temp_in, fac;   /* reduced dt, target T, dt/taut */
     double rcut_atom, rlist_atom, rcut_bead, rlist_bead;
     int nonbexc_atom, nonbexc_bead;
     int nupdate, halt_drift, nmts;
     int mts_quirk;             /* 1 = reproduce MOD==0 backward extrapolation */
+    double pressure, pfac;     /* NPT: target pressure and BETA*DT*LIMAVP/TAUP, reduced */
 } ox_desc;
 
 typedef struct ox_ctx ox_ctx;
@@ -93,7 +94,7 @@ int ox_run(ox_ctx *c, int nsteps, int step0);
 /* Accessors.  obs layout (double[32]):
  *   0..2 V_NB, 3..5 V_BOND, 6..8 V_ANGLE, 9..11 V_TORSION, 12..14 V_OOP,
  *   15..20 PT11 PT22 PT33 PT12 PT13 PT23, 21..22 EK, 23 TEMP, 24 TEMP_ATOM,
- *   25 TEMP_BEAD, 26 volume, 27 bond overextension count */
+ *   25 TEMP_BEAD, 26 volume, 27 bond overextension count, 28..30 the box */
 void ox_get_obs(ox_ctx *c, double *obs);
 int ox_nitems(ox_ctx *c);
 void ox_get_forces(ox_ctx *c, double *f);   /* nitems*3 */

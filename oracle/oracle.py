@@ -37,6 +37,7 @@ class _Desc(C.Structure):
         ("rcut_atom", C.c_double), ("rlist_atom", C.c_double), ("rcut_bead", C.c_double),
         ("rlist_bead", C.c_double), ("nonbexc_atom", C.c_int), ("nonbexc_bead", C.c_int),
         ("nupdate", C.c_int), ("halt_drift", C.c_int), ("nmts", C.c_int), ("mts_quirk", C.c_int),
+        ("pressure", C.c_double), ("pfac", C.c_double),
     ]
 
 
@@ -142,6 +143,7 @@ class Oracle:
         d.nvirtbonds, d.virtbonds = a["nvirtbonds"], arr(a["virtbonds"], C.c_int)
         d.nvirtangles, d.virtangles = a["nvirtangles"], arr(a["virtangles"], C.c_int)
         d.ensemble, d.dt, d.temp_in, d.fac = a["ensemble"], a["dt"], a["temp_in"], a["fac"]
+        d.pressure, d.pfac = a["pressure"], a["pfac"]
         d.rcut_atom, d.rlist_atom = a["rcut_atom"], a["rlist_atom"]
         d.rcut_bead, d.rlist_bead = a["rcut_bead"], a["rlist_bead"]
         d.nonbexc_atom, d.nonbexc_bead = a["nonbexc_atom"], a["nonbexc_bead"]
@@ -203,7 +205,7 @@ class Oracle:
         self.L.ox_get_obs(self.h, _p(o, C.c_double))
         return dict(vnb=o[0:3].copy(), vbond=o[3:6].copy(), vangle=o[6:9].copy(), vtors=o[9:12].copy(),
                     voop=o[12:15].copy(), pt=o[15:21].copy(), ek=o[21:23].copy(), temp=o[23],
-                    temp_atom=o[24], temp_bead=o[25], volume=o[26], overext=int(o[27]))
+                    temp_atom=o[24], temp_bead=o[25], volume=o[26], overext=int(o[27]), box=o[28:31].copy())
 
     def forces(self):
         f = np.zeros((self.nitems, 3))

@@ -61,6 +61,9 @@ struct ox_ctx {
     int *vtype_natoms;
     /* control */
     int ensemble, nupdate, halt_drift, nmts, mts_quirk;
+    double pressure, pfac;      /* NPT barostat: target pressure, BETA*DT*LIMAVP/TAUP (reduced) */
+    double sp[10];              /* the last LIMAVP instantaneous pressures (SCALEBP.f90:18-21) */
+    int iavp;                   /* SCALEBP.f90's IAVP: entries written so far, modulo LIMAVP */
     double dt, temp_in, fac;
     double mktemp, mktemp_atom, mktemp_bead;
     /* observables */

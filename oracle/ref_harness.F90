@@ -4,8 +4,8 @@
 ! TEST INFRASTRUCTURE ONLY.  It reproduces the set-up sequence of the
 ! reference's main program (main.F90:107-234) without its file output, then
 !   force        one NEW_FORCE, dump forces, energies, virial, lists (ref_dump.bin)
-!   run N        N steps in NEW_LOOP order (MTS_FORCE when mts>0), dump per-step
-!                observables (ref_obs.bin) and the final state (ref_state.bin)
+!   run N        N steps in NEW_LOOP order (MTS_FORCE when mts>0, SCALEBP for NPT), dump per-step
+!                observables (ref_obs.bin), the final state (ref_state.bin) and box (ref_box.bin)
 !   time N       the same N steps timed with OMP_GET_WTIME (prints LOOP_SECONDS)
 !   trz N        N steps, then the reference's WRITETRJ twice at that state (s-md.trj with two frames,
 !                config.xyz, restart), and the values it wrote them from (ref_trz_vals.bin)
@@ -148,6 +148,9 @@ PROGRAM REF_HARNESS
      WRITE(22) NITEMS, NATOMS
      WRITE(22) SXYZ, VXYZ, FXYZ
      CLOSE(22)
+     OPEN(24, FILE='ref_box.bin', ACCESS='STREAM', FORM='UNFORMATTED', STATUS='REPLACE')
+     WRITE(24) DBLE(BOX(1)), DBLE(BOX(2)), DBLE(BOX(3))
+     CLOSE(24)
   ELSE IF (MODE == 'trz') THEN
      DO K = 1, NRUN
         STEP = K
@@ -178,7 +181,7 @@ PROGRAM REF_HARNESS
 
 CONTAINS
 
-  ! Step body in the order of NEW_LOOP.F90:66-113 / MTS_LOOP.f90:26-40 (no output).
+  ! Step body in the order of NEW_LOOP.F90:66-122 / MTS_LOOP.f90:26-44 (no output).
   SUBROUTINE ONE_STEP(ST)
     INTEGER, INTENT(IN) :: ST
     CALL SHIFT()
@@ -191,6 +194,7 @@ CONTAINS
     END IF
     CALL MOVE()
     IF (MOD(ST, HALT_DRIFT) .eq. 0) CALL MOMENTUM()
+    IF (ENSEMBLE .eq. 2) CALL SCALEBP(ST)
   END SUBROUTINE ONE_STEP
 
   SUBROUTINE FORCE()

@@ -13,7 +13,10 @@ generated comb polymer whose branch atoms carry out-of-plane terms, and
 comb_oop_only the same with every other table zero (out-of-plane forces alone);
 hybridPA_mts runs hybridPA through MTS_LOOP's force (MTS_FORCE, NMTS 2) for 12 steps;
 cg_ortho is a melt in an orthorhombic box; trz_cg2000 keeps the trajectory
-frame and restart file the reference's WRITETRJ writes after 10 steps of cg2000.
+frame and restart file the reference's WRITETRJ writes after 10 steps of cg2000;
+hybridPA_npt runs the testsuite's hybridPA input as shipped (NPT: SCALEBP every 10
+steps), and hybridPA_npt_expected keeps the testsuite's own NPT expectations
+(testsuite/tests.py:19-39) as numbers.
 
 Inputs are stored in single precision, which is what the reference's REAL*4
 readers see, so the oracle and the reference start from identical numbers.
@@ -23,6 +26,7 @@ readers see, so the oracle and the reference start from identical numbers.
 """
 from __future__ import annotations
 
+import ast
 import hashlib
 import os
 import re
@@ -306,6 +310,20 @@ def trz_case(nsteps=10):
         shutil.rmtree(d, ignore_errors=True)
 
 
+def hybrid_npt_expected():
+    """The reference testsuite's own NPT expectations for hybridPA (testsuite/tests.py:19-39): coordinates and
+    velocities of the first three atoms and the box after 10 and 40 steps, as MDAnalysis reads them from the
+    TRZ file (angstrom, angstrom/ps)."""
+    src = open(os.path.join(REF, "testsuite", "tests.py")).read()
+    body = src[src.index("class reference_hybrid_data"):src.index("class reference_atomistic_data")]
+    out = {}
+    for key in ("ref_coords_short", "ref_velos_short", "ref_box_short", "ref_coords_long", "ref_velos_long",
+                "ref_box_long"):
+        m = re.search(key + r"\s*=\s*np\.array\((\[.*?\])\s*(?:,\s*dtype=np\.float32)?\)", body, re.S)
+        out[key] = np.array(ast.literal_eval(m.group(1)), dtype=np.float32)
+    return out
+
+
 def make_case(name, sysm, runs=()):
     sysm = single(sysm)
     d = tempfile.mkdtemp(prefix="ibx_gold_")
@@ -322,6 +340,7 @@ def make_case(name, sysm, runs=()):
             data[f"run{n}_s"] = s[:keep]
             data[f"run{n}_v"] = v[:keep]
             data[f"run{n}_obs"] = obs
+            data[f"run{n}_box"] = np.fromfile(os.path.join(d, "ref_box.bin"), "<f8")
         data.update({"sys_" + k: v for k, v in system.to_arrays(sysm, f32=True).items()})
         np.savez_compressed(os.path.join(HERE, f"{name}.npz"), **data)
         print(f"{name}: {res['nitems']} items, {res['pairs0_count']} + {res['pairs1_count']} pairs")
@@ -350,6 +369,9 @@ def main(only=()):
                 make_case("comb_oop_only", system.comb_melt(nchains=36, seed=1, oop_only=True), runs=(10,))
             if name == "trz_cg2000":
                 trz_case()
+            if name == "hybridPA_npt":
+                make_case("hybridPA_npt", testsuite_case("hybridPA"), runs=(10, 40))
+                np.savez_compressed(os.path.join(HERE, "hybridPA_npt_expected.npz"), **hybrid_npt_expected())
         return
     make_case("cg2000", system.cg_melt(nchains=40, nbeads=50, seed=3, avoid=0.8), runs=(10,))
     ua = system.ua_melt(nchains=20, nbeads=100, seed=5, avoid=0.6)   # 3 cells per edge at rlist 1.3
@@ -373,6 +395,8 @@ def main(only=()):
     make_case("hybridPA_virt", virt_case(), runs=(10,))
     make_case("ua_collinear", collinear_case(), runs=(10,))
     trz_case()
+    make_case("hybridPA_npt", testsuite_case("hybridPA"), runs=(10, 40))
+    np.savez_compressed(os.path.join(HERE, "hybridPA_npt_expected.npz"), **hybrid_npt_expected())
 
 
 if __name__ == "__main__":

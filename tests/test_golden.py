@@ -332,3 +332,58 @@ def test_mts_cycle_matches_reference(oracle_mod):
     sr, vr = z["run12_s"].astype(np.float64), z["run12_v"].astype(np.float64)
     assert np.abs(s[:len(sr)] - sr).max() <= POS_ATOL
     assert np.abs(v[:len(vr)] - vr).max() / np.abs(vr).max() <= VEL_RTOL
+
+
+BOX_RTOL = 1e-6           # the box after SCALEBP's rescalings, relative
+
+
+@pytest.mark.parametrize("n", (10, 40))
+def test_npt_runs(n, oracle_mod):
+    """hybridPA in NPT (testsuite/hybridPA-NPT: PRESSURE 101.3 kPa, TAUP 5 ps, BETA 1e-6): SCALEBP.f90:6-44
+    rescales the box and the positions every 10 steps by the cube root of (1 + (P - P0) PFAC).  The oracle
+    against the reference's own run of the same input: box, positions, velocities and energies.  The box
+    grows by 1.5e-3 (10 steps) and 2.6e-3 (40 steps) relative, so a missing rescaling, a wrong sign or an
+    exponent other than 1/3 fails the box by orders of magnitude more than BOX_RTOL."""
+    z, sysm = _load("hybridPA_npt")
+    assert sysm.control.ensemble == "NPT"
+    o = oracle_mod.Oracle(sysm)
+    o.run(n)
+    s, v, _ = o.state()
+    ob = o.obs()
+    box_ref = z[f"run{n}_box"]
+    dbox = np.abs(ob["box"] - box_ref).max() / box_ref.max()
+    grow = np.abs(box_ref - np.asarray(sysm.box)).max() / box_ref.max()
+    sr, vr = z[f"run{n}_s"].astype(np.float64), z[f"run{n}_v"].astype(np.float64)
+    ds = np.abs(s[:len(sr)] - sr).max()
+    dv = np.abs(v[:len(vr)] - vr).max() / np.abs(vr).max()
+    print("npt", n, "box", dbox, "growth", grow, "pos", ds, "vel", dv)
+    assert grow > 1000 * BOX_RTOL
+    assert dbox <= BOX_RTOL
+    assert ds <= POS_ATOL
+    assert dv <= VEL_RTOL
+    mine = np.concatenate([ob[k] for k in ("vnb", "vbond", "vangle", "vtors", "voop")])
+    ref = z[f"run{n}_obs"][-1]
+    assert np.all(np.abs(mine - ref[:15]) <= OBS_RTOL * np.abs(ref[:15]).sum())
+
+
+def test_npt_testsuite_numbers(oracle_mod):
+    """testsuite/tests.py (hybridPA-NPT): box edge and first three atoms (Angstrom, Angstrom/ps) after 10 and
+    40 steps.  As for the atomistic case the printed numbers do not come from the revision under test: the
+    reference run here is off them by 1.5e-5 / 5.8e-5 relative in the box, 1.9e-3 / 1.3e-2 A in coordinates
+    and 0.15 / 0.50 A/ps in velocities.  The oracle must land no further off than that (plus 5 %)."""
+    z, sysm = _load("hybridPA_npt")
+    ex = np.load(os.path.join(HERE, "hybridPA_npt_expected.npz"))
+    for n, key in ((10, "short"), (40, "long")):
+        o = oracle_mod.Oracle(sysm)
+        o.run(n)
+        s, v, _ = o.state()
+        box = o.obs()["box"]
+        sr, vr = z[f"run{n}_s"].astype(np.float64), z[f"run{n}_v"].astype(np.float64)
+        eb = ex["ref_box_" + key][:3]
+        errs = (np.abs(box * 10 - eb).max() / eb.max(), np.abs(s[:3] * 10 - ex["ref_coords_" + key]).max(),
+                np.abs(vel_from_reduced(v[:3]) * 10 - ex["ref_velos_" + key]).max())
+        refs = (np.abs(z[f"run{n}_box"] * 10 - eb).max() / eb.max(), np.abs(sr[:3] * 10 - ex["ref_coords_" + key]).max(),
+                np.abs(vel_from_reduced(vr[:3]) * 10 - ex["ref_velos_" + key]).max())
+        print(key, "oracle", errs, "reference run", refs)
+        for e, r in zip(errs, refs):
+            assert e <= 1.05 * r + 1e-6

@@ -108,6 +108,8 @@ struct ibx_ctx {
     int nterms = 0, term_nwg = 0, ncontrib = 0;
     DBuf<double> s, v, mass, invmass, vs_coeff, vs_invmass, part_bd, part_mv, fsum, obs, avg, sysstate;
     DBuf<double> vs_com;             // per virtual site (SoA like s): the mass-weighted centre of its atoms
+    DBuf<double> npt;                // [system][kNptState] barostat state (pressure ring, scale factor)
+    bool npt_on = false;             // some system is NPT: the box rescaling runs after every move
     double *part_bd_buf(int par) { return part_bd.p + (size_t)par * (part_bd.n / 2); }   // two buffers
     DBuf<float> fnb, ftot;
     DBuf<float4> nb_rows, bd_rows;
@@ -119,6 +121,7 @@ struct ibx_ctx {
     // rdf
     DBuf<unsigned long long> rdf;
     DBuf<int> rdf_samples;           // histograms accumulated per system (counted on the device)
+    DBuf<double> rdf_ivol;           // per system: the sum over the samples of 1 / volume
     int rdf_nbin = 0, rdf_every = 0;
     bool skin_tune = true;           // list-skin autotuner at every scheduled rebuild
     int respa = 1;                   // nonbonded force every respa steps, with weight respa
@@ -281,6 +284,7 @@ void set_sys_constants(const HostSystem &h, DevSys &D) {
     D.hybrid = h.hybrid;
     D.ngroup = h.ngroup;
     D.dt = h.dt; D.temp_in = h.temp_in; D.fac = h.fac; D.ensemble = h.ensemble;
+    D.pressure = h.pressure; D.pfac = h.pfac;
     D.mktemp = h.mktemp; D.mktemp_atom = h.mktemp_atom; D.mktemp_bead = h.mktemp_bead;
     D.volume = h.box[0] * h.box[1] * h.box[2];
     D.halt_drift = h.halt_drift < 0 ? -h.halt_drift : h.halt_drift;
@@ -299,6 +303,7 @@ void set_group_constants(const HostSystem &h, int g, double unit, DevGroup &dg) 
         nc[k] = (int)(kCellDiv * h.box[k] / rl);   // cells of edge >= rlist/kCellDiv (the reference: rlist)
         if (nc[k] < 1) nc[k] = 1;
         dg.nc[k] = nc[k];
+        dg.ncmax[k] = nc[k];
     }
     dg.all_pairs = (nc[0] < kMinCells || nc[1] < kMinCells || nc[2] < kMinCells) ? 1 : 0;
     dg.ncell = nc[0] * nc[1] * nc[2];
@@ -586,6 +591,10 @@ int build_device(ibx_ctx *c) {
         st[8 * s + 5] = 1.0;   // the next move advances the system
     }
     if ((rc = upload(c, c->sysstate, st))) return rc;
+    HIPCHK(c, c->npt.alloc((size_t)S * kNptState));
+    HIPCHK(c, memset_st(c, c->npt.p, 0, c->npt.bytes()));
+    c->npt_on = false;
+    for (int s = 0; s < S; ++s) c->npt_on = c->npt_on || c->hs[s].ensemble == 2;
     HIPCHK(c, c->dmax.alloc(2 * (size_t)S));
     {
         std::vector<int> inf(2 * (size_t)S, 0x7f800000);   // no bound until the first rebuild
@@ -684,7 +693,7 @@ size_t device_bytes(const ibx_ctx *c) {
                c->vs_invmass.bytes() + c->part_bd.bytes() + c->part_mv.bytes() + c->fsum.bytes() + c->obs.bytes() +
                c->avg.bytes() + c->sysstate.bytes() + c->fnb.bytes() + c->ftot.bytes() + c->nb_rows.bytes() +
                c->bd_rows.bytes() + c->bd_x.bytes() + c->nb_tabs.bytes() + c->nb_xoff.bytes() + c->bd_tabs.bytes() +
-               c->rdf.bytes() + c->rdf_samples.bytes() + c->mts_f.bytes() + c->mts_part.bytes() + c->vs_com.bytes();
+               c->rdf.bytes() + c->rdf_samples.bytes() + c->rdf_ivol.bytes() + c->mts_f.bytes() + c->mts_part.bytes() + c->vs_com.bytes() + c->npt.bytes();
     for (const GroupDev &G : c->gr)
         b += G.nneb.bytes() + G.mem.bytes() + G.mem_cell.bytes() + G.cell_count.bytes() + G.cell_start.bytes() +
              G.cell_fill.bytes() + G.tile_sum.bytes() + G.tmp.bytes() + G.perm.bytes() + G.slot_cell.bytes() + G.inv_perm.bytes() +
@@ -798,6 +807,7 @@ RebuildArgs rebuild_args(ibx_ctx *c, int g) {
     a.only = -1;
     a.rdf_hist = nullptr;
     a.rdf_samples = nullptr;
+    a.rdf_ivol = nullptr;
     a.rdf_nbin = 0;
     a.rdf_inv_dr = 0.f;
     return a;
@@ -817,6 +827,7 @@ void enqueue_rebuild(ibx_ctx *c, bool tune = false) {
         if (rdf && g == 0) {
             a.rdf_hist = c->rdf.p;
             a.rdf_samples = c->rdf_samples.p;
+            a.rdf_ivol = c->rdf_ivol.p;
             a.rdf_nbin = c->rdf_nbin;
             a.rdf_inv_dr = (float)(1.0 / c->rdf_dr);
         }
@@ -892,6 +903,7 @@ FinLaunch fin_launch(ibx_ctx *c, int mode) {
     f.snb0 = per * c->gr[0].nwg; f.snb1 = per * G1.nwg; f.sbd = c->term_nwg; f.smv = c->item_nwg;
     f.fsum = c->fsum.p; f.obs = c->obs.p; f.avg = c->avg.p; f.sysstate = c->sysstate.p;
     f.step = c->step.p; f.mode = mode; f.nsys = c->nsys;
+    f.npt = c->npt.p;
     return f;
 }
 
@@ -929,7 +941,12 @@ void enqueue_rest(ibx_ctx *c, int mode, std::vector<hipEvent_t> *ev, float nb_we
                         c->gr[1].pos4_ref.p, c->gr[1].inv_perm.p, c->dmax.p + c->nsys);
     c->fin = fin_launch(c, mode);
     c->fin_pending = true;
-    if (!defer) flush_fin(c);
+    // NPT: the field sums decide the box rescaling, which must land before the next step's forces
+    const bool npt = mode == 0 && c->npt_on;
+    if (!defer || npt) flush_fin(c);
+    if (npt)
+        launch_npt(c->st, c->item_nwg, c->sys.p, c->wg_items.p, c->nsys, c->npt.p, c->s.p, c->vs_com.p, c->itype.p,
+                   c->label.p, c->n_items, c->nb_tabs.p, c->dmax.p);
 }
 
 int check_flags(ibx_ctx *c) {
@@ -1389,6 +1406,13 @@ hipError_t put(ibx_ctx *c, DBuf<T> &b, size_t off, const std::vector<T> &h) {
 // n ints of value v from p on (the fill kernel; also zeroes floats and doubles).
 void fill_words(ibx_ctx *c, void *p, size_t nwords, int v) { launch_fill_int(c->st, (int *)p, (int)nwords, v); }
 
+// The device's record of system sys into c->dsys[sys]: an NPT system's box, and the constants that follow it,
+// change on the device.  The caller has synchronised the stream.
+int pull_sys(ibx_ctx *c, int sys) {
+    HIPCHK(c, memcpy_st(c, &c->dsys[sys], c->sys.p + sys, sizeof(DevSys), hipMemcpyDeviceToHost));
+    return 0;
+}
+
 }  // namespace
 
 // Place k of a batch taken over by system h (see ibx_assign).
@@ -1584,9 +1608,11 @@ int ibx::assign_system(ibx_ctx *c, int k, HostSystem &&h, int step0) {
     fill_words(c, G.inv_perm.p + o, np, -1);
     fill_words(c, c->obs.p + (size_t)k * kObs, 2 * (size_t)kObs, 0);
     fill_words(c, c->avg.p + (size_t)k * kObs, 2 * (size_t)kObs, 0);
+    fill_words(c, c->npt.p + (size_t)k * kNptState, 2 * (size_t)kNptState, 0);
     if (c->rdf_nbin) {
         fill_words(c, c->rdf.p + (size_t)k * c->rdf_nbin, 2 * (size_t)c->rdf_nbin, 0);
         fill_words(c, c->rdf_samples.p + k, 1, 0);
+        fill_words(c, c->rdf_ivol.p + k, 2, 0);
     }
     std::vector<double> st(8, 0.0);
     st[0] = h.temp_atom0;
@@ -1608,6 +1634,10 @@ int ibx::assign_system(ibx_ctx *c, int k, HostSystem &&h, int step0) {
         cubic_after = cubic_after && X.rho[1] == 1.0f && X.rho[2] == 1.0f;
     }
     if (cubic_before != cubic_after) drop_graph(c);
+    if (h.ensemble == 2 && !c->npt_on) {   // the steps gain the box rescaling: graphs are captured again
+        c->npt_on = true;
+        drop_graph(c);
+    }
     c->dsys[k] = D;
     c->hs[k] = std::move(h);
     // the place's first lists; the other systems' lists are untouched
@@ -1670,8 +1700,17 @@ int ibx_get_observables(ibx_ctx *c, int sys, double obs[32]) {
     HIPCHK(c, memcpy_st(c, obs, c->obs.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
     double a[kObs];
     HIPCHK(c, memcpy_st(c, a, c->avg.p + (size_t)sys * kObs, kObs * sizeof(double), hipMemcpyDeviceToHost));
+    if (int rc = pull_sys(c, sys)) return rc;
     obs[26] = c->dsys[sys].volume;
     obs[28] = a[28];
+    return 0;
+}
+
+int ibx_get_box(ibx_ctx *c, int sys, double box[3]) {
+    if (!c || sys < 0 || sys >= c->nsys || !box) return fail(c, IBX_E_INVALID, "invalid arguments");
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (int rc = pull_sys(c, sys)) return rc;
+    for (int k = 0; k < 3; ++k) box[k] = c->dsys[sys].box[k];
     return 0;
 }
 
@@ -1794,9 +1833,10 @@ int ibx_set_nb_table(ibx_ctx *c, int sys, int t, int nrow, const double *x, cons
     std::vector<float4> rows;
     std::vector<float> xoff;
     append_nb_rows(rows, nt);
+    HIPCHK(c, hipStreamSynchronize(c->st));
+    if (int rc = pull_sys(c, sys)) return rc;   // the current fixed-point unit
     const NbTab &hdr = c->h_nb_tabs[c->dsys[sys].nb_tab_off + t];
     const NbTab nh = nb_header(nt, c->dsys[sys].unit, hdr.row_off, xoff);
-    HIPCHK(c, hipStreamSynchronize(c->st));
     HIPCHK(c, memcpy_st(c, c->nb_rows.p + hdr.row_off, rows.data(), rows.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIPCHK(c, memcpy_st(c, c->nb_xoff.p + hdr.row_off, xoff.data(), xoff.size() * sizeof(float), hipMemcpyHostToDevice));
     c->h_nb_tabs[c->dsys[sys].nb_tab_off + t] = nh;
@@ -1877,6 +1917,8 @@ int ibx_rdf_enable(ibx_ctx *c, int nbin, double dr, int every) {
         HIPCHK(c, memset_st(c, c->rdf.p, 0, c->rdf.bytes()));
         HIPCHK(c, c->rdf_samples.alloc(c->nsys));
         HIPCHK(c, memset_st(c, c->rdf_samples.p, 0, c->rdf_samples.bytes()));
+        HIPCHK(c, c->rdf_ivol.alloc(c->nsys));
+        HIPCHK(c, memset_st(c, c->rdf_ivol.p, 0, c->rdf_ivol.bytes()));
     }
     drop_graph(c);
     return 0;
@@ -1888,20 +1930,21 @@ int ibx_rdf_get(ibx_ctx *c, int sys, double *g, int reset) {
     std::vector<unsigned long long> h(c->rdf_nbin);
     HIPCHK(c, memcpy_st(c, h.data(), c->rdf.p + (size_t)sys * c->rdf_nbin, h.size() * sizeof(unsigned long long),
                         hipMemcpyDeviceToHost));
-    int ns = 0;
-    HIPCHK(c, memcpy_st(c, &ns, c->rdf_samples.p + sys, sizeof(int), hipMemcpyDeviceToHost));
+    double ivol = 0.0;
+    HIPCHK(c, memcpy_st(c, &ivol, c->rdf_ivol.p + sys, sizeof(double), hipMemcpyDeviceToHost));
     const HostSystem &hs = c->hs[sys];
     const double n = (double)hs.members[0].size();
-    const double vol = hs.box[0] * hs.box[1] * hs.box[2];
-    const double samples = std::max(1, ns);
+    // samples at volumes V_i expect n (n / V_i) shell pairs each; no sample yet: an empty histogram
+    if (!(ivol > 0.0)) ivol = 1.0 / (hs.box[0] * hs.box[1] * hs.box[2]);
     for (int b = 0; b < c->rdf_nbin; ++b) {
         const double r0 = b * c->rdf_dr, r1 = (b + 1) * c->rdf_dr;
         const double shell = 4.0 / 3.0 * M_PI * (r1 * r1 * r1 - r0 * r0 * r0);
-        g[b] = (double)h[b] / (samples * n * (n / vol) * shell);
+        g[b] = (double)h[b] / (n * n * ivol * shell);
     }
     if (reset) {
         HIPCHK(c, memset_st(c, c->rdf.p + (size_t)sys * c->rdf_nbin, 0, c->rdf_nbin * sizeof(unsigned long long)));
         HIPCHK(c, memset_st(c, c->rdf_samples.p + sys, 0, sizeof(int)));
+        HIPCHK(c, memset_st(c, c->rdf_ivol.p + sys, 0, sizeof(double)));
     }
     return 0;
 }

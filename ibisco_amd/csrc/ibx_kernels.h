@@ -36,6 +36,11 @@ constexpr int kStencil = 2 * kReach + 1;
 constexpr int kListRanges = 2 * kStencil * kStencil;   // candidate slot ranges per member (columns, at most 2 each)
 constexpr int kMinCells = kStencil;       // cells per axis for the stencil (fewer: every member scans the group)
 constexpr double kInnerFrac = 1.0 / 3.0;  // delta = this fraction of the list skin rlist - rc
+// NPT (SCALEBP.f90): LIMAVP pressures averaged per box rescaling (ALLOCATEVAR.f90:17); the per-system state holds
+// that ring, then the scale factor of the step just finished and whether it rescales the box
+constexpr int kLimAvp = 10;
+constexpr int kNptState = 16;
+constexpr int kNptMiu = kLimAvp, kNptGo = kLimAvp + 1;
 
 // Position of entry k of sorted slot i in the Verlet list.  Entries of the
 // kAtomsPerWave slots of one wave are interleaved so that the kLanes lanes
@@ -53,6 +58,7 @@ struct DevGroup {
     int mem_off;                   // offset into the member list
     int cell_off, ncell;           // global cell range
     int nc[3];
+    int ncmax[3];                  // the grid the cell range was sized for (NPT: the grid never grows past it)
     int all_pairs;                 // fewer than kMinCells cells along an axis: every member scans the group
     double rl2;                    // rlist^2 (FP64 list criterion)
     float rl2u_lo, rl2u_hi;        // FP32 pre-classification thresholds, working units^2
@@ -82,7 +88,8 @@ struct DevSys {
     int ntype, nbmap_off;          // ntype*ntype map -> global table header index
     // integration
     double dt, temp_in, fac;
-    int ensemble;
+    int ensemble;                  // 0 NVE, 1 NVT, 2 NPT
+    double pressure, pfac;         // NPT: target pressure and BETA*DT*LIMAVP/TAUP, reduced units
     double mktemp, mktemp_atom, mktemp_bead;
     double volume;
     int halt_drift, nmts, mts_quirk, nupdate;

@@ -386,7 +386,8 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
                                                        int cap, int *__restrict__ list, int2 *__restrict__ nneb,
                                                        int *__restrict__ flags, int nblk, int only,
                                                        unsigned long long *__restrict__ rdf_hist,
-                                                       int *__restrict__ rdf_samples, int nbin, float inv_dr) {
+                                                       int *__restrict__ rdf_samples, double *__restrict__ rdf_ivol,
+                                                       int nbin, float inv_dr) {
     // range r of a member: {candidates up to its end, slot of candidate p in it minus p} (one LDS read)
     __shared__ int2 rng[kListAtoms][kListRanges];
     extern __shared__ unsigned int rdf_h[];   // nbin counts of this workgroup's listed pairs (RDF on)
@@ -401,7 +402,10 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
     constexpr bool rdf = kRdf;   // the RDF of the listed pairs (kRdf: rdf_hist holds the histograms)
     if (rdf) {
         for (int b = threadIdx.x; b < nbin; b += kBlock) rdf_h[b] = 0;
-        if (threadIdx.x == 0 && slot0 == G.slot_off) atomicAdd(&rdf_samples[s_id], 1);   // one sample a build
+        if (threadIdx.x == 0 && slot0 == G.slot_off) {   // one sample a build, and its 1/volume (NPT: the box varies)
+            atomicAdd(&rdf_samples[s_id], 1);
+            atomicAdd(&rdf_ivol[s_id], 1.0 / S.volume);
+        }
     }
     const float inv_dr_u = (float)S.unit * inv_dr;   // bins in this system's working units
     const int sub = threadIdx.x & (kListLanes - 1), grp = threadIdx.x / kListLanes;
@@ -1378,7 +1382,8 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
         } else {
             const int pop = (lab == 2) ? 1 : 0;
             double lam = 1.0;
-            if (ensemble == 1) lam = sqrt(1.0 + fac * ((temp_in / (pop ? st[1] : st[0])) - 1.0));
+            if (ensemble == 1 || ensemble == 2)   // NVT or NPT (MOVE.F90:32)
+                lam = sqrt(1.0 + fac * ((temp_in / (pop ? st[1] : st[0])) - 1.0));
             double vt[3];
             for (int k = 0; k < 3; ++k) {
                 const double v0 = vin[k] - st[2 + k] * im;   // deferred MOMENTUM
@@ -1467,6 +1472,85 @@ __global__ void __launch_bounds__(kBlock) k_vs_place(const DevSys *__restrict__ 
     block_max_to(disp, dmax + s_id, mx);
 }
 
+// SCALEBP.f90:36-43 on the device, after the field sums of a step: where the system rescales its box this step
+// (the field sums set kNptGo and the factor MIU), every item's position is multiplied by MIU (the reference
+// scales the atoms; its virtual sites follow at the next VIRTUAL_DEF, which here ran already, so they are
+// scaled with them, as are the centres of their atoms).  The fixed-point copies keep their values: a position's
+// fraction of the box does not change.  One thread per item.
+__global__ void __launch_bounds__(kBlock) k_npt_positions(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
+                                                          const double *__restrict__ npt, double *__restrict__ s,
+                                                          double *__restrict__ vs_com, const int *__restrict__ itype,
+                                                          const int *__restrict__ label, int nitems_total) {
+    const int w = wg_sys[blockIdx.x];
+    if (w & kWgRetired) return;
+    const DevSys &S = sys[w];
+    if (S.ensemble != 2 || !alive_at(S, *S.step_ctr)) return;   // after the move: the counter holds this step
+    const double *q = npt + (size_t)w * kNptState;
+    if (q[kNptGo] == 0.0) return;
+    const double miu = q[kNptMiu];
+    const int it = blockIdx.x * kBlock + threadIdx.x, n = nitems_total;
+    if (itype[it] < 0) return;
+    for (int k = 0; k < 3; ++k) s[k * n + it] *= miu;
+    if (label[it] == 3)
+        for (int k = 0; k < 3; ++k) vs_com[k * n + it] *= miu;
+}
+
+// The system's constants after its box is rescaled by MIU (SCALEBP.f90:38-40 and the cell grid of
+// UPDATE_NEIGHBOURLIST.F90:14-48): box, its inverse and half, the fixed-point unit and every radius
+// expressed in it, the nonbonded tables' abscissa scale, and the cell grid, which keeps cells of at least
+// rlist / kCellDiv (it shrinks with the box and grows back to the grid the cell range holds).  Pair distances
+// change by MIU while the fixed-point displacements do not, so the back run of each list is scanned until the
+// next rebuild.  One thread per system, after k_npt_positions (which reads kNptGo; cleared here).
+__global__ void __launch_bounds__(kBlock) k_npt_box(DevSys *__restrict__ sys, int nsys, double *__restrict__ npt,
+                                                    NbTab *__restrict__ nb_tabs, int *__restrict__ dmax) {
+    const int s_id = blockIdx.x * kBlock + threadIdx.x;
+    if (s_id >= nsys) return;
+    DevSys &S = sys[s_id];
+    if (S.ensemble != 2 || !alive_at(S, *S.step_ctr)) return;
+    double *q = npt + (size_t)s_id * kNptState;
+    if (q[kNptGo] == 0.0) return;
+    q[kNptGo] = 0.0;
+    const double miu = q[kNptMiu];
+    for (int k = 0; k < 3; ++k) {
+        S.box[k] *= miu;
+        S.boxinv[k] = 1.0 / S.box[k];
+        S.box2[k] = S.box[k] / 2.0;
+        S.inv_box_frac[k] = 1.0 / S.box[k];
+    }
+    S.volume = S.box[0] * S.box[1] * S.box[2];
+    const double u_old = S.unit;
+    S.unit = S.box[0] / 4294967296.0;
+    const double r = u_old / S.unit;   // a length in the new units is r times its value in the old ones
+    const float r2 = (float)(r * r);
+    for (int g = 0; g < S.ngroup; ++g) {
+        DevGroup &G = S.g[g];
+        const double rl = sqrt(G.rl2), rc = sqrt(G.rc2);
+        for (int k = 0; k < 3; ++k) {
+            int nc = (int)(kCellDiv * S.box[k] / rl);
+            nc = nc < 1 ? 1 : (nc > G.ncmax[k] ? G.ncmax[k] : nc);
+            G.nc[k] = nc;
+        }
+        G.ncell = G.nc[0] * G.nc[1] * G.nc[2];
+        G.all_pairs = (G.nc[0] < kMinCells || G.nc[1] < kMinCells || G.nc[2] < kMinCells) ? 1 : 0;
+        const double u = S.unit;
+        const double rl2u = (rl / u) * (rl / u);
+        const double lband = 4e-6 + 16.0 * u / rl;
+        G.rl2u_lo = (float)(rl2u * (1.0 - lband));
+        G.rl2u_hi = (float)(rl2u * (1.0 + lband));
+        const double band = 1e-6 + 16.0 * u / rc;
+        G.rc2u = (float)((rc / u) * (rc / u));
+        G.rc2u_lo = (float)((rc / u) * (rc / u) * (1.0 - band));
+        G.rc2u_hi = (float)((rc / u) * (rc / u) * (1.0 + band));
+        G.rin2u *= r2;     // the same distances rc + delta and 0.49 delta, in the new units
+        G.dmax2u *= r2;
+        dmax[g * nsys + s_id] = 0x7f800000;   // +inf: scan the back run until the next rebuild
+    }
+    for (int t = 0; t < S.nb_ntab; ++t) {
+        NbTab &T = nb_tabs[S.nb_tab_off + t];
+        T.c *= S.unit / u_old;
+    }
+}
+
 // Sum of p[w] for w = a + t, a + t + stride, ... < b, eight loads in flight;
 // the order of the additions is fixed.
 __device__ __forceinline__ double strided_sum(const double *__restrict__ p, int a, int b, int t, int stride) {
@@ -1487,7 +1571,7 @@ __device__ __forceinline__ double strided_sum(const double *__restrict__ p, int 
 // correction when the step is a HALT_DRIFT step.
 __device__ void finalize_system(const DevSys &S, int s_id, const double *red, double *__restrict__ obs,
                                 double *__restrict__ avg, double *__restrict__ sysstate, const int *__restrict__ step,
-                                int mode) {
+                                int mode, double *__restrict__ npt) {
     // the field sums: 0..8 nonbonded, 9..21 bonded energies and overextension, 22..27 bonded
     // virial, 28..39 kinetic tensor of each population, 40..42 momentum
     double acc[kFields];
@@ -1516,6 +1600,22 @@ __device__ void finalize_system(const DevSys &S, int s_id, const double *red, do
     st[0] = o[24];
     st[1] = o[25];
     const int cur = *step - S.step0;   // this system's own step
+    if (S.ensemble == 2) {
+        // SCALEBP.f90:18-34: this step's pressure into the ring; every LIMAVP-th step the ring's mean sets the
+        // box scale MIU, applied after these sums (k_npt_positions, k_npt_box).  The volume the step reports
+        // (and AVERAGE's window sums, taken after SCALEBP) is the rescaled one.
+        double *q = npt + (size_t)s_id * kNptState;
+        q[((cur - 1) % kLimAvp + kLimAvp) % kLimAvp] = (o[15] + o[16] + o[17]) / 3.0;
+        double miu = 1.0;
+        if (cur % kLimAvp == 0) {
+            double avp = 0.0;
+            for (int k = 0; k < kLimAvp; ++k) avp += q[k];
+            miu = cbrt(1.0 + (avp / kLimAvp - S.pressure) * S.pfac);
+        }
+        q[kNptMiu] = miu;
+        q[kNptGo] = cur % kLimAvp == 0 ? 1.0 : 0.0;
+        o[26] = S.volume * miu * miu * miu;
+    }
     if (cur % S.halt_drift == 0) {
         for (int k = 0; k < 3; ++k) st[2 + k] = mom[k] / (double)S.natoms;
     } else {
@@ -1538,9 +1638,9 @@ __device__ void finalize_system(const DevSys &S, int s_id, const double *red, do
     const DevSys *__restrict__ sys, const double *__restrict__ part_nb0, const double *__restrict__ part_nb1,      \
         const double *__restrict__ part_bd, const double *__restrict__ part_mv, int snb0, int snb1, int sbd,      \
         int smv, double *__restrict__ fsum, double *__restrict__ obs, double *__restrict__ avg,                   \
-        double *__restrict__ sysstate, const int *__restrict__ step, int mode, int nsys
+        double *__restrict__ sysstate, const int *__restrict__ step, int mode, int nsys, double *__restrict__ npt
 #define IBX_FIN_ARGS sys, part_nb0, part_nb1, part_bd, part_mv, snb0, snb1, sbd, smv, fsum, obs, avg, sysstate, step, \
-                     mode, nsys
+                     mode, nsys, npt
 
 // Workgroup b of the field sums (also one role of k_step_forces).
 __device__ __forceinline__ void field_sums_block(const int b, IBX_FIN_PARAMS) {
@@ -1570,7 +1670,7 @@ __device__ __forceinline__ void field_sums_block(const int b, IBX_FIN_PARAMS) {
         for (int k = 0; k < kFields; ++k) fsum[(size_t)s_id * kFields + k] = red[k];
         // the step summed is the counter's (the moves of its step are done, the next step's are not)
         const int t = *step;
-        if (mode == 1 ? S.live : alive_at(S, t)) finalize_system(S, s_id, red, obs, avg, sysstate, step, mode);
+        if (mode == 1 ? S.live : alive_at(S, t)) finalize_system(S, s_id, red, obs, avg, sysstate, step, mode, npt);
         if (mode == 0 && S.live) sysstate[(size_t)s_id * 8 + 5] = alive_at(S, t + 1) ? 1.0 : 0.0;   // next move
     }
 }
@@ -1622,7 +1722,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     int b = (int)blockIdx.x;
     if (b < fin_blocks) {
         field_sums_block(b, fk.sys, fk.part_nb0, fk.part_nb1, fk.part_bd, fk.part_mv, fk.snb0, fk.snb1, fk.sbd, fk.smv,
-                         fk.fsum, fk.obs, fk.avg, fk.sysstate, fk.step, fk.mode, fk.nsys);
+                         fk.fsum, fk.obs, fk.avg, fk.sysstate, fk.step, fk.mode, fk.nsys, fk.npt);
         return;
     }
     b -= fin_blocks;
@@ -1771,7 +1871,7 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
     auto go = [&](auto kernel) {
         hipLaunchKernelGGL(kernel, dim3(xcd_grid(nwg_list)), dim3(kBlock), rdf_lds, st, a.sys, a.wg_sys, a.g, a.perm,
                            a.inv_perm, a.pos4, a.r64, a.cell_start, a.excl_off, a.excl, a.cap, a.list, a.nneb, a.flags,
-                           nwg_list, a.only, a.rdf_hist, a.rdf_samples, a.rdf_nbin, a.rdf_inv_dr);
+                           nwg_list, a.only, a.rdf_hist, a.rdf_samples, a.rdf_ivol, a.rdf_nbin, a.rdf_inv_dr);
     };
     if (a.rdf_hist) a.cubic ? go(k_build_list<true, true>) : go(k_build_list<false, true>);
     else a.cubic ? go(k_build_list<true, false>) : go(k_build_list<false, false>);
@@ -1837,10 +1937,18 @@ void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_s
                        vs_moff, vs_members, mass, vs_invmass, vs_com, pos4, pos4_ref, inv_perm, dmax);
 }
 
+void launch_npt(hipStream_t st, int nwg, DevSys *sys, const int *wg_sys, int nsys, double *npt, double *s,
+                double *vs_com, const int *itype, const int *label, int nitems_total, NbTab *nb_tabs, int *dmax) {
+    hipLaunchKernelGGL(k_npt_positions, dim3(nwg), dim3(kBlock), 0, st, sys, wg_sys, npt, s, vs_com, itype, label,
+                       nitems_total);
+    hipLaunchKernelGGL(k_npt_box, dim3((nsys + kBlock - 1) / kBlock), dim3(kBlock), 0, st, sys, nsys, npt, nb_tabs,
+                       dmax);
+}
+
 void launch_finalize(hipStream_t st, const FinLaunch &f) {
     hipLaunchKernelGGL(k_field_sums, dim3(f.nsys), dim3(kBlock), 0, st, f.sys, f.part_nb0, f.part_nb1,
                        f.part_bd, f.part_mv, f.snb0, f.snb1, f.sbd, f.smv, f.fsum, f.obs, f.avg, f.sysstate,
-                       f.step, f.mode, f.nsys);
+                       f.step, f.mode, f.nsys, f.npt);
 }
 
 template <bool M, bool L, bool K>

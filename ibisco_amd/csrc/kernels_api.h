@@ -35,6 +35,7 @@ struct RebuildArgs {
     // RDF of the listed group-0 pairs, accumulated by the list build (null: none)
     unsigned long long *rdf_hist;
     int *rdf_samples;
+    double *rdf_ivol;   // per system: the sum over samples of 1 / volume
     int rdf_nbin;
     float rdf_inv_dr;   // 1 / bin width (nm^-1)
 };
@@ -102,6 +103,7 @@ struct FinLaunch {
     double *obs, *avg, *sysstate;
     const int *step;
     int mode, nsys;
+    double *npt;        // [system][kNptState] barostat state
 };
 
 void launch_mts(hipStream_t st, bool save, float *fnb, const int *label, int n, double *part, int npart,
@@ -116,6 +118,9 @@ void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_s
                      const double *mass, const double *vs_invmass, double *vs_com, int4 *pos4, const int4 *pos4_ref,
                      const int *inv_perm, int *dmax);
 void launch_finalize(hipStream_t st, const FinLaunch &f);
+// The NPT box rescaling after the field sums of a step (k_npt_positions over the item blocks, then k_npt_box).
+void launch_npt(hipStream_t st, int nwg, DevSys *sys, const int *wg_sys, int nsys, double *npt, double *s,
+                double *vs_com, const int *itype, const int *label, int nitems_total, NbTab *nb_tabs, int *dmax);
 // The force phase of a step in one launch: the group-0 nonbonded force (nb null: none this step), the bonded
 // terms, and the field sums of the previous step (fin null: none).
 void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const int *wg_terms, const TermLaunch &b,

@@ -152,9 +152,10 @@ int ibx_write_trz(ibx_ctx *ctx, int sys, const char *path, const char *title, in
     const ibx::HostSystem *h = ibx::host_system(ctx, sys);
     if (!h) return oerr(IBX_E_INVALID, "invalid context or system index");
     std::vector<double> s(3 * (size_t)h->nitems), v(3 * (size_t)h->natoms);
-    double obs[32];
+    double obs[32], box[3];
     int rc = ibx_get_state(ctx, sys, s.data(), v.data());
     if (!rc) rc = ibx_get_observables(ctx, sys, obs);
+    if (!rc) rc = ibx_get_box(ctx, sys, box);   // NPT: the box of the last step
     if (rc) return oerr(rc, ibx_last_error(ctx));
     // obs: 0..14 V_NB, V_BOND, V_ANGLE, V_TORSION, V_OOP (3 classes each), 15..20 the pressure tensor, 21..22
     // the kinetic energy of each population
@@ -162,7 +163,7 @@ int ibx_write_trz(ibx_ctx *ctx, int sys, const char *path, const char *title, in
     for (int k = 0; k < 15; ++k) epot += obs[k];
     const double pt[6] = {obs[15], obs[16], obs[17], obs[18], obs[19], obs[20]};
     const double time_ps = (double)step * h->dt * kTimeScale * 1.0e12 + initime_ps;   // WRITETRJ.F90:57
-    return ibx_trz_write_frame(path, title, frame, ntrj, h->natoms, time_ps, h->box, pt, epot, obs[21] + obs[22],
+    return ibx_trz_write_frame(path, title, frame, ntrj, h->natoms, time_ps, box, pt, epot, obs[21] + obs[22],
                                h->mktemp, s.data(), v.data());
 }
 
@@ -171,10 +172,12 @@ int ibx_write_restart(ibx_ctx *ctx, int sys, const char *path, const char *title
     const ibx::HostSystem *h = ibx::host_system(ctx, sys);
     if (!h) return oerr(IBX_E_INVALID, "invalid context or system index");
     std::vector<double> s(3 * (size_t)h->nitems), v(3 * (size_t)h->natoms);
+    double box[3];
     int rc = ibx_get_state(ctx, sys, s.data(), v.data());
+    if (!rc) rc = ibx_get_box(ctx, sys, box);
     if (rc) return oerr(rc, ibx_last_error(ctx));
     const double time_ps = (double)step * h->dt * kTimeScale * 1.0e12 + initime_ps;
-    return ibx_restart_write(path, title, time_ps, h->box, h->natoms, h->itype.data(), h->nbonds.data(),
+    return ibx_restart_write(path, title, time_ps, box, h->natoms, h->itype.data(), h->nbonds.data(),
                              h->jbond.data(), s.data(), v.data(), nmol, natm, names);
 }
 

@@ -349,7 +349,6 @@ int build_host_system(const ibx_system_desc *d, HostSystem &hs, std::string &err
     hs.pressure = d->pressure;
     hs.pfac = d->pfac;
     if (hs.ensemble == 2 && !(hs.pfac >= 0.0)) { err = "NPT needs a non-negative BETA*DT*LIMAVP/TAUP"; return IBX_E_INVALID; }
-    if (hs.ensemble == 2) { err = "NPT is not supported on the device path yet"; return IBX_E_INVALID; }
 
     // initial temperatures (main.F90:173-195) drive the first Berendsen factor
     hs.mktemp = 2.0 / (3.0 * (na - 1));

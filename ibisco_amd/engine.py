@@ -60,7 +60,7 @@ EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_s
            "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes",
            "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa", "ibx_get_counters", "ibx_step_noalloc",
            "ibx_assign", "ibx_get_capacity", "ibx_server_timing", "ibx_set_last_step", "ibx_trz_write_frame",
-           "ibx_restart_write", "ibx_write_trz", "ibx_write_restart", "ibx_output_last_error")
+           "ibx_restart_write", "ibx_write_trz", "ibx_write_restart", "ibx_output_last_error", "ibx_get_box")
 
 _lib = None
 
@@ -85,6 +85,7 @@ def load_library(path=LIB_PATH):
     L.ibx_compute_forces.argtypes = [vp, C.c_int]
     L.ibx_rebuild_lists.argtypes = [vp]
     L.ibx_get_observables.argtypes = [vp, C.c_int, _dp]
+    L.ibx_get_box.argtypes = [vp, C.c_int, _dp]
     L.ibx_get_averages.argtypes = [vp, C.c_int, _dp, C.c_int]
     L.ibx_get_forces.argtypes = [vp, C.c_int, _dp]
     L.ibx_get_state.argtypes = [vp, C.c_int, _dp, _dp]
@@ -247,6 +248,12 @@ class Engine:
         o = np.zeros(32)
         self._chk(self.L.ibx_get_observables(self.h, sys, _p(o, C.c_double)))
         return obs_dict(o)
+
+    def box(self, sys=0):
+        """The current box edges (nm); an NPT system's box follows its barostat."""
+        b = np.zeros(3)
+        self._chk(self.L.ibx_get_box(self.h, sys, _p(b, C.c_double)))
+        return b
 
     def averages(self, sys=0, reset=False):
         o = np.zeros(32)

@@ -143,6 +143,10 @@ int ibx_rebuild_lists(ibx_ctx *ctx);
  *   25 TEMP_BEAD, 26 volume, 27 bond over-extension count,
  *   28 steps accumulated in the window sums. */
 int ibx_get_observables(ibx_ctx *ctx, int sys, double obs[32]);
+/* The current box edges of system sys (nm): an NPT system's box is rescaled on
+ * the device every LIMAVP = 10 steps (SCALEBP.f90:36-40; the reference prints
+ * it with its averages, AVERAGE.f90:23-26).  obs[26] is its volume. */
+int ibx_get_box(ibx_ctx *ctx, int sys, double box[3]);
 /* Window sums of obs[0..25] over the steps since the last reset (AVERAGE.f90). */
 int ibx_get_averages(ibx_ctx *ctx, int sys, double sums[32], int reset);
 

@@ -60,3 +60,21 @@ def test_missing_table_file(tmp_path):
                 os.remove(os.path.join(d, f))
     e = _load(_dir_with(tmp_path, edit))
     assert e.code == IBX_E_IO and "does not exist" in str(e)
+
+
+@pytest.mark.parametrize("key", ["pressure", "pressure_coupling_time", "isothermal_compressibility"])
+def test_npt_needs_barostat_keywords(tmp_path, key):
+    """RDCONTROL.f90:307-345: NPT requires the barostat's keywords (NVE/NVT do not)."""
+    def edit(d):
+        p = os.path.join(d, "control")
+        lines = []
+        for ln in open(p):
+            tok = ln.split()
+            if tok and tok[0] == "ensemble":
+                ln = "ensemble NPT\n"
+            if tok and tok[0] == key:
+                continue
+            lines.append(ln)
+        open(p, "w").writelines(lines)
+    e = _load(_dir_with(tmp_path, edit))
+    assert e.code == IBX_E_INVALID and f"missing control keyword {key}" in str(e)

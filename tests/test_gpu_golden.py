@@ -9,8 +9,8 @@ and virtangles files, and a torsion melt with exactly collinear atoms (the
 0.99 perturbations of TORSIONS.inc:47-59 and :75-87).  The oracle is pinned to
 the reference's outputs on these inputs in test_golden.py; here the device path
 must match the oracle to the same gates as test_gpu_parity.py.  Forces do not
-depend on the ensemble, so the NPT input (hybridPA, which the device path does
-not integrate) is checked for forces with its ensemble set to NVE.
+depend on the ensemble, so the NPT input (hybridPA) is checked for forces with
+its ensemble set to NVE; its NPT runs are in test_gpu_npt.py.
 """
 import os
 

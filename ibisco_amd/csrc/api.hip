@@ -763,13 +763,13 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, const SlotSizes &res, int
                + nterm_pad * (4 + 16 + 16) + (nterm_pad / kBlock) * 4   // meta, members, slots, wg_terms
                + at_least1(members) * 16                           // contributions
                + 2 * (nterm_pad / kBlock) * kBdPart * 8 + (N / kBlock) * kMvPart * 8
-               + (size_t)(kFields + 2 * kObs + 8) * 8 + 2 * 4 + 8 * 4 + 4   // fsum obs avg sysstate dmax flags
+               + (size_t)(kFields + 2 * kObs + 8 + kNptState) * 8 + 2 * 4 + 8 * 4 + 4   // fsum obs avg sysstate npt dmax flags
                                                                        // step
                + at_least1(bd_rows) * (16 + 8) + at_least1(cap(need.bd_tabs, res.bd_tabs)) * sizeof(BdTab)
                + at_least1(nb_rows) * (16 + 4) + at_least1(cap(need.nb_tabs, res.nb_tabs)) * sizeof(NbTab) +
                at_least1(cap(need.nb_map, res.nb_map)) * 4
                + (h.nvs > 0 ? 24 * N : 8);                          // vs_com
-    if (rdf_nbin > 0) b += (size_t)rdf_nbin * 8 + 4;
+    if (rdf_nbin > 0) b += (size_t)rdf_nbin * 8 + 4 + 8;   // histogram, samples, sum of 1/volume
     if (h.hybrid && h.nmts > 0 && h.ngroup > 1 && !h.members[1].empty())   // MTS slots
         b += 9 * N * 4 + 27 * (size_t)(pad_to((int)h.members[1].size(), kBlock) / kBlock) * (kBlock / kNbAtomsPerBlock) * 8;
     for (int g = 0; g < 2; ++g) {

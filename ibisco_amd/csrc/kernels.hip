@@ -364,15 +364,15 @@ __device__ __forceinline__ float cell_gap(int d, float f, float a) {
 // bit: a fixed-point test settles pairs far from the boundary and the FP64
 // criterion of the reference decides the rest.
 //
-// Cells have an edge of at least rlist/2 and are stored x-major, so the cells
-// of one (x, y) column are consecutive slots.  For member i the columns within
-// two cells whose distance from i is below rlist give at most two slot ranges
-// each (the z extent reachable from i, split where it wraps), and kListLanes
-// lanes walk their concatenation, 16 candidates per step.  Hits are compacted
-// with a wave ballot, so each list is in candidate order and deterministic.
-// Pairs closer than rc + delta when the list is built fill the front run, the
-// others the back run.  With fewer than five cells along an axis every member scans the
-// whole group once.
+// Cells have an edge of at least rlist/kCellDiv and are stored x-major, so the
+// cells of one (x, y) column are consecutive slots.  For member i the columns
+// within kReach cells whose distance from i is below rlist give at most two slot
+// ranges each (the z extent reachable from i, split where it wraps), and
+// kListLanes lanes walk their concatenation, 16 candidates per step.  Hits are
+// compacted with a wave ballot, so each list is in candidate order and
+// deterministic.  Pairs closer than rc + delta when the list is built fill the
+// front run, the others the back run.  With fewer than kMinCells cells along an
+// axis every member scans the whole group once.
 template <bool kCubic, bool kRdf>
 // Held to 64 VGPRs (eight waves per SIMD, measured faster than seven: the candidate loop waits on one position
 // gather per round, and more waves hide more of it).

@@ -55,7 +55,7 @@ def test_footprint_estimate_without_device():
     from ibisco_amd import system
     s = system.cg_melt(nchains=20, nbeads=50, seed=4, avoid=0.8, dt=0.002)
     b0, b1 = engine.estimate_bytes(s), engine.estimate_bytes(s, 150)
-    assert b1 - b0 == 150 * 8 + 4
+    assert b1 - b0 == 150 * 8 + 4 + 8   # the histogram, its sample count and its sum of 1/volume
     import math
     nbr = 4.0 / 3.0 * math.pi * 1.8 ** 3 * 8.0
     cap = (int(1.5 * nbr) + 64 + 7) // 8 * 8

@@ -170,6 +170,16 @@ def sample_indices(systems, n):
     return [order[round(q * (len(order) - 1) / (n - 1))] for q in range(n)]
 
 
+def tile_record(e):
+    """The force launch layout the tile autotuner chose for a context, and what it measured."""
+    t, ms = e.tiles()
+    names = ["one workgroup per block", "persistent, 5 workgroups per CU, 1 block a take",
+             "persistent, 5 per CU, 2 blocks a take", "persistent, 4 per CU, 1 block a take"]
+    chosen = names[0] if t[0] == 0 else f"persistent, {t[2]} workgroups per CU, {t[1]} block(s) a take"
+    return {"force_launch": chosen, "tiles": t,
+            "tuner_ms_per_force_launch": {n: round(m, 4) for n, m in zip(names, ms)}}
+
+
 def batched_leg(args, seed=3, with_plain=True):
     """Config 3: the replicas in one context (every kernel launch covers all of them), then each alone."""
     import copy
@@ -186,6 +196,7 @@ def batched_leg(args, seed=3, with_plain=True):
     print("[bench] replicas equilibrated", file=sys.stderr, flush=True)
     eb.rdf_enable(150, 0.01, every=1)
     secs, tc = steady_run(eb, args.steps, args.warmup, graph=not args.eager)
+    tiles = tile_record(eb)
     value = total * args.steps / secs
     print(f"[bench] batched: {args.steps} steps in {secs:.4f} s ({value:.3e} atom-steps/s), timed counters {tc}",
           file=sys.stderr, flush=True)
@@ -222,7 +233,7 @@ def batched_leg(args, seed=3, with_plain=True):
         print(f"[bench] replicas: batched {value:.3e}, plain {vp:.3e} atom-steps/s", file=sys.stderr, flush=True)
     return dict(systems=systems, states=states, total=total, secs=secs, value=value, timed_counters=tc, tm=tm,
                 nb_ms=nb_ms, prl=prl, accuracy=acc, plain_value=vp, plain_secs=tp, plain_counters=plain_tc,
-                check=idx)
+                check=idx, tiles=tiles)
 
 
 def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=3, nsample=12, ncheck=4, step_band=1.5,
@@ -453,6 +464,7 @@ def ua_leg(args):
     e = Engine(sysm)
     workloads.equilibrate(e, sysm, stage_steps=200, final_steps=1000)
     secs, tc = steady_run(e, args.ua_steps, args.warmup, graph=not args.eager)
+    tiles = tile_record(e)
     e.set_respa(4)
     nr = max(80, args.ua_steps // 40 * 40)
     tr, _ = steady_run(e, nr, 40, graph=not args.eager)
@@ -469,7 +481,7 @@ def ua_leg(args):
     print(f"[bench] UA melt: {n * args.ua_steps / secs:.3e} atom-steps/s, NB {nb_ms * 1e3:.1f} us", file=sys.stderr,
           flush=True)
     return {"workload": "ua_melt_100k", "sites": n, "steps": args.ua_steps, "value": n * args.ua_steps / secs,
-            "ms_per_step": 1e3 * secs / args.ua_steps, "timed_counters": tc,
+            "ms_per_step": 1e3 * secs / args.ua_steps, "timed_counters": tc, "tile_autotuner": tiles,
             "step_breakdown_ms": {k: round(tm[k] / tm["nsteps_timed"], 5) for k in
                                   ("nonbonded", "bonded", "integrate", "rebuild", "step")},
             "nonbonded_roofline": {"achieved_GBs": round(ach, 1), "frac": round(ach / HBM_PEAK_GBS, 4),
@@ -575,6 +587,7 @@ def main():
                                                       "HIP graph replays captured in the warm-up"),
                    "precision": "FP32 forces, FP64 integration and reductions"},
         "timed_counters": B["timed_counters"],
+        "tile_autotuner": B["tiles"],
         "step_breakdown_ms": {k: round(tm[k] / tm["nsteps_timed"], 5) for k in
                               ("nonbonded", "bonded", "integrate", "rebuild", "step")},
         "accuracy": B["accuracy"],

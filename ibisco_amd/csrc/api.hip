@@ -124,6 +124,14 @@ struct ibx_ctx {
     DBuf<double> rdf_ivol;           // per system: the sum over the samples of 1 / volume
     int rdf_nbin = 0, rdf_every = 0;
     bool skin_tune = true;           // list-skin autotuner at every scheduled rebuild
+    // force launch tiles (ibx_set_tiles): requested (-1: the tile autotuner's choice) and in use, the work
+    // queues of the persistent launch, the CUs of the device, and the tuner's timings per candidate (ms)
+    int tiles_req[IBX_NTILES] = {-1, -1, -1};
+    ForceTiles ftiles;
+    bool tiles_pending = true;
+    DBuf<int> fqueue;
+    int ncu = 0;
+    float tile_ms[kTileCandidates] = {-1.f, -1.f, -1.f, -1.f};
     int respa = 1;                   // nonbonded force every respa steps, with weight respa
     bool mts = false;                // MTS cycle of a hybrid system (NMTS > 0)
     DBuf<float> mts_f;               // [3 slots][xyz][items] bead-group forces of the explicit steps
@@ -602,6 +610,9 @@ int build_device(ibx_ctx *c) {
     }
     HIPCHK(c, c->flags.alloc(8));
     HIPCHK(c, memset_st(c, c->flags.p, 0, c->flags.bytes()));
+    HIPCHK(c, c->fqueue.alloc(kQueueInts));
+    HIPCHK(c, memset_st(c, c->fqueue.p, 0, c->fqueue.bytes()));
+    c->ftiles.queue = c->fqueue.p;
     HIPCHK(c, c->step.alloc(1));
     HIPCHK(c, memset_st(c, c->step.p, 0, c->step.bytes()));
     for (DevSys &D : c->dsys) D.step_ctr = c->step.p;
@@ -687,7 +698,7 @@ int build_device(ibx_ctx *c) {
 size_t device_bytes(const ibx_ctx *c) {
     size_t b = c->sys.bytes() + c->wg_items.bytes() + c->itype.bytes() + c->label.bytes() + c->excl_off.bytes() +
                c->excl.bytes() + c->vs_of.bytes() + c->vs_center.bytes() + c->vs_moff.bytes() + c->vs_members.bytes() +
-               c->term_meta.bytes() + c->inc_off.bytes() + c->wg_terms.bytes() + c->nb_map.bytes() + c->flags.bytes() +
+               c->term_meta.bytes() + c->inc_off.bytes() + c->wg_terms.bytes() + c->nb_map.bytes() + c->flags.bytes() + c->fqueue.bytes() +
                c->step.bytes() + c->dmax.bytes() + c->term_slot.bytes() + c->term_mem.bytes() + c->contrib.bytes() +
                c->s.bytes() + c->v.bytes() + c->mass.bytes() + c->invmass.bytes() + c->vs_coeff.bytes() +
                c->vs_invmass.bytes() + c->part_bd.bytes() + c->part_mv.bytes() + c->fsum.bytes() + c->obs.bytes() +
@@ -765,6 +776,7 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, const SlotSizes &res, int
                + 2 * (nterm_pad / kBlock) * kBdPart * 8 + (N / kBlock) * kMvPart * 8
                + (size_t)(kFields + 2 * kObs + 8 + kNptState) * 8 + 2 * 4 + 8 * 4 + 4   // fsum obs avg sysstate npt dmax flags
                                                                        // step
+               + kQueueInts * 4                                       // work queues of the persistent force launch
                + at_least1(bd_rows) * (16 + 8) + at_least1(cap(need.bd_tabs, res.bd_tabs)) * sizeof(BdTab)
                + at_least1(nb_rows) * (16 + 4) + at_least1(cap(need.nb_tabs, res.nb_tabs)) * sizeof(NbTab) +
                at_least1(cap(need.nb_map, res.nb_map)) * 4
@@ -1014,9 +1026,71 @@ void enqueue_step(ibx_ctx *c, int step, std::vector<hipEvent_t> *ev) {
     c->fin_pending = false;
     launch_step_forces(c->st, g0 ? &a0 : nullptr, c->term_nwg, c->wg_terms.p, term_launch(c), c->s.p, c->n_items,
                        c->nterms, c->contrib.p, c->ncontrib, c->part_bd_buf(c->bd_par), c->flags.p,
-                       fin ? &f : nullptr);
+                       fin ? &f : nullptr, c->ftiles);
     if (nb_step) enqueue_nb_groups(c, 0, step, 1);
     enqueue_rest(c, 0, nullptr, w, true);
+}
+
+// The force launch layouts the tile autotuner times: one workgroup per block, and persistent grids of
+// 5 or 4 workgroups per CU taking one or two blocks at a time.
+ForceTiles tile_candidate(const ibx_ctx *c, int k) {
+    ForceTiles t;
+    t.queue = c->fqueue.p;
+    if (k == 0) return t;
+    t.persistent = true;
+    t.chunk = k == 2 ? 2 : 1;
+    t.grid = c->ncu * (k == 3 ? 4 : 5);
+    return t;
+}
+
+int ensure_events(ibx_ctx *c, size_t n);
+
+// Tile autotuner (no counterpart in the reference).  Times each candidate
+// layout of the force launch over a few launches of its force roles alone
+// (nonbonded and bonded, no field sums) at the current positions and keeps the
+// fastest.  Every layout computes the same bits, and these launches only
+// overwrite what the next force evaluation overwrites again: the forces, the
+// bonded contributions and the partial buffers that evaluation will write.
+int tune_tiles(ibx_ctx *c) {
+    c->tiles_pending = false;
+    if (c->tiles_req[0] >= 0) {
+        c->ftiles = tile_candidate(c, 0);
+        if (c->tiles_req[0] == 1) {
+            c->ftiles.persistent = true;
+            c->ftiles.chunk = c->tiles_req[1] > 0 ? c->tiles_req[1] : 1;
+            c->ftiles.grid = c->ncu * (c->tiles_req[2] > 0 ? c->tiles_req[2] : 5);
+        }
+        return 0;
+    }
+    int rc = ensure_events(c, 2);
+    if (rc) return rc;
+    const bool g0 = c->gr[0].present;
+    NbArgs a0;
+    if (g0) {
+        a0 = nb_args(c, 0);
+        a0.part = c->gr[0].part_buf(c->nb_par ^ 1);
+    }
+    constexpr int kReps = 3;
+    int best = 0;
+    for (int k = 0; k < kTileCandidates; ++k) {
+        const ForceTiles t = tile_candidate(c, k);
+        auto go = [&]() {
+            launch_step_forces(c->st, g0 ? &a0 : nullptr, c->term_nwg, c->wg_terms.p, term_launch(c), c->s.p,
+                               c->n_items, c->nterms, c->contrib.p, c->ncontrib, c->part_bd_buf(c->bd_par ^ 1),
+                               c->flags.p, nullptr, t);
+        };
+        go();   // first touch
+        HIPCHK(c, hipEventRecord(c->evpool[0], c->st));
+        for (int r = 0; r < kReps; ++r) go();
+        HIPCHK(c, hipEventRecord(c->evpool[1], c->st));
+        HIPCHK(c, hipEventSynchronize(c->evpool[1]));
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->evpool[0], c->evpool[1]));
+        c->tile_ms[k] = ms / kReps;
+        if (c->tile_ms[k] < c->tile_ms[best]) best = k;
+    }
+    c->ftiles = tile_candidate(c, best);
+    return 0;
 }
 
 int ensure_events(ibx_ctx *c, size_t n) {
@@ -1054,6 +1128,7 @@ int ibx::create_context(std::vector<HostSystem> &&hs, ibx_ctx **out) {
     std::unique_ptr<ibx_ctx> c(new ibx_ctx());
     c->nsys = nsys;
     c->device = dev;
+    c->ncu = prop.multiProcessorCount;
     c->hs = std::move(hs);
     for (int s = 0; s < nsys; ++s) {
         if (c->hs[s].nupdate != c->hs[0].nupdate)
@@ -1140,6 +1215,10 @@ int ibx_compute_forces(ibx_ctx *c, int rebuild) {
 
 int ibx_step(ibx_ctx *c, int nsteps, int step0) {
     if (!c || nsteps < 0) return fail(c, IBX_E_INVALID, "invalid arguments");
+    if (c->tiles_pending && nsteps > 0) {
+        int rc = tune_tiles(c);
+        if (rc) return rc;
+    }
     HIPCHK(c, hipMemsetAsync(c->flags.p, 0, c->flags.bytes(), c->st));
     HIPCHK(c, hipMemcpyAsync(c->step.p, &step0, sizeof(int), hipMemcpyHostToDevice, c->st));
     HIPCHK(c, hipStreamSynchronize(c->st));
@@ -1328,6 +1407,10 @@ int run_graph_blocks(ibx_ctx *c, int steps, int nblk, int cur) {
 
 int ibx_step_graph(ibx_ctx *c, int nsteps, int step0) {
     if (!c || nsteps < 0 || step0 < 0) return fail(c, IBX_E_INVALID, "invalid arguments");
+    if (c->tiles_pending && nsteps > 0) {   // before any capture
+        int rc = tune_tiles(c);
+        if (rc) return rc;
+    }
     // graph blocks of several block periods, then of one period, from step0 whatever its phase; eager tail
     const int period = block_period(c);
     const int longer = long_block(c);
@@ -1363,6 +1446,32 @@ int ibx_set_skin_tuning(ibx_ctx *c, int on) {
     if (!c) return fail(nullptr, IBX_E_INVALID, "null context");
     c->skin_tune = on != 0;
     drop_graph(c);
+    return 0;
+}
+
+int ibx_set_tiles(ibx_ctx *c, const int tiles[IBX_NTILES]) {
+    if (!c || !tiles) return fail(c, IBX_E_INVALID, "null argument");
+    if (tiles[0] < -1 || tiles[0] > 1) return fail(c, IBX_E_INVALID, "force launch layout %d is not -1, 0 or 1", tiles[0]);
+    if (tiles[1] < -1 || tiles[1] == 0 || tiles[1] > 64) return fail(c, IBX_E_INVALID, "blocks per take %d", tiles[1]);
+    if (tiles[2] < -1 || tiles[2] == 0 || tiles[2] > 16) return fail(c, IBX_E_INVALID, "workgroups per CU %d", tiles[2]);
+    for (int k = 0; k < IBX_NTILES; ++k) c->tiles_req[k] = tiles[k];
+    c->tiles_pending = true;   // applied (or tuned) at the next step call
+    for (float &m : c->tile_ms) m = -1.f;
+    drop_graph(c);
+    return 0;
+}
+
+int ibx_get_tiles(ibx_ctx *c, int tiles[IBX_NTILES], float ms[IBX_TILE_CANDIDATES]) {
+    if (!c || !tiles) return fail(c, IBX_E_INVALID, "null argument");
+    if (c->tiles_pending) {
+        for (int k = 0; k < IBX_NTILES; ++k) tiles[k] = -1;
+    } else {
+        tiles[0] = c->ftiles.persistent ? 1 : 0;
+        tiles[1] = c->ftiles.persistent ? c->ftiles.chunk : 0;
+        tiles[2] = c->ftiles.persistent && c->ncu > 0 ? c->ftiles.grid / c->ncu : 0;
+    }
+    if (ms)
+        for (int k = 0; k < IBX_TILE_CANDIDATES; ++k) ms[k] = c->tile_ms[k];
     return 0;
 }
 

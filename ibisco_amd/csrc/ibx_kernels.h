@@ -41,6 +41,8 @@ constexpr double kInnerFrac = 1.0 / 3.0;  // delta = this fraction of the list s
 constexpr int kLimAvp = 10;
 constexpr int kNptState = 16;
 constexpr int kNptMiu = kLimAvp, kNptGo = kLimAvp + 1;
+constexpr int kQueueInts = kXcd + 1;       // work queues of the persistent force launch, then its finished count
+constexpr int kTileCandidates = 4;         // force launch layouts the tile autotuner times
 
 // Position of entry k of sorted slot i in the Verlet list.  Entries of the
 // kAtomsPerWave slots of one wave are interleaved so that the kLanes lanes

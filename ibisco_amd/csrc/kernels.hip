@@ -788,7 +788,8 @@ __device__ __forceinline__ void nb_block(const int bidx, const DevSys *__restric
                                                       const float *__restrict__ nb_xoff,
                                                       const NbTab *__restrict__ nb_tabs, const int *__restrict__ nb_map,
                                                       int nitems_total, const double *__restrict__ s,
-                                                      float *__restrict__ fnb, double *__restrict__ part, int nblk) {
+                                                      float *__restrict__ fnb, double *__restrict__ part, int nblk,
+                                                      int &lds_sys, float *acc_lds) {
     extern __shared__ float4 lds_rows[];
     __shared__ NbTab tabs[kMultiTabsLds(kMulti)];
     __shared__ int map[kMulti ? kMaxNbTypes * kMaxNbTypes : 1];
@@ -807,7 +808,11 @@ __device__ __forceinline__ void nb_block(const int bidx, const DevSys *__restric
     const DevSys &S = sys[sid];
     if (!alive_at(S, *S.step_ctr + 1)) return;
     const DevGroup &G = S.g[kGroup];
-    if (kLds) {
+    // lds_sys: the system whose tables the LDS holds (a workgroup that works on several blocks in turn
+    // stages them once per system)
+    const bool restage = sid != lds_sys;
+    lds_sys = sid;
+    if (kLds && restage) {
         // four rows per thread in flight at once (the last row is read again past the end)
         constexpr int kU = 4;
         const int nrow = S.nb_nrow;
@@ -819,12 +824,12 @@ __device__ __forceinline__ void nb_block(const int bidx, const DevSys *__restric
             for (int u = 0; u < kU; ++u) lds_rows[min(r0 + u * kBlock, nrow - 1)] = v[u];   // same value twice at most
         }
     }
-    for (int t = threadIdx.x; t < S.nb_ntab && t < kMultiTabsLds(kMulti); t += kBlock) {
+    for (int t = threadIdx.x; restage && t < S.nb_ntab && t < kMultiTabsLds(kMulti); t += kBlock) {
         NbTab tb = nb_tabs[S.nb_tab_off + t];
         tb.row_off -= S.nb_row_off;
         tabs[t] = tb;
     }
-    if (kMulti)
+    if (kMulti && restage)
         for (int t = threadIdx.x; t < S.ntype * S.ntype; t += kBlock) {
             const int m = nb_map[S.nbmap_off + t];
             map[t] = m < 0 ? 0 : m - S.nb_tab_off;
@@ -881,11 +886,14 @@ __device__ __forceinline__ void nb_block(const int bidx, const DevSys *__restric
     }
     // Block sums of the nine energy and virial accumulators through LDS (this
     // kernel is VALU-bound, so no cross-lane arithmetic): the lane values go to
-    // the staging area, the table being no longer needed; sixteen threads per
-    // field add every sixteenth value, then one thread per field adds those
-    // sixteen sums, in FP64 and in a fixed order.
+    // acc_lds (null: the table's staging area, no longer needed); sixteen
+    // threads per field add every sixteenth value, then one thread per field
+    // adds those sixteen sums, in FP64 and in a fixed order.
     const float acc[kNbFields] = {e[0], e[1], e[2], w[0], w[1], w[2], w[3], w[4], w[5]};
-    float *acc_lds = reinterpret_cast<float *>(lds_rows);
+    if (!acc_lds) {
+        acc_lds = reinterpret_cast<float *>(lds_rows);
+        lds_sys = -1;   // the tables are overwritten
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < kNbFields; ++k) acc_lds[k * kBlock + threadIdx.x] = acc[k];
@@ -909,7 +917,8 @@ __device__ __forceinline__ void nb_block(const int bidx, const DevSys *__restric
 
 template <bool kMulti, bool kLds, int kGroup, bool kCubic>
 __global__ void __launch_bounds__(kBlock) k_nonbonded(IBX_NB_PARAMS) {
-    nb_block<kMulti, kLds, kGroup, kCubic>((int)blockIdx.x, IBX_NB_ARGS);
+    int lds_sys = -1;
+    nb_block<kMulti, kLds, kGroup, kCubic>((int)blockIdx.x, IBX_NB_ARGS, lds_sys, nullptr);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1713,13 +1722,13 @@ struct TermLaunchK {   // the arguments of k_terms
 // the field sums of the previous step (none on the first step of a run).  The
 // roles share the device with one another the way kernels on two streams
 // would, but with no dependency between streams to wait on; the field sums
-// read the other of the two nonbonded and bonded partial buffers.
+// read the other of the two nonbonded and bonded partial buffers.  Block b of
+// the launch's blocks: the previous step's field sums first (a whole number of
+// XCD rounds, so the nonbonded blocks keep their XCD-aware numbering), so they
+// run beside the nonbonded role rather than after it.
 template <bool kMulti, bool kLds, bool kCubic>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_step_forces(NbLaunchK nb, int nb_grid, TermLaunchK tk, FinLaunch fk,
-                                                        int fin_blocks) {
-    // the previous step's field sums first (a whole number of XCD rounds, so the nonbonded workgroups keep
-    // their XCD-aware numbering), so they run beside the nonbonded role rather than after it
-    int b = (int)blockIdx.x;
+__device__ __forceinline__ void step_forces_block(int b, const NbLaunchK &nb, int nb_grid, const TermLaunchK &tk,
+                                                  const FinLaunch &fk, int fin_blocks, int &lds_sys, float *acc_lds) {
     if (b < fin_blocks) {
         field_sums_block(b, fk.sys, fk.part_nb0, fk.part_nb1, fk.part_bd, fk.part_mv, fk.snb0, fk.snb1, fk.sbd, fk.smv,
                          fk.fsum, fk.obs, fk.avg, fk.sysstate, fk.step, fk.mode, fk.nsys, fk.npt);
@@ -1729,13 +1738,79 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))
     if (b < nb_grid) {
         nb_block<kMulti, kLds, 0, kCubic>(b, nb.sys, nb.wg_sys, nb.perm, nb.pos4, nb.list, nb.nneb, nb.dmax, nb.nsys,
                                           nb.cap, nb.rows, nb.xoff, nb.tabs, nb.map, nb.nitems_total, nb.s, nb.fnb,
-                                          nb.part, nb.nblk);
+                                          nb.part, nb.nblk, lds_sys, acc_lds);
         return;
     }
     b -= nb_grid;
     if (b < tk.nwg)
         terms_block(b, tk.nwg, tk.wg_sys, tk.A, tk.s, tk.nitems_total, tk.nterms, tk.contrib, tk.ncontrib, tk.part,
                     tk.flags);
+}
+
+// One workgroup per block.
+template <bool kMulti, bool kLds, bool kCubic>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5))) k_step_forces(NbLaunchK nb, int nb_grid, TermLaunchK tk, FinLaunch fk,
+                                                        int fin_blocks) {
+    int lds_sys = -1;
+    step_forces_block<kMulti, kLds, kCubic>((int)blockIdx.x, nb, nb_grid, tk, fk, fin_blocks, lds_sys, nullptr);
+}
+
+// Persistent: a grid sized to what the device holds at once works through the
+// same blocks from per-XCD work queues.  Block b belongs to queue b % kXcd
+// (the XCD the one-block-per-workgroup launch would run it on), and a
+// workgroup takes `chunk` blocks at a time from its own XCD's queue in block
+// order, then from the other XCDs' queues until every queue is drained; the
+// nonbonded tables stay in LDS while consecutive blocks belong to one system.
+// Every block computes what it computes in k_step_forces, so the results are
+// the same bits.  queue[0 .. kXcd) count the blocks taken from each queue and
+// queue[kXcd] the workgroups that have finished; the last to finish clears
+// them all for the next launch (no memset, so graphs may hold the launch).
+// Every workgroup ends once every queue is drained: no workgroup waits for
+// another.
+template <bool kMulti, bool kLds, bool kCubic>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(5)))
+k_step_forces_q(NbLaunchK nb, int nb_grid, TermLaunchK tk, FinLaunch fk, int fin_blocks, int *__restrict__ queue,
+                int chunk, int acc_off) {
+    extern __shared__ float4 lds_dyn[];
+    __shared__ int take, left[kXcd];
+    float *const acc_lds = reinterpret_cast<float *>(lds_dyn + acc_off);
+    const int total = fin_blocks + nb_grid + tk.nwg;
+    int lds_sys = -1;
+    int x = (int)blockIdx.x % kXcd;
+    for (;;) {
+        const int nx = total > x ? (total - x + kXcd - 1) / kXcd : 0;   // blocks x, x + kXcd, ... of queue x
+        for (;;) {
+            __syncthreads();   // the last block's LDS is no longer read, and take is free
+            if (threadIdx.x == 0) take = atomicAdd(&queue[x], chunk);
+            __syncthreads();
+            const int k0 = take;
+            if (k0 >= nx) break;
+            const int k1 = min(k0 + chunk, nx);
+            for (int k = k0; k < k1; ++k) {
+                if (k > k0) __syncthreads();
+                step_forces_block<kMulti, kLds, kCubic>(x + kXcd * k, nb, nb_grid, tk, fk, fin_blocks, lds_sys,
+                                                        acc_lds);
+            }
+        }
+        // queue x is drained: move to the next XCD's queue that still holds blocks (one read of each)
+        if (threadIdx.x < kXcd) {
+            const int y = threadIdx.x;
+            const int ny = total > y ? (total - y + kXcd - 1) / kXcd : 0;
+            left[y] = __hip_atomic_load(&queue[y], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < ny;
+        }
+        __syncthreads();
+        int nxt = -1;
+        for (int h = 1; h < kXcd && nxt < 0; ++h)
+            if (left[(x + h) % kXcd]) nxt = (x + h) % kXcd;
+        if (nxt < 0) break;
+        x = nxt;
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        if (atomicAdd(&queue[kXcd], 1) == (int)gridDim.x - 1) {
+            for (int y = 0; y <= kXcd; ++y) atomicExch(&queue[y], 0);
+        }
+    }
 }
 
 
@@ -1953,13 +2028,18 @@ void launch_finalize(hipStream_t st, const FinLaunch &f) {
 
 template <bool M, bool L, bool K>
 static void step_forces_go(hipStream_t st, dim3 grid, size_t lds, const NbLaunchK &nb, int nb_grid,
-                           const TermLaunchK &tk, const FinLaunch &fk, int fin_blocks) {
-    hipLaunchKernelGGL((k_step_forces<M, L, K>), grid, dim3(kBlock), lds, st, nb, nb_grid, tk, fk, fin_blocks);
+                           const TermLaunchK &tk, const FinLaunch &fk, int fin_blocks, const ForceTiles &ft,
+                           int acc_off) {
+    if (ft.persistent)
+        hipLaunchKernelGGL((k_step_forces_q<M, L, K>), grid, dim3(kBlock), lds, st, nb, nb_grid, tk, fk, fin_blocks,
+                           ft.queue, ft.chunk, acc_off);
+    else
+        hipLaunchKernelGGL((k_step_forces<M, L, K>), grid, dim3(kBlock), lds, st, nb, nb_grid, tk, fk, fin_blocks);
 }
 
 void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const int *wg_terms, const TermLaunch &b,
                         const double *s, int nitems_total, int nterms, float4 *contrib, int ncontrib, double *part_bd,
-                        int *flags, const FinLaunch *fin) {
+                        int *flags, const FinLaunch *fin, const ForceTiles &ft) {
     NbLaunchK k{};
     int nb_grid = 0;
     bool multi = false, lds = false, cubic = true;
@@ -1978,18 +2058,24 @@ void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const in
     FinLaunch f{};
     const int fin_blocks = fin ? xcd_grid(fin->nsys) : 0;
     if (fin) f = *fin;
-    const dim3 grid(nb_grid + term_nwg + fin_blocks);
+    dim3 grid(nb_grid + term_nwg + fin_blocks);
     if (grid.x == 0) return;
+    int acc_off = 0;   // persistent: the block sums are staged past the table rows (float4 units)
+    if (ft.persistent) {
+        grid.x = std::min<unsigned>(grid.x, (unsigned)xcd_grid(std::max(ft.grid, kXcd)));
+        acc_off = lds ? nb->max_rows_lds : 0;
+        bytes = (size_t)acc_off * sizeof(float4) + (size_t)kNbFields * kBlock * sizeof(float);
+    }
     if (multi) {
-        if (lds) cubic ? step_forces_go<true, true, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks)
-                       : step_forces_go<true, true, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks);
-        else cubic ? step_forces_go<true, false, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks)
-                   : step_forces_go<true, false, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks);
+        if (lds) cubic ? step_forces_go<true, true, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks, ft, acc_off)
+                       : step_forces_go<true, true, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks, ft, acc_off);
+        else cubic ? step_forces_go<true, false, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks, ft, acc_off)
+                   : step_forces_go<true, false, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks, ft, acc_off);
     } else {
-        if (lds) cubic ? step_forces_go<false, true, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks)
-                       : step_forces_go<false, true, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks);
-        else cubic ? step_forces_go<false, false, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks)
-                   : step_forces_go<false, false, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks);
+        if (lds) cubic ? step_forces_go<false, true, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks, ft, acc_off)
+                       : step_forces_go<false, true, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks, ft, acc_off);
+        else cubic ? step_forces_go<false, false, true>(st, grid, bytes, k, nb_grid, t, f, fin_blocks, ft, acc_off)
+                   : step_forces_go<false, false, false>(st, grid, bytes, k, nb_grid, t, f, fin_blocks, ft, acc_off);
     }
 }
 

@@ -123,9 +123,17 @@ void launch_npt(hipStream_t st, int nwg, DevSys *sys, const int *wg_sys, int nsy
                 double *vs_com, const int *itype, const int *label, int nitems_total, NbTab *nb_tabs, int *dmax);
 // The force phase of a step in one launch: the group-0 nonbonded force (nb null: none this step), the bonded
 // terms, and the field sums of the previous step (fin null: none).
+// How the force launch is laid out: one workgroup per block, or a persistent grid of `grid` workgroups taking
+// `chunk` blocks at a time from per-XCD work queues (queue: kXcd + 1 ints, zero between launches).
+struct ForceTiles {
+    bool persistent = false;
+    int chunk = 1;
+    int grid = 0;
+    int *queue = nullptr;
+};
 void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const int *wg_terms, const TermLaunch &b,
                         const double *s, int nitems_total, int nterms, float4 *contrib, int ncontrib, double *part_bd,
-                        int *flags, const FinLaunch *fin);
+                        int *flags, const FinLaunch *fin, const ForceTiles &ft);
 // Sets n ints to v (a kernel, not a memset node, so that code a caller may capture stays replayable).
 void launch_fill_int(hipStream_t st, int *p, int n, int v);
 

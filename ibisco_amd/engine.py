@@ -60,7 +60,8 @@ EXPORTS = ("ibx_create", "ibx_load_dir", "ibx_destroy", "ibx_last_error", "ibx_s
            "ibx_server_stats", "ibx_server_destroy", "ibx_server_last_error", "ibx_estimate_bytes",
            "ibx_set_skin_tuning", "ibx_get_skin", "ibx_set_respa", "ibx_get_counters", "ibx_step_noalloc",
            "ibx_assign", "ibx_get_capacity", "ibx_server_timing", "ibx_set_last_step", "ibx_trz_write_frame",
-           "ibx_restart_write", "ibx_write_trz", "ibx_write_restart", "ibx_output_last_error", "ibx_get_box")
+           "ibx_restart_write", "ibx_write_trz", "ibx_write_restart", "ibx_output_last_error", "ibx_get_box",
+           "ibx_set_tiles", "ibx_get_tiles")
 
 _lib = None
 
@@ -113,6 +114,8 @@ def load_library(path=LIB_PATH):
     L.ibx_output_last_error.restype = C.c_char_p
     L.ibx_get_capacity.argtypes = [vp, C.c_int, C.POINTER(C.c_int64)]
     L.ibx_set_skin_tuning.argtypes = [vp, C.c_int]
+    L.ibx_set_tiles.argtypes = [vp, _ip]
+    L.ibx_get_tiles.argtypes = [vp, _ip, C.POINTER(C.c_float)]
     L.ibx_set_respa.argtypes = [vp, C.c_int]
     L.ibx_get_skin.argtypes = [vp, C.c_int, _dp]
     L.ibx_server_create.argtypes = [C.POINTER(IbxServerConfig), C.POINTER(vp)]
@@ -376,6 +379,19 @@ class Engine:
 
     def set_skin_tuning(self, on=True):
         self._chk(self.L.ibx_set_skin_tuning(self.h, int(on)))
+
+    def set_tiles(self, layout=-1, chunk=-1, per_cu=-1):
+        """Force launch layout (ibx_set_tiles): 0 one workgroup per block, 1 persistent work queues taking
+        ``chunk`` blocks with ``per_cu`` workgroups per CU; -1 leaves the choice to the tile autotuner."""
+        t = np.array([layout, chunk, per_cu], dtype=np.int32)
+        self._chk(self.L.ibx_set_tiles(self.h, _p(t, C.c_int)))
+
+    def tiles(self):
+        """(layout in use [layout, chunk, workgroups per CU], the autotuner's ms per candidate)."""
+        t = np.zeros(3, dtype=np.int32)
+        ms = np.zeros(4, dtype=np.float32)
+        self._chk(self.L.ibx_get_tiles(self.h, _p(t, C.c_int), _p(ms, C.c_float)))
+        return [int(x) for x in t], [float(x) for x in ms]
 
     def skin(self, sys=0):
         d = np.zeros(2)

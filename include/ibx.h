@@ -29,6 +29,9 @@ extern "C" {
 #define IBX_E_IO (-6)            /* input file missing or unreadable */
 #define IBX_E_PENDING (-7)       /* ibx_server_wait: the job has not finished within the timeout */
 
+#define IBX_NTILES 3              /* ibx_set_tiles: force launch layout, blocks per take, workgroups per CU */
+#define IBX_TILE_CANDIDATES 4     /* force launch layouts the tile autotuner times */
+
 typedef struct ibx_ctx ibx_ctx;
 
 /* Raw tables of one kind: ntab tables whose rows (x as in the table file,
@@ -212,6 +215,26 @@ int ibx_set_respa(ibx_ctx *ctx, int k);
 int ibx_set_skin_tuning(ibx_ctx *ctx, int on);
 /* Current inner skin delta (nm) of each group of system sys. */
 int ibx_get_skin(ibx_ctx *ctx, int sys, double delta[2]);
+
+/* Tile autotuner and the layout of the force launch (no counterpart in the
+ * reference).  The force phase of a step (nonbonded blocks of 64 slots, bonded
+ * blocks of 256 terms, the previous step's field sums) runs either as one
+ * workgroup per block (tiles[0] = 0) or as a persistent grid of tiles[2]
+ * workgroups per CU that take tiles[1] blocks at a time from per-XCD device
+ * work queues (tiles[0] = 1), keeping a system's nonbonded tables in LDS from
+ * one block to the next.  Every layout computes each block the same way, so
+ * results are the same bits.  tiles[0] = -1 (the default) leaves the choice to
+ * the autotuner: at the first ibx_step / ibx_step_graph call after creation
+ * (or after this call) it times one-workgroup-per-block and persistent grids
+ * of 5 or 4 workgroups per CU taking 1 or 2 blocks, on the context's own
+ * systems, and keeps the fastest.  Values -1 in tiles[1] and tiles[2] take 1
+ * and 5. */
+int ibx_set_tiles(ibx_ctx *ctx, const int tiles[IBX_NTILES]);
+/* The layout in use (all -1 until the first step call applies or tunes it)
+ * and, when ms is not null, the autotuner's time per force launch of each
+ * candidate in milliseconds (-1: not timed): one workgroup per block,
+ * persistent 5 per CU taking 1 block, 5 per CU taking 2, 4 per CU taking 1. */
+int ibx_get_tiles(ibx_ctx *ctx, int tiles[IBX_NTILES], float ms[IBX_TILE_CANDIDATES]);
 
 /* Stops advancing system sys of a batch: its workgroups return at once, and
  * its state, observables, window sums and RDF keep the values of its last step

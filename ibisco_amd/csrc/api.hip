@@ -49,7 +49,7 @@ struct DBuf {
     }
     hipError_t alloc(size_t count) {
         release();
-        n = count ? count : 1;
+        n = (size_t)upload_count((long)count);   // never empty: kernels may read element 0 (ibx_index.h)
         std::lock_guard<std::mutex> lk(g_capture_mu);
         return hipMalloc((void **)&p, n * sizeof(T));
     }

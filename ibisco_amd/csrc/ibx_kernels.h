@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <cstdint>
 
+#include "ibx_index.h"
+
 namespace ibx {
 
 constexpr int kBlock = 256;        // threads per workgroup for every kernel
@@ -20,8 +22,6 @@ constexpr int kBdPart = 24;        // doubles per workgroup partial (bonded: ene
 constexpr int kMvPart = 16;        // doubles per workgroup partial (integrate: 15 fields)
 constexpr int kObs = 32;
 constexpr int kFields = 43;        // reduced fields per system: 9 nonbonded, 19 bonded, 15 integrate
-constexpr int kTermVsShift = 26;  // bonded term meta: kind (2 bits) | table << 2 | virtual-site members << 26
-constexpr int kTermTabMask = (1 << (kTermVsShift - 2)) - 1;
 constexpr int kMaxNbTabs = 64;     // nonbonded tables per system
 constexpr int kMaxNbTypes = 32;    // particle types per system with several nonbonded tables
 enum TermKind { KBOND = 0, KANGLE = 1, KTORS = 2, KOOP = 3 };

@@ -439,9 +439,9 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
         // it indexes inv_perm
         int xi[8];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) xi[e] = excl[ex0 + e < ex1 ? ex0 + e : 0];
+        for (int e = 0; e < 8; ++e) xi[e] = excl[excl_entry(ex0, e, ex1)];
 #pragma unroll
-        for (int e = 0; e < 8; ++e) xi[e] = inv_perm[ex0 + e < ex1 ? xi[e] : 0];
+        for (int e = 0; e < 8; ++e) xi[e] = inv_perm[excl_item(ex0, e, ex1, xi[e])];
 #pragma unroll
         for (int e = 0; e < 8; ++e) ex[e] = ex0 + e < ex1 ? xi[e] : -1;
     }
@@ -1180,10 +1180,10 @@ __device__ __forceinline__ void terms_block(const int b, const int nwg, const in
     double P[4][3];   // unused members (and padding terms) read item 0, never used
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        const int mr = (meta >= 0 && mem[r] >= 0) ? mem[r] : 0;
+        const int mr = term_member(meta, mem[r]);
         for (int k = 0; k < 3; ++k) P[r][k] = s[k * n + mr];
     }
-    const BdTab tb = A.tabs[(max(meta, 0) >> 2) & kTermTabMask];
+    const BdTab tb = A.tabs[term_table(meta)];
     issue_here(s, A.tabs, A.mem);
     if (!live) return;
     double en[19];   // 0..11 unused here: a term's energy is vt in class ek (see the sums below)
@@ -1311,7 +1311,7 @@ __device__ __forceinline__ void gather_contrib(const float4 *__restrict__ contri
     for (int q = a; q < b; q += 12) {
         float4 c[12];
 #pragma unroll
-        for (int u = 0; u < 12; ++u) c[u] = contrib[q + u < b ? q + u : 0];
+        for (int u = 0; u < 12; ++u) c[u] = contrib[contrib_entry(q + u, b)];
 #pragma unroll
         for (int u = 0; u < 12; ++u) {
             if (q + u < b) {
@@ -1373,7 +1373,7 @@ __global__ void __launch_bounds__(kBlock) k_move(const DevSys *__restrict__ sys,
         gather_contrib(A.contrib, A.ncontrib, A.inc_off[vs], A.inc_off[vs + 1], fvs);
     }
     int4 ref[2];
-    for (int g = 0; g < 2; ++g) ref[g] = A.pos4_ref[g][max(slot_g[g], 0)];   // used only where slot_g[g] >= 0
+    for (int g = 0; g < 2; ++g) ref[g] = A.pos4_ref[g][member_slot(slot_g[g])];   // used only where slot_g[g] >= 0
     issue_here();
     if (!live) return;
     double acc[kNf];

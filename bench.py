@@ -83,21 +83,52 @@ def host_threads():
 def cpu_baseline(systems, states, target_s=4.0, runs=3):
     """The FP32 OpenMP restatement of the reference loop (oracle/cpu_port.c) on the sampled replicas, run one
     after the other from their equilibrated states; rate = sum of beads x steps / sum of loop seconds, median
-    of ``runs`` runs, steps chosen so one run takes about ``target_s`` seconds."""
+    of ``runs`` runs, steps chosen so one run takes about ``target_s`` seconds.  It runs in a child process:
+    the OpenMP runtime binds the thread that starts it to a core, and every thread created after that (the
+    serving loop's among them) would inherit the binding."""
     import copy
-    from ibisco_amd import units
+    import subprocess
+    import tempfile
+
+    import numpy as np
+    from ibisco_amd import system as sysmod, units
     threads = host_threads()
-    os.environ["OMP_NUM_THREADS"] = str(threads)
-    os.environ.setdefault("OMP_PROC_BIND", "close")
-    os.environ.setdefault("OMP_PLACES", "cores")
+    with tempfile.TemporaryDirectory() as d:
+        path = os.path.join(d, "cpu_sample.npz")
+        arrays = {"nsys": np.array(len(systems), np.int32)}
+        for k, (sy, (s, v)) in enumerate(zip(systems, states)):
+            one = copy.copy(sy)
+            one.s, one.v = s[:sy.natoms].copy(), units.vel_from_reduced(v)
+            for key, val in sysmod.to_arrays(one).items():
+                arrays[f"sys{k}_{key}"] = val
+        np.savez(path, **arrays)
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--cpu-sample", path, "--cpu-seconds",
+                            str(target_s), "--cpu-runs", str(runs)], env=env, capture_output=True, text=True,
+                           timeout=max(600.0, 20 * target_s * runs))
+    if r.returncode != 0:
+        raise RuntimeError(f"CPU baseline failed ({r.returncode}): {r.stderr[-2000:]}")
+    return json.loads([x for x in r.stdout.splitlines() if x.startswith("{")][-1])
+
+
+def cpu_sample_run(path, target_s, runs):
+    """Child process of cpu_baseline: times the OpenMP restatement on the systems saved in ``path``."""
+    import copy
+
+    import numpy as np
+    from ibisco_amd import system as sysmod
     from oracle.oracle import CpuPort
+    with np.load(path) as z:
+        n = int(z["nsys"])
+        systems = []
+        for k in range(n):
+            pre = f"sys{k}_"
+            systems.append(sysmod.from_arrays({key[len(pre):]: z[key] for key in z.files if key.startswith(pre)}))
 
     def one_run(nsteps):
         secs, work = 0.0, 0
-        for sy, (s, v) in zip(systems, states):
-            one = copy.copy(sy)
-            one.s, one.v = s[:sy.natoms].copy(), units.vel_from_reduced(v)
-            p = CpuPort(one)
+        for sy in systems:
+            p = CpuPort(copy.copy(sy))
             secs += p.run(nsteps, 0)
             work += sy.natoms * nsteps
             del p
@@ -112,11 +143,12 @@ def cpu_baseline(systems, states, target_s=4.0, runs=3):
         secs, work = one_run(nsteps)
         rates.append(work / secs)
     rates.sort()
-    return {"value": rates[len(rates) // 2], "unit": "atom-steps/s", "cores": CpuPort.threads(), "kind": "port",
+    threads = CpuPort.threads()
+    return {"value": rates[len(rates) // 2], "unit": "atom-steps/s", "cores": threads, "kind": "port",
             "sample": f"{len(systems)} of the replicas ({sum(sy.natoms for sy in systems)} beads) one after the "
                       f"other, {nsteps} steps each from their equilibrated states (list rebuilt every {nupdate}), "
                       f"median of {runs} runs (rates {', '.join(f'{r:.3e}' for r in rates)}); FP32 OpenMP "
-                      f"restatement of the reference loop, gcc -O2 -fopenmp",
+                      f"restatement of the reference loop, gcc -O2 -fopenmp, in a child process",
             "cpu_model": cpu_model(), "nproc": os.cpu_count(), "omp_threads": threads}
 
 
@@ -529,6 +561,8 @@ def main():
     ap.add_argument("--serve-band", type=float, default=1.5,
                     help="serving: batch only jobs whose step counts are within this factor (<= 1: off)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline: seconds per run (3 runs)")
+    ap.add_argument("--cpu-runs", type=int, default=3, help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-sample", default="", help=argparse.SUPPRESS)   # the CPU baseline's child process
     ap.add_argument("--calib-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB five times (HBM counter calibration)")
     ap.add_argument("--only-ua", action="store_true",
@@ -543,6 +577,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"calib_copy_bytes_read": a.numel() * 4, "calib_copy_bytes_written": b.numel() * 4}))
+        return
+    if args.cpu_sample:
+        print(json.dumps(cpu_sample_run(args.cpu_sample, args.cpu_seconds, args.cpu_runs)))
         return
     if args.gpus != 1:
         print(f"bench.py measures one GPU only (asked for {args.gpus}); replicas and jobs shard across GPUs "

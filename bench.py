@@ -558,8 +558,9 @@ def main():
     ap.add_argument("--hybrid-steps", type=int, default=500)
     ap.add_argument("--serve-seconds", type=float, default=20.0)
     ap.add_argument("--serve-cohorts", type=int, default=3, help="batches the serving loop runs at once")
-    ap.add_argument("--serve-band", type=float, default=1.5,
-                    help="serving: batch only jobs whose step counts are within this factor (<= 1: off)")
+    ap.add_argument("--serve-band", type=float, default=1.0,
+                    help="serving: batch only jobs whose step counts are within this factor (<= 1: off; off "
+                         "measured the higher capacity, with free places refilled as jobs end)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline: seconds per run (3 runs)")
     ap.add_argument("--cpu-runs", type=int, default=3, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-sample", default="", help=argparse.SUPPRESS)   # the CPU baseline's child process

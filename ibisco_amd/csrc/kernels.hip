@@ -582,10 +582,17 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
             bool cand = r2u < lo;
             // the member's own FP64 position is read again here (rare) rather than held in registers
             if (r2u >= lo && r2u <= hi) cand = exact_r2(r64[i], r64[j], box0, box1, box2, bi0, bi1, bi2) < rl2;
-            bool skip = (j == i) || (hybrid && li == 3 && pos_label(pj.w) == 3);
+            // excluded: the member itself, and its exclusions -- the first eight held in registers (unused ones
+            // -1) and compared without branches, four of them when no member of the wave has more
+            bool skip = j == i;
+            if (nex <= 4) {
 #pragma unroll
-            for (int e = 0; e < 8; ++e)
-                if (e < nex) skip |= (ex[e] == j);
+                for (int e = 0; e < 4; ++e) skip |= ex[e] == j;
+            } else {
+#pragma unroll
+                for (int e = 0; e < 8; ++e) skip |= ex[e] == j;
+            }
+            if (hybrid && li == 3) skip |= pos_label(pj.w) == 3;   // VS-VS
             if (cand && !skip && ex1 - ex0 > 8)
                 for (int e = ex0 + 8; e < ex1; ++e) skip |= (inv_perm[excl[e]] == j);
             hit = cand && !skip;
@@ -593,10 +600,10 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
         }
         const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
         const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
-        const int k_in = n_in + __popc(gi & below);
-        const int k_out = cap - 1 - n_out - __popc(go & below);
-        const int k = inner ? k_in : k_out;
-        if (hit && (inner ? k_in < cap : k_out >= 0)) lrow[((k >> 2) << 6) + (k & 3)] = j;   // list_pos(i, k, cap)
+        if (hit) {   // entry k of the row: the front run from 0 up, the back run from cap - 1 down
+            const int k = inner ? n_in + __popc(gi & below) : cap - 1 - n_out - __popc(go & below);
+            if ((unsigned)k < (unsigned)cap) lrow[((k >> 2) << 6) + (k & 3)] = j;   // list_pos(i, k, cap)
+        }
         if (rdf && hit) {
             const int b = rdf_bin(r2u, inv_dr_u);
             if (b < nbin) atomicAdd(&rdf_h[b], 1u);

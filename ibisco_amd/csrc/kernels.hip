@@ -559,11 +559,12 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
     const unsigned below = (1u << sub) - 1u;
     int *const lrow = list + (valid ? list_pos(i, 0, cap) : 0);   // entry k of the row: lrow[kofs(k)]
     int n_in = 0, n_out = 0, r = 0;
-    // slot of candidate p (p ascending: the range walk only moves forward)
+    // slot of candidate p (p ascending: the range walk only moves forward); the lane's current range stays in
+    // registers, so LDS is read only when a candidate lies past its end
+    int2 cur = rng[grp][0];
     auto cand_slot = [&](int p) {
-        int2 e = rng[grp][r];
-        while (p >= e.x) e = rng[grp][++r];
-        return e.y + p;
+        while (p >= cur.x) cur = rng[grp][++r];
+        return cur.y + p;
     };
     for (int t0 = 0; t0 < tmax; t0 += kListLanes) {
         const int p = t0 + sub;

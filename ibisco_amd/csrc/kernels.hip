@@ -146,6 +146,12 @@ __device__ __forceinline__ int xcd_block(int n, int b) {
 __device__ __forceinline__ int xcd_block(int n) { return xcd_block(n, (int)blockIdx.x); }
 static inline int xcd_grid(int n) { return ((n + kXcd - 1) / kXcd) * kXcd; }
 
+// Position word of slot j, addressed by a 32-bit byte offset from the array
+// base so the gather keeps the base in scalar registers.
+__device__ __forceinline__ int4 ld_pos(const int4 *__restrict__ p, int j) {
+    return *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(p) + (uint32_t)j * 16u);
+}
+
 __device__ __forceinline__ int pos_type(int w) { return w & 0xffff; }
 __device__ __forceinline__ int pos_label(int w) { return (w >> 16) & 0xff; }
 
@@ -574,7 +580,7 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
         float r2u = 0.f;
         if (p < T) {
             j = cand_slot(p);
-            pj = pos4[j];
+            pj = ld_pos(pos4, j);
             float dx = (float)(int)((uint32_t)pi.x - (uint32_t)pj.x);
             float dy = (float)(int)((uint32_t)pi.y - (uint32_t)pj.y);
             float dz = (float)(int)((uint32_t)pi.z - (uint32_t)pj.z);
@@ -601,10 +607,11 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
         }
         const unsigned gi = (unsigned)(__ballot(hit && inner) >> gshift) & 0xFFFFu;
         const unsigned go = (unsigned)(__ballot(hit && !inner) >> gshift) & 0xFFFFu;
-        if (hit) {   // entry k of the row: the front run from 0 up, the back run from cap - 1 down
-            const int k = inner ? n_in + __popc(gi & below) : cap - 1 - n_out - __popc(go & below);
-            if ((unsigned)k < (unsigned)cap) lrow[((k >> 2) << 6) + (k & 3)] = j;   // list_pos(i, k, cap)
-        }
+        // entry k of the row: the front run from 0 up, the back run from cap - 1 down (both formed, one selected)
+        const int k_in = n_in + __popc(gi & below);
+        const int k_out = cap - 1 - n_out - __popc(go & below);
+        const int k = inner ? k_in : k_out;
+        if (hit && (unsigned)k < (unsigned)cap) lrow[((k >> 2) << 6) + (k & 3)] = j;   // list_pos(i, k, cap)
         if (rdf && hit) {
             const int b = rdf_bin(r2u, inv_dr_u);
             if (b < nbin) atomicAdd(&rdf_h[b], 1u);
@@ -719,11 +726,6 @@ __device__ __forceinline__ void nb_pair(const NbCtx &C, const int4 pi, int j, co
     w[5] = fmaf(fdy, dz, w[5]);
 }
 
-// Position word of slot j, addressed by a 32-bit byte offset from the array
-// base so the gather keeps the base in scalar registers.
-__device__ __forceinline__ int4 ld_pos(const int4 *__restrict__ p, int j) {
-    return *reinterpret_cast<const int4 *>(reinterpret_cast<const char *>(p) + (uint32_t)j * 16u);
-}
 
 // One run of a slot's list: entries q, q + kLanes, ... at l[0], l[kStep], ...
 // (kStep = +kWave for the front run, -kWave for the back run), kNbUnroll

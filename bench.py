@@ -269,7 +269,7 @@ def batched_leg(args, seed=3, with_plain=True):
 
 
 def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=3, nsample=12, ncheck=4, step_band=1.5,
-                 capacity=True):
+                 capacity=True, spare_places=0):
     """Serving: Poisson arrivals of CG melt jobs (1k-100k beads, 500-5000 steps, each with its own
     corrected table) submitted to the serving loop at their arrival times; every job returns its RDF and mean
     pressure.  Job states are equilibrated (batched, untimed) before the stream starts.  A sample of the jobs
@@ -304,7 +304,8 @@ def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=3, nsamp
             batch, nb = [], 0
     sample = sorted(range(nj), key=lambda k: systems[k].natoms)[::max(1, nj // nsample)][:nsample]
     check = set(sorted(sample, key=lambda k: systems[k].natoms)[:ncheck])
-    srv = Server(hbm_budget=budget, max_cohorts=max_cohorts, rdf_nbin=150, rdf_dr=0.01, step_band=step_band)
+    srv = Server(hbm_budget=budget, max_cohorts=max_cohorts, rdf_nbin=150, rdf_dr=0.01, step_band=step_band,
+                 spare_places=spare_places)
     results = [None] * nj
     ids = queue.Queue()
 
@@ -365,7 +366,8 @@ def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=3, nsamp
     print(f"[bench] serving: {vs:.3e} atom-steps/s, p50 {np.percentile(lat, 50):.2f} s, "
           f"p99 {np.percentile(lat, 99):.2f} s; plain sample {vp:.3e}", file=sys.stderr, flush=True)
     pres = [units.pressure_to_bar(r["mean_pressure"]) for r in ok]
-    cap = serve_capacity(systems, stream, vp, budget, max_cohorts, step_band, rate_hz) if capacity else None
+    cap = serve_capacity(systems, stream, vp, budget, max_cohorts, step_band, rate_hz,
+                         spare_places=spare_places) if capacity else None
     return {"workload": "cg_serving_poisson", "rate_jobs_per_s": rate_hz, "arrival_window_s": duration_s,
             "jobs": nj, "jobs_failed": nj - len(ok), "atom_steps": work,
             "offered_atom_steps_per_s": sum(sy.natoms * st_[2] for sy, st_ in zip(systems, stream)) / duration_s,
@@ -390,7 +392,8 @@ def serve_stream(rate_hz, duration_s, seed=4, budget=200e9, max_cohorts=3, nsamp
             "capacity": cap}
 
 
-def serve_capacity(systems, stream, vp, budget, max_cohorts, step_band, rate_hz, factor=3.0, duration_s=8.0,
+def serve_capacity(systems, stream, vp, budget, max_cohorts, step_band, rate_hz, spare_places=0, factor=3.0,
+                   duration_s=8.0,
                    seed=5):
     """The serving loop's capacity: jobs of the stream (cycled) arriving ``factor`` times faster than the stream
     (Poisson) for ``duration_s`` seconds, to a loop set up as for the stream, so that jobs are always queued:
@@ -408,7 +411,8 @@ def serve_capacity(systems, stream, vp, budget, max_cohorts, step_band, rate_hz,
         arrivals.append(t)
     n = len(systems)
     offered = sum(systems[k % n].natoms * stream[k % n][2] for k in range(len(arrivals))) / duration_s
-    srv = Server(hbm_budget=budget, max_cohorts=max_cohorts, rdf_nbin=150, rdf_dr=0.01, step_band=step_band)
+    srv = Server(hbm_budget=budget, max_cohorts=max_cohorts, rdf_nbin=150, rdf_dr=0.01, step_band=step_band,
+                 spare_places=spare_places)
     samples, ids = [], []
     t0 = time.perf_counter()
     nxt = 0.0
@@ -561,6 +565,8 @@ def main():
     ap.add_argument("--serve-band", type=float, default=1.0,
                     help="serving: batch only jobs whose step counts are within this factor (<= 1: off; off "
                          "measured the higher capacity, with free places refilled as jobs end)")
+    ap.add_argument("--serve-spares", type=int, default=0,
+                    help="serving: spare places per size class in a new batch (0: the loop's default)")
     ap.add_argument("--cpu-seconds", type=float, default=4.0, help="CPU baseline: seconds per run (3 runs)")
     ap.add_argument("--cpu-runs", type=int, default=3, help=argparse.SUPPRESS)
     ap.add_argument("--cpu-sample", default="", help=argparse.SUPPRESS)   # the CPU baseline's child process
@@ -641,7 +647,7 @@ def main():
                                    "plain_timed_counters": B["plain_counters"]}
     if "served" in legs and args.serve_seconds > 0:
         out["served_vs_plain"] = serve_stream(20.0, args.serve_seconds, max_cohorts=args.serve_cohorts,
-                                              step_band=args.serve_band)
+                                              step_band=args.serve_band, spare_places=args.serve_spares)
     if "ua" in legs:
         out["ua_melt_100k"] = ua_leg(args)
     if "hybrid" in legs and args.hybrid_steps > 0:

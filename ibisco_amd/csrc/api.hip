@@ -308,12 +308,12 @@ void set_group_constants(const HostSystem &h, int g, double unit, DevGroup &dg) 
     const double rc = h.rcut[g], rl = h.rlist[g];
     int nc[3];
     for (int k = 0; k < 3; ++k) {
-        nc[k] = (int)(kCellDiv * h.box[k] / rl);   // cells of edge >= rlist/kCellDiv (the reference: rlist)
+        nc[k] = (int)(cell_div(k) * h.box[k] / rl);   // cells of edge >= rlist/cell_div(k) (the reference: rlist)
         if (nc[k] < 1) nc[k] = 1;
         dg.nc[k] = nc[k];
         dg.ncmax[k] = nc[k];
     }
-    dg.all_pairs = (nc[0] < kMinCells || nc[1] < kMinCells || nc[2] < kMinCells) ? 1 : 0;
+    dg.all_pairs = (nc[0] < min_cells(0) || nc[1] < min_cells(1) || nc[2] < min_cells(2)) ? 1 : 0;
     dg.ncell = nc[0] * nc[1] * nc[2];
     dg.rl2 = rl * rl;
     const double rl2u = (rl / unit) * (rl / unit);
@@ -726,7 +726,7 @@ SlotSizes ibx::slot_needs(const HostSystem &h) {
         n.members[g] = g < h.ngroup ? (int)h.members[g].size() : 0;
         if (n.members[g] == 0) continue;
         int cells = 1;
-        for (int k = 0; k < 3; ++k) cells *= std::max(1, (int)(kCellDiv * h.box[k] / h.rlist[g]));
+        for (int k = 0; k < 3; ++k) cells *= std::max(1, (int)(cell_div(k) * h.box[k] / h.rlist[g]));
         n.cells[g] = cells;
     }
     for (const Table &t : h.nt) n.nb_rows += t.ndat + 1;

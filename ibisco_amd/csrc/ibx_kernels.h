@@ -30,11 +30,15 @@ constexpr int kAtomsPerWave = kWave / kLanes;
 constexpr int kNbAtomsPerBlock = kBlock / kLanes;
 constexpr int kListLanes = 16;            // lanes testing the candidates of one member in the list build
 constexpr int kListAtoms = kBlock / kListLanes;
-constexpr int kCellDiv = 3;               // cells per rlist along each axis (cell edge >= rlist/3)
-constexpr int kReach = kCellDiv;          // the list stencil: +-kReach cells around the member's cell
+constexpr int kCellDiv = 3;               // cells per rlist along x and y (cell edge >= rlist/3)
+constexpr int kCellDivZ = 6;              // cells per rlist along z: a column's candidate window ends closer to rlist
+constexpr int kReach = kCellDiv;          // the list stencil: +-kReach columns around the member's cell in x and y,
+constexpr int kReachZ = kCellDivZ;        // and +-kReachZ cells of each column in z
 constexpr int kStencil = 2 * kReach + 1;
 constexpr int kListRanges = 2 * kStencil * kStencil;   // candidate slot ranges per member (columns, at most 2 each)
-constexpr int kMinCells = kStencil;       // cells per axis for the stencil (fewer: every member scans the group)
+// cells per rlist along axis k, and the fewest cells along it for the stencil (fewer: every member scans the group)
+__host__ __device__ constexpr int cell_div(int k) { return k == 2 ? kCellDivZ : kCellDiv; }
+__host__ __device__ constexpr int min_cells(int k) { return 2 * cell_div(k) + 1; }
 constexpr double kInnerFrac = 1.0 / 3.0;  // delta = this fraction of the list skin rlist - rc
 // NPT (SCALEBP.f90): LIMAVP pressures averaged per box rescaling (ALLOCATEVAR.f90:17); the per-system state holds
 // that ring, then the scale factor of the step just finished and whether it rescales the box
@@ -61,7 +65,7 @@ struct DevGroup {
     int cell_off, ncell;           // global cell range
     int nc[3];
     int ncmax[3];                  // the grid the cell range was sized for (NPT: the grid never grows past it)
-    int all_pairs;                 // fewer than kMinCells cells along an axis: every member scans the group
+    int all_pairs;                 // fewer than min_cells(k) cells along an axis: every member scans the group
     double rl2;                    // rlist^2 (FP64 list criterion)
     float rl2u_lo, rl2u_hi;        // FP32 pre-classification thresholds, working units^2
     float rc2u;                    // rcut^2 in working units^2

@@ -370,15 +370,16 @@ __device__ __forceinline__ float cell_gap(int d, float f, float a) {
 // bit: a fixed-point test settles pairs far from the boundary and the FP64
 // criterion of the reference decides the rest.
 //
-// Cells have an edge of at least rlist/kCellDiv and are stored x-major, so the
-// cells of one (x, y) column are consecutive slots.  For member i the columns
-// within kReach cells whose distance from i is below rlist give at most two slot
-// ranges each (the z extent reachable from i, split where it wraps), and
+// Cells have an edge of at least rlist/kCellDiv in x and y and rlist/kCellDivZ in
+// z, and are stored x-major, so the cells of one (x, y) column are consecutive
+// slots.  For member i the columns within kReach cells whose distance from i is
+// below rlist give at most two slot ranges each (the z extent reachable from i,
+// within kReachZ cells, split where it wraps), and
 // kListLanes lanes walk their concatenation, 16 candidates per step.  Hits are
 // compacted with a wave ballot, so each list is in candidate order and
 // deterministic.  Pairs closer than rc + delta when the list is built fill the
-// front run, the others the back run.  With fewer than kMinCells cells along an
-// axis every member scans the whole group once.
+// front run, the others the back run.  With fewer than min_cells(k) cells along
+// axis k every member scans the whole group once.
 template <bool kCubic, bool kRdf>
 // Held to 64 VGPRs (eight waves per SIMD, measured faster than seven: the candidate loop waits on one position
 // gather per round, and more waves hide more of it).
@@ -488,11 +489,11 @@ k_build_list(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys,
             const float gxy = gx * gx + gy * gy;
             if (gxy >= reach2) continue;
             int dz0 = 0, dz1 = 0;
-            for (int d = -kReach; d < 0; ++d) {
+            for (int d = -kReachZ; d < 0; ++d) {
                 const float gz = cell_gap(d, f[2], a[2]);
                 if (gxy + gz * gz < reach2) { dz0 = d; break; }
             }
-            for (int d = kReach; d > 0; --d) {
+            for (int d = kReachZ; d > 0; --d) {
                 const float gz = cell_gap(d, f[2], a[2]);
                 if (gxy + gz * gz < reach2) { dz1 = d; break; }
             }
@@ -1517,7 +1518,7 @@ __global__ void __launch_bounds__(kBlock) k_npt_positions(const DevSys *__restri
 // The system's constants after its box is rescaled by MIU (SCALEBP.f90:38-40 and the cell grid of
 // UPDATE_NEIGHBOURLIST.F90:14-48): box, its inverse and half, the fixed-point unit and every radius
 // expressed in it, the nonbonded tables' abscissa scale, and the cell grid, which keeps cells of at least
-// rlist / kCellDiv (it shrinks with the box and grows back to the grid the cell range holds).  Pair distances
+// rlist / cell_div(k) (it shrinks with the box and grows back to the grid the cell range holds).  Pair distances
 // change by MIU while the fixed-point displacements do not, so the back run of each list is scanned until the
 // next rebuild.  One thread per system, after k_npt_positions (which reads kNptGo; cleared here).
 __global__ void __launch_bounds__(kBlock) k_npt_box(DevSys *__restrict__ sys, int nsys, double *__restrict__ npt,
@@ -1545,12 +1546,12 @@ __global__ void __launch_bounds__(kBlock) k_npt_box(DevSys *__restrict__ sys, in
         DevGroup &G = S.g[g];
         const double rl = sqrt(G.rl2), rc = sqrt(G.rc2);
         for (int k = 0; k < 3; ++k) {
-            int nc = (int)(kCellDiv * S.box[k] / rl);
+            int nc = (int)(cell_div(k) * S.box[k] / rl);
             nc = nc < 1 ? 1 : (nc > G.ncmax[k] ? G.ncmax[k] : nc);
             G.nc[k] = nc;
         }
         G.ncell = G.nc[0] * G.nc[1] * G.nc[2];
-        G.all_pairs = (G.nc[0] < kMinCells || G.nc[1] < kMinCells || G.nc[2] < kMinCells) ? 1 : 0;
+        G.all_pairs = (G.nc[0] < min_cells(0) || G.nc[1] < min_cells(1) || G.nc[2] < min_cells(2)) ? 1 : 0;
         const double u = S.unit;
         const double rl2u = (rl / u) * (rl / u);
         const double lband = 4e-6 + 16.0 * u / rl;

@@ -2,7 +2,7 @@
 
 - Batched replicas (the replicas workload): eight CG melts of 500-50,000 beads,
   each with its own Gaussian-corrected table (the smallest has fewer than 7
-  list cells per axis, so its members scan the whole system, and fewer than 3
+  list cells along x and y, so its members scan the whole system, and fewer than 3
   reference cells, so the oracle uses its all-pairs build), equilibrated
   together on the device, then each replica of the batch against the oracle on
   the same positions: pair sets bit-exact, forces <= 1e-5 relative RMS,

@@ -282,7 +282,7 @@ __global__ void __launch_bounds__(kBlock) k_place(const DevSys *__restrict__ sys
 
 // Orders each cell by member index so the sorted layout (and every sum over
 // it) is deterministic.  One workgroup per cell.
-constexpr int kSortLanes = 16;   // lanes per cell in the cell sort
+constexpr int kSortLanes = 4;    // lanes per cell in the cell sort (cells hold about one member on average)
 
 __global__ void __launch_bounds__(kBlock) k_sort_cells(const int *__restrict__ cell_sys, const DevSys *__restrict__ sys, int g,
                                                        const int *__restrict__ cell_start,
@@ -290,7 +290,7 @@ __global__ void __launch_bounds__(kBlock) k_sort_cells(const int *__restrict__ c
                                                        int *__restrict__ perm, int *__restrict__ slot_cell,
                                                        int *__restrict__ flags, int ncell, int only) {
     // kSortLanes lanes per cell (cells hold a few members each), kBlock / kSortLanes cells per workgroup
-    constexpr int kCap = 256;
+    constexpr int kCap = 64;   // members of a cell ranked in LDS; a more crowded cell is ranked in global memory
     __shared__ int vals[kBlock / kSortLanes][kCap];
     const int w = threadIdx.x / kSortLanes, lane = threadIdx.x & (kSortLanes - 1);
     const int gc = blockIdx.x * (kBlock / kSortLanes) + w;

@@ -183,3 +183,32 @@ def test_back_run_after_large_displacement(oracle_mod):
     rr = _rel_rms(fg, o.forces())
     print("force rel rms after", moved, "nm:", rr)
     assert rr <= FORCE_RTOL
+
+
+def test_crowded_cell(oracle_mod):
+    """A cell holding more members than the cell sort ranks in LDS (64): 200 of 2,000 single beads placed within
+    0.01 nm of one list cell's centre (cells of rlist/3 along x and y, rlist/6 along z) are ranked in global
+    memory.  The pair set is bit-exact against the oracle, and the nonbonded energy, virial and forces of the
+    system batched behind another equal its own alone bit for bit (the slot order is deterministic)."""
+    s = system.cg_melt(nchains=2000, nbeads=1, seed=21, avoid=0.8)
+    rl, L = s.control.rlist, np.asarray(s.box, float)
+    nc = np.array([int(3 * L[0] / rl), int(3 * L[1] / rl), int(6 * L[2] / rl)])
+    centre = ((nc // 2) + 0.5) / nc * L - 0.5 * L
+    rng = np.random.default_rng(4)
+    s.s[:200] = centre + rng.uniform(-0.01, 0.01, (200, 3))
+    folded = s.s - L * np.round(s.s / L)
+    cell = np.clip(((folded / L + 0.5) * nc).astype(int), 0, nc - 1)
+    _, counts = np.unique(cell[:, 0] * nc[1] * nc[2] + cell[:, 1] * nc[2] + cell[:, 2], return_counts=True)
+    assert counts.max() >= 200, counts.max()
+    o = oracle_mod.Oracle(s)
+    e = Engine(s)
+    po, pg = o.pairs(0), e.pairs(0, 0)
+    print("crowded cell: pairs", len(po), len(pg), "largest cell", counts.max(), flush=True)
+    assert np.array_equal(po, pg)
+    e.compute_forces(rebuild=True)
+    other = system.cg_melt(nchains=500, nbeads=1, seed=22, avoid=0.8)
+    batch = Engine([other, s])
+    batch.compute_forces(rebuild=True)
+    oa, ob = e.obs(), batch.obs(1)
+    assert np.array_equal(oa["vnb"], ob["vnb"]) and np.array_equal(oa["pt"], ob["pt"])
+    assert np.array_equal(e.forces(), batch.forces(1))

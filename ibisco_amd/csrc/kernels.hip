@@ -227,8 +227,9 @@ __global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *sys, const 
 // is where cell c ends) in two passes over tiles of kBlock cells, so that every
 // pass is spread over many workgroups (the counts of a large grid take one
 // workgroup most of its time just to read): k_scan_tiles scans each tile,
-// writes its total and clears the counts for the next rebuild, k_scan_add adds
-// the totals of the tiles before each one.  Integer sums, so the order is free.
+// writes its total and clears the counts for the next rebuild, k_scan_mid turns
+// the tile totals (one per kBlock cells) into their prefix in one workgroup, and
+// k_scan_add adds each tile's prefix.  Integer sums, so the order is free.
 __global__ void __launch_bounds__(kBlock) k_scan_tiles(int *__restrict__ count, int *__restrict__ start,
                                                       int *__restrict__ fill, int *__restrict__ tile_sum, int n) {
     __shared__ int wtot[kBlock / kWave];
@@ -249,19 +250,52 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(int *__restrict__ count, 
     if (threadIdx.x == kBlock - 1) tile_sum[blockIdx.x] = pre + x;
 }
 
-__global__ void __launch_bounds__(kBlock) k_scan_add(int *__restrict__ start, const int *__restrict__ tile_sum, int n) {
+// The tile sums replaced by their exclusive prefix, in one workgroup: each thread sums a contiguous run of
+// tiles, the runs' totals are scanned across the workgroup, and each thread rewrites its run (linear in the
+// number of tiles).
+__global__ void __launch_bounds__(kBlock) k_scan_mid(int *__restrict__ tile_sum, int ntile) {
     __shared__ int wtot[kBlock / kWave];
+    constexpr int U = 8;   // loads in flight per thread (indices clamped into the run, values past it dropped)
+    const int per = (ntile + kBlock - 1) / kBlock;
+    const int t0 = threadIdx.x * per, t1 = min(t0 + per, ntile);
+    int c = 0;
+    for (int t = t0; t < t1; t += U) {
+        int v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = tile_sum[min(t + u, t1 - 1)];
+#pragma unroll
+        for (int u = 0; u < U; ++u) c += t + u < t1 ? v[u] : 0;
+    }
+    const int lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
+    int x = c;   // inclusive scan over the wave
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int y = __shfl_up(x, off, kWave);
+        if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) wtot[wid] = x;
+    __syncthreads();
+    int run = x - c;
+    for (int w = 0; w < wid; ++w) run += wtot[w];
+    for (int t = t0; t < t1; t += U) {
+        int v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = tile_sum[min(t + u, t1 - 1)];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (t + u < t1) {
+                tile_sum[t + u] = run;
+                run += v[u];
+            }
+        }
+    }
+}
+
+// Adds each tile's offset (the exclusive prefix of the tile sums) to its starts.
+__global__ void __launch_bounds__(kBlock) k_scan_add(int *__restrict__ start, const int *__restrict__ tile_sum, int n) {
     const int b = blockIdx.x;
     if (b == 0) return;
-    int acc = 0;
-    for (int t = threadIdx.x; t < b; t += kBlock) acc += tile_sum[t];
-    for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_xor(acc, off, kWave);
-    if ((threadIdx.x & (kWave - 1)) == 0) wtot[threadIdx.x / kWave] = acc;
-    __syncthreads();
-    int off = 0;
-    for (int w = 0; w < kBlock / kWave; ++w) off += wtot[w];
     const int i = b * kBlock + threadIdx.x;
-    if (i <= n) start[i] += off;
+    if (i <= n) start[i] += tile_sum[b];
 }
 
 __global__ void __launch_bounds__(kBlock) k_place(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, int g,
@@ -1944,6 +1978,7 @@ void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
     const int ntile = (a.ncell_total + 1 + kBlock - 1) / kBlock;   // the starts have one more entry
     hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(kBlock), 0, st, a.cell_count, a.cell_start, a.cell_fill,
                        a.tile_sum, a.ncell_total);
+    hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(kBlock), 0, st, a.tile_sum, ntile);
     hipLaunchKernelGGL(k_scan_add, dim3(ntile), dim3(kBlock), 0, st, a.cell_start, a.tile_sum, a.ncell_total);
     hipLaunchKernelGGL(k_place, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.mem, a.mem_cell, a.cell_start,
                        a.cell_fill, a.tmp, a.only);

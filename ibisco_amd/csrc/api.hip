@@ -104,7 +104,7 @@ struct ibx_ctx {
     DBuf<int> dmax;                  // [group][system] largest displacement^2 since the rebuild (float bits)
     DBuf<int4> term_slot;           // contribution slot of each member of a term
     DBuf<int4> term_mem;
-    DBuf<float4> contrib;           // bonded contribution slots {x, y, z, 0}
+    DBuf<Contrib> contrib;          // bonded contribution slots
     int nterms = 0, term_nwg = 0, ncontrib = 0;
     DBuf<double> s, v, mass, invmass, vs_coeff, vs_invmass, part_bd, part_mv, fsum, obs, avg, sysstate;
     DBuf<double> vs_com;             // per virtual site (SoA like s): the mass-weighted centre of its atoms
@@ -772,7 +772,7 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, const SlotSizes &res, int
                + 3 * (N + 1) * 4                                  // excl_off vs_moff inc_off
                + at_least1(cap(need.excl, res.excl)) * 4 + at_least1(h.vs_atoms.size()) * 4
                + nterm_pad * (4 + 16 + 16) + (nterm_pad / kBlock) * 4   // meta, members, slots, wg_terms
-               + at_least1(members) * 16                           // contributions
+               + at_least1(members) * sizeof(Contrib)             // contributions
                + 2 * (nterm_pad / kBlock) * kBdPart * 8 + (N / kBlock) * kMvPart * 8
                + (size_t)(kFields + 2 * kObs + 8 + kNptState) * 8 + 2 * 4 + 8 * 4 + 4   // fsum obs avg sysstate npt dmax flags
                                                                        // step
@@ -1707,7 +1707,7 @@ int ibx::assign_system(ibx_ctx *c, int k, HostSystem &&h, int step0) {
         return fail(c, IBX_E_DEVICE, "assign: %s", hipGetErrorString(e));
     }
     // state of the place left by the system before: cleared
-    fill_words(c, c->contrib.p + L.contrib_off, 4 * (size_t)L.cap.contrib, 0);
+    fill_words(c, c->contrib.p + L.contrib_off, sizeof(Contrib) / sizeof(int) * (size_t)L.cap.contrib, 0);
     for (int q = 0; q < 3; ++q) {
         fill_words(c, c->fnb.p + (size_t)q * N + o, np, 0);
         fill_words(c, c->ftot.p + (size_t)q * N + o, np, 0);

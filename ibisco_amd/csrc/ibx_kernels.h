@@ -48,6 +48,11 @@ constexpr int kNptMiu = kLimAvp, kNptGo = kLimAvp + 1;
 constexpr int kQueueInts = kXcd + 1;       // work queues of the persistent force launch, then its finished count
 constexpr int kTileCandidates = 4;         // force launch layouts the tile autotuner times
 
+// One bonded contribution slot: the force of one term on one member (12 bytes; the owning item gathers its slots).
+struct Contrib {
+    float x, y, z;
+};
+
 // Position of entry k of sorted slot i in the Verlet list.  Entries of the
 // kAtomsPerWave slots of one wave are interleaved so that the kLanes lanes
 // serving a slot, and the slots of a wave, read consecutive words.

@@ -1207,7 +1207,7 @@ __device__ __forceinline__ void oop_term(const BdTab &t, const float4 *rows, con
 // Workgroup b of nwg (also one role of k_step_forces).
 __device__ __forceinline__ void terms_block(const int b, const int nwg, const int *__restrict__ wg_sys, const TermArgs &A,
                                             const double *__restrict__ s, int nitems_total, int nterms,
-                                            float4 *__restrict__ contrib, int ncontrib, double *__restrict__ part,
+                                            Contrib *__restrict__ contrib, int ncontrib, double *__restrict__ part,
                                             int *__restrict__ flags) {
     __shared__ double red[19 * 4];
     const int wsys = wg_sys[b];
@@ -1260,7 +1260,7 @@ __device__ __forceinline__ void terms_block(const int b, const int nwg, const in
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             if (r >= nm) break;
-            contrib[sl[r]] = make_float4((float)fm[r][0], (float)fm[r][1], (float)fm[r][2], 0.f);
+            contrib[sl[r]] = Contrib{(float)fm[r][0], (float)fm[r][1], (float)fm[r][2]};
         }
         // NEW_FORCE.F90:73-80 sums the bonded virial over atoms after DISTRIBUTE_VSFORCE.f90:27-37 has moved a
         // virtual site's force to its atoms (m_j/M each): a site's force counts at the centre of its atoms
@@ -1310,7 +1310,7 @@ __device__ __forceinline__ void terms_block(const int b, const int nwg, const in
 }
 
 __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys, TermArgs A, const double *__restrict__ s,
-                                                  int nitems_total, int nterms, float4 *__restrict__ contrib,
+                                                  int nitems_total, int nterms, Contrib *__restrict__ contrib,
                                                   int ncontrib, double *__restrict__ part, int *__restrict__ flags) {
     terms_block((int)blockIdx.x, (int)gridDim.x, wg_sys, A, s, nitems_total, nterms, contrib, ncontrib, part, flags);
 }
@@ -1326,7 +1326,7 @@ __global__ void __launch_bounds__(kBlock) k_terms(const int *__restrict__ wg_sys
 struct MoveArgs {
     double *s, *v;
     const float *fnb;
-    const float4 *contrib;  // bonded contribution slots {x, y, z, 0}; item i owns [inc_off[i], inc_off[i+1])
+    const Contrib *contrib;  // bonded contribution slots; item i owns [inc_off[i], inc_off[i+1])
     const int *inc_off;
     int ncontrib;
     float *ftot;
@@ -1351,10 +1351,10 @@ struct MoveArgs {
 // twelve slots at a time so their latencies overlap.  The loads are
 // unconditional (slots past b read slot 0 and are dropped): a load under a
 // per-lane condition would make the compiler wait for it before the next one.
-__device__ __forceinline__ void gather_contrib(const float4 *__restrict__ contrib, int nc, int a, int b, double f[3]) {
+__device__ __forceinline__ void gather_contrib(const Contrib *__restrict__ contrib, int nc, int a, int b, double f[3]) {
     (void)nc;
     for (int q = a; q < b; q += 12) {
-        float4 c[12];
+        Contrib c[12];
 #pragma unroll
         for (int u = 0; u < 12; ++u) c[u] = contrib[contrib_entry(q + u, b)];
 #pragma unroll
@@ -1755,7 +1755,7 @@ struct TermLaunchK {   // the arguments of k_terms
     TermArgs A;
     const double *s;
     int nitems_total, nterms;
-    float4 *contrib;
+    Contrib *contrib;
     int ncontrib;
     double *part;
     int *flags;
@@ -2032,7 +2032,7 @@ void launch_nonbonded(hipStream_t st, const NbArgs &a) {
 }
 
 void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &b, const double *s, int nitems_total,
-                  int nterms, float4 *contrib, int ncontrib, double *part, int *flags) {
+                  int nterms, Contrib *contrib, int ncontrib, double *part, int *flags) {
     TermArgs A{b.sys, b.mem, b.meta, b.slot, b.tabs, b.rows, b.xs, b.vs_com};
     hipLaunchKernelGGL(k_terms, dim3(nwg), dim3(kBlock), 0, st, wg_sys, A, s, nitems_total, nterms, contrib, ncontrib,
                        part, flags);
@@ -2084,7 +2084,7 @@ static void step_forces_go(hipStream_t st, dim3 grid, size_t lds, const NbLaunch
 }
 
 void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const int *wg_terms, const TermLaunch &b,
-                        const double *s, int nitems_total, int nterms, float4 *contrib, int ncontrib, double *part_bd,
+                        const double *s, int nitems_total, int nterms, Contrib *contrib, int ncontrib, double *part_bd,
                         int *flags, const FinLaunch *fin, const ForceTiles &ft) {
     NbLaunchK k{};
     int nb_grid = 0;

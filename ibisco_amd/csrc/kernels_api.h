@@ -74,7 +74,7 @@ struct TermLaunch {
 struct MoveLaunch {
     double *s, *v;
     const float *fnb;
-    const float4 *contrib;
+    const Contrib *contrib;
     const int *inc_off;
     int ncontrib;
     float *ftot;
@@ -111,7 +111,7 @@ void launch_mts(hipStream_t st, bool save, float *fnb, const int *label, int n, 
 void launch_rebuild(hipStream_t st, const RebuildArgs &a);
 void launch_nonbonded(hipStream_t st, const NbArgs &a);
 void launch_terms(hipStream_t st, int nwg, const int *wg_sys, const TermLaunch &b, const double *s, int nitems_total,
-                  int nterms, float4 *contrib, int ncontrib, double *part, int *flags);
+                  int nterms, Contrib *contrib, int ncontrib, double *part, int *flags);
 void launch_move(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, const MoveLaunch &m);
 void launch_vs_place(hipStream_t st, int nwg, const DevSys *sys, const int *wg_sys, double *s, int nitems_total,
                      const int *label, const int *itype, const int *vs_center, const int *vs_moff, const int *vs_members,
@@ -132,7 +132,7 @@ struct ForceTiles {
     int *queue = nullptr;
 };
 void launch_step_forces(hipStream_t st, const NbArgs *nb, int term_nwg, const int *wg_terms, const TermLaunch &b,
-                        const double *s, int nitems_total, int nterms, float4 *contrib, int ncontrib, double *part_bd,
+                        const double *s, int nitems_total, int nterms, Contrib *contrib, int ncontrib, double *part_bd,
                         int *flags, const FinLaunch *fin, const ForceTiles &ft);
 // Sets n ints to v (a kernel, not a memset node, so that code a caller may capture stays replayable).
 void launch_fill_int(hipStream_t st, int *p, int n, int v);

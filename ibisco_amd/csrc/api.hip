@@ -62,7 +62,7 @@ struct GroupDev {
     int nslot_pad = 0, nwg = 0, ncell_total = 0, cap = 0, max_rows = 0;
     bool multi = false, present = false;
     DBuf<int2> nneb;
-    DBuf<int> mem, mem_cell, cell_count, cell_start, cell_fill, tile_sum, tmp, perm, slot_cell, inv_perm, list, wg_sys,
+    DBuf<int> mem, mem_cell, mem_rank, cell_count, cell_start, tile_sum, tmp, perm, slot_cell, inv_perm, list, wg_sys,
         cell_sys;
     DBuf<double4> r64;
     DBuf<int4> pos4, pos4_ref;
@@ -672,10 +672,10 @@ int build_device(ibx_ctx *c) {
             (rc = upload(c, G.perm, perm_init)))
             return rc;
         HIPCHK(c, G.mem_cell.alloc(slot));
+        HIPCHK(c, G.mem_rank.alloc(slot));
         HIPCHK(c, G.cell_count.alloc(cell));
         HIPCHK(c, memset_st(c, G.cell_count.p, 0, G.cell_count.bytes()));   // each scan clears it for the next rebuild
         HIPCHK(c, G.cell_start.alloc(cell + 1));
-        HIPCHK(c, G.cell_fill.alloc(cell));
         HIPCHK(c, G.tile_sum.alloc((cell + 1 + kBlock - 1) / kBlock));
         HIPCHK(c, G.tmp.alloc(slot));
         HIPCHK(c, G.slot_cell.alloc(slot));
@@ -707,7 +707,7 @@ size_t device_bytes(const ibx_ctx *c) {
                c->rdf.bytes() + c->rdf_samples.bytes() + c->rdf_ivol.bytes() + c->mts_f.bytes() + c->mts_part.bytes() + c->vs_com.bytes() + c->npt.bytes();
     for (const GroupDev &G : c->gr)
         b += G.nneb.bytes() + G.mem.bytes() + G.mem_cell.bytes() + G.cell_count.bytes() + G.cell_start.bytes() +
-             G.cell_fill.bytes() + G.tile_sum.bytes() + G.tmp.bytes() + G.perm.bytes() + G.slot_cell.bytes() + G.inv_perm.bytes() +
+             G.mem_rank.bytes() + G.tile_sum.bytes() + G.tmp.bytes() + G.perm.bytes() + G.slot_cell.bytes() + G.inv_perm.bytes() +
              G.list.bytes() + G.wg_sys.bytes() + G.cell_sys.bytes() + G.r64.bytes() + G.pos4.bytes() +
              G.pos4_ref.bytes() + G.part.bytes();
     return b;
@@ -792,8 +792,8 @@ size_t ibx::estimate_device_bytes(const HostSystem &h, const SlotSizes &res, int
         const double nbr = 4.0 / 3.0 * M_PI * h.rlist[g] * h.rlist[g] * h.rlist[g] * nm /
                            (h.box[0] * h.box[1] * h.box[2]);
         const size_t lcap = (size_t)pad_to((int)(std::max(1.5, h.list_margin) * nbr) + 64, 8);
-        b += slots * (4 * 5 + 32 + 16 + 16 + 8)                  // mem mem_cell tmp perm slot_cell r64 pos4 pos4_ref nneb
-             + N * 4 + nwg * 4 + ncell * 16 + 4                   // inv_perm, wg_sys, cell arrays (one more start)
+        b += slots * (4 * 6 + 32 + 16 + 16 + 8)                  // mem mem_cell mem_rank tmp perm slot_cell r64 pos4 pos4_ref nneb
+             + N * 4 + nwg * 4 + ncell * 12 + 4                   // inv_perm, wg_sys, cell arrays (one more start)
              + (ncell + 1 + kBlock - 1) / kBlock * 4              // scan tile sums
              + 2 * nwg * (kBlock / kNbAtomsPerBlock) * kNbPart * 8 + slots * lcap * 4;
     }
@@ -811,7 +811,7 @@ RebuildArgs rebuild_args(ibx_ctx *c, int g) {
     for (const DevSys &D : c->dsys) a.cubic = a.cubic && D.rho[1] == 1.0f && D.rho[2] == 1.0f;
     a.mem = G.mem.p; a.s = c->s.p; a.itype = c->itype.p; a.label = c->label.p;
     a.mem_cell = G.mem_cell.p; a.cell_count = G.cell_count.p; a.cell_start = G.cell_start.p;
-    a.cell_fill = G.cell_fill.p; a.tile_sum = G.tile_sum.p; a.tmp = G.tmp.p; a.perm = G.perm.p; a.slot_cell = G.slot_cell.p;
+    a.mem_rank = G.mem_rank.p; a.tile_sum = G.tile_sum.p; a.tmp = G.tmp.p; a.perm = G.perm.p; a.slot_cell = G.slot_cell.p;
     a.r64 = G.r64.p; a.pos4 = G.pos4.p; a.inv_perm = G.inv_perm.p; a.pos4_ref = G.pos4_ref.p;
     a.dmax = c->dmax.p + (size_t)g * c->nsys; a.nsys = c->nsys;
     a.tune = -1; a.nmove = c->nupdate; a.tune_sys = c->sys.p; a.dmax_all = c->dmax.p;

@@ -197,7 +197,8 @@ __device__ void tune_skin(DevSys *sys, int nsys, const int *dmax, int nmove, int
 __global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *sys, const int *__restrict__ wg_sys, int nwg,
                                                        int g, const int *__restrict__ mem, const double *__restrict__ s,
                                                        int nitems_total, int *__restrict__ mem_cell,
-                                                       int *__restrict__ cell_count, DevSys *tune_sys, int nsys,
+                                                       int *__restrict__ mem_rank, int *__restrict__ cell_count,
+                                                       DevSys *tune_sys, int nsys,
                                                        const int *__restrict__ dmax_all, int nmove, int tune,
                                                        int only) {
     if ((int)blockIdx.x >= nwg) {
@@ -220,7 +221,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *sys, const 
     }
     int cell = (c[0] * G.nc[1] + c[1]) * G.nc[2] + c[2];
     mem_cell[m] = cell;
-    atomicAdd(&cell_count[G.cell_off + cell], 1);
+    mem_rank[m] = atomicAdd(&cell_count[G.cell_off + cell], 1);   // the member's place among its cell's arrivals
 }
 
 // Exclusive scan of the cell counts into cell starts (n + 1 of them: start[c + 1]
@@ -231,7 +232,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_count(const DevSys *sys, const 
 // the tile totals (one per kBlock cells) into their prefix in one workgroup, and
 // k_scan_add adds each tile's prefix.  Integer sums, so the order is free.
 __global__ void __launch_bounds__(kBlock) k_scan_tiles(int *__restrict__ count, int *__restrict__ start,
-                                                      int *__restrict__ fill, int *__restrict__ tile_sum, int n) {
+                                                      int *__restrict__ tile_sum, int n) {
     __shared__ int wtot[kBlock / kWave];
     const int i = blockIdx.x * kBlock + threadIdx.x, lane = threadIdx.x & (kWave - 1), wid = threadIdx.x / kWave;
     const int c = i < n ? count[i] : 0;
@@ -246,7 +247,6 @@ __global__ void __launch_bounds__(kBlock) k_scan_tiles(int *__restrict__ count, 
     int pre = 0;
     for (int w = 0; w < wid; ++w) pre += wtot[w];
     if (i <= n) start[i] = pre + x - c;
-    if (i < n) fill[i] = 0;
     if (threadIdx.x == kBlock - 1) tile_sum[blockIdx.x] = pre + x;
 }
 
@@ -300,7 +300,7 @@ __global__ void __launch_bounds__(kBlock) k_scan_add(int *__restrict__ start, co
 
 __global__ void __launch_bounds__(kBlock) k_place(const DevSys *__restrict__ sys, const int *__restrict__ wg_sys, int g,
                                                   const int *__restrict__ mem, const int *__restrict__ mem_cell,
-                                                  const int *__restrict__ cell_start, int *__restrict__ cell_fill,
+                                                  const int *__restrict__ cell_start, const int *__restrict__ mem_rank,
                                                   int *__restrict__ tmp, int only) {
     if (only >= 0 && wg_sys[blockIdx.x] != only) return;
     const DevSys &S = sys[wg_sys[blockIdx.x]];
@@ -310,7 +310,7 @@ __global__ void __launch_bounds__(kBlock) k_place(const DevSys *__restrict__ sys
     if (mem[m] < 0) return;
     int gc = G.cell_off + mem_cell[m];
     int local = cell_start[gc] - cell_start[G.cell_off];
-    int pos = atomicAdd(&cell_fill[gc], 1);
+    const int pos = mem_rank[m];
     tmp[G.slot_off + local + pos] = m;
 }
 
@@ -1973,15 +1973,15 @@ void launch_fill_int(hipStream_t st, int *p, int n, int v) {
 void launch_rebuild(hipStream_t st, const RebuildArgs &a) {
     const int ntune = a.tune >= 0 ? (2 * a.nsys + kBlock - 1) / kBlock : 0;
     hipLaunchKernelGGL(k_cell_count, dim3(a.nwg + ntune), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.nwg, a.g, a.mem, a.s,
-                       a.nitems_total, a.mem_cell, a.cell_count, a.tune_sys, a.nsys, a.dmax_all, a.nmove, a.tune,
+                       a.nitems_total, a.mem_cell, a.mem_rank, a.cell_count, a.tune_sys, a.nsys, a.dmax_all, a.nmove, a.tune,
                        a.only);
     const int ntile = (a.ncell_total + 1 + kBlock - 1) / kBlock;   // the starts have one more entry
-    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(kBlock), 0, st, a.cell_count, a.cell_start, a.cell_fill,
+    hipLaunchKernelGGL(k_scan_tiles, dim3(ntile), dim3(kBlock), 0, st, a.cell_count, a.cell_start,
                        a.tile_sum, a.ncell_total);
     hipLaunchKernelGGL(k_scan_mid, dim3(1), dim3(kBlock), 0, st, a.tile_sum, ntile);
     hipLaunchKernelGGL(k_scan_add, dim3(ntile), dim3(kBlock), 0, st, a.cell_start, a.tile_sum, a.ncell_total);
     hipLaunchKernelGGL(k_place, dim3(a.nwg), dim3(kBlock), 0, st, a.sys, a.wg_sys, a.g, a.mem, a.mem_cell, a.cell_start,
-                       a.cell_fill, a.tmp, a.only);
+                       a.mem_rank, a.tmp, a.only);
     hipLaunchKernelGGL(k_sort_cells, dim3((a.ncell_total + kBlock / kSortLanes - 1) / (kBlock / kSortLanes)), dim3(kBlock), 0, st,
                        a.cell_sys, a.sys, a.g, a.cell_start,
                        a.tmp, a.mem, a.perm, a.slot_cell, a.flags, a.ncell_total, a.only);

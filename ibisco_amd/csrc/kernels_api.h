@@ -15,7 +15,7 @@ struct RebuildArgs {
     const int *mem;
     const double *s;
     const int *itype, *label;
-    int *mem_cell, *cell_count, *cell_start, *cell_fill, *tmp, *perm, *slot_cell;
+    int *mem_cell, *mem_rank, *cell_count, *cell_start, *tmp, *perm, *slot_cell;
     int *tile_sum;      // one per kBlock cells (the cell-start scan)
     double4 *r64;
     int4 *pos4, *pos4_ref;
